@@ -1,0 +1,450 @@
+// Species storage kernels: native load, stable compaction, stable counting
+// sort by cell, emitter injection.
+//
+// Compaction keeps the order of the live particles, which is what the
+// reference's vector::erase removal does (species.cpp:95-121).  The counting
+// sort orders particles by cell key i*(ny-1)+j and, inside a cell, by their
+// previous storage index: a stable sort, equal element for element to
+// std::stable_sort by key on the CPU.  Absorbed particles get the key ncell and
+// are dropped, so a sort also compacts.
+#include <algorithm>
+
+#include "device_common.hpp"
+#include "epic_internal.hpp"
+
+namespace epic {
+
+constexpr int AUX_BLOCK = 256;
+constexpr int CMP_PPT = 8;  // elements per thread in compaction
+constexpr int CMP_CH = AUX_BLOCK * CMP_PPT;
+
+// ---- native load ------------------------------------------------------------
+template <typename P>
+__global__ void load_kernel(P* x, P* y, P* vx, P* vy, P* vz, uint32_t* id, int64_t n, double x0, double y0,
+                            double Lx, double Ly, double sig, double ux, double uy, int vz_thermal, uint32_t k0,
+                            uint32_t k1, uint32_t sp) {
+  const P xhi = (P)(x0 + Lx), yhi = (P)(y0 + Ly);
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
+    const u32x4 r0 = philox4x32_10({lo, hi, RNG_LOAD_POS, sp}, k0, k1);
+    const u32x4 r1 = philox4x32_10({lo, hi, RNG_LOAD_VEL, sp}, k0, k1);
+    const u32x4 r2 = philox4x32_10({lo, hi, RNG_LOAD_VEL2, sp}, k0, k1);
+    P px = (P)(x0 + Lx * u01_53(r0.a, r0.b));
+    P py = (P)(y0 + Ly * u01_53(r0.c, r0.d));
+    if (px >= xhi) px = nextafter(xhi, (P)x0);
+    if (py >= yhi) py = nextafter(yhi, (P)y0);
+    // Box-Muller: (1 - u) in (0, 1]
+    const double u1 = 1.0 - u01_53(r1.a, r1.b), u2 = u01_53(r1.c, r1.d);
+    const double u3 = 1.0 - u01_53(r2.a, r2.b), u4 = u01_53(r2.c, r2.d);
+    const double m1 = sqrt(-2.0 * log(u1)), m2 = sqrt(-2.0 * log(u3));
+    const double g1 = m1 * cos(2.0 * M_PI * u2), g2 = m1 * sin(2.0 * M_PI * u2);
+    const double g3 = m2 * cos(2.0 * M_PI * u4);
+    x[p] = px;
+    y[p] = py;
+    vx[p] = (P)(sig * g1 + ux);
+    vy[p] = (P)(sig * g2 + uy);
+    vz[p] = (P)(vz_thermal ? sig * g3 : 0.0);
+    id[p] = (uint32_t)p;
+  }
+}
+
+__global__ void set_count_kernel(int64_t* d_n, int64_t v) {
+  d_n[0] = v;
+  d_n[1] = v;
+}
+
+hipError_t launch_load(Ctx* c, int s, int64_t n, int vz_thermal) {
+  SpeciesDev& S = c->sp[s];
+  const epic_domain_desc& d = c->dom;
+  const double Lx = (d.nx - 1) * d.dx, Ly = (d.ny - 1) * d.dy;
+  // sigma = sqrt(kT/m): the spread of SampleVel's Irwin-Hall sum
+  // v_th*sqrt(2)*(R1+R2+R3-1.5) with v_th = sqrt(2 kT/m) (init.cpp:104-108)
+  const double kT = S.desc.temp * QE;
+  const double sig = sqrt(kT / S.desc.mass) / d.vel_norm;
+  const double ux = S.desc.vsx, uy = S.desc.vsy;
+  const uint32_t k0 = (uint32_t)d.seed, k1 = (uint32_t)(d.seed >> 32);
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  if (d.precision == EPIC_FP64)
+    hipLaunchKernelGGL(load_kernel<double>, dim3(std::max(grid, 1)), dim3(256), 0, c->stream, (double*)S.x,
+                       (double*)S.y, (double*)S.vx, (double*)S.vy, (double*)S.vz, S.id, n, d.x0, d.y0, Lx, Ly, sig,
+                       ux, uy, vz_thermal, k0, k1, (uint32_t)s);
+  else
+    hipLaunchKernelGGL(load_kernel<float>, dim3(std::max(grid, 1)), dim3(256), 0, c->stream, (float*)S.x,
+                       (float*)S.y, (float*)S.vx, (float*)S.vy, (float*)S.vz, S.id, n, d.x0, d.y0, Lx, Ly, sig, ux,
+                       uy, vz_thermal, k0, k1, (uint32_t)s);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(1), 0, c->stream, S.d_n, n);
+  S.n_host = n;
+  return hipGetLastError();
+}
+
+// ---- block-level exclusive scan helper --------------------------------------
+// Exclusive scan of one int per thread over a 256-thread block.
+__device__ __forceinline__ int block_excl_scan(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) sh[wid] = inc;
+  __syncthreads();
+  int wbase = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    if (w < wid) wbase += sh[w];
+    tot += sh[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return wbase + inc - v;
+}
+
+// ---- compaction -------------------------------------------------------------
+template <typename P>
+__global__ __launch_bounds__(AUX_BLOCK) void live_count_kernel(const P* x, const int64_t* d_n, int* blk) {
+  __shared__ int sh[AUX_BLOCK / 64];
+  const int64_t n = *d_n;
+  const int64_t base = (int64_t)blockIdx.x * CMP_CH + (int64_t)threadIdx.x * CMP_PPT;
+  int c = 0;
+  for (int k = 0; k < CMP_PPT; k++) {
+    const int64_t i = base + k;
+    if (i < n && !isnan(x[i])) c++;
+  }
+  int tot;
+  block_excl_scan(c, sh, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+// Exclusive scan of m ints in place (one block of 1024); writes the total to
+// out[m].
+__global__ __launch_bounds__(1024) void scan_kernel(int* a, int64_t m) {
+  __shared__ long long sh[1024 / 64];
+  const int T = blockDim.x;
+  const int64_t per = (m + T - 1) / T;
+  const int64_t lo = threadIdx.x * per, hi = std::min<int64_t>(lo + per, m);
+  long long s = 0;
+  for (int64_t i = lo; i < hi; i++) s += a[i];
+  // block exclusive scan of s (64-bit)
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = T >> 6;
+  long long inc = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) sh[wid] = inc;
+  __syncthreads();
+  long long base = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    if (w < wid) base += sh[w];
+    tot += sh[w];
+  }
+  long long run = base + inc - s;
+  for (int64_t i = lo; i < hi; i++) {
+    const int v = a[i];
+    a[i] = (int)run;
+    run += v;
+  }
+  if (threadIdx.x == 0) a[m] = (int)tot;
+}
+
+template <typename P>
+__global__ __launch_bounds__(AUX_BLOCK) void compact_scatter_kernel(const P* x, const P* y, const P* vx, const P* vy,
+                                                                    const P* vz, const uint32_t* id, P* x2, P* y2,
+                                                                    P* vx2, P* vy2, P* vz2, uint32_t* id2,
+                                                                    const int64_t* d_n, const int* blk_off) {
+  __shared__ int sh[AUX_BLOCK / 64];
+  const int64_t n = *d_n;
+  const int64_t base = (int64_t)blockIdx.x * CMP_CH + (int64_t)threadIdx.x * CMP_PPT;
+  int c = 0;
+  for (int k = 0; k < CMP_PPT; k++) {
+    const int64_t i = base + k;
+    if (i < n && !isnan(x[i])) c++;
+  }
+  int tot;
+  int off = blk_off[blockIdx.x] + block_excl_scan(c, sh, &tot);
+  for (int k = 0; k < CMP_PPT; k++) {
+    const int64_t i = base + k;
+    if (i < n && !isnan(x[i])) {
+      x2[off] = x[i]; y2[off] = y[i]; vx2[off] = vx[i]; vy2[off] = vy[i]; vz2[off] = vz[i]; id2[off] = id[i];
+      off++;
+    }
+  }
+}
+
+__global__ void set_count_from_int_kernel(int64_t* d_n, const int* v) { d_n[0] = (int64_t)*v; }
+
+static void swap_buffers(SpeciesDev& S) {
+  std::swap(S.x, S.x2); std::swap(S.y, S.y2); std::swap(S.vx, S.vx2);
+  std::swap(S.vy, S.vy2); std::swap(S.vz, S.vz2); std::swap(S.id, S.id2);
+}
+
+template <typename P>
+static hipError_t compact_t(Ctx* c, int s) {
+  SpeciesDev& S = c->sp[s];
+  const int64_t n = S.n_host;
+  const int64_t nb = (n + CMP_CH - 1) / CMP_CH;
+  if (nb == 0) return hipSuccess;
+  if (nb + 1 > c->blk_cap) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(live_count_kernel<P>, dim3(nb), dim3(AUX_BLOCK), 0, c->stream, (const P*)S.x, S.d_n, c->blk_count);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->blk_count, nb);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(compact_scatter_kernel<P>, dim3(nb), dim3(AUX_BLOCK), 0, c->stream, (const P*)S.x,
+                     (const P*)S.y, (const P*)S.vx, (const P*)S.vy, (const P*)S.vz, S.id, (P*)S.x2, (P*)S.y2,
+                     (P*)S.vx2, (P*)S.vy2, (P*)S.vz2, S.id2, S.d_n, c->blk_count);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(set_count_from_int_kernel, dim3(1), dim3(1), 0, c->stream, S.d_n, c->blk_count + nb);
+  EPIC_HIP_TRY(hipGetLastError());
+  swap_buffers(S);
+  int64_t nn;
+  EPIC_HIP_TRY(hipMemcpyAsync(&nn, S.d_n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
+  S.n_host = nn;
+  return hipSuccess;
+}
+
+hipError_t launch_compact(Ctx* c, int s) {
+  if (c->dom.precision == EPIC_FP64) return compact_t<double>(c, s);
+  return compact_t<float>(c, s);
+}
+
+// ---- counting sort by cell ----------------------------------------------------
+template <typename P>
+__device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, int ncx, int ncy) {
+  if (isnan(x)) return ncx * ncy;
+  const P lx = (x - x0) / dx, ly = (y - y0) / dy;
+  const int i = (int)lx, j = (int)ly;
+  if (!(lx >= 0) || !(ly >= 0) || i >= ncx || j >= ncy) return ncx * ncy;
+  return i * ncy + j;
+}
+
+template <typename P>
+__global__ void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0, P y0, P dx, P dy,
+                                 int ncx, int ncy) {
+  const int64_t n = *d_n;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&count[cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy)], 1);
+}
+
+template <typename P>
+__global__ void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, const int* start, int* fill,
+                                 uint32_t* perm, P x0, P y0, P dx, P dy, int ncx, int ncy) {
+  const int64_t n = *d_n;
+  const int dead = ncx * ncy;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy);
+    if (k == dead) continue;
+    const int slot = start[k] + atomicAdd(&fill[k], 1);
+    perm[slot] = (uint32_t)p;
+  }
+}
+
+// One wavefront per cell: order the cell's particle indices ascending by
+// rank counting (rank = number of smaller indices; indices are unique).
+// Cells of up to FIX_LDS particles are staged in LDS; larger ones are staged
+// in `scratch` (same offsets as perm).
+constexpr int FIX_LDS = 1024;
+constexpr int FIX_Q = FIX_LDS / 64;
+__global__ __launch_bounds__(256) void sort_fix_kernel(const int* start, uint32_t* perm, uint32_t* scratch,
+                                                        int ncell) {
+  __shared__ uint32_t buf[4][FIX_LDS];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int cell = blockIdx.x * 4 + w; cell < ncell; cell += gridDim.x * 4) {
+    const int b = start[cell], e = start[cell + 1], m = e - b;
+    if (m <= 1) continue;
+    if (m <= FIX_LDS) {
+      for (int k = lane; k < m; k += 64) buf[w][k] = perm[b + k];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int cnt = (m + 63) / 64;
+      uint32_t mine[FIX_Q];
+      int rank[FIX_Q];
+#pragma unroll
+      for (int q = 0; q < FIX_Q; q++) {
+        const int k = lane + q * 64;
+        mine[q] = (q < cnt && k < m) ? buf[w][k] : 0xFFFFFFFFu;
+        rank[q] = 0;
+      }
+      for (int t = 0; t < m; t++) {
+        const uint32_t o = buf[w][t];
+#pragma unroll
+        for (int q = 0; q < FIX_Q; q++) rank[q] += (q < cnt && o < mine[q]) ? 1 : 0;
+      }
+#pragma unroll
+      for (int q = 0; q < FIX_Q; q++) {
+        const int k = lane + q * 64;
+        if (q < cnt && k < m) perm[b + rank[q]] = mine[q];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      for (int k = lane; k < m; k += 64) scratch[b + k] = perm[b + k];
+      __threadfence();
+      __builtin_amdgcn_wave_barrier();
+      for (int k0 = 0; k0 < m; k0 += 64) {
+        const int k = k0 + lane;
+        const uint32_t me = k < m ? __builtin_nontemporal_load(&scratch[b + k]) : 0xFFFFFFFFu;
+        int rk = 0;
+        for (int t = 0; t < m; t++) rk += (__builtin_nontemporal_load(&scratch[b + t]) < me) ? 1 : 0;
+        if (k < m) perm[b + rk] = me;
+      }
+      __threadfence();
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+template <typename P>
+__global__ void sort_gather_kernel(const uint32_t* perm, const int* total, const P* x, const P* y, const P* vx,
+                                   const P* vy, const P* vz, const uint32_t* id, P* x2, P* y2, P* vx2, P* vy2,
+                                   P* vz2, uint32_t* id2) {
+  const int64_t n = *total;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = perm[i];
+    x2[i] = x[p]; y2[i] = y[p]; vx2[i] = vx[p]; vy2[i] = vy[p]; vz2[i] = vz[p]; id2[i] = id[p];
+  }
+}
+
+__global__ void zero_int_kernel(int* p, int64_t n) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = 0;
+}
+
+template <typename P>
+static hipError_t sort_t(Ctx* c, int s) {
+  SpeciesDev& S = c->sp[s];
+  const epic_domain_desc& d = c->dom;
+  const int ncx = d.nx - 1, ncy = d.ny - 1, ncell = ncx * ncy;
+  const int64_t n = S.n_host;
+  if (n == 0) return hipSuccess;
+  const P x0 = (P)d.x0, y0 = (P)d.y0, dx = (P)d.dx, dy = (P)d.dy;
+  const int g = std::max(1, (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->num_cu * 16));
+  const int gc = std::max(1, std::min((ncell + 2 + 255) / 256, 4096));
+  hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_count, (int64_t)ncell + 2);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_fill, (int64_t)ncell + 2);
+  hipLaunchKernelGGL(sort_hist_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
+                     c->sort_count, x0, y0, dx, dy, ncx, ncy);
+  EPIC_HIP_TRY(hipGetLastError());
+  // start[k] = exclusive scan; start[ncell] = live total; start[ncell+1] = all
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->sort_count, (int64_t)ncell + 1);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(sort_fill_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
+                     c->sort_count, c->sort_fill, c->sort_perm, x0, y0, dx, dy, ncx, ncy);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(sort_fix_kernel, dim3(std::min((ncell + 3) / 4, 65536)), dim3(256), 0, c->stream,
+                     c->sort_count, c->sort_perm, c->sort_perm + c->sort_cap, ncell);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(sort_gather_kernel<P>, dim3(g), dim3(256), 0, c->stream, c->sort_perm, c->sort_count + ncell,
+                     (const P*)S.x, (const P*)S.y, (const P*)S.vx, (const P*)S.vy, (const P*)S.vz, S.id, (P*)S.x2,
+                     (P*)S.y2, (P*)S.vx2, (P*)S.vy2, (P*)S.vz2, S.id2);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(set_count_from_int_kernel, dim3(1), dim3(1), 0, c->stream, S.d_n, c->sort_count + ncell);
+  EPIC_HIP_TRY(hipGetLastError());
+  swap_buffers(S);
+  int64_t nn;
+  EPIC_HIP_TRY(hipMemcpyAsync(&nn, S.d_n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
+  S.n_host = nn;
+  return hipSuccess;
+}
+
+hipError_t launch_sort(Ctx* c, int s) {
+  if (c->dom.precision == EPIC_FP64) return sort_t<double>(c, s);
+  return sort_t<float>(c, s);
+}
+
+// ---- emitter injection (Emitter::inject, emitter.cpp:5-36) -------------------
+// Per emitter and step, numparticle particles at (unirand(x0,x1),
+// unirand(y0,y1)) for species_idx1 and, when different, the same positions for
+// species_idx2; velocity SampleVel(temp) + vd*vel_norm, normalised, vz = 0.
+// Positions and velocities come from Philox keyed by (emitter, step, k) in
+// place of the reference's random_device-seeded draws.
+template <typename P>
+__global__ void inject_kernel(P* x, P* y, P* vx, P* vy, P* vz, uint32_t* id, int64_t* d_n, int64_t cap, int k, epic_emitter_desc e, double vth_sqrt2, double vel_norm,
+                              uint32_t step, uint32_t emitter, uint32_t which, uint32_t k0, uint32_t k1,
+                              unsigned long long* Q, int nx, int ny, double x0, double y0, double dx, double dy,
+                              int fix_bits, unsigned long long* errc) {
+  const int64_t n = d_n[0];
+  const uint32_t id0 = (uint32_t)d_n[1];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < k; t += gridDim.x * blockDim.x) {
+    const int64_t slot = n + t;
+    if (slot >= cap) {
+      atomicAdd(&errc[EPIC_ERRC_CAPACITY], 1ull);
+      continue;
+    }
+    const u32x4 rp = philox4x32_10({(uint32_t)t, step, RNG_INJECT, emitter}, k0, k1);
+    const double px = e.x0 + (e.x1 - e.x0) * u01_53(rp.a, rp.b);
+    const double py = e.y0 + (e.y1 - e.y0) * u01_53(rp.c, rp.d);
+    const u32x4 r1 = philox4x32_10({(uint32_t)t, step, RNG_INJECT2 + 2 * which, emitter}, k0, k1);
+    const u32x4 r2 = philox4x32_10({(uint32_t)t, step, RNG_INJECT2 + 2 * which + 1, emitter}, k0, k1);
+    const double sx = u01_53(r1.a, r1.b) + u01_53(r1.c, r1.d) + u01_53(r2.a, r2.b) - 1.5;
+    const double sy = u01_53(r2.c, r2.d) + u01_24(r1.a ^ r2.d) + u01_24(r1.c ^ r2.b) - 1.5;
+    const double vxp = (vth_sqrt2 * sx + e.vdx * vel_norm) / vel_norm;
+    const double vyp = (vth_sqrt2 * sy + e.vdy * vel_norm) / vel_norm;
+    x[slot] = (P)px; y[slot] = (P)py; vx[slot] = (P)vxp; vy[slot] = (P)vyp; vz[slot] = (P)0;
+    id[slot] = id0 + (uint32_t)t;
+    if (Q) {
+      const double lx = (px - x0) / dx, ly = (py - y0) / dy;
+      const int i = (int)lx, j = (int)ly;
+      if (i < 0 || i + 1 >= nx || j < 0 || j + 1 >= ny) {
+        atomicAdd(&errc[EPIC_ERRC_DEPOSIT_OOB], 1ull);
+        continue;
+      }
+      const double di = lx - i, dj = ly - j, sc = ldexp(1.0, fix_bits);
+      const long long one = 1ll << fix_bits;
+      const long long q00 = (long long)rint((1 - di) * (1 - dj) * sc);
+      const long long q10 = (long long)rint(di * (1 - dj) * sc);
+      const long long q01 = (long long)rint((1 - di) * dj * sc);
+      const long long q11 = one - q00 - q10 - q01;
+      unsigned long long* b = Q + (size_t)i * ny + j;
+      atomicAdd(b, (unsigned long long)q00);
+      atomicAdd(b + ny, (unsigned long long)q10);
+      atomicAdd(b + 1, (unsigned long long)q01);
+      atomicAdd(b + ny + 1, (unsigned long long)q11);
+    }
+  }
+}
+
+__global__ void bump_count_kernel(int64_t* d_n, int64_t k, int64_t cap) {
+  d_n[0] = std::min(d_n[0] + k, cap);
+  d_n[1] += k;
+}
+
+hipError_t launch_inject(Ctx* c, int step) {
+  const epic_domain_desc& d = c->dom;
+  const uint32_t k0 = (uint32_t)d.seed, k1 = (uint32_t)(d.seed >> 32);
+  for (size_t ei = 0; ei < c->emitters.size(); ei++) {
+    const epic_emitter_desc& e = c->emitters[ei].d;
+    const int targets[2] = {e.species_idx1, e.species_idx2};
+    const int nt = (e.species_idx1 != e.species_idx2) ? 2 : 1;
+    for (int w = 0; w < nt; w++) {
+      const int s = targets[w];
+      SpeciesDev& S = c->sp[s];
+      // Init::SampleVel(species, temp): v_th = sqrt(2 K_b T EV_to_K / m)
+      const double v_th = sqrt(2 * K_b * e.temp * (QE / K_b) / S.desc.mass);
+      const double vts = v_th * sqrt(2);
+      const int k = e.numparticle;
+      const int grid = std::max(1, std::min((k + 255) / 256, 1024));
+      unsigned long long* Q = c->deposit_valid ? S.Q : nullptr;
+      if (d.precision == EPIC_FP64)
+        hipLaunchKernelGGL(inject_kernel<double>, dim3(grid), dim3(256), 0, c->stream, (double*)S.x, (double*)S.y,
+                           (double*)S.vx, (double*)S.vy, (double*)S.vz, S.id, S.d_n, S.cap, k, e, vts,
+                           d.vel_norm, (uint32_t)step, (uint32_t)ei, (uint32_t)w, k0, k1, Q, d.nx, d.ny, d.x0, d.y0,
+                           d.dx, d.dy, c->fix_bits, c->errc);
+      else
+        hipLaunchKernelGGL(inject_kernel<float>, dim3(grid), dim3(256), 0, c->stream, (float*)S.x, (float*)S.y,
+                           (float*)S.vx, (float*)S.vy, (float*)S.vz, S.id, S.d_n, S.cap, k, e, vts,
+                           d.vel_norm, (uint32_t)step, (uint32_t)ei, (uint32_t)w, k0, k1, Q, d.nx, d.ny, d.x0, d.y0,
+                           d.dx, d.dy, c->fix_bits, c->errc);
+      EPIC_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(bump_count_kernel, dim3(1), dim3(1), 0, c->stream, S.d_n, (int64_t)k, S.cap);
+      EPIC_HIP_TRY(hipGetLastError());
+      S.n_host = std::min<int64_t>(S.n_host + k, S.cap);
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace epic

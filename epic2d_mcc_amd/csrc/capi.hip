@@ -1,0 +1,636 @@
+// extern "C" boundary of libepic2d: context lifetime, particle upload and
+// download, the per-step operators and diagnostics.  Status codes only; no
+// exception crosses the boundary.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "epic_internal.hpp"
+
+struct epic_ctx {
+  epic::Ctx c;
+};
+
+namespace epic {
+
+static int fail(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+static int hip_fail(Ctx* c, hipError_t e, const char* where) {
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  return fail(c, EPIC_ERR_HIP, m);
+}
+
+#define CK(expr, where)                                       \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) return hip_fail(c, _e, where);      \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemset(*p, 0, count * sizeof(T));
+}
+
+static size_t psize(const Ctx* c) { return c->dom.precision == EPIC_FP64 ? 8 : 4; }
+
+hipError_t sync_counts(Ctx* c) {
+  for (auto& S : c->sp) {
+    int64_t n[2];
+    EPIC_HIP_TRY(hipMemcpyAsync(n, S.d_n, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
+    S.n_host = n[0];
+  }
+  return hipSuccess;
+}
+
+void stage_begin(Ctx* c) {
+  if (c->timing) hipEventRecord(c->ev0, c->stream);
+}
+void stage_end(Ctx* c, int stage) {
+  if (!c->timing) return;
+  hipEventRecord(c->ev1, c->stream);
+  hipEventSynchronize(c->ev1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->stage_ms[stage] += ms;
+  c->stage_calls[stage]++;
+}
+
+static void free_all(Ctx* c) {
+  for (auto& S : c->sp) {
+    void* ps[] = {S.x, S.y, S.vx, S.vy, S.vz, S.id, S.x2, S.y2, S.vx2, S.vy2, S.vz2, S.id2, S.Q, S.den, S.d_n};
+    for (void* p : ps)
+      if (p) hipFree(p);
+  }
+  void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
+                c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
+                c->red_part, c->red_out, c->errc, c->mcc.sigma, c->mcc.counts, c->mcc.events,
+                c->mcc.event_count, c->sp_S, c->sp_T, c->sp_lam};
+  for (void* p : gs)
+    if (p) hipFree(p);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+}  // namespace epic
+
+using namespace epic;
+
+extern "C" {
+
+int epic_device_name(char* buf, int len) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return EPIC_ERR_HIP;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return EPIC_ERR_HIP;
+  snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return EPIC_OK;
+}
+
+int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, epic_ctx** out) {
+  if (!dom || !sp || !out || nsp < 1 || nsp > 16) return EPIC_ERR_ARG;
+  if (dom->nx < 3 || dom->ny < 3) return EPIC_ERR_ARG;
+  if (dom->precision != EPIC_FP64 && dom->precision != EPIC_FP32) return EPIC_ERR_ARG;
+  if (dom->solver < EPIC_SOLVER_PCG || dom->solver > EPIC_SOLVER_SPECTRAL) return EPIC_ERR_ARG;
+  if (dom->bc < EPIC_BC_PBC || dom->bc > EPIC_BC_REFLECTIVE) return EPIC_ERR_ARG;
+  if (dom->shape != EPIC_SHAPE_CIC && dom->shape != EPIC_SHAPE_NGP) return EPIC_ERR_ARG;
+  if ((int64_t)dom->nx * dom->ny > (1ll << 30)) return EPIC_ERR_ARG;
+  epic_ctx* h = new (std::nothrow) epic_ctx();
+  if (!h) return EPIC_ERR_CAPACITY;
+  Ctx* c = &h->c;
+  *out = nullptr;
+  c->dom = *dom;
+  if (c->dom.sor_omega == 0) c->dom.sor_omega = 1.4;
+  c->nsp = nsp;
+  c->fix_bits = dom->precision == EPIC_FP64 ? FIX_BITS_FP64 : FIX_BITS_FP32;
+  c->nnodes = (size_t)dom->nx * dom->ny;
+  hipError_t e;
+#define CKC(expr, where)                          \
+  do {                                            \
+    e = (expr);                                   \
+    if (e != hipSuccess) {                        \
+      c->err = std::string(where) + ": " + hipGetErrorString(e); \
+      free_all(c);                                \
+      delete h;                                   \
+      return EPIC_ERR_HIP;                        \
+    }                                             \
+  } while (0)
+  CKC(hipGetDevice(&c->device), "hipGetDevice");
+  hipDeviceProp_t prop;
+  CKC(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+  c->num_cu = prop.multiProcessorCount;
+  c->max_solver_blocks = c->num_cu;
+  CKC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  CKC(hipEventCreate(&c->ev0), "hipEventCreate");
+  CKC(hipEventCreate(&c->ev1), "hipEventCreate");
+  const size_t nn = c->nnodes;
+  const size_t ps = psize(c);
+  int64_t maxcap = 1;
+  c->sp.resize(nsp);
+  for (int s = 0; s < nsp; s++) {
+    SpeciesDev& S = c->sp[s];
+    S.desc = sp[s];
+    S.cap = sp[s].capacity > 0 ? sp[s].capacity : std::max<int64_t>(sp[s].numparticle, 1);
+    if (S.cap > (int64_t)0x7fffffff) {
+      free_all(c);
+      delete h;
+      return EPIC_ERR_CAPACITY;
+    }
+    maxcap = std::max(maxcap, S.cap);
+    // pad to whole 32-byte runs so vector loads of the last run stay inside
+    const size_t cap_pad = (size_t)((S.cap + 7) / 8 * 8);
+    char** arrs[] = {(char**)&S.x, (char**)&S.y, (char**)&S.vx, (char**)&S.vy, (char**)&S.vz,
+                     (char**)&S.x2, (char**)&S.y2, (char**)&S.vx2, (char**)&S.vy2, (char**)&S.vz2};
+    for (char** a : arrs) CKC(dalloc(a, cap_pad * ps), "particle arrays");
+    CKC(dalloc(&S.id, cap_pad), "ids");
+    CKC(dalloc(&S.id2, cap_pad), "ids");
+    CKC(dalloc(&S.Q, nn), "Q");
+    CKC(dalloc(&S.den, nn), "den");
+    CKC(dalloc(&S.d_n, 2), "count");
+  }
+  CKC(dalloc(&c->rho, nn), "rho");
+  CKC(dalloc(&c->phi, nn), "phi");
+  CKC(dalloc(&c->efx, nn), "efx");
+  CKC(dalloc(&c->efy, nn), "efy");
+  CKC(dalloc((char**)&c->E2, 2 * nn * ps), "E2");
+  CKC(dalloc(&c->b64, nn), "b");
+  CKC(dalloc(&c->r64, nn), "r");
+  CKC(dalloc((double**)&c->ws_v, 2 * nn), "v");
+  CKC(dalloc((double**)&c->ws_w, nn), "w");
+  CKC(dalloc((double**)&c->ws_x, nn), "x");
+  CKC(dalloc(&c->ws_part, (size_t)16 * std::max(c->max_solver_blocks, 1024)), "partials");
+  CKC(dalloc(&c->ws_scalar, 8), "scalars");
+  CKC(dalloc(&c->ws_iter, 1), "iters");
+  const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
+  CKC(dalloc(&c->sort_count, ncell + 2), "sort");
+  CKC(dalloc(&c->sort_fill, ncell + 2), "sort");
+  c->sort_cap = maxcap;
+  CKC(dalloc(&c->sort_perm, 2 * (size_t)maxcap), "sort");
+  c->blk_cap = maxcap / 2048 + 4;
+  CKC(dalloc(&c->blk_count, (size_t)c->blk_cap), "compact");
+  c->red_blocks = c->num_cu * 4;
+  CKC(dalloc(&c->red_part, (size_t)8 * c->red_blocks), "reduce");
+  CKC(dalloc(&c->red_out, 8), "reduce");
+  CKC(dalloc(&c->errc, EPIC_NERR), "errc");
+#undef CKC
+  *out = h;
+  return EPIC_OK;
+}
+
+int epic_destroy(epic_ctx* h) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete h;
+  return EPIC_OK;
+}
+
+const char* epic_last_error(const epic_ctx* h) { return h ? h->c.err.c_str() : "null context"; }
+
+int epic_sync(epic_ctx* h) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  CK(hipStreamSynchronize(c->stream), "sync");
+  return EPIC_OK;
+}
+
+// ---- particles ------------------------------------------------------------
+int epic_upload_particles(epic_ctx* h, int s, const double* x, const double* y, const double* vx, const double* vy,
+                          const double* vz, int64_t n) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp || n < 0) return fail(c, EPIC_ERR_ARG, "bad species or count");
+  SpeciesDev& S = c->sp[s];
+  if (n > S.cap) return fail(c, EPIC_ERR_CAPACITY, "upload exceeds species capacity");
+  const double* src[5] = {x, y, vx, vy, vz};
+  void* dst[5] = {S.x, S.y, S.vx, S.vy, S.vz};
+  if (c->dom.precision == EPIC_FP64) {
+    for (int k = 0; k < 5; k++) {
+      if (!src[k]) return fail(c, EPIC_ERR_ARG, "null array");
+      CK(hipMemcpy(dst[k], src[k], n * sizeof(double), hipMemcpyHostToDevice), "upload");
+    }
+  } else {
+    std::vector<float> tmp(n);
+    for (int k = 0; k < 5; k++) {
+      if (!src[k]) return fail(c, EPIC_ERR_ARG, "null array");
+      for (int64_t i = 0; i < n; i++) tmp[i] = (float)src[k][i];
+      CK(hipMemcpy(dst[k], tmp.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
+    }
+  }
+  std::vector<uint32_t> ids(n);
+  for (int64_t i = 0; i < n; i++) ids[i] = (uint32_t)i;
+  CK(hipMemcpy(S.id, ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice), "upload ids");
+  int64_t cnt[2] = {n, n};
+  CK(hipMemcpy(S.d_n, cnt, sizeof(cnt), hipMemcpyHostToDevice), "upload count");
+  S.n_host = n;
+  c->deposit_valid = false;
+  return EPIC_OK;
+}
+
+int epic_download_particles(epic_ctx* h, int s, double* x, double* y, double* vx, double* vy, double* vz,
+                            uint32_t* id, int64_t cap, int64_t* n_out) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  CK(sync_counts(c), "counts");
+  SpeciesDev& S = c->sp[s];
+  const int64_t n = S.n_host;
+  std::vector<double> a[5];
+  void* srcs[5] = {S.x, S.y, S.vx, S.vy, S.vz};
+  for (int k = 0; k < 5; k++) {
+    a[k].resize(n);
+    if (c->dom.precision == EPIC_FP64) {
+      CK(hipMemcpy(a[k].data(), srcs[k], n * sizeof(double), hipMemcpyDeviceToHost), "download");
+    } else {
+      std::vector<float> t(n);
+      CK(hipMemcpy(t.data(), srcs[k], n * sizeof(float), hipMemcpyDeviceToHost), "download");
+      for (int64_t i = 0; i < n; i++) a[k][i] = t[i];
+    }
+  }
+  std::vector<uint32_t> ids(n);
+  CK(hipMemcpy(ids.data(), S.id, n * sizeof(uint32_t), hipMemcpyDeviceToHost), "download ids");
+  int64_t m = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (std::isnan(a[0][i])) continue;
+    if (m < cap) {
+      if (x) x[m] = a[0][i];
+      if (y) y[m] = a[1][i];
+      if (vx) vx[m] = a[2][i];
+      if (vy) vy[m] = a[3][i];
+      if (vz) vz[m] = a[4][i];
+      if (id) id[m] = ids[i];
+    }
+    m++;
+  }
+  if (n_out) *n_out = m;
+  if (m > cap) return fail(c, EPIC_ERR_CAPACITY, "download buffer too small");
+  return EPIC_OK;
+}
+
+int epic_count(epic_ctx* h, int s, int64_t* n_out) {
+  if (!h || !n_out) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  double m[7];
+  CK(launch_particle_moments(c, s, m), "count");
+  *n_out = (int64_t)llround(m[6]);
+  return EPIC_OK;
+}
+
+int epic_load_species(epic_ctx* h, int s, int64_t n, int vz_thermal) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp || n < 0) return fail(c, EPIC_ERR_ARG, "bad species or count");
+  if (n > c->sp[s].cap) return fail(c, EPIC_ERR_CAPACITY, "load exceeds capacity");
+  CK(launch_load(c, s, n, vz_thermal), "load");
+  c->deposit_valid = false;
+  return EPIC_OK;
+}
+
+// ---- per-step operators --------------------------------------------------
+int epic_scatter(epic_ctx* h, int s) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  stage_begin(c);
+  CK(launch_zero_Q(c), "zero");
+  CK(launch_deposit(c, s), "deposit");
+  CK(launch_den_only(c, s), "den");
+  stage_end(c, 0);
+  c->deposit_valid = false;
+  return EPIC_OK;
+}
+
+int epic_compute_rho(epic_ctx* h) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  stage_begin(c);
+  CK(launch_den_only(c, -1), "rho");
+  stage_end(c, 1);
+  return EPIC_OK;
+}
+
+int epic_solve_potential(epic_ctx* h, int* iters, double* err) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  int it = 0;
+  double e = 0;
+  stage_begin(c);
+  CK(launch_solve(c, &it, &e), "solve");
+  stage_end(c, 2);
+  if (iters) *iters = it;
+  if (err) *err = e;
+  return EPIC_OK;
+}
+
+int epic_efield(epic_ctx* h) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  stage_begin(c);
+  CK(launch_efield(c), "efield");
+  stage_end(c, 3);
+  return EPIC_OK;
+}
+
+int epic_push(epic_ctx* h, int s, int sub_cycle) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  stage_begin(c);
+  CK(launch_push(c, s, sub_cycle, false, 0.0), "push");
+  stage_end(c, 4);
+  c->deposit_valid = false;
+  return EPIC_OK;
+}
+
+int epic_rewind(epic_ctx* h, int s) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  CK(launch_push(c, s, 1, false, 1.0), "rewind");
+  return EPIC_OK;
+}
+
+int epic_add_emitter(epic_ctx* h, const epic_emitter_desc* e) {
+  if (!h || !e) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (e->species_idx1 < 0 || e->species_idx1 >= c->nsp || e->species_idx2 < 0 || e->species_idx2 >= c->nsp ||
+      e->numparticle < 0)
+    return fail(c, EPIC_ERR_ARG, "bad emitter");
+  c->emitters.push_back(Emitter{*e});
+  return EPIC_OK;
+}
+
+int epic_inject(epic_ctx* h, int step) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  stage_begin(c);
+  CK(launch_inject(c, step), "inject");
+  stage_end(c, 7);
+  return EPIC_OK;
+}
+
+int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
+  if (!h || !d) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (d->ntab < 2 || !d->sigma || !(d->e_min > 0) || !(d->e_max > d->e_min)) return fail(c, EPIC_ERR_ARG, "bad table");
+  if (d->electron_sp >= c->nsp || d->ion_sp >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  MccState& M = c->mcc;
+  M.d = *d;
+  std::vector<double> sig(d->sigma, d->sigma + (size_t)EPIC_MCC_NPROC * d->ntab);
+  if (M.sigma) hipFree(M.sigma);
+  M.sigma = nullptr;
+  CK(dalloc(&M.sigma, sig.size()), "mcc table");
+  CK(hipMemcpy(M.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice), "mcc table");
+  M.d.sigma = nullptr;
+  M.event_cap = d->event_cap > 0 ? d->event_cap : 16384;
+  if (!M.counts) CK(dalloc(&M.counts, EPIC_MCC_NPROC + 2), "mcc counts");
+  if (M.events) hipFree(M.events);
+  M.events = nullptr;
+  CK(dalloc(&M.events, (size_t)M.event_cap * 8), "mcc events");
+  if (!M.event_count) CK(dalloc(&M.event_count, 1), "mcc events");
+  M.nu_max_e = d->electron_sp >= 0 ? mcc_nu_max(M.d, sig, ME, true) : 0.0;
+  M.nu_max_i = d->ion_sp >= 0 ? mcc_nu_max(M.d, sig, c->sp[d->ion_sp].desc.mass, false) : 0.0;
+  M.on = true;
+  return EPIC_OK;
+}
+
+int epic_collide(epic_ctx* h, int step) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  stage_begin(c);
+  CK(launch_mcc(c, step), "collide");
+  stage_end(c, 5);
+  return EPIC_OK;
+}
+
+int epic_mcc_get_stats(epic_ctx* h, epic_mcc_stats* out) {
+  if (!h || !out) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  unsigned long long v[EPIC_MCC_NPROC + 2] = {};
+  if (c->mcc.counts) {
+    CK(hipMemcpyAsync(v, c->mcc.counts, sizeof(v), hipMemcpyDeviceToHost, c->stream), "stats");
+    CK(hipStreamSynchronize(c->stream), "stats");
+  }
+  for (int p = 0; p < EPIC_MCC_NPROC; p++) out->count[p] = (int64_t)v[p];
+  out->births = (int64_t)v[EPIC_MCC_NPROC];
+  out->eloss_ev = v[EPIC_MCC_E_EXCITATION] * c->mcc.d.e_exc_th + v[EPIC_MCC_E_IONIZATION] * c->mcc.d.e_ion_th;
+  return EPIC_OK;
+}
+
+int epic_sort(epic_ctx* h, int s) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  stage_begin(c);
+  CK(sync_counts(c), "counts");
+  CK(launch_sort(c, s), "sort");
+  stage_end(c, 6);
+  return EPIC_OK;
+}
+
+int epic_compact(epic_ctx* h, int s) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  CK(sync_counts(c), "counts");
+  CK(launch_compact(c, s), "compact");
+  return EPIC_OK;
+}
+
+// One pass of main.cpp:395-423: inject, scatter, ComputeRho, PotentialSolver,
+// CalculateEfield, push (next deposit fused), collide; sort on schedule.
+int epic_step(epic_ctx* h, int ts, int* iters) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!c->emitters.empty()) {
+    stage_begin(c);
+    CK(launch_inject(c, ts), "inject");
+    stage_end(c, 7);
+  }
+  if (!c->deposit_valid) {
+    stage_begin(c);
+    CK(launch_zero_Q(c), "zero");
+    for (int s = 0; s < c->nsp; s++) CK(launch_deposit(c, s), "deposit");
+    stage_end(c, 0);
+  }
+  stage_begin(c);
+  CK(launch_rho_finalize(c), "rho");
+  stage_end(c, 1);
+  int it = 0;
+  double err = 0;
+  stage_begin(c);
+  CK(launch_solve(c, &it, &err), "solve");
+  stage_end(c, 2);
+  if (iters) *iters = it;
+  stage_begin(c);
+  CK(launch_efield(c), "efield");
+  stage_end(c, 3);
+  stage_begin(c);
+  for (int s = 0; s < c->nsp; s++) {
+    if (c->sp[s].desc.pushed) CK(launch_push(c, s, 1, true, 0.0), "push");
+    else CK(launch_deposit(c, s), "deposit");
+  }
+  stage_end(c, 4);
+  c->deposit_valid = true;
+  if (c->mcc.on) {
+    stage_begin(c);
+    CK(launch_mcc(c, ts), "collide");
+    stage_end(c, 5);
+  }
+  c->step_count++;
+  if (c->dom.sort_interval > 0 && c->step_count % c->dom.sort_interval == 0) {
+    stage_begin(c);
+    CK(sync_counts(c), "counts");
+    for (int s = 0; s < c->nsp; s++) CK(launch_sort(c, s), "sort");
+    stage_end(c, 6);
+  }
+  return EPIC_OK;
+}
+
+// main.cpp:366-382: scatter every species, ComputeRho, solve, E, rewind.
+int epic_initial_field(epic_ctx* h) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  CK(launch_zero_Q(c), "zero");
+  for (int s = 0; s < c->nsp; s++) CK(launch_deposit(c, s), "deposit");
+  CK(launch_rho_finalize(c), "rho");
+  int it;
+  double err;
+  CK(launch_solve(c, &it, &err), "solve");
+  CK(launch_efield(c), "efield");
+  for (int s = 0; s < c->nsp; s++) CK(launch_push(c, s, 1, false, 1.0), "rewind");
+  c->deposit_valid = false;
+  return EPIC_OK;
+}
+
+// ---- diagnostics -----------------------------------------------------------
+int epic_compute_ke(epic_ctx* h, int s, int ns, double out3[3]) {
+  if (!h || !out3) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp || ns < 0 || ns >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  double m[7];
+  CK(launch_particle_moments(c, s, m), "ke");
+  const epic_species_desc& a = c->sp[s].desc;
+  const epic_species_desc& b = c->sp[ns].desc;
+  const double vn = c->dom.vel_norm;
+  const double Th = (b.temp * eV) * (b.spwt) * (double)b.numparticle;  // species.cpp:264
+  for (int k = 0; k < 3; k++) out3[k] = ((m[k] * vn) * vn) * (0.5 * (a.spwt * a.mass)) / Th;
+  return EPIC_OK;
+}
+
+int epic_compute_momentum(epic_ctx* h, int s, int ns, double out3[3]) {
+  if (!h || !out3) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (s < 0 || s >= c->nsp || ns < 0 || ns >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  double m[7];
+  CK(launch_particle_moments(c, s, m), "momentum");
+  const epic_species_desc& a = c->sp[s].desc;
+  const epic_species_desc& b = c->sp[ns].desc;
+  const double vn = c->dom.vel_norm;
+  const double Thp = sqrt(b.temp * eV / ME) * b.spwt * (double)b.numparticle * ME;  // species.cpp:290
+  for (int k = 0; k < 3; k++) out3[k] = ((m[3 + k] * vn) * (a.spwt * a.mass)) / Thp;
+  return EPIC_OK;
+}
+
+int epic_compute_pe(epic_ctx* h, int ns, double* out) {
+  if (!h || !out) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (ns < 0 || ns >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
+  double se2;
+  CK(launch_field_energy(c, &se2), "pe");
+  const epic_domain_desc& d = c->dom;
+  const epic_species_desc& b = c->sp[ns].desc;
+  const double pe = se2 * (d.L * d.L);
+  const double unnorm = (d.density * QE * d.L) / EPS_0;  // domain.cpp:269
+  const double pe_norm = 0.5 * EPS_0 * pe * unnorm * unnorm;
+  const double Th = (b.temp * eV) * b.spwt * (double)b.numparticle;
+  *out = pe_norm / Th;
+  return EPIC_OK;
+}
+
+static double* field_ptr(Ctx* c, int which) {
+  switch (which) {
+    case 0: return c->rho;
+    case 1: return c->phi;
+    case 2: return c->efx;
+    case 3: return c->efy;
+    default:
+      if (which >= 10 && which < 10 + c->nsp) return c->sp[which - 10].den;
+      return nullptr;
+  }
+}
+
+int epic_get_field(epic_ctx* h, int which, double* out) {
+  if (!h || !out) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  double* p = field_ptr(c, which);
+  if (!p) return fail(c, EPIC_ERR_ARG, "bad field");
+  CK(hipMemcpyAsync(out, p, c->nnodes * sizeof(double), hipMemcpyDeviceToHost, c->stream), "get field");
+  CK(hipStreamSynchronize(c->stream), "get field");
+  return EPIC_OK;
+}
+
+int epic_set_field(epic_ctx* h, int which, const double* in) {
+  if (!h || !in) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  double* p = field_ptr(c, which);
+  if (!p) return fail(c, EPIC_ERR_ARG, "bad field");
+  CK(hipMemcpyAsync(p, in, c->nnodes * sizeof(double), hipMemcpyHostToDevice, c->stream), "set field");
+  if (which == 2 || which == 3) CK(launch_pack_E2(c), "pack E");
+  CK(hipStreamSynchronize(c->stream), "set field");
+  return EPIC_OK;
+}
+
+int epic_error_counters(epic_ctx* h, int64_t out[EPIC_NERR]) {
+  if (!h || !out) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  unsigned long long v[EPIC_NERR];
+  CK(hipMemcpyAsync(v, c->errc, sizeof(v), hipMemcpyDeviceToHost, c->stream), "errc");
+  CK(hipStreamSynchronize(c->stream), "errc");
+  for (int k = 0; k < EPIC_NERR; k++) out[k] = (int64_t)v[k];
+  return EPIC_OK;
+}
+
+int epic_clear_error_counters(epic_ctx* h) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  CK(hipMemsetAsync(c->errc, 0, EPIC_NERR * sizeof(unsigned long long), c->stream), "errc");
+  CK(hipStreamSynchronize(c->stream), "errc");
+  return EPIC_OK;
+}
+
+int epic_timing_enable(epic_ctx* h, int on) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  c->timing = on != 0;
+  for (int k = 0; k < EPIC_NSTAGE; k++) {
+    c->stage_ms[k] = 0;
+    c->stage_calls[k] = 0;
+  }
+  return EPIC_OK;
+}
+
+int epic_timing_read(epic_ctx* h, double ms[EPIC_NSTAGE], int64_t calls[EPIC_NSTAGE]) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  for (int k = 0; k < EPIC_NSTAGE; k++) {
+    if (ms) ms[k] = c->stage_ms[k];
+    if (calls) calls[k] = c->stage_calls[k];
+  }
+  return EPIC_OK;
+}
+
+}  // extern "C"

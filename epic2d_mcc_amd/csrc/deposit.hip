@@ -1,0 +1,129 @@
+// Fixed-point deposit finalisation: Q (int64, 2^B per particle) -> species
+// density den_s and charge density rho.
+//
+//   den_s = Q * 2^-B * spwt / (dx*dy*L^2*density)        species.cpp:188-190
+//   pbc: each node takes the sum over its periodic images  species.cpp:193-205
+//        (done on the integers, so the fold is exact)
+//   open / reflective: edge nodes x 1/2, corners x 1/4     species.cpp:206-225
+//   rho = 0; rho += den_s * (charge_s * (1/QE))             domain.cpp:194-202
+#include "device_common.hpp"
+#include "epic_internal.hpp"
+
+namespace epic {
+
+constexpr int MAXSP = 16;
+
+struct FinArgs {
+  const unsigned long long* Q[MAXSP];
+  double* den[MAXSP];
+  double scale[MAXSP];   // 2^-B * spwt
+  double factor[MAXSP];  // charge * (1/QE)
+  int nsp;
+  unsigned mask;         // species whose den is (re)computed from Q
+  int do_rho;            // 1: rho from den of all species
+  double norm;           // dx*dy*L^2*density
+  int nx, ny, bc;
+  double* rho;
+  unsigned long long* errc;
+};
+
+__global__ void rho_finalize_kernel(FinArgs a) {
+  const int64_t nn = (int64_t)a.nx * a.ny;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nn; k += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(k / a.ny), j = (int)(k % a.ny);
+    const bool iedge = (i == 0 || i == a.nx - 1), jedge = (j == 0 || j == a.ny - 1);
+    double rho = 0;
+    for (int s = 0; s < a.nsp; s++) {
+      double den;
+      if (a.mask & (1u << s)) {
+        const unsigned long long* Q = a.Q[s];
+        unsigned long long q;
+        if (a.bc == EPIC_BC_PBC) {
+          const int ia0 = iedge ? 0 : i, ia1 = iedge ? a.nx - 1 : i;
+          const int ja0 = jedge ? 0 : j, ja1 = jedge ? a.ny - 1 : j;
+          q = Q[(size_t)ia0 * a.ny + ja0];
+          if (ja1 != ja0) q += Q[(size_t)ia0 * a.ny + ja1];
+          if (ia1 != ia0) {
+            q += Q[(size_t)ia1 * a.ny + ja0];
+            if (ja1 != ja0) q += Q[(size_t)ia1 * a.ny + ja1];
+          }
+        } else {
+          q = Q[k];
+        }
+        const long long qs = (long long)q;
+        if (qs < 0) atomicAdd(&a.errc[EPIC_ERRC_FIXED_OVF], 1ull);
+        den = ((double)qs * a.scale[s]) / a.norm;
+        if (a.bc != EPIC_BC_PBC) {
+          if (iedge) den *= 0.5;
+          if (jedge) den *= 0.5;
+        }
+        a.den[s][k] = den;
+      } else {
+        den = a.den[s][k];
+      }
+      if (a.do_rho) rho += den * a.factor[s];
+    }
+    if (a.do_rho) a.rho[k] = rho;
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    p[k] = 0ull;
+}
+
+static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
+  const epic_domain_desc& d = c->dom;
+  FinArgs a{};
+  a.nsp = c->nsp;
+  a.mask = mask;
+  a.do_rho = do_rho;
+  for (int s = 0; s < c->nsp; s++) {
+    a.Q[s] = c->sp[s].Q;
+    a.den[s] = c->sp[s].den;
+    a.scale[s] = ldexp(1.0, -c->fix_bits) * c->sp[s].desc.spwt;
+    a.factor[s] = c->sp[s].desc.charge * (1 / QE);
+  }
+  const double cell_area = d.dx * d.dy;
+  const double L2 = d.L * d.L;
+  a.norm = (cell_area * L2 * d.density);
+  a.nx = d.nx; a.ny = d.ny; a.bc = d.bc;
+  a.rho = c->rho;
+  a.errc = c->errc;
+  const int64_t nn = (int64_t)d.nx * d.ny;
+  int grid = (int)std::min<int64_t>((nn + 255) / 256, 4096);
+  hipLaunchKernelGGL(rho_finalize_kernel, dim3(grid), dim3(256), 0, c->stream, a);
+  EPIC_HIP_TRY(hipGetLastError());
+  for (int s = 0; s < c->nsp; s++) {
+    if (!(mask & (1u << s))) continue;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid), dim3(256), 0, c->stream, c->sp[s].Q, nn);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+// All species' Q -> den and rho (the fused-step path).
+hipError_t launch_rho_finalize(Ctx* c) {
+  unsigned mask = 0;
+  for (int s = 0; s < c->nsp; s++) mask |= 1u << s;
+  return finalize(c, mask, 1);
+}
+
+// One species' Q -> den (ScatterSpecies_serial) when called with s >= 0;
+// rho from the stored den of all species (ComputeRho) when s < 0.
+hipError_t launch_den_only(Ctx* c, int s) {
+  if (s >= 0) return finalize(c, 1u << s, 0);
+  return finalize(c, 0u, 1);
+}
+
+hipError_t launch_zero_Q(Ctx* c) {
+  const int64_t nn = (int64_t)c->dom.nx * c->dom.ny;
+  int grid = (int)std::min<int64_t>((nn + 255) / 256, 4096);
+  for (int s = 0; s < c->nsp; s++) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid), dim3(256), 0, c->stream, c->sp[s].Q, nn);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+}  // namespace epic

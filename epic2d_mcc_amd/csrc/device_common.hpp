@@ -1,0 +1,130 @@
+// Device helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace epic {
+
+// ---- Philox4x32-10 ---------------------------------------------------------
+// Counter-based generator of Salmon, Moraes, Dror and Shaw (SC'11): 10 rounds
+// of multiply-hi/lo mixing with a Weyl-sequence key schedule.  Stateless: a
+// draw is a pure function of (counter, key), so every particle gets its own
+// reproducible stream keyed by (particle id, step).
+struct u32x4 {
+  uint32_t a, b, c, d;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 ctr, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, ctr.a);
+    const uint32_t lo0 = 0xD2511F53u * ctr.a;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, ctr.c);
+    const uint32_t lo1 = 0xCD9E8D57u * ctr.c;
+    u32x4 n;
+    n.a = hi1 ^ ctr.b ^ k0;
+    n.b = lo1;
+    n.c = hi0 ^ ctr.d ^ k1;
+    n.d = lo0;
+    ctr = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return ctr;
+}
+
+// 53 random bits -> [0, 1)
+__device__ __forceinline__ double u01_53(uint32_t a, uint32_t b) {
+  const uint64_t m = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);
+  return (double)m * (1.0 / 9007199254740992.0);
+}
+// 24 random bits -> [0, 1)
+__device__ __forceinline__ float u01_24(uint32_t a) { return (float)(a >> 8) * (1.0f / 16777216.0f); }
+
+// Stream purposes: the third counter word separates independent uses.
+enum : uint32_t {
+  RNG_LOAD_POS = 1,
+  RNG_LOAD_VEL = 2,
+  RNG_LOAD_VEL2 = 3,
+  RNG_MCC = 4,
+  RNG_MCC2 = 5,
+  RNG_MCC3 = 6,
+  RNG_INJECT = 7,
+  RNG_MCC_ION_SCATTER = 12,
+  RNG_MCC_NEWION = 13,
+  RNG_MCC_NEWION2 = 14,
+  RNG_INJECT2 = 16,  // 16 + 2*target, 17 + 2*target
+};
+
+// ---- reductions ------------------------------------------------------------
+// Fixed butterfly over the 64-lane wavefront: the same lanes are combined in
+// the same order every run, so FP sums are reproducible.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block sum of NV doubles; result valid in every thread.  `sh` holds
+// NV * (blockDim/64) doubles.  Deterministic (fixed order).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0;
+    for (int w = 0; w < nw; w++) s += sh[k * nw + w];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {
+  atomicAdd(p, v);
+}
+
+// XCD-aware chunk walk: MI355X dispatches consecutive workgroups round-robin
+// over its 8 XCDs.  Giving XCD x a contiguous run of chunks keeps a chunk
+// range's grid-cell working set in that XCD's own L2.
+__device__ __forceinline__ void xcd_chunk_range(int64_t nchunks, int64_t* begin, int64_t* end,
+                                                int64_t* step) {
+  const int nxcd = 8;
+  const int g = gridDim.x;
+  if (g % nxcd != 0 || g < nxcd) {
+    *begin = blockIdx.x;
+    *end = nchunks;
+    *step = g;
+    return;
+  }
+  const int xcd = blockIdx.x % nxcd;
+  const int local = blockIdx.x / nxcd;
+  const int per = g / nxcd;
+  const int64_t span = (nchunks + nxcd - 1) / nxcd;
+  const int64_t lo = (int64_t)xcd * span;
+  int64_t hi = lo + span;
+  if (hi > nchunks) hi = nchunks;
+  *begin = lo + local;
+  *end = hi;
+  *step = per;
+}
+
+}  // namespace epic

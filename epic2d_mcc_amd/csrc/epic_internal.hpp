@@ -1,0 +1,169 @@
+// Internal declarations shared by the HIP sources of libepic2d.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/epic2d.h"
+
+namespace epic {
+
+// Physical constants, the values of include/domain.h:204-215.
+constexpr double EPS_0 = 8.85418782e-12;
+constexpr double QE = 1.602176565e-19;
+constexpr double ME = 9.10938215e-31;
+constexpr double K_b = 1.380648e-23;
+constexpr double PI = 3.141592653;  // the reference's truncated pi
+constexpr double eV = 1.602176565e-19;
+
+// Fixed-point deposit scale (bits after the binary point of one particle's
+// unit weight).  FP64 storage: 2^44 (per-contribution rounding < 3e-14 of a
+// particle, 2^19 particles per node before overflow).  FP32 storage: 2^24,
+// the FP32 mantissa.
+constexpr int FIX_BITS_FP64 = 44;
+constexpr int FIX_BITS_FP32 = 24;
+
+// LDS deposit window: 8192 int64 node accumulators (64 KiB) per workgroup.
+constexpr int DEP_WINDOW = 8192;
+constexpr int PART_BLOCK = 256;  // threads per workgroup in particle kernels
+
+struct SpeciesDev {
+  epic_species_desc desc;
+  int64_t cap = 0;
+  int64_t n_host = 0;       // host copy of the live+dead slot count (authoritative unless births)
+  void* x = nullptr;        // P*  (P = double | float)
+  void* y = nullptr;
+  void* vx = nullptr;
+  void* vy = nullptr;
+  void* vz = nullptr;
+  uint32_t* id = nullptr;
+  // sort/compaction double buffers
+  void* x2 = nullptr;
+  void* y2 = nullptr;
+  void* vx2 = nullptr;
+  void* vy2 = nullptr;
+  void* vz2 = nullptr;
+  uint32_t* id2 = nullptr;
+  unsigned long long* Q = nullptr;  // fixed-point deposit accumulator, nnodes
+  double* den = nullptr;            // normalised density, nnodes
+  int64_t* d_n = nullptr;           // device [slot count, next id] (births append here)
+};
+
+struct Emitter {
+  epic_emitter_desc d;
+};
+
+struct MccState {
+  bool on = false;
+  epic_mcc_desc d{};
+  double* sigma = nullptr;               // device NPROC * ntab
+  double nu_max_e = 0, nu_max_i = 0;     // s^-1
+  unsigned long long* counts = nullptr;  // device NPROC + 2 (births, dropped)
+  double* events = nullptr;              // device event_cap * 8
+  int* event_count = nullptr;            // device
+  int event_cap = 0;
+};
+
+struct Ctx {
+  epic_domain_desc dom{};
+  int nsp = 0;
+  std::vector<SpeciesDev> sp;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cu = 0;
+  int fix_bits = FIX_BITS_FP64;
+  size_t nnodes = 0;
+
+  double* rho = nullptr;
+  double* phi = nullptr;
+  double* efx = nullptr;
+  double* efy = nullptr;
+  void* E2 = nullptr;  // interleaved (ex, ey) per node in particle precision
+
+  // Poisson workspace
+  double* b64 = nullptr;
+  double* r64 = nullptr;
+  void* ws_v = nullptr;   // 2 * nnodes of T (double-buffered search direction)
+  void* ws_w = nullptr;   // nnodes of T
+  void* ws_x = nullptr;   // nnodes of T (inner solution)
+  double* ws_part = nullptr;  // per-block partial sums (3 * max_blocks)
+  double* ws_scalar = nullptr;  // solver scalars written back to host
+  int* ws_iter = nullptr;
+  int max_solver_blocks = 0;
+  // spectral solver workspace
+  double* sp_S = nullptr;       // sine-transform matrices
+  double* sp_T = nullptr;       // temporaries
+  double* sp_lam = nullptr;
+  int sp_m = 0, sp_n = 0;
+  void* blas = nullptr;
+
+  // sort workspace
+  int* sort_count = nullptr;   // ncell + 2
+  int* sort_start = nullptr;
+  int* sort_fill = nullptr;
+  uint32_t* sort_perm = nullptr;
+  int64_t sort_cap = 0;
+  int* blk_count = nullptr;    // compaction per-block counts
+  int64_t blk_cap = 0;
+
+  // diagnostics
+  double* red_part = nullptr;  // per-block partials
+  double* red_out = nullptr;   // final sums
+  int red_blocks = 0;
+
+  unsigned long long* errc = nullptr;  // EPIC_NERR device counters
+
+  std::vector<Emitter> emitters;
+  MccState mcc;
+
+  bool deposit_valid = false;  // Q holds the deposit of the current positions
+  int step_count = 0;
+
+  // timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double stage_ms[EPIC_NSTAGE] = {};
+  int64_t stage_calls[EPIC_NSTAGE] = {};
+
+  std::string err;
+};
+
+// ---- launchers (one .hip file each) ---------------------------------------
+// deposit.hip
+hipError_t launch_deposit(Ctx* c, int s);
+hipError_t launch_rho_finalize(Ctx* c);
+hipError_t launch_den_only(Ctx* c, int s);
+hipError_t launch_zero_Q(Ctx* c);
+// particles.hip
+hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind_factor);
+hipError_t launch_load(Ctx* c, int s, int64_t n, int vz_thermal);
+hipError_t launch_compact(Ctx* c, int s);
+hipError_t launch_sort(Ctx* c, int s);
+hipError_t launch_inject(Ctx* c, int step);
+// field.hip
+hipError_t launch_solve(Ctx* c, int* iters, double* err);
+hipError_t launch_efield(Ctx* c);
+hipError_t launch_pack_E2(Ctx* c);
+// diag.hip
+hipError_t launch_particle_moments(Ctx* c, int s, double out7[7]);
+hipError_t launch_field_energy(Ctx* c, double* sum_e2);
+// mcc.hip
+hipError_t launch_mcc(Ctx* c, int step);
+double mcc_nu_max(const epic_mcc_desc& d, const std::vector<double>& sig, double m, bool electron);
+
+// helpers (capi.hip)
+int64_t live_count(Ctx* c, int s, hipError_t* e);
+hipError_t sync_counts(Ctx* c);
+void stage_begin(Ctx* c);
+void stage_end(Ctx* c, int stage);
+
+}  // namespace epic
+
+#define EPIC_HIP_TRY(expr)                 \
+  do {                                     \
+    hipError_t _e = (expr);                \
+    if (_e != hipSuccess) return _e;       \
+  } while (0)

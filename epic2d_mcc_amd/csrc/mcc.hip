@@ -1,0 +1,394 @@
+// Null-collision Monte-Carlo collisions against tabulated cross sections.
+//
+// Replaces CollisionHandler::handle_collisions (collision.cpp:190-207), which
+// draws P = 1 - exp(-nu(E) dt) per electron.  Here every particle of the
+// colliding species draws from its own Philox stream keyed by
+// (particle id, step): it is a candidate with probability
+// P_null = 1 - exp(-nu_max dt), and a candidate collides for real with
+// probability nu(E)/nu_max (else a null collision).  Per process the expected
+// rate is the reference's to first order in nu dt.
+//
+// Electron-neutral kinematics follow CollisionHandler::collision_electron
+// (collision.cpp:60-188): centre-of-mass split F1 = m_e/(m_e+M), F2 = M/(m_e+M),
+// Euler angles of g, isotropic chi = acos(1-2R), excitation and ionisation
+// thresholds, ejected-electron energy 10 tan(R atan(E/20)) eV.  Ionisation
+// births are written to an event list, ranked by parent slot and appended in
+// that order, so slots and ids of new particles do not depend on scheduling.
+//
+// Ion-neutral (Ar+ - Ar): isotropic elastic scattering in the centre-of-mass
+// frame and resonant charge exchange (the ion takes the neutral's velocity),
+// against Maxwellian neutrals at gas_temp_K.  No reference counterpart.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "device_common.hpp"
+#include "epic_internal.hpp"
+
+namespace epic {
+
+constexpr int MCC_BLOCK = 256;
+constexpr int EVW = 8;  // doubles per ionisation event: slot, x, y, vex, vey, vez, (ion from RNG)
+
+struct MccTab {
+  const double* sigma;  // NPROC * n
+  int n;
+  double loge0, inv_dloge;
+  __device__ __forceinline__ double at(int p, double e) const {
+    if (!(e > 0)) return sigma[p * n];
+    double t = (log(e) - loge0) * inv_dloge;
+    if (t <= 0) return sigma[p * n];
+    if (t >= n - 1) return sigma[p * n + n - 1];
+    const int k = (int)t;
+    const double f = t - k;
+    return sigma[p * n + k] * (1 - f) + sigma[p * n + k + 1] * f;
+  }
+};
+
+template <typename P>
+struct MccArgs {
+  P* x;
+  P* y;
+  P* vx;
+  P* vy;
+  P* vz;
+  const uint32_t* id;
+  const int64_t* d_n;
+  MccTab tab;
+  int kind;             // 0 electron, 1 ion
+  unsigned enable;      // bit per process
+  int ref_selection;
+  double nu_max, p_null;
+  double vel_norm, mass, gas_mass, gas_density, gas_vth;  // gas_vth normalised sqrt(kT/M)
+  double e_exc, e_ion;  // eV
+  double ion_vth;       // normalised thermal spread of newborn ions
+  uint32_t step, sp, k0, k1;
+  unsigned long long* counts;
+  double* events;
+  int* event_count;
+  int event_cap;
+};
+
+__device__ __forceinline__ void rotate(double g, double chi, double eta, double st, double ct, double sp, double cp,
+                                       double& gx, double& gy, double& gz) {
+  const double sc = sin(chi), cc = cos(chi), se = sin(eta), ce = cos(eta);
+  gx = g * (ct * cc - st * sc * ce);
+  gy = g * (st * cp * cc + ct * cp * sc * ce - sp * sc * se);
+  gz = g * (st * sp * cc + ct * sp * sc * ce + cp * sc * se);
+}
+
+__device__ __forceinline__ double gauss(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double* second) {
+  const double u1 = 1.0 - u01_53(a, b), u2 = u01_53(c, d);
+  const double m = sqrt(-2.0 * log(u1));
+  *second = m * sin(2.0 * M_PI * u2);
+  return m * cos(2.0 * M_PI * u2);
+}
+
+template <typename P>
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
+  const int64_t n = a.d_n[0];
+  unsigned long long cnt[EPIC_MCC_NPROC] = {0, 0, 0, 0, 0};
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t pid = a.id[p];
+    const u32x4 r = philox4x32_10({pid, a.step, RNG_MCC, a.sp}, a.k0, a.k1);
+    const double R0 = u01_53(r.a, r.b);
+    if (R0 >= a.p_null) continue;
+    const P xp = a.x[p];
+    if (isnan(xp)) continue;
+    double vxe = a.vx[p], vye = a.vy[p], vze = a.vz[p];
+    const double R1 = u01_53(r.c, r.d);
+    const u32x4 r2 = philox4x32_10({pid, a.step, RNG_MCC2, a.sp}, a.k0, a.k1);
+    const u32x4 r3 = philox4x32_10({pid, a.step, RNG_MCC3, a.sp}, a.k0, a.k1);
+    if (a.kind == 0) {
+      // electron - neutral (neutral at rest)
+      const double v_sqr = (vxe * vxe + vye * vye + vze * vze) * a.vel_norm * a.vel_norm;
+      const double velocity = sqrt(v_sqr);
+      const double energy = (0.5 * ME * v_sqr) / eV;
+      double s_el = (a.enable & 1) ? a.tab.at(0, energy) : 0.0;
+      double s_ex = (a.enable & 2) && energy > a.e_exc ? a.tab.at(1, energy) : 0.0;
+      double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, energy) : 0.0;
+      const double t0 = s_el, t1 = t0 + s_ex, t2 = t1 + s_io;
+      const double nu = a.gas_density * t2 * velocity;
+      if (R1 * a.nu_max >= nu) continue;  // null collision
+      int type;
+      if (a.ref_selection) {
+        // two independent draws (collision.cpp:117,125)
+        if (u01_53(r2.a, r2.b) < (t0 / t2)) type = 0;
+        else if (u01_53(r2.c, r2.d) < (t1 / t2)) type = 1;
+        else type = 2;
+      } else {
+        const double R = u01_53(r2.a, r2.b) * t2;
+        type = R < t0 ? 0 : (R < t1 ? 1 : 2);
+      }
+      const double F1 = ME / (ME + a.gas_mass), F2 = a.gas_mass / (ME + a.gas_mass);
+      double gx = vxe, gy = vye, gz = vze;
+      double g = sqrt(gx * gx + gy * gy + gz * gz);
+      const double wx = F1 * vxe, wy = F1 * vye, wz = F1 * vze;
+      const double theta = (gx == 0) ? 0.5 * PI : atan2(sqrt(gy * gy + gz * gz), gx);
+      const double phi = (gy == 0) ? (gz > 0 ? 0.5 * PI : -0.5 * PI) : atan2(gz, gy);
+      const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+      double chi, eta;
+      const double Ra = u01_53(r3.a, r3.b), Rb = u01_53(r3.c, r3.d);
+      if (type == 0) {
+        chi = acos(1.0 - 2.0 * Ra);
+        eta = 2 * PI * Rb;
+      } else if (type == 1) {
+        double en = 0.5 * ME * g * g * a.vel_norm * a.vel_norm;
+        en = fabs(en - a.e_exc * eV);
+        g = sqrt(2.0 * en / ME) / a.vel_norm;
+        chi = acos(1.0 - 2.0 * Ra);
+        eta = 2 * PI * Rb;
+      } else {
+        double en = 0.5 * ME * g * g * a.vel_norm * a.vel_norm;
+        en = fabs(en - a.e_ion * eV);
+        const double e_ej = 10.0 * tan(Ra * atan(en / eV / 20.0)) * eV;
+        const double e_sc = fabs(en - e_ej);
+        g = sqrt(2.0 * e_sc / ME) / a.vel_norm;
+        const double g2 = sqrt(2.0 * e_ej / ME) / a.vel_norm;
+        chi = acos(sqrt(e_sc / en));
+        const double chi2 = acos(sqrt(e_ej / en));
+        eta = 2 * PI * Rb;
+        const double eta2 = eta + PI;
+        double hx, hy, hz;
+        rotate(g2, chi2, eta2, st, ct, sp, cp, hx, hy, hz);
+        const int e = atomicAdd(a.event_count, 1);
+        if (e < a.event_cap) {
+          double* ev = a.events + (size_t)e * EVW;
+          ev[0] = (double)p;
+          ev[1] = (double)xp;
+          ev[2] = (double)a.y[p];
+          ev[3] = wx + F2 * hx;
+          ev[4] = wy + F2 * hy;
+          ev[5] = wz + F2 * hz;
+          ev[6] = (double)pid;
+          ev[7] = 0;
+        }
+      }
+      double ngx, ngy, ngz;
+      rotate(g, chi, eta, st, ct, sp, cp, ngx, ngy, ngz);
+      a.vx[p] = (P)(wx + F2 * ngx);
+      a.vy[p] = (P)(wy + F2 * ngy);
+      a.vz[p] = (P)(wz + F2 * ngz);
+      cnt[type]++;
+    } else {
+      // ion - neutral against a Maxwellian neutral
+      double g2nd, g3nd;
+      const double vnx = a.gas_vth * gauss(r2.a, r2.b, r2.c, r2.d, &g2nd);
+      const double vny = a.gas_vth * g2nd;
+      const double vnz = a.gas_vth * gauss(r3.a, r3.b, r3.c, r3.d, &g3nd);
+      const double gx = vxe - vnx, gy = vye - vny, gz = vze - vnz;
+      const double g2 = gx * gx + gy * gy + gz * gz;
+      const double g = sqrt(g2);
+      const double energy = 0.5 * a.mass * g2 * a.vel_norm * a.vel_norm / eV;
+      const double s_el = (a.enable & 8) ? a.tab.at(3, energy) : 0.0;
+      const double s_cx = (a.enable & 16) ? a.tab.at(4, energy) : 0.0;
+      const double nu_el = a.gas_density * s_el * g * a.vel_norm;
+      const double nu_cx = a.gas_density * s_cx * g * a.vel_norm;
+      const double R = R1 * a.nu_max;
+      if (R < nu_el) {
+        const u32x4 r4 = philox4x32_10({pid, a.step, RNG_MCC_ION_SCATTER, a.sp}, a.k0, a.k1);
+        const double M = a.gas_mass, m = a.mass;
+        const double cx = (m * vxe + M * vnx) / (m + M), cy = (m * vye + M * vny) / (m + M),
+                     cz = (m * vze + M * vnz) / (m + M);
+        const double ct = 1.0 - 2.0 * u01_53(r4.a, r4.b), stt = sqrt(fmax(0.0, 1.0 - ct * ct));
+        const double ph = 2.0 * M_PI * u01_53(r4.c, r4.d);
+        const double f = M / (m + M);
+        a.vx[p] = (P)(cx + f * g * stt * cos(ph));
+        a.vy[p] = (P)(cy + f * g * stt * sin(ph));
+        a.vz[p] = (P)(cz + f * g * ct);
+        cnt[3]++;
+      } else if (R < nu_el + nu_cx) {
+        a.vx[p] = (P)vnx;
+        a.vy[p] = (P)vny;
+        a.vz[p] = (P)vnz;
+        cnt[4]++;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < EPIC_MCC_NPROC; q++) {
+    unsigned long long v = cnt[q];
+    // wave reduction, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&a.counts[q], v);
+  }
+}
+
+// Append ionisation births in parent-slot order: one workgroup ranks the
+// events (slots are unique) and writes the new electron at n_e + rank and the
+// new ion at n_i + rank, ids next_id + rank, and deposits both.
+constexpr int APPEND_BLOCK = 1024;
+constexpr int APPEND_LDS = 16384;
+
+template <typename P>
+__global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_kernel(
+    const double* events, int* event_count, int cap_events, P* ex, P* ey, P* evx, P* evy, P* evz, uint32_t* eid,
+    int64_t* e_n, int64_t e_cap, P* ix, P* iy, P* ivx, P* ivy, P* ivz, uint32_t* iid, int64_t* i_n, int64_t i_cap,
+    double ion_vth, uint32_t step, uint32_t k0, uint32_t k1, unsigned long long* Qe, unsigned long long* Qi, int nx,
+    int ny, double x0, double y0, double dx, double dy, int fix_bits, unsigned long long* counts,
+    unsigned long long* errc) {
+  __shared__ double slots[APPEND_LDS];
+  const int E = min(*event_count, cap_events);
+  for (int k = threadIdx.x; k < E; k += blockDim.x) slots[k] = events[(size_t)k * EVW];
+  __syncthreads();
+  const int64_t ne = e_n[0], ni = i_n[0];
+  const uint32_t ide = (uint32_t)e_n[1], idi = (uint32_t)i_n[1];
+  int dropped = 0;
+  for (int k = threadIdx.x; k < E; k += blockDim.x) {
+    const double me = slots[k];
+    int rank = 0;
+    for (int t = 0; t < E; t++) rank += slots[t] < me ? 1 : 0;
+    const double* ev = events + (size_t)k * EVW;
+    const int64_t se = ne + rank, si = ni + rank;
+    if (se >= e_cap || si >= i_cap) {
+      dropped++;
+      continue;
+    }
+    const double px = ev[1], py = ev[2];
+    ex[se] = (P)px; ey[se] = (P)py;
+    evx[se] = (P)ev[3]; evy[se] = (P)ev[4]; evz[se] = (P)ev[5];
+    eid[se] = ide + (uint32_t)rank;
+    // new ion: Maxwellian at the ion species temperature (normalised)
+    const uint32_t pid = (uint32_t)ev[6];
+    const u32x4 r = philox4x32_10({pid, step, RNG_MCC_NEWION, 0x1u}, k0, k1);
+    const u32x4 r2 = philox4x32_10({pid, step, RNG_MCC_NEWION2, 0x1u}, k0, k1);
+    double s1, s2;
+    const double g0 = gauss(r.a, r.b, r.c, r.d, &s1);
+    const double g2 = gauss(r2.a, r2.b, r2.c, r2.d, &s2);
+    ix[si] = (P)px; iy[si] = (P)py;
+    ivx[si] = (P)(ion_vth * g0); ivy[si] = (P)(ion_vth * s1); ivz[si] = (P)(ion_vth * g2);
+    iid[si] = idi + (uint32_t)rank;
+    if (Qe) {
+      const double lx = (px - x0) / dx, ly = (py - y0) / dy;
+      const int i = (int)lx, j = (int)ly;
+      if (i < 0 || i + 1 >= nx || j < 0 || j + 1 >= ny) {
+        atomicAdd(&errc[EPIC_ERRC_DEPOSIT_OOB], 2ull);
+        continue;
+      }
+      const double di = lx - i, dj = ly - j, sc = ldexp(1.0, fix_bits);
+      const long long one = 1ll << fix_bits;
+      const long long q00 = (long long)rint((1 - di) * (1 - dj) * sc);
+      const long long q10 = (long long)rint(di * (1 - dj) * sc);
+      const long long q01 = (long long)rint((1 - di) * dj * sc);
+      const long long q11 = one - q00 - q10 - q01;
+      const size_t b = (size_t)i * ny + j;
+      unsigned long long* Qs[2] = {Qe, Qi};
+      for (int w = 0; w < 2; w++) {
+        atomicAdd(Qs[w] + b, (unsigned long long)q00);
+        atomicAdd(Qs[w] + b + ny, (unsigned long long)q10);
+        atomicAdd(Qs[w] + b + 1, (unsigned long long)q01);
+        atomicAdd(Qs[w] + b + ny + 1, (unsigned long long)q11);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int ok = E;
+    e_n[0] = min(ne + ok, e_cap);
+    e_n[1] += ok;
+    i_n[0] = min(ni + ok, i_cap);
+    i_n[1] += ok;
+    counts[EPIC_MCC_NPROC] += (unsigned long long)ok;
+    const int over = *event_count - E;
+    if (over > 0) atomicAdd(&errc[EPIC_ERRC_CAPACITY], (unsigned long long)over);
+    *event_count = 0;
+  }
+  if (dropped) atomicAdd(&errc[EPIC_ERRC_CAPACITY], (unsigned long long)dropped);
+}
+
+template <typename P>
+static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
+  MccState& M = c->mcc;
+  SpeciesDev& S = c->sp[s];
+  const epic_domain_desc& d = c->dom;
+  MccArgs<P> a{};
+  a.x = (P*)S.x; a.y = (P*)S.y; a.vx = (P*)S.vx; a.vy = (P*)S.vy; a.vz = (P*)S.vz;
+  a.id = S.id; a.d_n = S.d_n;
+  a.tab.sigma = M.sigma; a.tab.n = M.d.ntab;
+  a.tab.loge0 = log(M.d.e_min);
+  a.tab.inv_dloge = (M.d.ntab - 1) / (log(M.d.e_max) - log(M.d.e_min));
+  a.kind = kind;
+  unsigned en = 0;
+  for (int p = 0; p < EPIC_MCC_NPROC; p++)
+    if (M.d.enable[p]) en |= 1u << p;
+  a.enable = en;
+  a.ref_selection = M.d.ref_selection;
+  const double dt = d.DT / d.W;  // seconds (collision.cpp:201)
+  a.nu_max = kind == 0 ? M.nu_max_e : M.nu_max_i;
+  a.p_null = 1.0 - exp(-a.nu_max * dt);
+  a.vel_norm = d.vel_norm;
+  a.mass = S.desc.mass;
+  a.gas_mass = M.d.gas_mass;
+  a.gas_density = M.d.gas_density;
+  a.gas_vth = M.d.gas_temp_K > 0 ? sqrt(K_b * M.d.gas_temp_K / M.d.gas_mass) / d.vel_norm : 0.0;
+  a.e_exc = M.d.e_exc_th;
+  a.e_ion = M.d.e_ion_th;
+  a.step = (uint32_t)step;
+  a.sp = (uint32_t)s;
+  a.k0 = (uint32_t)d.seed;
+  a.k1 = (uint32_t)(d.seed >> 32);
+  a.counts = M.counts;
+  a.events = M.events;
+  a.event_count = M.event_count;
+  a.event_cap = M.event_cap;
+  if (a.nu_max <= 0) return hipSuccess;
+  const int grid = c->num_cu * 8;
+  hipLaunchKernelGGL(mcc_kernel<P>, dim3(grid), dim3(MCC_BLOCK), 0, c->stream, a);
+  EPIC_HIP_TRY(hipGetLastError());
+  if (kind == 0 && (en & 4)) {
+    SpeciesDev& I = c->sp[M.d.ion_sp];
+    const double ion_vth = sqrt(I.desc.temp * QE / I.desc.mass) / d.vel_norm;
+    unsigned long long* Qe = c->deposit_valid ? S.Q : nullptr;
+    unsigned long long* Qi = c->deposit_valid ? I.Q : nullptr;
+    hipLaunchKernelGGL(mcc_append_kernel<P>, dim3(1), dim3(APPEND_BLOCK), 0, c->stream, M.events, M.event_count,
+                       std::min(M.event_cap, APPEND_LDS), (P*)S.x, (P*)S.y, (P*)S.vx, (P*)S.vy, (P*)S.vz, S.id,
+                       S.d_n, S.cap, (P*)I.x, (P*)I.y, (P*)I.vx, (P*)I.vy, (P*)I.vz, I.id, I.d_n, I.cap, ion_vth,
+                       (uint32_t)step, a.k0, a.k1, Qe, Qi, d.nx, d.ny, d.x0, d.y0, d.dx, d.dy, c->fix_bits,
+                       M.counts, c->errc);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_mcc(Ctx* c, int step) {
+  if (!c->mcc.on) return hipSuccess;
+  const epic_mcc_desc& d = c->mcc.d;
+  const bool fp64 = c->dom.precision == EPIC_FP64;
+  if (d.electron_sp >= 0) {
+    EPIC_HIP_TRY(fp64 ? mcc_species<double>(c, d.electron_sp, 0, step) : mcc_species<float>(c, d.electron_sp, 0, step));
+  }
+  if (d.ion_sp >= 0 && (d.enable[EPIC_MCC_I_ELASTIC] || d.enable[EPIC_MCC_I_CX])) {
+    EPIC_HIP_TRY(fp64 ? mcc_species<double>(c, d.ion_sp, 1, step) : mcc_species<float>(c, d.ion_sp, 1, step));
+  }
+  return hipSuccess;
+}
+
+// nu_max over the table for a particle of mass m (s^-1), 5% margin for the
+// interpolation between table points and for ion-neutral relative speeds.
+double mcc_nu_max(const epic_mcc_desc& d, const std::vector<double>& sig, double m, bool electron) {
+  double best = 0;
+  const int n = d.ntab;
+  const double l0 = log(d.e_min), l1 = log(d.e_max);
+  for (int k = 0; k < n; k++) {
+    const double e = exp(l0 + (l1 - l0) * k / (n - 1));
+    double s = 0;
+    if (electron) {
+      for (int p = 0; p < 3; p++)
+        if (d.enable[p]) s += sig[(size_t)p * n + k];
+    } else {
+      for (int p = 3; p < 5; p++)
+        if (d.enable[p]) s += sig[(size_t)p * n + k];
+    }
+    const double v = sqrt(2.0 * e * eV / m);
+    best = std::max(best, d.gas_density * s * v);
+  }
+  // above e_max the cross section is held: nu keeps growing with v; cover
+  // speeds up to 4x the table's top speed
+  double stop = 0;
+  for (int p = electron ? 0 : 3; p < (electron ? 3 : 5); p++)
+    if (d.enable[p]) stop += sig[(size_t)p * n + n - 1];
+  best = std::max(best, d.gas_density * stop * sqrt(2.0 * d.e_max * eV / m) * 4.0);
+  return best * 1.05;
+}
+
+}  // namespace epic

@@ -1,0 +1,124 @@
+"""GPU parity against the oracle (FP64 CPU restatement of the reference):
+rho, phi, E and particle states after each of the first steps match within
+1e-12 normwise relative; PCG iteration counts match exactly."""
+import numpy as np
+import pytest
+
+from epic2d_mcc_amd import _lib
+from harness import reference_load, rel, small_setup
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _pair(st):
+    o, parts = reference_load(st)
+    ctx = _lib.Context(st.domain, st.species)
+    for s, p in enumerate(parts):
+        ctx.upload(s, *p)
+    return o, ctx
+
+
+def _compare_fields(o, ctx, tag, tol=TOL):
+    for which, name in ((0, "rho"), (1, "phi"), (2, "efx"), (3, "efy")):
+        r = rel(ctx.get_field(which), o.grid(which))
+        assert r < tol, f"{tag}: {name} rel {r:.3e}"
+
+
+def _compare_particles(o, ctx, nsp, tag, tol=TOL):
+    for s in range(nsp):
+        g = ctx.download(s)
+        c = o.particles(s)
+        assert len(g[0]) == len(c[0]), f"{tag}: species {s} count {len(g[0])} vs {len(c[0])}"
+        for k in range(4):
+            r = rel(g[k], c[k])
+            assert r < tol, f"{tag}: species {s} component {k} rel {r:.3e}"
+
+
+@pytest.mark.parametrize("bc,shape,N", [("pbc", "CIC", 2000), ("open", "CIC", 2000), ("reflective", "CIC", 2000),
+                                        ("pbc", "NGP", 2000)])
+def test_stepwise_parity(bc, shape, N):
+    print(f"[parity] bc={bc} shape={shape} N={N}", flush=True)
+    st = small_setup(N=N, bc=bc, shape=shape, tol=1e-10, vsx_beam=2.0 if bc != "pbc" else 0.0)
+    o, ctx = _pair(st)
+    nsp = 2
+    for s in range(nsp):
+        ctx.scatter(s)
+        o.scatter(s)
+    ctx.compute_rho()
+    o.compute_rho()
+    itg, _ = ctx.solve()
+    ito, _ = o.solve()
+    assert itg == ito
+    ctx.efield()
+    o.efield()
+    _compare_fields(o, ctx, "initial")
+    for s in range(nsp):
+        ctx.rewind(s)
+        o.rewind(s)
+    _compare_particles(o, ctx, nsp, "rewind")
+    for ts in range(10):
+        for s in range(nsp):
+            ctx.scatter(s)
+            o.scatter(s)
+        ctx.compute_rho()
+        o.compute_rho()
+        itg, _ = ctx.solve()
+        ito, _ = o.solve()
+        assert itg == ito, f"step {ts}: iterations {itg} vs {ito}"
+        ctx.efield()
+        o.efield()
+        _compare_fields(o, ctx, f"step {ts}")
+        for s in range(nsp):
+            ctx.push(s)
+            o.push(s)
+        _compare_particles(o, ctx, nsp, f"step {ts}")
+    assert not ctx.errors().any()
+
+
+def test_fused_step_matches_oracle():
+    """epic_step (deposit fused into the push) against the oracle's loop."""
+    print("[parity] fused step", flush=True)
+    st = small_setup(N=4000, Nx=32, Ny=32, tol=1e-10)
+    o, ctx = _pair(st)
+    ctx.initial_field()
+    o.initial_field()
+    for ts in range(10):
+        itg = ctx.step(ts)
+        ito = o.step()
+        assert itg == ito, f"step {ts}: iterations {itg} vs {ito}"
+    _compare_fields(o, ctx, "after 10 fused steps")
+    _compare_particles(o, ctx, 2, "after 10 fused steps")
+
+
+def test_c1_full_size_ten_steps():
+    """The 64x64 periodic workload at full size (409,600 particles per species),
+    reference load from mt19937(0), PCG, first 10 steps."""
+    print("[parity] C1 64x64 819,200 particles", flush=True)
+    st = small_setup(N=409600, Nx=64, Ny=64, tol=1e-6)
+    o, ctx = _pair(st)
+    ctx.initial_field()
+    o.initial_field()
+    for ts in range(10):
+        itg = ctx.step(ts)
+        ito = o.step()
+        assert itg == ito, f"step {ts}: iterations {itg} vs {ito}"
+        _compare_fields(o, ctx, f"C1 step {ts}")
+    _compare_particles(o, ctx, 2, "C1 after 10 steps")
+
+
+def test_deposit_is_deterministic():
+    st = small_setup(N=20000, Nx=32, Ny=32)
+    o, parts = reference_load(st)
+    outs = []
+    for rep in range(2):
+        ctx = _lib.Context(st.domain, st.species)
+        for s, p in enumerate(parts):
+            ctx.upload(s, *p)
+        ctx.scatter(0)
+        ctx.scatter(1)
+        ctx.compute_rho()
+        outs.append(ctx.get_field(0).copy())
+        ctx.close()
+    assert np.array_equal(outs[0], outs[1])

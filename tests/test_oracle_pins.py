@@ -1,0 +1,402 @@
+"""Pins of the oracle (CPU restatement) against what fixes it independently:
+the reference's own compiled pcg(), published known answers, closed forms,
+brute force on tiny inputs.  Each check is chosen so that a plausible mistake
+(dropped term, wrong sign, swapped index, transposed operand) fails."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from epic2d_mcc_amd import config
+from harness import small_setup
+from oracle_api import OracleSim, OrcInputs, OrcSetup, dp, oracle, ref_pcg_lib
+
+
+# ---- random number streams ---------------------------------------------------
+def test_mt19937_known_answer():
+    # C++ [rand.predef]: the 10000th output of a default-constructed mt19937
+    assert oracle().orc_mt19937_nth(5489, 10000) == 4123659995
+
+
+def test_uniform_draws_in_unit_interval_and_seeded():
+    a = np.empty(20000)
+    oracle().orc_rnd_draws(0, 20000, dp(a))
+    assert a.min() >= 0.0 and a.max() < 1.0
+    assert abs(a.mean() - 0.5) < 0.01
+    b = np.empty(20000)
+    oracle().orc_rnd_draws(0, 20000, dp(b))
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("ctr,key,out", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ctr, key, out):
+    """Random123 kat_vectors for philox4x32_10."""
+    C = (ctypes.c_uint32 * 4)(*ctr)
+    K = (ctypes.c_uint32 * 2)(*key)
+    O = (ctypes.c_uint32 * 4)()
+    oracle().orc_philox4x32_10(C, K, O)
+    assert list(O) == out
+
+
+# ---- set-up / normalisation -------------------------------------------------------
+def _orc_setup(specs, Nx, Ny, density, DT_coeff, norm_scheme=1):
+    n = len(specs)
+    mass = (ctypes.c_double * n)(*[s.mass for s in specs])
+    npart = (ctypes.c_int * n)(*[s.N for s in specs])
+    temps = (ctypes.c_double * n)(*[s.temp for s in specs])
+    signs = (ctypes.c_int * n)(*[s.sign for s in specs])
+    frac = (ctypes.c_double * n)(*[s.frac for s in specs])
+    vsx = (ctypes.c_double * n)(*[s.vsx for s in specs])
+    vsy = (ctypes.c_double * n)(*[s.vsy for s in specs])
+    inp = OrcInputs(Nx=Nx, Ny=Ny, x0=0.0, y0=0.0, NUM_TS=10, DT_coeff=DT_coeff, density=density,
+                    norm_scheme=norm_scheme, L_scale=0.005, T_scale_kind=0, T_scale_value=0.0, nsp=n, mass=mass,
+                    nparticles=npart, temps=temps, charge_signs=signs, frac=frac, vsx=vsx, vsy=vsy)
+    out = OrcSetup()
+    assert oracle().orc_compute_setup(ctypes.byref(inp), ctypes.byref(out)) == 0
+    return out
+
+
+def test_setup_closed_form_c1():
+    """1e13 m^-3, Te = 1 eV: textbook Debye length sqrt(eps0 T/(n e)), plasma
+    frequency sqrt(n e^2/(eps0 m)), and spwt = n Lx Ly L^2 / N."""
+    specs = [config.SpeciesSpec("electron", config.ME, 409600, 1.0, -1, 1.0),
+             config.SpeciesSpec("ion", 1836 * config.ME, 409600, 0.026, 1, 0.0)]
+    o = _orc_setup(specs, 64, 64, 1e13, 0.05)
+    lde = math.sqrt(8.8541878128e-12 * 1.0 / (1e13 * 1.602176634e-19))
+    wpe = math.sqrt(1e13 * 1.602176634e-19 ** 2 / (8.8541878128e-12 * 9.1093837e-31))
+    assert abs(o.LDe / lde - 1) < 1e-6
+    assert abs(o.wpe / wpe - 1) < 1e-6
+    assert abs(o.vel_norm / (lde * wpe) - 1) < 1e-6
+    assert o.dx == 1.0 and o.dy == 1.0 and o.nx == 65 and o.Lx == 64.0
+    assert abs(o.DT - 0.05) < 1e-15
+    assert abs(o.spwt[0] / (1e13 * 64 * 64 * lde * lde / 409600) - 1) < 1e-6
+    # the survey's derived figures for this workload
+    assert abs(o.LDe - 2.3508e-3) < 1e-7 and abs(o.wpe - 1.7840e8) < 1e4
+    # quasi-neutrality k = sum(-sign*frac): electrons get density/k
+    assert o.normden[0] == 1e13 and o.normden[1] == 1e13
+
+
+def test_python_setup_matches_oracle_bitwise():
+    specs = [config.SpeciesSpec("electron", config.ME, 1000, 1.0, -1, 1.0),
+             config.SpeciesSpec("ion", 1836 * config.ME, 1000, 0.026, 1, 0.0),
+             config.SpeciesSpec("negion", 1836 * config.ME, 1000, 0.026, -1, 0.1),
+             config.SpeciesSpec("beam", config.ME, 1000, 1.0, -1, 0.05, 10.0)]
+    for ns in (1, 2, 4):
+        o = _orc_setup(specs, 32, 24, 1e13, 0.05, norm_scheme=ns)
+        st = config.derive(32, 24, 0.0, specs, 1e13, 0.05, norm_scheme=ns)
+        d = st.domain
+        assert (d.L, d.W, d.vel_norm, d.dx, d.DT) == (o.L, o.W, o.vel_norm, o.dx, o.DT)
+        for i in range(4):
+            assert st.species[i].spwt == o.spwt[i]
+        # quasi-neutral: ion density = sum of negative densities
+        nd = st.info["normden"]
+        assert abs(nd[1] - (nd[0] + nd[2] + nd[3])) < 1e-6 * nd[1]
+
+
+def test_ini_parser_and_setup():
+    text = """[file]
+output = data/
+[time]
+NUM_TS = 200
+DT_coeff = 0.05
+[domain]
+Nx       = 64
+Ny       = 64
+x0       = 0
+y0       = 0
+[normalization]
+norm_scheme = 1
+vel_norm_scheme = 1
+lenght_scale = 0.005
+time_scale = omegape
+[solver]
+solvertype = cg
+tolerance = 1e-8   # inline comment: stod stops at the '#'
+max_iteration = 10000
+[simulation]
+shapefunction = CIC
+density = 1E13
+bc = pbc
+ionfixed = 0
+[collision]
+elastic = false
+GAS_DENSITY = 0
+[species]
+# name, mass, N, T, sign, frac, vsx, vsy, load
+electron,9.10938215E-31,409600,1,-1,1,0,0,random
+ion,1.6724825e-27,409600,0.026,1,0,0,0,random
+"""
+    sec = config.parse_ini_text(text)
+    assert sec["solver"]["tolerance"].startswith("1e-8")
+    st = config.setup_from_ini(sec)
+    assert st.domain.nx == 65 and st.domain.tolerance == 1e-8 and st.domain.bc == 0
+    assert st.info["NUM_TS"] == 200
+    assert [s.numparticle for s in st.species] == [409600, 409600]
+    assert config.loadtypeextract("sin(3)") == ("sin", 3)
+    assert config.loadtypeextract("random") == ("random", 0)
+    with pytest.raises(KeyError):
+        del sec["solver"]["max_iteration"]
+        config.setup_from_ini(sec)
+
+
+# ---- deposit ---------------------------------------------------------------------------
+def _single(st, x, y):
+    o = OracleSim(st.domain, st.species)
+    o.set_particles(0, [x], [y], [0.0], [0.0], [0.0])
+    o.set_particles(1, [], [], [], [], [])
+    o.scatter(0)
+    return o.grid(10)
+
+
+def test_cic_weights_brute_force():
+    st = small_setup(N=1, Nx=8, Ny=8, bc="open")
+    d = st.domain
+    norm = d.dx * d.dy * d.L * d.L * d.density
+    spwt = st.species[0].spwt
+    den = _single(st, 3.25, 5.5)
+    exp = np.zeros((9, 9))
+    exp[3, 5] = 0.75 * 0.5
+    exp[4, 5] = 0.25 * 0.5
+    exp[3, 6] = 0.75 * 0.5
+    exp[4, 6] = 0.25 * 0.5
+    assert np.allclose(den, exp * spwt / norm, rtol=1e-15, atol=0)
+    # edge node (open): x 1/2 ; corner x 1/4
+    den = _single(st, 0.0, 0.0)
+    assert den[0, 0] == pytest.approx(0.25 * spwt / norm, rel=1e-15)
+    den = _single(st, 0.0, 4.0)
+    assert den[0, 4] == pytest.approx(0.5 * spwt / norm, rel=1e-15)
+
+
+def test_cic_periodic_fold_conserves_charge():
+    st = small_setup(N=1, Nx=8, Ny=8, bc="pbc")
+    d = st.domain
+    norm = d.dx * d.dy * d.L * d.L * d.density
+    spwt = st.species[0].spwt
+    den = _single(st, 7.5, 7.75)  # touches the last column and row -> folded
+    uniq = den[:-1, :-1]
+    assert uniq.sum() * norm == pytest.approx(spwt, rel=1e-14)
+    # the periodic images hold the same value
+    assert np.array_equal(den[0, :], den[-1, :]) and np.array_equal(den[:, 0], den[:, -1])
+    assert den[0, 0] == pytest.approx(0.5 * 0.75 * spwt / norm, rel=1e-14)
+
+
+def test_ngp_rounds_to_nearest_node():
+    st = small_setup(N=1, Nx=8, Ny=8, bc="open", shape="NGP")
+    den = _single(st, 3.49, 5.51)
+    assert den[3, 6] > 0 and np.count_nonzero(den) == 1
+
+
+# ---- Poisson ---------------------------------------------------------------------------
+def _dense_A(nx, ny):
+    n = nx * ny
+    A = np.zeros((n, n))
+    for i in range(n):
+        r, c = divmod(i, nx)
+        A[i, i] = -4
+        if c > 0: A[i, i - 1] = 1
+        if c < nx - 1: A[i, i + 1] = 1
+        if r > 0: A[i, i - nx] = 1
+        if r < ny - 1: A[i, i + nx] = 1
+    return A
+
+
+@pytest.mark.parametrize("nx,ny", [(9, 9), (17, 17), (33, 33), (12, 9)])
+def test_pcg_bitwise_equal_to_reference_pcg(nx, ny):
+    r = ref_pcg_lib()
+    if r is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(nx * 1000 + ny)
+    b = rng.standard_normal((nx, ny))
+    b[0, :] = b[-1, :] = 0
+    b[:, 0] = b[:, -1] = 0
+    b = b.ravel().copy()
+    x1 = np.zeros_like(b)
+    x2 = np.zeros_like(b)
+    err = ctypes.c_double()
+    for tol in (1e-3, 1e-10):
+        oracle().orc_pcg(nx, ny, dp(b), dp(x1), 10000, tol)
+        r.ref_pcg(nx, ny, dp(b), dp(x2), 10000, tol, ctypes.byref(err))
+        assert np.array_equal(x1, x2)
+
+
+def test_pcg_solves_dense_system():
+    nx, ny = 9, 9
+    A = _dense_A(nx, ny)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(nx * ny)
+    x = np.zeros_like(b)
+    oracle().orc_pcg(nx, ny, dp(b), dp(x), 10000, 1e-13)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=0, atol=1e-11)
+
+
+def test_pcg_eigenmode():
+    """sin(pi k (i+1)/(nx+1)) sin(pi l (j+1)/(ny+1)) with eigenvalue
+    -4 + 2cos(pi k/(nx+1)) + 2cos(pi l/(ny+1)) on the zero-ghost grid."""
+    nx = ny = 17
+    k, l = 3, 5
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny)[None, :]
+    u = np.sin(np.pi * k * (i + 1) / (nx + 1)) * np.sin(np.pi * l * (j + 1) / (ny + 1))
+    lam = -4 + 2 * np.cos(np.pi * k / (nx + 1)) + 2 * np.cos(np.pi * l / (ny + 1))
+    b = (lam * u).ravel().copy()
+    x = np.zeros_like(b)
+    oracle().orc_pcg(nx, ny, dp(b), dp(x), 10000, 1e-14)
+    assert np.allclose(x, u.ravel(), atol=1e-12)
+
+
+def test_gs_manufactured_dirichlet():
+    st = small_setup(N=1, Nx=16, Ny=16, solver=1, tol=1e-12, maxit=100000)
+    d = st.domain
+    nx, ny = d.nx, d.ny
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny)[None, :]
+    phi = np.sin(np.pi * i / (nx - 1)) * np.sin(2 * np.pi * j / (ny - 1))
+    lap = np.zeros_like(phi)
+    lap[1:-1, 1:-1] = (phi[2:, 1:-1] - 2 * phi[1:-1, 1:-1] + phi[:-2, 1:-1]) + \
+                      (phi[1:-1, 2:] - 2 * phi[1:-1, 1:-1] + phi[1:-1, :-2])
+    s = (d.L * d.L) / (d.LDe * d.LDe)
+    rho = -lap / s
+    o = OracleSim(d, st.species)
+    o.set_grid(0, rho)
+    o.solve()
+    assert np.allclose(o.grid(1), phi, atol=1e-10)
+
+
+def test_efield_exact_for_quadratic():
+    st = small_setup(N=1, Nx=8, Ny=8, bc="open")
+    d = st.domain
+    i = np.arange(d.nx)[:, None].astype(float)
+    j = np.arange(d.ny)[None, :].astype(float)
+    phi = 0.5 * i * i + 3 * j + 0 * i
+    o = OracleSim(d, st.species)
+    o.set_grid(1, phi)
+    o.efield()
+    ex = o.grid(2)
+    ey = o.grid(3)
+    assert np.allclose(ex[1:-1, :], -i[1:-1] * np.ones((1, d.ny)))  # central difference of i^2/2 is exact
+    assert np.allclose(ey, -3.0)
+    assert np.allclose(ex[0, :], -0.5) and np.allclose(ex[-1, :], -(0.5 * 64 - 0.5 * 49))
+
+
+def test_efield_periodic_edges():
+    st = small_setup(N=1, Nx=8, Ny=8, bc="pbc")
+    d = st.domain
+    i = np.arange(d.nx)[:, None].astype(float)
+    phi = np.sin(2 * np.pi * i / (d.nx - 1)) * np.ones((1, d.ny))
+    o = OracleSim(d, st.species)
+    o.set_grid(1, phi)
+    o.efield()
+    ex = o.grid(2)
+    exp = -(np.sin(2 * np.pi * 1 / 8) - np.sin(2 * np.pi * 7 / 8)) / 2
+    assert ex[0, 3] == pytest.approx(exp) and ex[-1, 3] == ex[0, 3]
+
+
+# ---- push / rewind -------------------------------------------------------------------
+def test_push_harmonic_oscillator_frequency():
+    """A linear field E = k (x - xc) (exact under CIC) pulls an electron
+    (q < 0) back to xc: leapfrog oscillates at 2 asin(w dt/2)/dt with
+    w^2 = |q/m| k K c3 in the normalised units of the push."""
+    st = small_setup(N=1, Nx=32, Ny=8, bc="pbc", DT_coeff=0.1)
+    d = st.domain
+    o = OracleSim(d, st.species)
+    kk = 0.02
+    i = np.arange(d.nx)[:, None].astype(float)
+    efx = kk * (i - 16.0) * np.ones((1, d.ny))
+    o.set_grid(2, efx)
+    o.set_grid(3, np.zeros((d.nx, d.ny)))
+    o.set_particles(0, [18.0], [4.0], [0.0], [0.0], [0.0])
+    o.set_particles(1, [], [], [], [], [])
+    xs = []
+    for _ in range(4000):
+        o.push(0)
+        xs.append(o.particles(0)[0][0])
+    xs = np.array(xs) - 16.0
+    qm = st.species[0].charge / st.species[0].mass
+    K = (d.density * config.QE * d.L) / (config.EPS_0 * d.W * d.vel_norm)
+    c3 = d.vel_norm / (d.L * d.W)
+    w2 = -(qm * K * c3) * kk  # d2x/dt2 = c3*qm*K*E = -w2 (x - xc)
+    assert w2 > 0
+    w = math.sqrt(w2)
+    dt = d.DT
+    wnum = 2 * math.asin(w * dt / 2) / dt
+    crossings = np.where(np.diff(np.sign(xs)) != 0)[0]
+    period = 2 * np.mean(np.diff(crossings)) * dt
+    assert abs(2 * math.pi / period - wnum) / wnum < 5e-3
+
+
+def test_push_uniform_field_and_wrap():
+    st = small_setup(N=1, Nx=8, Ny=8, bc="pbc")
+    d = st.domain
+    o = OracleSim(d, st.species)
+    o.set_grid(2, np.full((d.nx, d.ny), 0.5))
+    o.set_grid(3, np.zeros((d.nx, d.ny)))
+    o.set_particles(0, [7.99], [1.0], [3.0], [0.0], [0.7])
+    o.set_particles(1, [], [], [], [], [])
+    o.push(0)
+    x, y, vx, vy, vz = [a[0] for a in o.particles(0)]
+    qm = st.species[0].charge / st.species[0].mass
+    K = (d.density * config.QE * d.L) / (config.EPS_0 * d.W * d.vel_norm)
+    v1 = 3.0 + qm * K * 0.5 * d.DT
+    assert vx == pytest.approx(v1, rel=1e-15)
+    assert vz == 0.7  # vz untouched (species.cpp:44-45)
+    assert x == pytest.approx(7.99 + v1 * d.DT - 8.0, rel=1e-12)  # wrapped once
+    # open boundary: the particle is removed
+    st2 = small_setup(N=1, Nx=8, Ny=8, bc="open")
+    o2 = OracleSim(st2.domain, st2.species)
+    o2.set_grid(2, np.zeros((9, 9)))
+    o2.set_grid(3, np.zeros((9, 9)))
+    o2.set_particles(0, [7.99, 4.0], [1.0, 4.0], [3.0, 0.0], [0.0, 0.0], [0.0, 0.0])
+    o2.set_particles(1, [], [], [], [], [])
+    o2.push(0)
+    assert len(o2.particles(0)[0]) == 1 and o2.particles(0)[0][0] == 4.0
+
+
+# ---- diagnostics -----------------------------------------------------------------------
+def test_ke_closed_form():
+    """Electrons at v = vel_norm = v_th,e along x carry KE_x = m v^2/2 per
+    particle = T_e/2: normalised by N T_e it is exactly 1/2."""
+    st = small_setup(N=100, Nx=8, Ny=8)
+    o = OracleSim(st.domain, st.species)
+    n = 100
+    o.set_particles(0, np.full(n, 1.0), np.full(n, 1.0), np.ones(n), np.zeros(n), np.zeros(n))
+    ke = o.ke(0, 0)
+    assert ke[0] == pytest.approx(0.5, rel=1e-5) and ke[1] == 0 and ke[2] == 0
+    p = o.momentum(0, 0)
+    # p = N m vn spwt / (sqrt(T eV/ME) spwt N ME) = vn/sqrt(T eV/ME) = 1 for electrons at 1 eV
+    assert p[0] == pytest.approx(1.0, rel=1e-5)
+
+
+def test_pe_closed_form():
+    """Uniform normalised field E0 on every node: PE = 1/2 eps0 |E|^2 per node
+    area L^2, summed over all nx*ny nodes (domain.cpp:270-277)."""
+    st = small_setup(N=100, Nx=8, Ny=8)
+    d = st.domain
+    o = OracleSim(d, st.species)
+    E0 = 0.01
+    o.set_grid(2, np.full((d.nx, d.ny), E0))
+    o.set_grid(3, np.zeros((d.nx, d.ny)))
+    Ephys = E0 * d.density * config.QE * d.L / config.EPS_0
+    energy = 0.5 * config.EPS_0 * Ephys ** 2 * d.L ** 2 * d.nx * d.ny
+    sp = st.species[0]
+    assert o.pe(0) == pytest.approx(energy / (sp.temp * config.eV * sp.spwt * sp.numparticle), rel=1e-12)
+
+
+# ---- cross sections / direct MCC --------------------------------------------------------
+def test_cross_section_thresholds():
+    o = oracle()
+    assert o.orc_sigma_excitation(11.5) == 0.0 and o.orc_sigma_excitation(11.6) > 0
+    assert o.orc_sigma_ionization(15.8) == 0.0 and o.orc_sigma_ionization(16.0) > 0
+    e = np.logspace(-2, 3, 400)
+    s = np.array([o.orc_sigma_elastic(v) for v in e])
+    assert np.all(s > 0) and np.all(s < 1e-18)
+    # argon's Ramsauer-Townsend minimum of the elastic cross section lies
+    # near 0.2-0.3 eV
+    e = np.logspace(-2, 1, 3000)
+    s = np.array([o.orc_sigma_elastic(v) for v in e])
+    assert 0.15 < e[np.argmin(s)] < 0.35
