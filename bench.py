@@ -1,0 +1,184 @@
+"""Benchmark: particle-steps/s of the per-timestep PIC loop on one MI355X.
+
+Workload (default): the four-species bounded plasma -- 512x512 cells,
+absorbing walls with Dirichlet phi = 0, electrons + positive ions + negative
+ions (alpha 0.1) + negative beam (beta 0.05, 10 v_th,e drift), 100
+particles/cell/species = 104,857,600 particles, FP32 particles,
+mixed-precision PCG (FP32 iterations, FP64 refinement) to 1e-10 relative,
+counting sort every 20 steps.  A step is one full pass of the loop: scatter,
+rho, Poisson, E, gather+push (+fused next deposit), absorption.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
+runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from epic2d_mcc_amd import _lib, workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def gpu_run(st, steps, warmup, solver_name):
+    ctx = _lib.Context(st.domain, st.species)
+    nsp = len(st.species)
+    for s, sp in enumerate(st.species):
+        ctx.load(s, sp.numparticle, vz_thermal=False)
+    for s in range(nsp):
+        ctx.sort(s)
+    ctx.initial_field()
+    ts = 0
+    for _ in range(warmup):
+        ctx.step(ts)
+        ts += 1
+    n0 = sum(ctx.count(s) for s in range(nsp))
+    per_sp0 = [ctx.count(s) for s in range(nsp)]
+    ctx.timing(True)
+    iters = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        iters.append(ctx.step(ts))
+        ts += 1
+    ctx.sync()
+    t1 = time.perf_counter()
+    tm = ctx.timing_read()
+    ctx.timing(False)
+    n1 = sum(ctx.count(s) for s in range(nsp))
+    per_sp1 = [ctx.count(s) for s in range(nsp)]
+    errs = ctx.errors()
+    ctx.close()
+    return dict(t=t1 - t0, n0=n0, n1=n1, per_sp0=per_sp0, per_sp1=per_sp1, timing=tm, iters=iters,
+                errors=errs.tolist())
+
+
+def cpu_baseline(st, sample_frac=0.1):
+    """The oracle (FP64 CPU restatement of the reference loop, one thread) on a
+    bounded sample of the same workload: the full 512x512 grid and one step,
+    with `sample_frac` of the particles.  Particle stages (scatter, push) are
+    timed on the sample and scaled to the full count; the field stages
+    (rho, Poisson, E) are per step and counted once.  Poisson: the
+    matrix-free restatement of the reference's pcg() recurrence (dense PCG at
+    263,169 unknowns needs 1.1 TB), absolute tolerance = the workload's
+    relative one times ||b||."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_api import OracleSim
+    d = st.domain
+    dom = _lib.DomainDesc.from_buffer_copy(d)
+    dom.solver = _lib.SOLVER_PCG
+    o = OracleSim(dom, st.species)
+    rng = np.random.default_rng(7)
+    Lx = (d.nx - 1) * d.dx
+    Ly = (d.ny - 1) * d.dy
+    nsp = len(st.species)
+    n_full = sum(int(sp.numparticle) for sp in st.species)
+    n_samp = 0
+    for s, sp in enumerate(st.species):
+        n = max(1, int(sp.numparticle * sample_frac))
+        n_samp += n
+        sig = math.sqrt(sp.temp * 1.602176565e-19 / sp.mass) / d.vel_norm
+        o.set_particles(s, rng.random(n) * Lx, rng.random(n) * Ly, rng.normal(0, sig, n) + sp.vsx,
+                        rng.normal(0, sig, n) + sp.vsy, np.zeros(n))
+    for s in range(nsp):
+        o.scatter(s)
+    o.compute_rho()
+    rho = o.grid(0)
+    b = -rho[1:-1, 1:-1] * d.dx * d.dx * ((d.L * d.L) / (d.LDe * d.LDe))
+    o.set_tolerance(d.tolerance * float(np.sqrt((b * b).sum())))
+    t = time.perf_counter
+    t0 = t()
+    for s in range(nsp):
+        o.scatter(s)
+    t1 = t()
+    o.compute_rho()
+    it, _ = o.solve()
+    o.efield()
+    t2 = t()
+    for s in range(nsp):
+        o.push(s)
+    t3 = t()
+    scale = n_full / n_samp
+    t_particles = ((t1 - t0) + (t3 - t2)) * scale
+    t_fields = t2 - t1
+    t_step = t_particles + t_fields
+    return dict(value=n_full / t_step, t=t3 - t0, iters=it, npart=n_full, nsamp=n_samp,
+                t_particles=t_particles, t_fields=t_fields)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--solver", default="pcg_mixed", choices=["pcg_mixed", "spectral", "pcg", "gs"])
+    ap.add_argument("--ppc", type=int, default=100)
+    ap.add_argument("--cpu-sample", type=float, default=0.1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU (the per-step loop of one simulation); "
+              "--gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+    solver = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
+              "gs": _lib.SOLVER_GS}[args.solver]
+    st = workloads.bounded_512(ppc=args.ppc, solver=solver)
+    r = gpu_run(st, args.steps, args.warmup, args.solver)
+    nsp = len(st.species)
+    part_steps = 0.5 * (r["n0"] + r["n1"]) * args.steps
+    value = part_steps / r["t"]
+    # dominant kernel: the fused gather + push + deposit, one launch per species
+    push_ms, push_calls = r["timing"]["push"]
+    avg_launch_s = push_ms / max(push_calls, 1) / 1e3
+    parts_per_launch = 0.5 * (sum(r["per_sp0"]) + sum(r["per_sp1"])) / nsp
+    bytes_per_ps = 8 * 4  # read + write x, y, vx, vy in FP32
+    achieved = bytes_per_ps * parts_per_launch / avg_launch_s / 1e9
+    stage = {k: round(v[0] / max(args.steps, 1), 4) for k, v in r["timing"].items() if v[1]}
+    out = {
+        "metric": "particle-steps/s per GPU (+% HBM roofline)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["t"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
+        "config": {"workload": "bounded 512x512, 4 species x %d ppc (%d particles), open walls, "
+                               "Dirichlet phi=0, mixed PCG 1e-10, sort every 20 steps" % (
+                                   args.ppc, sum(int(s.numparticle) for s in st.species)),
+                   "solver": args.solver, "particles_start": r["n0"], "particles_end": r["n1"],
+                   "poisson_iterations_mean": float(np.mean(r["iters"])) if r["iters"] else 0,
+                   "stage_ms_per_step": stage},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "particle_kernel<float,8,MODE_PUSH,false,true> (gather+push+deposit)",
+                     "bytes_per_particle_step": bytes_per_ps, "avg_launch_ms": avg_launch_s * 1e3,
+                     "particles_per_launch": parts_per_launch},
+        "device_errors": r["errors"],
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(st, sample_frac=args.cpu_sample)
+        out["cpu_baseline"] = {
+            "value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "one step of the same workload by the FP64 oracle on one thread: full 512x512 grid, %d of "
+                      "the %d particles (particle stages %.1f s scaled x%.1f to the full count; field stages %.1f s "
+                      "incl. %d PCG iterations of the matrix-free pcg() restatement); %.1f s of CPU work" % (
+                          cb["nsamp"], cb["npart"], cb["t_particles"] * cb["nsamp"] / cb["npart"],
+                          cb["npart"] / cb["nsamp"], cb["t_fields"], cb["iters"], cb["t"])}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -22,6 +22,7 @@ BC = {"pbc": 0, "open": 1, "reflective": 2}
 SHAPE = {"CIC": 0, "NGP": 1}
 SOLVER_PCG, SOLVER_GS, SOLVER_PCG_MIXED, SOLVER_SPECTRAL = 0, 1, 2, 3
 OP_REF_GHOST, OP_DIRICHLET_WALL = 0, 1
+PRECOND_JACOBI, PRECOND_SPECTRAL = 0, 1
 FP64, FP32 = 0, 1
 NERR = 8
 NSTAGE = 8
@@ -39,7 +40,7 @@ class DomainDesc(ctypes.Structure):
                 ("solver", ctypes.c_int32), ("poisson_op", ctypes.c_int32),
                 ("tolerance", ctypes.c_double), ("tol_relative", ctypes.c_int32), ("max_iteration", ctypes.c_int32),
                 ("sor_omega", ctypes.c_double), ("precision", ctypes.c_int32), ("sort_interval", ctypes.c_int32),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("precond", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class SpeciesDesc(ctypes.Structure):

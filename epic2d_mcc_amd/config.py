@@ -109,7 +109,7 @@ class Setup:
 def derive(Nx, Ny, x0, species: List[SpeciesSpec], density, DT_coeff, norm_scheme=1, L_scale=0.005,
            T_scale="omegape", bc="pbc", shape="CIC", solver=_lib.SOLVER_PCG, poisson_op=None, tolerance=1e-8,
            tol_relative=0, max_iteration=10000, precision=_lib.FP64, sort_interval=0, seed=1, ionfixed=0,
-           capacity_factor=1.0, sor_omega=1.4) -> Setup:
+           capacity_factor=1.0, sor_omega=1.4, precond=0) -> Setup:
     """main.cpp:151-261 for the given ini values."""
     nsp = len(species)
     if nsp < 2:
@@ -169,7 +169,7 @@ def derive(Nx, Ny, x0, species: List[SpeciesSpec], density, DT_coeff, norm_schem
                           density=density, DT=DT, bc=_lib.BC[bc], shape=_lib.SHAPE[shape], solver=solver,
                           poisson_op=poisson_op, tolerance=tolerance, tol_relative=tol_relative,
                           max_iteration=max_iteration, sor_omega=sor_omega, precision=precision,
-                          sort_interval=sort_interval, seed=seed)
+                          sort_interval=sort_interval, seed=seed, precond=precond)
     descs = []
     for i, sp in enumerate(species):
         spwt = normden[i] * Lx * Ly * L * L / sp.N

@@ -53,20 +53,39 @@ hipError_t sync_counts(Ctx* c) {
   return hipSuccess;
 }
 
-void stage_begin(Ctx* c) {
-  if (c->timing) hipEventRecord(c->ev0, c->stream);
-}
-void stage_end(Ctx* c, int stage) {
-  if (!c->timing) return;
-  hipEventRecord(c->ev1, c->stream);
-  hipEventSynchronize(c->ev1);
-  float ms = 0;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
-  c->stage_ms[stage] += ms;
-  c->stage_calls[stage]++;
+// Resolve every recorded pair into the per-stage totals.
+static void resolve_events(Ctx* c) {
+  if (c->ev_used == 0) return;
+  hipEventSynchronize(c->ev_pool[2 * (c->ev_used - 1) + 1]);
+  for (int k = 0; k < c->ev_used; k++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]) == hipSuccess) {
+      c->stage_ms[c->ev_stage[k]] += ms;
+      c->stage_calls[c->ev_stage[k]]++;
+    }
+  }
+  c->ev_used = 0;
 }
 
+void stage_begin(Ctx* c) {
+  if (!c->timing) return;
+  if (c->ev_used * 2 + 2 > (int)c->ev_pool.size()) resolve_events(c);
+  if (c->ev_used * 2 + 2 > (int)c->ev_pool.size()) return;
+  hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream);
+  c->ev_open = c->ev_used;
+}
+void stage_end(Ctx* c, int stage) {
+  if (!c->timing || c->ev_open < 0) return;
+  hipEventRecord(c->ev_pool[2 * c->ev_open + 1], c->stream);
+  c->ev_stage[c->ev_open] = stage;
+  c->ev_used = c->ev_open + 1;
+  c->ev_open = -1;
+}
+
+void spec_destroy(Ctx* c);
+
 static void free_all(Ctx* c) {
+  spec_destroy(c);
   for (auto& S : c->sp) {
     void* ps[] = {S.x, S.y, S.vx, S.vy, S.vz, S.id, S.x2, S.y2, S.vx2, S.vy2, S.vz2, S.id2, S.Q, S.den, S.d_n};
     for (void* p : ps)
@@ -80,6 +99,8 @@ static void free_all(Ctx* c) {
     if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t ev : c->ev_pool) hipEventDestroy(ev);
+  c->ev_pool.clear();
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -171,7 +192,7 @@ int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int ns
   CKC(dalloc((double**)&c->ws_x, nn), "x");
   CKC(dalloc(&c->ws_part, (size_t)16 * std::max(c->max_solver_blocks, 1024)), "partials");
   CKC(dalloc(&c->ws_scalar, 8), "scalars");
-  CKC(dalloc(&c->ws_iter, 1), "iters");
+  CKC(dalloc(&c->ws_iter, 4), "iters");  // [0] iterations, [1] grid barrier
   const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
   CKC(dalloc(&c->sort_count, ncell + 2), "sort");
   CKC(dalloc(&c->sort_fill, ncell + 2), "sort");
@@ -478,12 +499,12 @@ int epic_step(epic_ctx* h, int ts, int* iters) {
   stage_begin(c);
   CK(launch_efield(c), "efield");
   stage_end(c, 3);
-  stage_begin(c);
   for (int s = 0; s < c->nsp; s++) {
+    stage_begin(c);
     if (c->sp[s].desc.pushed) CK(launch_push(c, s, 1, true, 0.0), "push");
     else CK(launch_deposit(c, s), "deposit");
+    stage_end(c, 4);
   }
-  stage_end(c, 4);
   c->deposit_valid = true;
   if (c->mcc.on) {
     stage_begin(c);
@@ -615,6 +636,13 @@ int epic_clear_error_counters(epic_ctx* h) {
 int epic_timing_enable(epic_ctx* h, int on) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
+  if (on && c->ev_pool.empty()) {
+    c->ev_pool.resize(2 * 2048);
+    c->ev_stage.assign(2048, 0);
+    for (auto& ev : c->ev_pool) CK(hipEventCreate(&ev), "hipEventCreate");
+  }
+  c->ev_used = 0;
+  c->ev_open = -1;
   c->timing = on != 0;
   for (int k = 0; k < EPIC_NSTAGE; k++) {
     c->stage_ms[k] = 0;
@@ -626,6 +654,7 @@ int epic_timing_enable(epic_ctx* h, int on) {
 int epic_timing_read(epic_ctx* h, double ms[EPIC_NSTAGE], int64_t calls[EPIC_NSTAGE]) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
+  resolve_events(c);
   for (int k = 0; k < EPIC_NSTAGE; k++) {
     if (ms) ms[k] = c->stage_ms[k];
     if (calls) calls[k] = c->stage_calls[k];

@@ -122,8 +122,13 @@ struct Ctx {
   bool deposit_valid = false;  // Q holds the deposit of the current positions
   int step_count = 0;
 
-  // timing
+  // timing: event pairs recorded on the stream without synchronising,
+  // resolved when read
   bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_stage;  // stage of each recorded pair
+  int ev_used = 0;
+  int ev_open = -1;           // index of the pair whose start was recorded
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double stage_ms[EPIC_NSTAGE] = {};
   int64_t stage_calls[EPIC_NSTAGE] = {};

@@ -7,7 +7,8 @@
 // beta = w.r, v = w + v (beta/alpha).  The stopping test ||r||_2 < tol is
 // taken before every iteration, iterations capped at min(n^2, max_iteration).
 //
-// One persistent (cooperative) kernel runs the whole solve.  Each thread owns
+// One persistent kernel runs the whole solve (a single workgroup on small
+// grids; one co-resident workgroup per CU with a counter barrier otherwise).  Each thread owns
 // a strided set of nodes and keeps x, r, w and v for them in registers; only
 // the search direction crosses threads, through a double-buffered global
 // array.  The next iteration's stencil rebuilds neighbour v from neighbour
@@ -17,15 +18,11 @@
 // sums the partials in the same fixed order, so all blocks hold identical
 // scalars and runs are reproducible.  Dots accumulate in FP64 whatever the
 // vector precision T.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
 
 #include "device_common.hpp"
 #include "epic_internal.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace epic {
 
@@ -46,6 +43,7 @@ struct PcgArgs {
   double tol;           // stop when ||r|| < tol (absolute)
   int relative;         // tol relative to ||b||
   int compute_error;    // ||A x - b|| / n (field.cpp:258-259)
+  unsigned* bar;        // grid barrier counter
 };
 
 template <int OP>
@@ -90,9 +88,27 @@ __device__ __forceinline__ T rd(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Grid barrier for a co-resident (cooperative) launch: a monotonic counter
+// in device memory, zeroed by the host before the launch.  Thread 0 of each
+// workgroup arrives with an agent-scope release and waits with agent-scope
+// acquires; the workgroup barriers order the other threads.
+struct GridBar {
+  unsigned* ctr;
+  unsigned epoch;
+  __device__ __forceinline__ void sync() {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      epoch += gridDim.x;
+      __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) __builtin_amdgcn_s_sleep(1);
+    }
+    __syncthreads();
+  }
+};
+
 // Reduce NV doubles over the whole grid.  `slot` selects a partial-sum region.
 template <int NV, bool COOP>
-__device__ __forceinline__ void grid_sum(double (&v)[NV], double* part, int slot, double* sh, cg::grid_group& grid) {
+__device__ __forceinline__ void grid_sum(double (&v)[NV], double* part, int slot, double* sh, GridBar& grid) {
   block_sum<NV>(v, sh);
   if (!COOP) return;
   const int G = gridDim.x;
@@ -101,7 +117,6 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double* part, int slot
 #pragma unroll
     for (int k = 0; k < NV; k++) p[k * G + blockIdx.x] = v[k];
   }
-  __threadfence();
   grid.sync();
   if (threadIdx.x < 64) {
 #pragma unroll
@@ -119,9 +134,8 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double* part, int slot
 }
 
 template <bool COOP>
-__device__ __forceinline__ void gsync(cg::grid_group& grid) {
+__device__ __forceinline__ void gsync(GridBar& grid) {
   if (COOP) {
-    __threadfence();
     grid.sync();
   } else {
     __syncthreads();
@@ -131,7 +145,7 @@ __device__ __forceinline__ void gsync(cg::grid_group& grid) {
 template <typename T, int OP, int NPT, bool COOP>
 __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
   __shared__ double sh[64];
-  cg::grid_group grid = cg::this_grid();
+  GridBar grid{a.bar, 0u};
   const int n = a.nx * a.ny;
   const int TT = gridDim.x * blockDim.x;
   const int g0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,12 +280,13 @@ struct SorArgs {
   double* part;
   double* scalars;
   int* iters_out;
+  unsigned* bar;
 };
 
 template <bool COOP>
 __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
   __shared__ double sh[64];
-  cg::grid_group grid = cg::this_grid();
+  GridBar grid{a.bar, 0u};
   const int n = a.nx * a.ny;
   const int TT = gridDim.x * blockDim.x;
   const int g0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -448,8 +463,11 @@ static int solver_blocks(Ctx* c, int n, int* npt) {
   return G;
 }
 
+__global__ void zero_bar_kernel(unsigned* b) { *b = 0u; }
+
 template <typename T, int OP, int NPT>
 static hipError_t run_pcg_t(Ctx* c, int G, PcgArgs<T>& a) {
+  hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
   if (G == 1) {
     hipLaunchKernelGGL((pcg_kernel<T, OP, NPT, false>), dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
     return hipGetLastError();
@@ -483,6 +501,7 @@ static hipError_t run_pcg(Ctx* c, const double* b, T* x_out, long long cap, doub
   a.vb = (T*)c->ws_v; a.wg = (T*)c->ws_w;
   a.part = c->ws_part; a.scalars = c->ws_scalar; a.iters_out = c->ws_iter;
   a.cap = cap; a.tol = tol; a.relative = relative; a.compute_error = compute_error;
+  a.bar = (unsigned*)(c->ws_iter + 1);
   if (d.poisson_op == EPIC_OP_REF_GHOST) return run_pcg_op<T, EPIC_OP_REF_GHOST>(c, G, npt, a);
   return run_pcg_op<T, EPIC_OP_DIRICHLET_WALL>(c, G, npt, a);
 }
@@ -493,7 +512,8 @@ static hipError_t read_scalars(Ctx* c, int* iters, double* scal3) {
   return hipStreamSynchronize(c->stream);
 }
 
-hipError_t launch_spectral(Ctx* c, const double* b, double* phi);
+hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32_inner, double tol, int max_ref,
+                                 int* solves, double* relres);
 
 hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   const epic_domain_desc& d = c->dom;
@@ -509,8 +529,10 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
     a.coeff_inv = 1 / ((2 / a.dx2) + (2 / a.dy2));
     a.s = s; a.omega = d.sor_omega; a.tol = d.tolerance; a.maxit = d.max_iteration;
     a.part = c->ws_part; a.scalars = c->ws_scalar; a.iters_out = c->ws_iter;
+    a.bar = (unsigned*)(c->ws_iter + 1);
     int npt;
     const int G = solver_blocks(c, n, &npt);
+    hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
     if (G == 1) {
       hipLaunchKernelGGL(sor_kernel<false>, dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
       EPIC_HIP_TRY(hipGetLastError());
@@ -535,11 +557,10 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
     *err = sc[1];
     return hipSuccess;
   }
-  if (d.solver == EPIC_SOLVER_SPECTRAL) {
-    EPIC_HIP_TRY(launch_spectral(c, c->b64, c->phi));
-    *iters = 1;
-    *err = 0;
-    return hipSuccess;
+  if (d.solver == EPIC_SOLVER_SPECTRAL || (d.solver == EPIC_SOLVER_PCG_MIXED && d.precond == EPIC_PRECOND_SPECTRAL)) {
+    const bool fp32 = d.solver == EPIC_SOLVER_PCG_MIXED;
+    const double tol = d.tol_relative ? d.tolerance : 1e-12;
+    return launch_spectral_solve(c, c->b64, c->phi, fp32, tol, fp32 ? 6 : 3, iters, err);
   }
 
   // Mixed precision: FP64 true residual + refinement around FP32 Jacobi-PCG
@@ -580,8 +601,3 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
 
 }  // namespace epic
 
-namespace epic {
-// Spectral solve: provided by spectral.hip when built; this weak default
-// reports the option as unavailable.
-__attribute__((weak)) hipError_t launch_spectral(Ctx*, const double*, double*) { return hipErrorNotSupported; }
-}  // namespace epic

@@ -49,8 +49,13 @@ extern "C" {
 #define EPIC_SOLVER_PCG_MIXED 2 /* FP32 Jacobi-PCG inner solves, FP64 true
                                    residual + iterative refinement             */
 #define EPIC_SOLVER_SPECTRAL 3  /* exact sine-transform solve of the same
-                                   operator, FP64 (sine transforms as FP64
-                                   matrix products on the matrix cores)        */
+                                   operator in FP64 (the transforms are FP64
+                                   matrix products on the matrix cores), plus
+                                   FP64 residual refinement                    */
+/* inner solve of EPIC_SOLVER_PCG_MIXED */
+#define EPIC_PRECOND_JACOBI 0   /* FP32 Jacobi-preconditioned CG               */
+#define EPIC_PRECOND_SPECTRAL 1 /* FP32 sine-transform solve (one application
+                                   per refinement step)                        */
 /* Poisson operators */
 #define EPIC_OP_REF_GHOST 0      /* pcgsolver's A: every node unknown, zero ghost
                                     ring, zero RHS on boundary (field.cpp:221-251) */
@@ -86,6 +91,8 @@ typedef struct epic_domain_desc {
   int32_t precision;         /* EPIC_FP64 | EPIC_FP32 particle storage */
   int32_t sort_interval;     /* counting sort by cell every k steps (0 = never) */
   uint64_t seed;             /* Philox key for native loads, MCC and injection */
+  int32_t precond;           /* EPIC_PRECOND_* for EPIC_SOLVER_PCG_MIXED */
+  int32_t reserved;
 } epic_domain_desc;
 
 /* Species constructor arguments (species.h:65-66). */
@@ -199,8 +206,10 @@ int epic_error_counters(epic_ctx* ctx, int64_t out[EPIC_NERR]);
 int epic_clear_error_counters(epic_ctx* ctx);
 
 /* ---- timing ------------------------------------------------------------ */
-/* Stage timers (HIP events on the context's stream).  Stages: 0 deposit,
- * 1 rho, 2 solve, 3 efield, 4 push(+deposit), 5 collide, 6 sort, 7 inject. */
+/* Stage timers: HIP event pairs recorded on the context's stream around
+ * each stage without synchronising; epic_timing_read resolves them.  Stages:
+ * 0 deposit, 1 rho, 2 solve, 3 efield, 4 push(+deposit, one pair per species
+ * launch), 5 collide, 6 sort, 7 inject. */
 #define EPIC_NSTAGE 8
 int epic_timing_enable(epic_ctx* ctx, int on);
 int epic_timing_read(epic_ctx* ctx, double ms[EPIC_NSTAGE], int64_t calls[EPIC_NSTAGE]);

@@ -889,4 +889,6 @@ void orc_collide_direct(void* h, int se, int st, long* counts, double* eloss_ev)
 
 int orc_last_iters(void* h) { return ((Sim*)h)->last_iters; }
 
+void orc_set_tolerance(void* h, double tol) { ((Sim*)h)->tolerance = tol; }
+
 }  // extern "C"

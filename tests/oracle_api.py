@@ -90,6 +90,7 @@ def oracle():
         o.orc_collide_direct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_long), D]
         o.orc_last_iters.argtypes = [ctypes.c_void_p]
+        o.orc_set_tolerance.argtypes = [ctypes.c_void_p, ctypes.c_double]
     return _o
 
 
@@ -125,6 +126,9 @@ class OracleSim:
             self.o.orc_destroy(self.h)
         except Exception:
             pass
+
+    def set_tolerance(self, tol):
+        self.o.orc_set_tolerance(self.h, tol)
 
     def init_species(self, s, gauss_seed=0):
         self.o.orc_init_species(self.h, s, gauss_seed)
