@@ -116,18 +116,15 @@ __global__ __launch_bounds__(AUX_BLOCK) void live_count_kernel(const P* x, const
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
 }
 
-// Exclusive scan of m ints in place (one block of 1024); writes the total to
-// out[m].
-__global__ __launch_bounds__(1024) void scan_kernel(int* a, int64_t m) {
-  __shared__ long long sh[1024 / 64];
-  const int T = blockDim.x;
-  const int64_t per = (m + T - 1) / T;
-  const int64_t lo = threadIdx.x * per, hi = std::min<int64_t>(lo + per, m);
-  long long s = 0;
-  for (int64_t i = lo; i < hi; i++) s += a[i];
-  // block exclusive scan of s (64-bit)
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = T >> 6;
-  long long inc = s;
+// Exclusive scan of m ints in place; a[m] receives the total.  Three
+// kernels: per-tile sums, a scan of the tile sums (one workgroup), and the
+// per-tile scan with its offset.  Tiles of 1024 elements.
+constexpr int SCAN_TILE = 1024;
+
+__device__ __forceinline__ long long block_scan_ll(long long v, long long* sh, long long* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  long long inc = v;
+#pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const long long t = __shfl_up(inc, off, 64);
     if (lane >= off) inc += t;
@@ -139,13 +136,51 @@ __global__ __launch_bounds__(1024) void scan_kernel(int* a, int64_t m) {
     if (w < wid) base += sh[w];
     tot += sh[w];
   }
-  long long run = base + inc - s;
-  for (int64_t i = lo; i < hi; i++) {
-    const int v = a[i];
-    a[i] = (int)run;
-    run += v;
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void scan_tile_sums_kernel(const int* a, int64_t m, long long* tiles) {
+  __shared__ long long sh[SCAN_TILE / 64];
+  const int64_t k = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
+  long long tot;
+  block_scan_ll(k < m ? a[k] : 0, sh, &tot);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void scan_tiles_kernel(long long* tiles, int nt) {
+  __shared__ long long sh[SCAN_TILE / 64];
+  long long run = 0;
+  for (int b0 = 0; b0 < nt; b0 += SCAN_TILE) {
+    const int k = b0 + threadIdx.x;
+    const long long v = k < nt ? tiles[k] : 0;
+    long long tot;
+    const long long ex = block_scan_ll(v, sh, &tot);
+    if (k < nt) tiles[k] = run + ex;
+    run += tot;
   }
-  if (threadIdx.x == 0) a[m] = (int)tot;
+  if (threadIdx.x == 0) tiles[nt] = run;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void scan_apply_kernel(int* a, int64_t m, const long long* tiles, int nt) {
+  __shared__ long long sh[SCAN_TILE / 64];
+  const int64_t k = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
+  const int v = k < m ? a[k] : 0;
+  long long tot;
+  const long long ex = block_scan_ll(v, sh, &tot);
+  if (k < m) a[k] = (int)(tiles[blockIdx.x] + ex);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a[m] = (int)tiles[nt];
+}
+
+static hipError_t scan_ints(Ctx* c, int* a, int64_t m) {
+  const int nt = (int)((m + SCAN_TILE - 1) / SCAN_TILE);
+  long long* tiles = c->scan_tiles;
+  if (nt + 1 > c->scan_tiles_cap) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(nt), dim3(SCAN_TILE), 0, c->stream, a, m, tiles);
+  hipLaunchKernelGGL(scan_tiles_kernel, dim3(1), dim3(SCAN_TILE), 0, c->stream, tiles, nt);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3(nt), dim3(SCAN_TILE), 0, c->stream, a, m, tiles, nt);
+  return hipGetLastError();
 }
 
 template <typename P>
@@ -188,8 +223,7 @@ static hipError_t compact_t(Ctx* c, int s) {
   if (nb + 1 > c->blk_cap) return hipErrorInvalidValue;
   hipLaunchKernelGGL(live_count_kernel<P>, dim3(nb), dim3(AUX_BLOCK), 0, c->stream, (const P*)S.x, S.d_n, c->blk_count);
   EPIC_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->blk_count, nb);
-  EPIC_HIP_TRY(hipGetLastError());
+  EPIC_HIP_TRY(scan_ints(c, c->blk_count, nb));
   hipLaunchKernelGGL(compact_scatter_kernel<P>, dim3(nb), dim3(AUX_BLOCK), 0, c->stream, (const P*)S.x,
                      (const P*)S.y, (const P*)S.vx, (const P*)S.vy, (const P*)S.vz, S.id, (P*)S.x2, (P*)S.y2,
                      (P*)S.vx2, (P*)S.vy2, (P*)S.vz2, S.id2, S.d_n, c->blk_count);
@@ -219,24 +253,57 @@ __device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, int nc
   return i * ncy + j;
 }
 
+// Runs of equal keys among consecutive lanes (particles are nearly sorted
+// between sorts): the run's first lane does one atomic for the whole run.
+struct LaneRun {
+  int head;  // lane that starts this lane's run
+  int len;   // run length (valid in the head lane)
+};
+__device__ __forceinline__ LaneRun lane_run(int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool h = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(h);
+  const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+  LaneRun r;
+  r.head = 63 - __builtin_clzll(upto);
+  const unsigned long long after = lane == 63 ? 0ull : (heads >> (lane + 1));
+  r.len = (after ? __builtin_ctzll(after) + 1 : 64 - lane);
+  return r;
+}
+
 template <typename P>
 __global__ void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0, P y0, P dx, P dy,
                                  int ncx, int ncy) {
   const int64_t n = *d_n;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&count[cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy)], 1);
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += stride) {
+    const int64_t p = base + threadIdx.x;
+    const int key = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
+    const LaneRun r = lane_run(key);
+    if (r.head == lane && key >= 0) atomicAdd(&count[key], r.len);
+  }
 }
 
+// Slots: a run's first lane reserves len consecutive slots with one atomic,
+// so particles keep their order within a wavefront; the order between
+// wavefronts is fixed afterwards by sort_fix_kernel.
 template <typename P>
 __global__ void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, const int* start, int* fill,
                                  uint32_t* perm, P x0, P y0, P dx, P dy, int ncx, int ncy) {
   const int64_t n = *d_n;
   const int dead = ncx * ncy;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-    const int k = cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy);
-    if (k == dead) continue;
-    const int slot = start[k] + atomicAdd(&fill[k], 1);
-    perm[slot] = (uint32_t)p;
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += stride) {
+    const int64_t p = base + threadIdx.x;
+    const int key = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
+    const LaneRun r = lane_run(key);
+    int b = 0;
+    if (r.head == lane && key >= 0 && key != dead) b = atomicAdd(&fill[key], r.len);
+    b = __shfl(b, r.head, 64);
+    if (key >= 0 && key != dead) perm[start[key] + b + (lane - r.head)] = (uint32_t)p;
   }
 }
 
@@ -253,6 +320,17 @@ __global__ __launch_bounds__(256) void sort_fix_kernel(const int* start, uint32_
   for (int cell = blockIdx.x * 4 + w; cell < ncell; cell += gridDim.x * 4) {
     const int b = start[cell], e = start[cell + 1], m = e - b;
     if (m <= 1) continue;
+    // already in order?  (common: the cell's particles came from one wavefront)
+    bool ordered = true;
+    for (int k0 = 0; k0 < m - 1; k0 += 64) {
+      const int k = k0 + lane;
+      const bool ok = k >= m - 1 || perm[b + k] < perm[b + k + 1];
+      if (__ballot(!ok)) {
+        ordered = false;
+        break;
+      }
+    }
+    if (ordered) continue;
     if (m <= FIX_LDS) {
       for (int k = lane; k < m; k += 64) buf[w][k] = perm[b + k];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -267,10 +345,18 @@ __global__ __launch_bounds__(256) void sort_fix_kernel(const int* start, uint32_
         mine[q] = (q < cnt && k < m) ? buf[w][k] : 0xFFFFFFFFu;
         rank[q] = 0;
       }
-      for (int t = 0; t < m; t++) {
-        const uint32_t o = buf[w][t];
+      if (cnt <= 2) {
+        for (int t = 0; t < m; t++) {
+          const uint32_t o = buf[w][t];
+          rank[0] += o < mine[0] ? 1 : 0;
+          rank[1] += o < mine[1] ? 1 : 0;
+        }
+      } else {
+        for (int t = 0; t < m; t++) {
+          const uint32_t o = buf[w][t];
 #pragma unroll
-        for (int q = 0; q < FIX_Q; q++) rank[q] += (q < cnt && o < mine[q]) ? 1 : 0;
+          for (int q = 0; q < FIX_Q; q++) rank[q] += (q < cnt && o < mine[q]) ? 1 : 0;
+        }
       }
 #pragma unroll
       for (int q = 0; q < FIX_Q; q++) {
@@ -328,8 +414,7 @@ static hipError_t sort_t(Ctx* c, int s) {
                      c->sort_count, x0, y0, dx, dy, ncx, ncy);
   EPIC_HIP_TRY(hipGetLastError());
   // start[k] = exclusive scan; start[ncell] = live total; start[ncell+1] = all
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->sort_count, (int64_t)ncell + 1);
-  EPIC_HIP_TRY(hipGetLastError());
+  EPIC_HIP_TRY(scan_ints(c, c->sort_count, (int64_t)ncell + 1));
   hipLaunchKernelGGL(sort_fill_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
                      c->sort_count, c->sort_fill, c->sort_perm, x0, y0, dx, dy, ncx, ncy);
   EPIC_HIP_TRY(hipGetLastError());

@@ -93,7 +93,7 @@ static void free_all(Ctx* c) {
   }
   void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
-                c->red_part, c->red_out, c->errc, c->mcc.sigma, c->mcc.counts, c->mcc.events,
+                c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
                 c->mcc.event_count, c->sp_S, c->sp_T, c->sp_lam};
   for (void* p : gs)
     if (p) hipFree(p);
@@ -200,6 +200,8 @@ int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int ns
   CKC(dalloc(&c->sort_perm, 2 * (size_t)maxcap), "sort");
   c->blk_cap = maxcap / 2048 + 4;
   CKC(dalloc(&c->blk_count, (size_t)c->blk_cap), "compact");
+  c->scan_tiles_cap = (int)(std::max<int64_t>((int64_t)ncell + 2, c->blk_cap) / 1024 + 4);
+  CKC(dalloc(&c->scan_tiles, (size_t)c->scan_tiles_cap), "scan");
   c->red_blocks = c->num_cu * 4;
   CKC(dalloc(&c->red_part, (size_t)8 * c->red_blocks), "reduce");
   CKC(dalloc(&c->red_out, 8), "reduce");

@@ -108,6 +108,8 @@ struct Ctx {
   int64_t sort_cap = 0;
   int* blk_count = nullptr;    // compaction per-block counts
   int64_t blk_cap = 0;
+  long long* scan_tiles = nullptr;  // multi-workgroup scan tile sums
+  int scan_tiles_cap = 0;
 
   // diagnostics
   double* red_part = nullptr;  // per-block partials

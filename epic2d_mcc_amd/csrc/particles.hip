@@ -151,13 +151,47 @@ __device__ __forceinline__ bool gather(const PartArgs<P>& a, P lx, P ly, int& ck
   return true;
 }
 
+// Cell of a position for the deposit: CIC base node (floor) or NGP node.
+// Returns false where the reference's bounds-checked scatter would throw.
+template <typename P>
+__device__ __forceinline__ bool dep_cell(const PartArgs<P>& a, P x, P y, int& i, int& j, P& di, P& dj) {
+  const P lx = (x - a.x0) / a.dx;
+  const P ly = (y - a.y0) / a.dy;
+  if (a.shape == EPIC_SHAPE_NGP) {
+    i = (int)round(lx);
+    j = (int)round(ly);
+    di = dj = 0;
+    return i >= 0 && i < a.nx && j >= 0 && j < a.ny;
+  }
+  i = (int)lx;
+  j = (int)ly;
+  di = lx - (P)i;
+  dj = ly - (P)j;
+  return i >= 0 && i + 1 < a.nx && j >= 0 && j + 1 < a.ny;
+}
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// WHOLE: the whole node grid is one LDS histogram per workgroup, flushed once
+// at the end (small grids).  Otherwise each wavefront owns an LDS window of
+// WAVE_WIN node accumulators placed on the bounding box of the cells its
+// 64*PPT particles deposit to; deposits outside go to global atomics.  A
+// wavefront never waits on the rest of its workgroup.
+constexpr int WAVE_WIN = 512;
+
 template <typename P, int PPT, int MODE, bool WHOLE, bool DEP>
 __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
-  __shared__ unsigned long long lds[DEP_WINDOW];
-  __shared__ int red[4 * (PART_BLOCK / 64)];
-  __shared__ int anchor[2];
+  constexpr int LDS_N = DEP ? (WHOLE ? DEP_WINDOW : (PART_BLOCK / 64) * WAVE_WIN) : 1;
+  __shared__ unsigned long long lds[LDS_N];
 
-  constexpr int CH = PART_BLOCK * PPT;
+  constexpr int CHW = 64 * PPT;          // particles per wavefront chunk
+  constexpr int CH = PART_BLOCK * PPT;   // per workgroup chunk
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long* wl = WHOLE ? lds : lds + wid * WAVE_WIN;
   const int64_t n = a.n_fixed >= 0 ? a.n_fixed : *a.d_n;
   const int64_t nchunks = (n + CH - 1) / CH;
   const int ny = a.ny;
@@ -174,7 +208,7 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
   int64_t cb, ce_, cs;
   xcd_chunk_range(nchunks, &cb, &ce_, &cs);
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
-    const int64_t i0 = chunk * CH + (int64_t)threadIdx.x * PPT;
+    const int64_t i0 = chunk * CH + (int64_t)wid * CHW + (int64_t)lane * PPT;
     P x[PPT], y[PPT], vx[PPT], vy[PPT];
     ld_run<P, PPT>(a.x, i0, n, x);
     ld_run<P, PPT>(a.y, i0, n, y);
@@ -237,66 +271,32 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
     if (!DEP) continue;
 
     // ---- deposit the (new) positions ------------------------------------
-    int bi[PPT], bj[PPT];
-    P w0[PPT], w1[PPT];  // di, dj (CIC) -- weights built below
-    int imin = 1 << 30, imax = -1, jmin = 1 << 30, jmax = -1;
     int bad = 0;
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      bi[k] = -1;
-      if (!live[k]) continue;
-      const P lx = (x[k] - a.x0) / a.dx;
-      const P ly = (y[k] - a.y0) / a.dy;
-      int i, j;
-      if (a.shape == EPIC_SHAPE_NGP) {
-        i = (int)round(lx);
-        j = (int)round(ly);
-        if (i < 0 || i >= a.nx || j < 0 || j >= a.ny) { bad++; continue; }
-        w0[k] = 0; w1[k] = 0;
-      } else {
-        i = (int)lx;
-        j = (int)ly;
-        if (i < 0 || i + 1 >= a.nx || j < 0 || j + 1 >= a.ny) { bad++; continue; }
-        w0[k] = lx - (P)i;
-        w1[k] = ly - (P)j;
-      }
-      bi[k] = i;
-      bj[k] = j;
-      imin = min(imin, i); imax = max(imax, i);
-      jmin = min(jmin, j); jmax = max(jmax, j);
-    }
-    if (bad) atomicAdd(&a.errc[EPIC_ERRC_DEPOSIT_OOB], (unsigned long long)bad);
-
     if (!WHOLE) {
-      // block bounding box of the base nodes
-      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-      const int r0 = wave_min(imin), r1 = wave_max(imax), r2 = wave_min(jmin), r3 = wave_max(jmax);
-      if (lane == 0) { red[wid] = r0; red[nw + wid] = r1; red[2 * nw + wid] = r2; red[3 * nw + wid] = r3; }
-      if (threadIdx.x == blockDim.x / 2) {
-        anchor[0] = bi[0] >= 0 ? bi[0] : imin;
-        anchor[1] = bi[0] >= 0 ? bj[0] : jmin;
+      int imin = 1 << 30, imax = -1, jmin = 1 << 30, jmax = -1;
+#pragma unroll
+      for (int k = 0; k < PPT; k++) {
+        if (!live[k]) continue;
+        int i, j;
+        P di, dj;
+        if (!dep_cell(a, x[k], y[k], i, j, di, dj)) continue;
+        imin = min(imin, i); imax = max(imax, i);
+        jmin = min(jmin, j); jmax = max(jmax, j);
       }
-      __syncthreads();
-      int gi0 = 1 << 30, gi1 = -1, gj0 = 1 << 30, gj1 = -1;
-      for (int w = 0; w < nw; w++) {
-        gi0 = min(gi0, red[w]); gi1 = max(gi1, red[nw + w]);
-        gj0 = min(gj0, red[2 * nw + w]); gj1 = max(gj1, red[3 * nw + w]);
-      }
-      if (gi1 < 0) {  // nothing live in this chunk
-        __syncthreads();
-        continue;
-      }
+      const int gi0 = wave_min(imin), gi1 = wave_max(imax);
+      if (gi1 < 0) continue;  // nothing to deposit in this wavefront chunk (uniform)
+      const int gj0 = wave_min(jmin), gj1 = wave_max(jmax);
       const int wi = gi1 - gi0 + 2, wj = gj1 - gj0 + 2;
-      if ((long long)wi * wj <= DEP_WINDOW) {
+      if (wi * wj <= WAVE_WIN) {
         win = Window{gi0, gj0, wi, wj};
       } else {
-        int ai = anchor[0], aj = anchor[1];
-        if (ai > a.nx) ai = gi0;
-        if (aj > a.ny) aj = gj0;
-        win = Window{max(0, ai - 44), max(0, aj - 44), 90, 90};
+        // too wide (drifted outliers): a window on the wavefront's first
+        // cells, the rest goes straight to global atomics
+        const int wjc = min(wj, 32);
+        win = Window{gi0, gj0, WAVE_WIN / wjc, wjc};
       }
-      for (int k = threadIdx.x; k < win.wi * win.wj; k += blockDim.x) lds[k] = 0ull;
-      __syncthreads();
+      for (int k = lane; k < win.wi * win.wj; k += 64) wl[k] = 0ull;
+      wave_fence();
     }
 
     // run-length accumulation over this thread's consecutive particles
@@ -304,17 +304,22 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
     unsigned long long acc[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-      if (bi[k] < 0) continue;
-      if (bi[k] != rk_i || bj[k] != rk_j) {
-        if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, lds, a.Q);
-        rk_i = bi[k];
-        rk_j = bj[k];
+      if (!live[k]) continue;
+      int i, j;
+      P di, dj;
+      if (!dep_cell(a, x[k], y[k], i, j, di, dj)) {
+        bad++;
+        continue;
+      }
+      if (i != rk_i || j != rk_j) {
+        if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, wl, a.Q);
+        rk_i = i;
+        rk_j = j;
         acc[0] = acc[1] = acc[2] = acc[3] = 0;
       }
       if (ncorner == 1) {
         acc[0] += (unsigned long long)one;
       } else {
-        const P di = w0[k], dj = w1[k];
         const P omdi = (P)1 - di, omdj = (P)1 - dj;
         const long long q00 = fix_round(omdi * omdj, scale);
         const long long q10 = fix_round(di * omdj, scale);
@@ -326,19 +331,20 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
         acc[3] += (unsigned long long)q11;
       }
     }
-    if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, lds, a.Q);
+    if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, wl, a.Q);
+    if (bad) atomicAdd(&a.errc[EPIC_ERRC_DEPOSIT_OOB], (unsigned long long)bad);
 
     if (!WHOLE) {
-      __syncthreads();
+      wave_fence();
       const int W = win.wi * win.wj;
-      for (int k = threadIdx.x; k < W; k += blockDim.x) {
-        const unsigned long long v = lds[k];
+      for (int k = lane; k < W; k += 64) {
+        const unsigned long long v = wl[k];
         if (v) {
           const int gi = win.i0 + k / win.wj, gj = win.j0 + k % win.wj;
           if (gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
         }
       }
-      __syncthreads();
+      wave_fence();
     }
   }
 
@@ -391,29 +397,42 @@ static void push_consts(const Ctx* c, int s, int sub_cycle, bool rewind, double*
   *c3 = (d.vel_norm) / (d.L * d.W);
 }
 
-static int particle_grid(Ctx* c) { return c->num_cu * 4; }
+// Persistent grid: as many workgroups as can be resident (a multiple of the
+// 8 XCDs, so the XCD-contiguous chunk walk applies).
+template <typename K>
+static int particle_grid(Ctx* c, K kernel) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, PART_BLOCK, 0) != hipSuccess || per_cu < 1)
+    per_cu = 2;
+  return c->num_cu * per_cu;
+}
 
 template <typename P, int PPT>
 static hipError_t launch_particles_t(Ctx* c, int s, int mode, bool dep, double c1, double c2, double c3) {
   PartArgs<P> a = make_args<P>(c, s);
   a.c1 = (P)c1; a.c2 = (P)c2; a.c3 = (P)c3;
   const bool whole = (size_t)c->dom.nx * c->dom.ny <= (size_t)DEP_WINDOW;
-  const int grid = particle_grid(c);
-  const dim3 g(grid), b(PART_BLOCK);
+  const dim3 b(PART_BLOCK);
   hipStream_t st = c->stream;
+#define EPIC_LAUNCH(M, W, D)                                                    \
+  do {                                                                         \
+    auto k = particle_kernel<P, PPT, M, W, D>;                                 \
+    hipLaunchKernelGGL(k, dim3(particle_grid(c, k)), b, 0, st, a);             \
+  } while (0)
   if (mode == MODE_PUSH) {
     if (dep) {
-      if (whole) hipLaunchKernelGGL((particle_kernel<P, PPT, MODE_PUSH, true, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((particle_kernel<P, PPT, MODE_PUSH, false, true>), g, b, 0, st, a);
+      if (whole) EPIC_LAUNCH(MODE_PUSH, true, true);
+      else EPIC_LAUNCH(MODE_PUSH, false, true);
     } else {
-      hipLaunchKernelGGL((particle_kernel<P, PPT, MODE_PUSH, false, false>), g, b, 0, st, a);
+      EPIC_LAUNCH(MODE_PUSH, false, false);
     }
   } else if (mode == MODE_REWIND) {
-    hipLaunchKernelGGL((particle_kernel<P, PPT, MODE_REWIND, false, false>), g, b, 0, st, a);
+    EPIC_LAUNCH(MODE_REWIND, false, false);
   } else {
-    if (whole) hipLaunchKernelGGL((particle_kernel<P, PPT, MODE_DEPOSIT, true, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((particle_kernel<P, PPT, MODE_DEPOSIT, false, true>), g, b, 0, st, a);
+    if (whole) EPIC_LAUNCH(MODE_DEPOSIT, true, true);
+    else EPIC_LAUNCH(MODE_DEPOSIT, false, true);
   }
+#undef EPIC_LAUNCH
   return hipGetLastError();
 }
 

@@ -1,0 +1,132 @@
+"""Species storage on the GPU: counting sort by cell (bit-exact against a CPU
+stable sort by key), order-preserving compaction, deposit invariance under
+reordering, and the Philox loader (recomputed on the CPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from epic2d_mcc_amd import _lib
+from harness import rel, reference_load, small_setup
+from oracle_api import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cell_keys(x, y, d):
+    lx = (x - d.x0) / d.dx
+    ly = (y - d.y0) / d.dy
+    ncx, ncy = d.nx - 1, d.ny - 1
+    i = np.floor(lx).astype(np.int64)
+    j = np.floor(ly).astype(np.int64)
+    return i * ncy + j
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32])
+def test_sort_equals_cpu_stable_sort(precision):
+    print(f"[storage] sort precision={precision}", flush=True)
+    st = small_setup(N=50000, Nx=32, Ny=24, precision=precision, bc="open")
+    d = st.domain
+    rng = np.random.default_rng(11)
+    n = 50000
+    # nearly sorted input (the steady state) mixed with random ones
+    x = np.sort(rng.random(n)) * 32
+    y = rng.random(n) * 24
+    x[rng.integers(0, n, 2000)] = rng.random(2000) * 32
+    if precision == _lib.FP32:
+        x = x.astype(np.float32).astype(np.float64)
+        y = y.astype(np.float32).astype(np.float64)
+    vx = rng.standard_normal(n)
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, x, y, vx, -vx, 2 * vx)
+    ctx.sort(0)
+    gx, gy, gvx, gvy, gvz, gid = ctx.download(0)
+    key = _cell_keys(x, y, d)
+    order = np.argsort(key, kind="stable")
+    assert np.array_equal(gid, order.astype(np.uint32))
+    assert np.array_equal(gx, x[order]) and np.array_equal(gvx, vx[order] if precision == _lib.FP64
+                                                            else vx[order].astype(np.float32))
+    # a second sort of sorted data changes nothing
+    ctx.sort(0)
+    g2 = ctx.download(0)
+    assert np.array_equal(g2[5], gid)
+
+
+def test_sort_drops_absorbed_and_keeps_deposit():
+    st = small_setup(N=20000, Nx=32, Ny=32, bc="open", vsx_beam=8.0)
+    o, parts = reference_load(st)
+    ctx = _lib.Context(st.domain, st.species)
+    for s, p in enumerate(parts):
+        ctx.upload(s, *p)
+    ctx.initial_field()
+    for ts in range(5):
+        ctx.step(ts)
+    n_live = [ctx.count(s) for s in range(2)]
+    before = [ctx.download(s) for s in range(2)]
+    ctx.scatter(0)
+    ctx.scatter(1)
+    ctx.compute_rho()
+    rho0 = ctx.get_field(0).copy()
+    for s in range(2):
+        ctx.sort(s)
+    after = [ctx.download(s) for s in range(2)]
+    for s in range(2):
+        assert len(after[s][0]) == n_live[s]
+        # same particle set (by id)
+        a = np.argsort(before[s][5])
+        b = np.argsort(after[s][5])
+        assert np.array_equal(before[s][5][a], after[s][5][b])
+        assert np.array_equal(before[s][0][a], after[s][0][b])
+    ctx.scatter(0)
+    ctx.scatter(1)
+    ctx.compute_rho()
+    # integer deposit: identical after reordering
+    assert np.array_equal(ctx.get_field(0), rho0)
+
+
+def test_compaction_keeps_order():
+    st = small_setup(N=30000, Nx=16, Ny=16, bc="open", vsx_beam=20.0)
+    o, parts = reference_load(st)
+    ctx = _lib.Context(st.domain, st.species)
+    for s, p in enumerate(parts):
+        ctx.upload(s, *p)
+    ctx.initial_field()
+    o.initial_field()
+    for ts in range(3):
+        ctx.step(ts)
+        o.step()
+    for s in range(2):
+        ctx.compact(s)
+        g = ctx.download(s)
+        c = o.particles(s)
+        assert len(g[0]) == len(c[0])
+        assert rel(g[0], c[0]) < 1e-12
+        assert np.all(np.diff(g[5].astype(np.int64)) > 0)  # ids stay in load order
+
+
+def _philox(ctr, key):
+    C = (ctypes.c_uint32 * 4)(*ctr)
+    K = (ctypes.c_uint32 * 2)(*key)
+    O = (ctypes.c_uint32 * 4)()
+    oracle().orc_philox4x32_10(C, K, O)
+    return list(O)
+
+
+def test_loader_positions_are_philox():
+    """The native loader's positions are x0 + Lx*u53(philox(p, 0, 1, s)),
+    recomputed here with the CPU Philox pinned by the Random123 vectors."""
+    st = small_setup(N=5000, Nx=16, Ny=16)
+    d = st.domain
+    d.seed = 0x1234567890ABCDEF
+    ctx = _lib.Context(d, st.species)
+    ctx.load(0, 5000, vz_thermal=True)
+    x, y, vx, vy, vz, ids = ctx.download(0)
+    k0, k1 = d.seed & 0xFFFFFFFF, d.seed >> 32
+    for p in (0, 1, 17, 4999):
+        r = _philox([p, 0, 1, 0], [k0, k1])
+        u = ((r[0] << 21) ^ (r[1] >> 11)) * (1.0 / 9007199254740992.0)
+        v = ((r[2] << 21) ^ (r[3] >> 11)) * (1.0 / 9007199254740992.0)
+        assert x[p] == d.x0 + 16.0 * u and y[p] == d.y0 + 16.0 * v
+    # Maxwellian spread sqrt(kT/m)/vel_norm per component (= 1 for electrons at Te)
+    for comp in (vx, vy, vz):
+        assert abs(np.std(comp) - 1.0) < 0.05 and abs(np.mean(comp)) < 0.05
