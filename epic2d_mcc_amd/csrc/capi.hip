@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -151,6 +152,7 @@ int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int ns
   hipDeviceProp_t prop;
   CKC(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
   c->num_cu = prop.multiProcessorCount;
+  if (const char* v = getenv("EPIC_PUSH_VARIANT")) c->push_variant = atoi(v);
   c->max_solver_blocks = c->num_cu;
   CKC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   CKC(hipEventCreate(&c->ev0), "hipEventCreate");

@@ -21,10 +21,11 @@ constexpr double eV = 1.602176565e-19;
 
 // Fixed-point deposit scale (bits after the binary point of one particle's
 // unit weight).  FP64 storage: 2^44 (per-contribution rounding < 3e-14 of a
-// particle, 2^19 particles per node before overflow).  FP32 storage: 2^24,
-// the FP32 mantissa.
+// particle, 2^19 particles per node before overflow).  FP32 storage: 2^28,
+// products of 14-bit cell offsets (2^-14 cell resolution, 2^35 particles per
+// node before overflow).
 constexpr int FIX_BITS_FP64 = 44;
-constexpr int FIX_BITS_FP32 = 24;
+constexpr int FIX_BITS_FP32 = 28;
 
 // LDS deposit window: 8192 int64 node accumulators (64 KiB) per workgroup.
 constexpr int DEP_WINDOW = 8192;
@@ -121,6 +122,7 @@ struct Ctx {
   std::vector<Emitter> emitters;
   MccState mcc;
 
+  int push_variant = 0;        // FP32 push kernel shape (see particles.hip)
   bool deposit_valid = false;  // Q holds the deposit of the current positions
   int step_count = 0;
 

@@ -38,6 +38,7 @@ struct PartArgs {
   int64_t n_fixed;     // >= 0: process [0, n_fixed) instead of [0, *d_n)
   const Vec2<P>* E;    // (ex, ey) per node, i*ny + j
   P x0, y0, dx, dy, xhi, yhi, Lx, Ly;
+  P idx, idy;          // 1/dx, 1/dy (FP32 deposit coordinates)
   int nx, ny;
   int bc, shape;
   P c1, c2, c3;        // dv = (c1*E)*c2 ; dpos = (c3*v)*c2
@@ -46,34 +47,54 @@ struct PartArgs {
   unsigned long long* errc;
 };
 
+// Particle k of a lane within its wavefront chunk of 64*PPT particles:
+// V = 16/sizeof(P) consecutive particles per 16-byte vector, group g = k/V
+// at offset g*64*V.  Load g of a wavefront is then one contiguous 1 KiB
+// block (fully coalesced), and each lane owns runs of V consecutive
+// particles (what the gather cell reuse and the run-length deposit rely on).
 template <typename P, int PPT>
-__device__ __forceinline__ void ld_run(const P* __restrict__ a, int64_t i0, int64_t n, P (&o)[PPT]) {
-  constexpr int NV = PPT * (int)sizeof(P) / 16;
-  if (i0 + PPT <= n) {
-    const v4u32* p = reinterpret_cast<const v4u32*>(a + i0);
-    v4u32 t[NV];
+__device__ __forceinline__ int64_t part_index(int64_t wbase, int lane, int k) {
+  constexpr int V = 16 / (int)sizeof(P);
+  return wbase + (int64_t)(k / V) * (64 * V) + (int64_t)lane * V + (k % V);
+}
+
+template <typename P, int PPT>
+__device__ __forceinline__ void ld_run(const P* __restrict__ a, int64_t wbase, int lane, int64_t n, P (&o)[PPT]) {
+  constexpr int V = 16 / (int)sizeof(P);
+  constexpr int NG = PPT / V;
+  if (wbase + 64 * PPT <= n) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) t[k] = __builtin_nontemporal_load(p + k);
-    __builtin_memcpy(&o[0], &t[0], sizeof(t));
+    for (int g = 0; g < NG; g++) {
+      const v4u32* p = reinterpret_cast<const v4u32*>(a + wbase + (int64_t)g * 64 * V + (int64_t)lane * V);
+      const v4u32 t = __builtin_nontemporal_load(p);
+      __builtin_memcpy(&o[g * V], &t, 16);
+    }
   } else {
 #pragma unroll
-    for (int k = 0; k < PPT; k++) o[k] = (i0 + k < n) ? a[i0 + k] : P(0);
+    for (int k = 0; k < PPT; k++) {
+      const int64_t i = part_index<P, PPT>(wbase, lane, k);
+      o[k] = i < n ? a[i] : P(0);
+    }
   }
 }
 
 template <typename P, int PPT>
-__device__ __forceinline__ void st_run(P* __restrict__ a, int64_t i0, int64_t n, const P (&o)[PPT]) {
-  constexpr int NV = PPT * (int)sizeof(P) / 16;
-  if (i0 + PPT <= n) {
-    v4u32* p = reinterpret_cast<v4u32*>(a + i0);
-    v4u32 t[NV];
-    __builtin_memcpy(&t[0], &o[0], sizeof(t));
+__device__ __forceinline__ void st_run(P* __restrict__ a, int64_t wbase, int lane, int64_t n, const P (&o)[PPT]) {
+  constexpr int V = 16 / (int)sizeof(P);
+  constexpr int NG = PPT / V;
+  if (wbase + 64 * PPT <= n) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) __builtin_nontemporal_store(t[k], p + k);
+    for (int g = 0; g < NG; g++) {
+      v4u32 t;
+      __builtin_memcpy(&t, &o[g * V], 16);
+      __builtin_nontemporal_store(t, reinterpret_cast<v4u32*>(a + wbase + (int64_t)g * 64 * V + (int64_t)lane * V));
+    }
   } else {
 #pragma unroll
-    for (int k = 0; k < PPT; k++)
-      if (i0 + k < n) a[i0 + k] = o[k];
+    for (int k = 0; k < PPT; k++) {
+      const int64_t i = part_index<P, PPT>(wbase, lane, k);
+      if (i < n) a[i] = o[k];
+    }
   }
 }
 
@@ -155,8 +176,9 @@ __device__ __forceinline__ bool gather(const PartArgs<P>& a, P lx, P ly, int& ck
 // Returns false where the reference's bounds-checked scatter would throw.
 template <typename P>
 __device__ __forceinline__ bool dep_cell(const PartArgs<P>& a, P x, P y, int& i, int& j, P& di, P& dj) {
-  const P lx = (x - a.x0) / a.dx;
-  const P ly = (y - a.y0) / a.dy;
+  // FP64: the reference's XtoL division; FP32: the hot kernel's reciprocal
+  const P lx = sizeof(P) == 8 ? (x - a.x0) / a.dx : (x - a.x0) * a.idx;
+  const P ly = sizeof(P) == 8 ? (y - a.y0) / a.dy : (y - a.y0) * a.idy;
   if (a.shape == EPIC_SHAPE_NGP) {
     i = (int)round(lx);
     j = (int)round(ly);
@@ -208,17 +230,17 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
   int64_t cb, ce_, cs;
   xcd_chunk_range(nchunks, &cb, &ce_, &cs);
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
-    const int64_t i0 = chunk * CH + (int64_t)wid * CHW + (int64_t)lane * PPT;
+    const int64_t wb = chunk * CH + (int64_t)wid * CHW;
     P x[PPT], y[PPT], vx[PPT], vy[PPT];
-    ld_run<P, PPT>(a.x, i0, n, x);
-    ld_run<P, PPT>(a.y, i0, n, y);
+    ld_run<P, PPT>(a.x, wb, lane, n, x);
+    ld_run<P, PPT>(a.y, wb, lane, n, y);
     if (MODE != MODE_DEPOSIT) {
-      ld_run<P, PPT>(a.vx, i0, n, vx);
-      ld_run<P, PPT>(a.vy, i0, n, vy);
+      ld_run<P, PPT>(a.vx, wb, lane, n, vx);
+      ld_run<P, PPT>(a.vy, wb, lane, n, vy);
     }
     bool live[PPT];
 #pragma unroll
-    for (int k = 0; k < PPT; k++) live[k] = (i0 + k < n) && !isnan(x[k]);
+    for (int k = 0; k < PPT; k++) live[k] = (part_index<P, PPT>(wb, lane, k) < n) && !isnan(x[k]);
 
     if (MODE != MODE_DEPOSIT) {
       int ckey = -1;
@@ -260,11 +282,11 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
         y[k] = py;
       }
       if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
-      st_run<P, PPT>(a.vx, i0, n, vx);
-      st_run<P, PPT>(a.vy, i0, n, vy);
+      st_run<P, PPT>(a.vx, wb, lane, n, vx);
+      st_run<P, PPT>(a.vy, wb, lane, n, vy);
       if (MODE == MODE_PUSH) {
-        st_run<P, PPT>(a.x, i0, n, x);
-        st_run<P, PPT>(a.y, i0, n, y);
+        st_run<P, PPT>(a.x, wb, lane, n, x);
+        st_run<P, PPT>(a.y, wb, lane, n, y);
       }
     }
 
@@ -319,6 +341,15 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
       }
       if (ncorner == 1) {
         acc[0] += (unsigned long long)one;
+      } else if (sizeof(P) == 4) {
+        // FP32 storage: 14-bit offsets, weights sum to 2^28 exactly
+        const unsigned dI = (unsigned)min((int)rintf((float)di * 16384.0f), 16384);
+        const unsigned dJ = (unsigned)min((int)rintf((float)dj * 16384.0f), 16384);
+        const unsigned oI = 16384u - dI, oJ = 16384u - dJ;
+        acc[0] += oI * oJ;
+        acc[1] += dI * oJ;
+        acc[2] += oI * dJ;
+        acc[3] += dI * dJ;
       } else {
         const P omdi = (P)1 - di, omdj = (P)1 - dj;
         const long long q00 = fix_round(omdi * omdj, scale);
@@ -359,6 +390,177 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
 }
 
 // ---------------------------------------------------------------------------
+// FP32 hot path: fused gather + leapfrog push + boundary + deposit of the
+// new position, for the storage precision of the large workloads.  Same
+// structure as particle_kernel, with the arithmetic cut to what an FP32
+// particle needs:
+//   logical coordinates by multiplication with 1/dx (no division),
+//   v += (c1 c2) E and x += (c3 c2) v as fused multiply-adds,
+//   bilinear gather as two lerps per component,
+//   integer CIC weights from 14-bit offsets dI = rint(di 2^14):
+//     w00 = (2^14-dI)(2^14-dJ), w10 = dI(2^14-dJ), w01 = (2^14-dI)dJ, w11 = dI dJ
+//   which sum to exactly 2^28 per particle (FIX_BITS_FP32).
+// The deposit stays an integer sum, so it is deterministic; the 2^-14 cell
+// resolution matches the FP32 position resolution at 512 cells.
+struct PushF32 {
+  float* x;
+  float* y;
+  float* vx;
+  float* vy;
+  const int64_t* d_n;
+  const float2* E;
+  float x0, y0, idx, idy, xhi, yhi, Lx, Ly;
+  float kv, kx;  // c1*c2, c3*c2
+  int nx, ny, bc;
+  unsigned long long* Q;
+  unsigned long long* errc;
+};
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+template <bool DEP, int PPT, int WAVES>
+__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a) {
+  __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
+  constexpr int CHW = 64 * PPT;
+  constexpr int CH = PART_BLOCK * PPT;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long* wl = lds + (DEP ? wid * WAVE_WIN : 0);
+  const int64_t n = *a.d_n;
+  const int64_t nchunks = (n + CH - 1) / CH;
+  const int ny = a.ny;
+  const float fmax_i = (float)(a.nx - 1), fmax_j = (float)(a.ny - 1);
+
+  int64_t cb, ce_, cs;
+  xcd_chunk_range(nchunks, &cb, &ce_, &cs);
+  for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
+    const int64_t wb = chunk * CH + (int64_t)wid * CHW;
+    float x[PPT], y[PPT], vx[PPT], vy[PPT];
+    ld_run<float, PPT>(a.x, wb, lane, n, x);
+    ld_run<float, PPT>(a.y, wb, lane, n, y);
+    ld_run<float, PPT>(a.vx, wb, lane, n, vx);
+    ld_run<float, PPT>(a.vy, wb, lane, n, vy);
+    unsigned live = 0;
+    int imin = 1 << 30, imax = -1, jmin = 1 << 30, jmax = -1;
+    int ckey = -1;
+    float c00x = 0, c10x = 0, c01x = 0, c11x = 0, c00y = 0, c10y = 0, c01y = 0, c11y = 0;
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      if (!(part_index<float, PPT>(wb, lane, k) < n) || isnan(x[k])) continue;
+      const float lx = (x[k] - a.x0) * a.idx;
+      const float ly = (y[k] - a.y0) * a.idy;
+      const int i = (int)lx, j = (int)ly;
+      float ex = 0.f, ey = 0.f;
+      if (i >= 0 && i + 1 < a.nx && j >= 0 && j + 1 < a.ny) {
+        const int key = i * ny + j;
+        if (key != ckey) {
+          const float2 e00 = a.E[key], e01 = a.E[key + 1], e10 = a.E[key + ny], e11 = a.E[key + ny + 1];
+          c00x = e00.x; c01x = e01.x; c10x = e10.x; c11x = e11.x;
+          c00y = e00.y; c01y = e01.y; c10y = e10.y; c11y = e11.y;
+          ckey = key;
+        }
+        const float di = lx - (float)i, dj = ly - (float)j;
+        ex = lerpf(lerpf(c00x, c10x, di), lerpf(c01x, c11x, di), dj);
+        ey = lerpf(lerpf(c00y, c10y, di), lerpf(c01y, c11y, di), dj);
+      } else {
+        bad++;
+      }
+      vx[k] = fmaf(a.kv, ex, vx[k]);
+      vy[k] = fmaf(a.kv, ey, vy[k]);
+      float px = fmaf(a.kx, vx[k], x[k]);
+      float py = fmaf(a.kx, vy[k], y[k]);
+      bool alive = true;
+      if (a.bc == EPIC_BC_PBC) {
+        px = px < a.x0 ? px + a.Lx : (px >= a.xhi ? px - a.Lx : px);
+        py = py < a.y0 ? py + a.Ly : (py >= a.yhi ? py - a.Ly : py);
+      } else if (a.bc == EPIC_BC_OPEN) {
+        alive = !(px < a.x0 || px >= a.xhi || py < a.y0 || py >= a.yhi);
+        if (!alive) px = __builtin_nanf("");
+      } else {
+        if (px < a.x0) { px = a.x0 + (a.x0 - px); vx[k] = -vx[k]; }
+        else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); vx[k] = -vx[k]; }
+        if (py < a.y0) { py = a.y0 + (a.y0 - py); vy[k] = -vy[k]; }
+        else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); vy[k] = -vy[k]; }
+      }
+      x[k] = px;
+      y[k] = py;
+      if (DEP && alive) {
+        const float nlx = (px - a.x0) * a.idx, nly = (py - a.y0) * a.idy;
+        // clamp to the last cell: a wrapped position can round onto the
+        // upper boundary node
+        if (nlx >= 0.f && nly >= 0.f && nlx <= fmax_i && nly <= fmax_j) {
+          live |= 1u << k;
+          const int ni = min((int)nlx, a.nx - 2), nj = min((int)nly, a.ny - 2);
+          imin = min(imin, ni); imax = max(imax, ni);
+          jmin = min(jmin, nj); jmax = max(jmax, nj);
+        } else {
+          bad++;
+        }
+      }
+    }
+    st_run<float, PPT>(a.vx, wb, lane, n, vx);
+    st_run<float, PPT>(a.vy, wb, lane, n, vy);
+    st_run<float, PPT>(a.x, wb, lane, n, x);
+    st_run<float, PPT>(a.y, wb, lane, n, y);
+    if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
+    if (!DEP) continue;
+
+    const int gi0 = wave_min(imin), gi1 = wave_max(imax);
+    if (gi1 < 0) continue;
+    const int gj0 = wave_min(jmin), gj1 = wave_max(jmax);
+    const int wi = gi1 - gi0 + 2, wj = gj1 - gj0 + 2;
+    Window win;
+    if (wi * wj <= WAVE_WIN) {
+      win = Window{gi0, gj0, wi, wj};
+    } else {
+      const int wjc = min(wj, 32);
+      win = Window{gi0, gj0, WAVE_WIN / wjc, wjc};
+    }
+    for (int k = lane; k < win.wi * win.wj; k += 64) wl[k] = 0ull;
+    wave_fence();
+
+    int rk_i = -1, rk_j = -1;
+    unsigned acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      if (!(live & (1u << k))) continue;
+      const float nlx = (x[k] - a.x0) * a.idx, nly = (y[k] - a.y0) * a.idy;
+      const int i = min((int)nlx, a.nx - 2), j = min((int)nly, a.ny - 2);
+      const int dI = min((int)rintf((nlx - (float)i) * 16384.0f), 16384);
+      const int dJ = min((int)rintf((nly - (float)j) * 16384.0f), 16384);
+      if (i != rk_i || j != rk_j) {
+        if (rk_i >= 0) {
+          const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
+          flush_block(rk_i, rk_j, ny, acc, 4, win, wl, a.Q);
+        }
+        rk_i = i;
+        rk_j = j;
+        acc0 = acc1 = acc2 = acc3 = 0;
+      }
+      const unsigned oI = 16384u - (unsigned)dI, oJ = 16384u - (unsigned)dJ;
+      acc0 += oI * oJ;
+      acc1 += (unsigned)dI * oJ;
+      acc2 += oI * (unsigned)dJ;
+      acc3 += (unsigned)dI * (unsigned)dJ;
+    }
+    if (rk_i >= 0) {
+      const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
+      flush_block(rk_i, rk_j, ny, acc, 4, win, wl, a.Q);
+    }
+    wave_fence();
+    const int W = win.wi * win.wj;
+    for (int k = lane; k < W; k += 64) {
+      const unsigned long long v = wl[k];
+      if (v) {
+        const int gi = win.i0 + k / win.wj, gj = win.j0 + k % win.wj;
+        if (gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
+      }
+    }
+    wave_fence();
+  }
+}
+
+// ---------------------------------------------------------------------------
 template <typename P>
 static PartArgs<P> make_args(Ctx* c, int s) {
   SpeciesDev& S = c->sp[s];
@@ -370,6 +572,7 @@ static PartArgs<P> make_args(Ctx* c, int s) {
   a.E = (const Vec2<P>*)c->E2;
   const double Lx = (d.nx - 1) * d.dx, Ly = (d.ny - 1) * d.dy;
   a.x0 = (P)d.x0; a.y0 = (P)d.y0; a.dx = (P)d.dx; a.dy = (P)d.dy;
+  a.idx = (P)(1.0 / d.dx); a.idy = (P)(1.0 / d.dy);
   a.xhi = (P)(d.x0 + Lx); a.yhi = (P)(d.y0 + Ly);
   a.Lx = (P)Lx; a.Ly = (P)Ly;
   a.nx = d.nx; a.ny = d.ny;
@@ -448,6 +651,47 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
     return launch_particles(c, s, MODE_REWIND, false, c1, c2, c3);
   }
   push_consts(c, s, sub_cycle, false, &c1, &c2, &c3);
+  const epic_domain_desc& d = c->dom;
+  const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
+  if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole) {
+    SpeciesDev& S = c->sp[s];
+    const double Lx = (d.nx - 1) * d.dx, Ly = (d.ny - 1) * d.dy;
+    PushF32 a{};
+    a.x = (float*)S.x; a.y = (float*)S.y; a.vx = (float*)S.vx; a.vy = (float*)S.vy;
+    a.d_n = S.d_n;
+    a.E = (const float2*)c->E2;
+    a.x0 = (float)d.x0; a.y0 = (float)d.y0;
+    a.idx = (float)(1.0 / d.dx); a.idy = (float)(1.0 / d.dy);
+    a.xhi = (float)(d.x0 + Lx); a.yhi = (float)(d.y0 + Ly);
+    a.Lx = (float)Lx; a.Ly = (float)Ly;
+    a.kv = (float)(c1 * c2);
+    a.kx = (float)(c3 * c2);
+    a.nx = d.nx; a.ny = d.ny; a.bc = d.bc;
+    a.Q = S.Q;
+    a.errc = c->errc;
+    // kernel shape: particles per thread x occupancy target (waves per SIMD);
+    // set by the tuner (epic_tune) or EPIC_PUSH_VARIANT
+    switch (c->push_variant) {
+#define EPIC_V(ID, PPT, WV)                                                               \
+  case ID:                                                                                \
+    if (deposit) {                                                                        \
+      auto k = push_f32_kernel<true, PPT, WV>;                                            \
+      hipLaunchKernelGGL(k, dim3(particle_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a); \
+    } else {                                                                              \
+      auto k = push_f32_kernel<false, PPT, WV>;                                           \
+      hipLaunchKernelGGL(k, dim3(particle_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a); \
+    }                                                                                     \
+    break;
+      EPIC_V(1, 8, 5)
+      EPIC_V(2, 4, 1)
+      EPIC_V(3, 4, 8)
+      EPIC_V(4, 16, 1)
+      default:
+      EPIC_V(0, 8, 1)
+#undef EPIC_V
+    }
+    return hipGetLastError();
+  }
   return launch_particles(c, s, MODE_PUSH, deposit, c1, c2, c3);
 }
 
