@@ -5,8 +5,9 @@
 // colliding species draws from its own Philox stream keyed by
 // (particle id, step): it is a candidate with probability
 // P_null = 1 - exp(-nu_max dt), and a candidate collides for real with
-// probability nu(E)/nu_max (else a null collision).  Per process the expected
-// rate is the reference's to first order in nu dt.
+// probability (1 - exp(-nu(E) dt)) / P_null (else a null collision), so each
+// particle's collision probability is exactly the reference's; only the
+// candidates (a fraction ~nu_max dt) evaluate cross sections.
 //
 // Electron-neutral kinematics follow CollisionHandler::collision_electron
 // (collision.cpp:60-188): centre-of-mass split F1 = m_e/(m_e+M), F2 = M/(m_e+M),
@@ -58,7 +59,7 @@ struct MccArgs {
   int kind;             // 0 electron, 1 ion
   unsigned enable;      // bit per process
   int ref_selection;
-  double nu_max, p_null;
+  double nu_max, p_null, dt;
   double vel_norm, mass, gas_mass, gas_density, gas_vth;  // gas_vth normalised sqrt(kT/M)
   double e_exc, e_ion;  // eV
   double ion_vth;       // normalised thermal spread of newborn ions
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
       double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, energy) : 0.0;
       const double t0 = s_el, t1 = t0 + s_ex, t2 = t1 + s_io;
       const double nu = a.gas_density * t2 * velocity;
-      if (R1 * a.nu_max >= nu) continue;  // null collision
+      if (R1 * a.p_null >= -expm1(-nu * a.dt)) continue;  // null collision
       int type;
       if (a.ref_selection) {
         // two independent draws (collision.cpp:117,125)
@@ -184,7 +185,11 @@ __global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
       const double s_cx = (a.enable & 16) ? a.tab.at(4, energy) : 0.0;
       const double nu_el = a.gas_density * s_el * g * a.vel_norm;
       const double nu_cx = a.gas_density * s_cx * g * a.vel_norm;
-      const double R = R1 * a.nu_max;
+      const double nu = nu_el + nu_cx;
+      const double pc = -expm1(-nu * a.dt);
+      if (R1 * a.p_null >= pc) continue;  // null collision
+      // real collision: process by frequency share
+      const double R = (R1 * a.p_null / pc) * nu;
       if (R < nu_el) {
         const u32x4 r4 = philox4x32_10({pid, a.step, RNG_MCC_ION_SCATTER, a.sp}, a.k0, a.k1);
         const double M = a.gas_mass, m = a.mass;
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
         a.vy[p] = (P)(cy + f * g * stt * sin(ph));
         a.vz[p] = (P)(cz + f * g * ct);
         cnt[3]++;
-      } else if (R < nu_el + nu_cx) {
+      } else {
         a.vx[p] = (P)vnx;
         a.vy[p] = (P)vny;
         a.vz[p] = (P)vnz;
@@ -315,7 +320,8 @@ static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
   a.ref_selection = M.d.ref_selection;
   const double dt = d.DT / d.W;  // seconds (collision.cpp:201)
   a.nu_max = kind == 0 ? M.nu_max_e : M.nu_max_i;
-  a.p_null = 1.0 - exp(-a.nu_max * dt);
+  a.p_null = -expm1(-a.nu_max * dt);
+  a.dt = dt;
   a.vel_norm = d.vel_norm;
   a.mass = S.desc.mass;
   a.gas_mass = M.d.gas_mass;

@@ -1,0 +1,159 @@
+"""Null-collision MCC on the GPU against the reference's direct method.
+
+The oracle restates CollisionHandler::handle_collisions (collision.cpp:190-
+207) with its 5,000,000-point tables and mt19937 draws; the GPU runs the
+null-collision scheme with Philox streams on log-spaced tables of the same
+analytic cross sections.  Per-process counts must agree within 3 sigma
+(sigma^2 = sum of the two Poisson counts); energy removed by thresholds is
+checked against the electrons' kinetic-energy drop; runs are repeatable."""
+import numpy as np
+import pytest
+
+from epic2d_mcc_amd import _lib, config, xsec
+from oracle_api import OracleSim, oracle
+
+pytestmark = pytest.mark.gpu
+
+ME = config.ME
+GAS = 2e21
+
+
+def _setup(N, Te_load, nsp_ion_N=1):
+    specs = [config.SpeciesSpec("electron", ME, N, 1.0, -1, 1.0),
+             config.SpeciesSpec("ion", xsec.M_AR, nsp_ion_N, 0.026, 1, 0.0)]
+    st = config.derive(16, 16, 0.0, specs, 1e15, 0.05, bc="pbc", capacity_factor=1.5)
+    st.species[1].capacity = int(N * 0.6) + 16
+    return st
+
+
+def _electrons(st, N, Te, seed):
+    rng = np.random.default_rng(seed)
+    d = st.domain
+    sig = np.sqrt(Te * config.QE / ME) / d.vel_norm
+    return (rng.random(N) * 16, rng.random(N) * 16, rng.normal(0, sig, N), rng.normal(0, sig, N),
+            rng.normal(0, sig, N))
+
+
+def _ke_ev(v, vel_norm):
+    return float(0.5 * ME * (v[2] ** 2 + v[3] ** 2 + v[4] ** 2).sum() * vel_norm ** 2 / config.eV)
+
+
+def test_counts_match_reference_direct_method():
+    print("[mcc] counts vs direct method", flush=True)
+    N, K = 200000, 150
+    st = _setup(N, 8.0)
+    d = st.domain
+    e = _electrons(st, N, 8.0, 1)
+    # GPU: null collision, reference analytic e-Ar tables
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, *e)
+    ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    ctx.mcc_setup(xsec.mcc_desc(xsec.reference_e_ar(), gas_density=GAS, gas_temp_K=0.0,
+                                enable=(1, 1, 1, 0, 0)))
+    for k in range(K):
+        ctx.collide(k)
+    g = ctx.mcc_stats()["count"]
+    # oracle: the reference's direct method
+    o = OracleSim(d, st.species, gas=(1, 1, 1, GAS))
+    o.set_particles(0, *e)
+    o.set_particles(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    oracle().orc_mcc_build_tables(o.h)
+    import ctypes
+    cnt = (ctypes.c_long * 3)(0, 0, 0)
+    el = ctypes.c_double()
+    for k in range(K):
+        oracle().orc_collide_direct(o.h, 0, 1, cnt, ctypes.byref(el))
+    c = list(cnt)
+    print(f"  gpu {g[:3]}  oracle {c}")
+    for p in range(3):
+        sigma = np.sqrt(g[p] + c[p])
+        assert abs(g[p] - c[p]) <= 3 * sigma + 1, (p, g[p], c[p])
+    assert g[0] > 10000 and g[1] > 500 and g[2] > 200
+    # births: one electron and one ion per ionisation
+    assert ctx.count(0) == N + g[2]
+    assert ctx.count(1) == 1 + g[2]
+
+
+def test_energy_removed_equals_thresholds():
+    """Electron KE drop = 11.5 eV per excitation + 15.8 eV per ionisation (the
+    ejected electron keeps its share), up to the 2 m_e/M elastic recoil."""
+    print("[mcc] energy bookkeeping", flush=True)
+    N, K = 100000, 100
+    st = _setup(N, 8.0)
+    d = st.domain
+    e = _electrons(st, N, 8.0, 2)
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, *e)
+    ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    ctx.mcc_setup(xsec.mcc_desc(xsec.reference_e_ar(), gas_density=GAS, gas_temp_K=0.0, enable=(1, 1, 1, 0, 0)))
+    ke0 = _ke_ev(e, d.vel_norm)
+    for k in range(K):
+        ctx.collide(k)
+    after = ctx.download(0)
+    ke1 = _ke_ev(after, d.vel_norm)
+    st_ = ctx.mcc_stats()
+    loss = st_["eloss_ev"]
+    print(f"  KE drop {ke0 - ke1:.1f} eV, thresholds {loss:.1f} eV, counts {st_['count'][:3]}")
+    assert loss > 0
+    # elastic recoil: ~2 m_e/M of the colliding energy per elastic collision
+    recoil = 2 * ME / xsec.M_AR * st_["count"][0] * 8.0 * 1.5
+    assert abs((ke0 - ke1) - loss) <= 0.01 * loss + 3 * recoil
+
+
+def test_mcc_is_deterministic():
+    N = 50000
+    st = _setup(N, 8.0)
+    e = _electrons(st, N, 8.0, 3)
+    outs = []
+    for rep in range(2):
+        ctx = _lib.Context(st.domain, st.species)
+        ctx.upload(0, *e)
+        ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+        ctx.mcc_setup(xsec.mcc_desc(xsec.reference_e_ar(), gas_density=GAS, gas_temp_K=0.0,
+                                    enable=(1, 1, 1, 0, 0)))
+        for k in range(30):
+            ctx.collide(k)
+        outs.append((ctx.download(0), ctx.mcc_stats()))
+        ctx.close()
+    for k in range(6):
+        assert np.array_equal(outs[0][0][k], outs[1][0][k])
+    assert outs[0][1] == outs[1][1]
+
+
+def test_ion_neutral_rates_match_monte_carlo_expectation():
+    """Ar+ - Ar elastic + charge exchange against 300 K neutrals: counts per
+    step against n_g <sigma(E_rel) g> dt estimated by numpy sampling."""
+    print("[mcc] ion-neutral", flush=True)
+    N, K = 200000, 50
+    specs = [config.SpeciesSpec("electron", ME, 16, 1.0, -1, 1.0),
+             config.SpeciesSpec("ion", xsec.M_AR, N, 0.026, 1, 0.0)]
+    st = config.derive(16, 16, 0.0, specs, 1e15, 0.05, bc="pbc")
+    d = st.domain
+    rng = np.random.default_rng(4)
+    kT = 0.026 * config.QE
+    sig_i = np.sqrt(kT / xsec.M_AR)
+    vi = rng.normal(0, sig_i, (N, 3))
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, np.full(16, 1.0), np.full(16, 1.0), np.zeros(16), np.zeros(16), np.zeros(16))
+    ctx.upload(1, rng.random(N) * 16, rng.random(N) * 16, vi[:, 0] / d.vel_norm, vi[:, 1] / d.vel_norm,
+               vi[:, 2] / d.vel_norm)
+    tab = xsec.synthetic_ar()
+    ctx.mcc_setup(xsec.mcc_desc(tab, electron_sp=-1, ion_sp=1, gas_density=1e21, gas_temp_K=300.0,
+                                enable=(0, 0, 0, 1, 1)))
+    for k in range(K):
+        ctx.collide(k)
+    g = ctx.mcc_stats()["count"]
+    # expectation: ions stay thermal at ~300 K (CX and elastic with 300 K gas)
+    vn = rng.normal(0, np.sqrt(config.K_b * 300.0 / xsec.M_AR), (N, 3))
+    gr = np.linalg.norm(vi - vn, axis=1)
+    E = 0.5 * xsec.M_AR * gr ** 2 / config.eV
+    e_min, e_max, T = tab
+    lg = np.log(xsec.log_grid(e_min, e_max, T.shape[1]))
+    s_el = np.interp(np.log(np.clip(E, e_min, e_max)), lg, T[_lib.MCC_I_ELASTIC])
+    s_cx = np.interp(np.log(np.clip(E, e_min, e_max)), lg, T[_lib.MCC_I_CX])
+    dt = d.DT / d.W
+    exp_el = (1e21 * s_el * gr * dt).sum() * K
+    exp_cx = (1e21 * s_cx * gr * dt).sum() * K
+    print(f"  gpu elastic {g[3]} cx {g[4]}; expected {exp_el:.0f} {exp_cx:.0f}")
+    assert abs(g[3] - exp_el) < 4 * np.sqrt(exp_el) + 0.02 * exp_el
+    assert abs(g[4] - exp_cx) < 4 * np.sqrt(exp_cx) + 0.02 * exp_cx
