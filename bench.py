@@ -61,7 +61,7 @@ def gpu_run(st, steps, warmup, solver_name):
                 errors=errs.tolist())
 
 
-def cpu_baseline(st, sample_frac=0.1):
+def cpu_baseline(st, sample_frac=1.0, steps=2):
     """The oracle (FP64 CPU restatement of the reference loop, one thread) on a
     bounded sample of the same workload: the full 512x512 grid and one step,
     with `sample_frac` of the particles.  Particle stages (scatter, push) are
@@ -95,23 +95,61 @@ def cpu_baseline(st, sample_frac=0.1):
     b = -rho[1:-1, 1:-1] * d.dx * d.dx * ((d.L * d.L) / (d.LDe * d.LDe))
     o.set_tolerance(d.tolerance * float(np.sqrt((b * b).sum())))
     t = time.perf_counter
-    t0 = t()
-    for s in range(nsp):
-        o.scatter(s)
-    t1 = t()
-    o.compute_rho()
-    it, _ = o.solve()
-    o.efield()
-    t2 = t()
-    for s in range(nsp):
-        o.push(s)
-    t3 = t()
+    tp = tf = 0.0
+    its = []
+    start = t()
+    for _ in range(steps):
+        t0 = t()
+        for s in range(nsp):
+            o.scatter(s)
+        t1 = t()
+        o.compute_rho()
+        it, _ = o.solve()
+        o.efield()
+        t2 = t()
+        for s in range(nsp):
+            o.push(s)
+        t3 = t()
+        tp += (t1 - t0) + (t3 - t2)
+        tf += t2 - t1
+        its.append(it)
+    wall = t() - start
     scale = n_full / n_samp
-    t_particles = ((t1 - t0) + (t3 - t2)) * scale
-    t_fields = t2 - t1
+    t_particles = tp * scale / steps
+    t_fields = tf / steps
     t_step = t_particles + t_fields
-    return dict(value=n_full / t_step, t=t3 - t0, iters=it, npart=n_full, nsamp=n_samp,
+    return dict(value=n_full / t_step, t=wall, iters=its, npart=n_full, nsamp=n_samp, steps=steps,
                 t_particles=t_particles, t_fields=t_fields)
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size (torch), timed with events:
+    the practical HBM rate and the reference point for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * nbytes
+    # a read-only pass of known size (reduction) for the read counter
+    e0.record()
+    for _ in range(reps):
+        tot = src.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_read = e0.elapsed_time(e1) / reps
+    del src, dst, tot
+    torch.cuda.empty_cache()
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": moved / ms / 1e6,
+            "read_only_ms": ms_read, "read_only_GB/s": nbytes / ms_read / 1e6}
 
 
 def main():
@@ -121,8 +159,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--solver", default="pcg_mixed", choices=["pcg_mixed", "spectral", "pcg", "gs"])
     ap.add_argument("--ppc", type=int, default=100)
-    ap.add_argument("--cpu-sample", type=float, default=0.1)
+    ap.add_argument("--cpu-sample", type=float, default=1.0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also time a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU (the per-step loop of one simulation); "
@@ -130,6 +171,7 @@ def main():
         sys.exit(2)
     solver = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
               "gs": _lib.SOLVER_GS}[args.solver]
+    copy_cal = copy_calibration() if args.copy_calibration else None
     st = workloads.bounded_512(ppc=args.ppc, solver=solver)
     r = gpu_run(st, args.steps, args.warmup, args.solver)
     nsp = len(st.species)
@@ -163,20 +205,23 @@ def main():
                    "stage_ms_per_step": stage},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "particle_kernel<float,8,MODE_PUSH,false,true> (gather+push+deposit)",
+                     "kernel": "push_f32_kernel<true,8,1> (CIC gather + leapfrog push + walls + deposit)",
                      "bytes_per_particle_step": bytes_per_ps, "avg_launch_ms": avg_launch_s * 1e3,
                      "particles_per_launch": parts_per_launch},
         "device_errors": r["errors"],
     }
+    if copy_cal:
+        out["copy_calibration"] = copy_cal
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(st, sample_frac=args.cpu_sample)
+        cb = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps)
         out["cpu_baseline"] = {
             "value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "one step of the same workload by the FP64 oracle on one thread: full 512x512 grid, %d of "
-                      "the %d particles (particle stages %.1f s scaled x%.1f to the full count; field stages %.1f s "
-                      "incl. %d PCG iterations of the matrix-free pcg() restatement); %.1f s of CPU work" % (
-                          cb["nsamp"], cb["npart"], cb["t_particles"] * cb["nsamp"] / cb["npart"],
-                          cb["npart"] / cb["nsamp"], cb["t_fields"], cb["iters"], cb["t"])}
+            "sample": "%d steps of the same workload by the FP64 oracle (restatement of the reference loop) on one "
+                      "thread: 512x512 grid, %d of the %d particles (particle stages scaled x%.2f to the full "
+                      "count), %.2f s particle + %.2f s field stages per step, Poisson by the matrix-free pcg() "
+                      "restatement (%s iterations); %.1f s of CPU work" % (
+                          cb["steps"], cb["nsamp"], cb["npart"], cb["npart"] / cb["nsamp"], cb["t_particles"],
+                          cb["t_fields"], cb["iters"], cb["t"])}
     print(json.dumps(out))
 
 

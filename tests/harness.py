@@ -10,11 +10,11 @@ ME = config.ME
 
 def small_setup(N=2000, Nx=16, Ny=16, bc="pbc", shape="CIC", solver=_lib.SOLVER_PCG, tol=1e-10,
                 precision=_lib.FP64, vsx_beam=0.0, capacity_factor=1.0, maxit=10000, poisson_op=None,
-                mass_ratio=1836.0, DT_coeff=0.05, sort_interval=0):
+                mass_ratio=1836.0, DT_coeff=0.05, sort_interval=0, ion_mass=None):
     """Electron-ion plasma at 1e13 m^-3, Te = 1 eV, Ti = 0.026 eV (the
     configuration of the 64x64 periodic workload, scaled down)."""
     specs = [config.SpeciesSpec("electron", ME, N, 1.0, -1, 1.0, vsx_beam, 0.0),
-             config.SpeciesSpec("ion", mass_ratio * ME, N, 0.026, 1, 0.0, 0.0, 0.0)]
+             config.SpeciesSpec("ion", ion_mass if ion_mass else mass_ratio * ME, N, 0.026, 1, 0.0, 0.0, 0.0)]
     return config.derive(Nx, Ny, 0.0, specs, 1e13, DT_coeff, bc=bc, shape=shape, solver=solver, tolerance=tol,
                          max_iteration=maxit, precision=precision, capacity_factor=capacity_factor,
                          poisson_op=poisson_op, sort_interval=sort_interval)
