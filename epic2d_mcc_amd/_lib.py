@@ -68,6 +68,31 @@ class MccStats(ctypes.Structure):
     _fields_ = [("count", ctypes.c_int64 * 5), ("eloss_ev", ctypes.c_double), ("births", ctypes.c_int64)]
 
 
+BATCH_NPUSH = 5
+BATCH_NNODE = 4
+BATCH_NODE_BLOCKS = (64, 128, 256, 512)
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("refine", ctypes.c_int), ("use_graph", ctypes.c_int), ("push_variant", ctypes.c_int),
+                ("node_block", ctypes.c_int)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("push_variant", ctypes.c_int), ("node_block", ctypes.c_int),
+                ("push_ms", ctypes.c_double * BATCH_NPUSH), ("node_ms", ctypes.c_double * BATCH_NNODE),
+                ("steps_used", ctypes.c_int)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("members", ctypes.c_int), ("groups", ctypes.c_int), ("tiles", ctypes.c_int),
+                ("segments", ctypes.c_int), ("chunks", ctypes.c_int), ("mcc_passes", ctypes.c_int),
+                ("push_variant", ctypes.c_int), ("node_block", ctypes.c_int),
+                ("steps", ctypes.c_int64), ("graph_replays", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
+                ("device_bytes", ctypes.c_int64), ("relres_max", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("stage_ms", ctypes.c_double * 4)]
+
+
 class EpicError(RuntimeError):
     pass
 
@@ -125,6 +150,22 @@ def lib():
         "epic_clear_error_counters": [P],
         "epic_timing_enable": [P, ctypes.c_int],
         "epic_timing_read": [P, D, ctypes.POINTER(I64)],
+        "epic_estimate_bytes": [ctypes.POINTER(DomainDesc), ctypes.POINTER(SpeciesDesc), ctypes.c_int,
+                                ctypes.POINTER(I64)],
+        "epic_device_bytes": [P, ctypes.POINTER(I64)],
+        "epic_batch_create": [ctypes.POINTER(BatchOpts), ctypes.POINTER(P)],
+        "epic_batch_destroy": [P],
+        "epic_batch_add": [P, ctypes.POINTER(DomainDesc), ctypes.POINTER(SpeciesDesc), ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_int)],
+        "epic_batch_member": [P, ctypes.c_int, ctypes.POINTER(P)],
+        "epic_batch_remove": [P, ctypes.c_int],
+        "epic_batch_step": [P, ctypes.c_int],
+        "epic_batch_sync": [P],
+        "epic_batch_tune": [P, ctypes.c_int, ctypes.POINTER(BatchTuning)],
+        "epic_batch_relres": [P, ctypes.c_int, D],
+        "epic_batch_member_steps": [P, ctypes.c_int, ctypes.POINTER(I64)],
+        "epic_batch_get_stats": [P, ctypes.POINTER(BatchStats)],
+        "epic_batch_timing_enable": [P, ctypes.c_int],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -132,6 +173,8 @@ def lib():
         f.restype = ctypes.c_int
     L.epic_last_error.argtypes = [P]
     L.epic_last_error.restype = ctypes.c_char_p
+    L.epic_batch_last_error.argtypes = [P]
+    L.epic_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -149,22 +192,33 @@ def check(ctx, rc, what):
 class Context:
     """One device context: a Domain with its Species, driven through the C-ABI."""
 
-    def __init__(self, domain: DomainDesc, species):
+    def __init__(self, domain: DomainDesc, species, handle=None):
         self.L = lib()
         self.domain = domain
         self.species = list(species)
+        self.nnodes = domain.nx * domain.ny
+        if handle is not None:
+            # a batch member: the batch owns and destroys it
+            self.h = handle
+            self.owned = False
+            return
         arr = (SpeciesDesc * len(self.species))(*self.species)
         h = ctypes.c_void_p()
         rc = self.L.epic_create(ctypes.byref(domain), arr, len(self.species), ctypes.byref(h))
         if rc != EPIC_OK:
             raise EpicError(f"epic_create failed: {ERR_NAMES.get(rc, rc)}")
         self.h = h
-        self.nnodes = domain.nx * domain.ny
+        self.owned = True
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             self.L.epic_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def device_bytes(self):
+        v = ctypes.c_int64()
+        self._c(self.L.epic_device_bytes(self.h, ctypes.byref(v)), "device_bytes")
+        return v.value
 
     def __del__(self):
         try:
@@ -304,3 +358,96 @@ def device_name():
     buf = ctypes.create_string_buffer(256)
     rc = lib().epic_device_name(buf, 256)
     return buf.value.decode() if rc == 0 else "unknown"
+
+
+def estimate_bytes(domain: DomainDesc, species):
+    """Device bytes a context for this description allocates at creation."""
+    arr = (SpeciesDesc * len(species))(*species)
+    v = ctypes.c_int64()
+    rc = lib().epic_estimate_bytes(ctypes.byref(domain), arr, len(species), ctypes.byref(v))
+    if rc != EPIC_OK:
+        raise EpicError(f"epic_estimate_bytes failed: {ERR_NAMES.get(rc, rc)}")
+    return v.value
+
+
+class Batch:
+    """A batched ensemble (epic_batch_*): members are Contexts that the batch
+    owns; every member advances one step per epic_batch_step."""
+
+    def __init__(self, refine=2, use_graph=True, push_variant=-1, node_block=0):
+        self.L = lib()
+        o = BatchOpts(refine, 1 if use_graph else 0, push_variant, node_block)
+        h = ctypes.c_void_p()
+        rc = self.L.epic_batch_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != EPIC_OK:
+            raise EpicError(f"epic_batch_create failed: {ERR_NAMES.get(rc, rc)}")
+        self.h = h
+        self.members = {}
+
+    def _c(self, rc, what):
+        if rc != EPIC_OK:
+            msg = self.L.epic_batch_last_error(self.h).decode() if self.h else ""
+            raise EpicError(f"{what} failed: {ERR_NAMES.get(rc, rc)} {msg}")
+
+    def close(self):
+        if self.h:
+            for m in self.members.values():
+                m.h = None
+            self.members = {}
+            self.L.epic_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, domain: DomainDesc, species):
+        """New member; returns (id, Context) for its per-simulation calls."""
+        species = list(species)
+        arr = (SpeciesDesc * len(species))(*species)
+        mid = ctypes.c_int()
+        self._c(self.L.epic_batch_add(self.h, ctypes.byref(domain), arr, len(species), ctypes.byref(mid)), "batch_add")
+        ch = ctypes.c_void_p()
+        self._c(self.L.epic_batch_member(self.h, mid.value, ctypes.byref(ch)), "batch_member")
+        ctx = Context(domain, species, handle=ch)
+        self.members[mid.value] = ctx
+        return mid.value, ctx
+
+    def remove(self, mid):
+        self._c(self.L.epic_batch_remove(self.h, mid), "batch_remove")
+        ctx = self.members.pop(mid)
+        ctx.h = None
+
+    def step(self, n=1):
+        self._c(self.L.epic_batch_step(self.h, n), "batch_step")
+
+    def sync(self):
+        self._c(self.L.epic_batch_sync(self.h), "batch_sync")
+
+    def tune(self, steps_per_candidate=2):
+        t = BatchTuning()
+        self._c(self.L.epic_batch_tune(self.h, steps_per_candidate, ctypes.byref(t)), "batch_tune")
+        return {"push_variant": t.push_variant, "node_block": t.node_block, "push_ms": list(t.push_ms),
+                "node_ms": list(t.node_ms), "steps_used": t.steps_used}
+
+    def relres(self, mid):
+        v = ctypes.c_double()
+        self._c(self.L.epic_batch_relres(self.h, mid, ctypes.byref(v)), "batch_relres")
+        return v.value
+
+    def member_steps(self, mid):
+        v = ctypes.c_int64()
+        self._c(self.L.epic_batch_member_steps(self.h, mid, ctypes.byref(v)), "batch_member_steps")
+        return v.value
+
+    def stats(self):
+        st = BatchStats()
+        self._c(self.L.epic_batch_get_stats(self.h, ctypes.byref(st)), "batch_stats")
+        out = {k: getattr(st, k) for k, _ in BatchStats._fields_}
+        out["stage_ms"] = list(st.stage_ms)
+        return out
+
+    def timing(self, on=True):
+        self._c(self.L.epic_batch_timing_enable(self.h, 1 if on else 0), "batch_timing")

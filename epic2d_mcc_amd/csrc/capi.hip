@@ -42,6 +42,13 @@ static hipError_t dalloc(T** p, size_t count) {
   return hipMemset(*p, 0, count * sizeof(T));
 }
 
+template <typename T>
+static hipError_t dalloc_c(Ctx* c, T** p, size_t count) {
+  if (count == 0) count = 1;
+  c->bytes += count * sizeof(T);
+  return dalloc(p, count);
+}
+
 static size_t psize(const Ctx* c) { return c->dom.precision == EPIC_FP64 ? 8 : 4; }
 
 hipError_t sync_counts(Ctx* c) {
@@ -102,7 +109,36 @@ static void free_all(Ctx* c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   for (hipEvent_t ev : c->ev_pool) hipEventDestroy(ev);
   c->ev_pool.clear();
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+}
+
+Ctx* ctx_of(epic_ctx* h) { return h ? &h->c : nullptr; }
+
+// Device bytes ctx_create allocates for this description (the allocations
+// below, in the same order); MCC tables and the sine-transform solver's
+// matrices are allocated later, on first use.
+size_t ctx_bytes_estimate(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int num_cu) {
+  const size_t nn = (size_t)dom->nx * dom->ny;
+  const size_t ps = dom->precision == EPIC_FP64 ? 8 : 4;
+  auto b = [](size_t count, size_t sz) { return (count == 0 ? 1 : count) * sz; };
+  size_t tot = 0;
+  int64_t maxcap = 1;
+  for (int s = 0; s < nsp; s++) {
+    const int64_t cap = sp[s].capacity > 0 ? sp[s].capacity : std::max<int64_t>(sp[s].numparticle, 1);
+    maxcap = std::max(maxcap, cap);
+    const size_t cap_pad = (size_t)((cap + 7) / 8 * 8);
+    tot += 10 * b(cap_pad * ps, 1) + 2 * b(cap_pad, 4) + 2 * b(nn, 8) + b(2, 8);
+  }
+  tot += 4 * b(nn, 8) + b(2 * nn * ps, 1) + 2 * b(nn, 8) + b(2 * nn, 8) + 2 * b(nn, 8);
+  tot += b((size_t)16 * std::max(num_cu, 1024), 8) + b(8, 8) + b(4, 4);
+  const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
+  tot += 2 * b(ncell + 2, 4) + b(2 * (size_t)maxcap, 4);
+  const int64_t blk_cap = maxcap / 2048 + 4;
+  tot += b((size_t)blk_cap, 4);
+  const int scan_cap = (int)(std::max<int64_t>((int64_t)ncell + 2, blk_cap) / 1024 + 4);
+  tot += b((size_t)scan_cap, 8);
+  tot += b((size_t)8 * num_cu * 4, 8) + b(8, 8) + b(EPIC_NERR, 8);
+  return tot;
 }
 
 }  // namespace epic
@@ -120,7 +156,11 @@ int epic_device_name(char* buf, int len) {
   return EPIC_OK;
 }
 
-int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, epic_ctx** out) {
+}  // extern "C"
+
+namespace epic {
+
+int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, hipStream_t ext, epic_ctx** out) {
   if (!dom || !sp || !out || nsp < 1 || nsp > 16) return EPIC_ERR_ARG;
   if (dom->nx < 3 || dom->ny < 3) return EPIC_ERR_ARG;
   if (dom->precision != EPIC_FP64 && dom->precision != EPIC_FP32) return EPIC_ERR_ARG;
@@ -154,7 +194,12 @@ int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int ns
   c->num_cu = prop.multiProcessorCount;
   if (const char* v = getenv("EPIC_PUSH_VARIANT")) c->push_variant = atoi(v);
   c->max_solver_blocks = c->num_cu;
-  CKC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (ext) {
+    c->stream = ext;
+    c->own_stream = false;
+  } else {
+    CKC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  }
   CKC(hipEventCreate(&c->ev0), "hipEventCreate");
   CKC(hipEventCreate(&c->ev1), "hipEventCreate");
   const size_t nn = c->nnodes;
@@ -175,41 +220,65 @@ int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int ns
     const size_t cap_pad = (size_t)((S.cap + 7) / 8 * 8);
     char** arrs[] = {(char**)&S.x, (char**)&S.y, (char**)&S.vx, (char**)&S.vy, (char**)&S.vz,
                      (char**)&S.x2, (char**)&S.y2, (char**)&S.vx2, (char**)&S.vy2, (char**)&S.vz2};
-    for (char** a : arrs) CKC(dalloc(a, cap_pad * ps), "particle arrays");
-    CKC(dalloc(&S.id, cap_pad), "ids");
-    CKC(dalloc(&S.id2, cap_pad), "ids");
-    CKC(dalloc(&S.Q, nn), "Q");
-    CKC(dalloc(&S.den, nn), "den");
-    CKC(dalloc(&S.d_n, 2), "count");
+    for (char** a : arrs) CKC(dalloc_c(c, a, cap_pad * ps), "particle arrays");
+    CKC(dalloc_c(c, &S.id, cap_pad), "ids");
+    CKC(dalloc_c(c, &S.id2, cap_pad), "ids");
+    CKC(dalloc_c(c, &S.Q, nn), "Q");
+    CKC(dalloc_c(c, &S.den, nn), "den");
+    CKC(dalloc_c(c, &S.d_n, 2), "count");
   }
-  CKC(dalloc(&c->rho, nn), "rho");
-  CKC(dalloc(&c->phi, nn), "phi");
-  CKC(dalloc(&c->efx, nn), "efx");
-  CKC(dalloc(&c->efy, nn), "efy");
-  CKC(dalloc((char**)&c->E2, 2 * nn * ps), "E2");
-  CKC(dalloc(&c->b64, nn), "b");
-  CKC(dalloc(&c->r64, nn), "r");
-  CKC(dalloc((double**)&c->ws_v, 2 * nn), "v");
-  CKC(dalloc((double**)&c->ws_w, nn), "w");
-  CKC(dalloc((double**)&c->ws_x, nn), "x");
-  CKC(dalloc(&c->ws_part, (size_t)16 * std::max(c->max_solver_blocks, 1024)), "partials");
-  CKC(dalloc(&c->ws_scalar, 8), "scalars");
-  CKC(dalloc(&c->ws_iter, 4), "iters");  // [0] iterations, [1] grid barrier
+  CKC(dalloc_c(c, &c->rho, nn), "rho");
+  CKC(dalloc_c(c, &c->phi, nn), "phi");
+  CKC(dalloc_c(c, &c->efx, nn), "efx");
+  CKC(dalloc_c(c, &c->efy, nn), "efy");
+  CKC(dalloc_c(c, (char**)&c->E2, 2 * nn * ps), "E2");
+  CKC(dalloc_c(c, &c->b64, nn), "b");
+  CKC(dalloc_c(c, &c->r64, nn), "r");
+  CKC(dalloc_c(c, (double**)&c->ws_v, 2 * nn), "v");
+  CKC(dalloc_c(c, (double**)&c->ws_w, nn), "w");
+  CKC(dalloc_c(c, (double**)&c->ws_x, nn), "x");
+  CKC(dalloc_c(c, &c->ws_part, (size_t)16 * std::max(c->max_solver_blocks, 1024)), "partials");
+  CKC(dalloc_c(c, &c->ws_scalar, 8), "scalars");
+  CKC(dalloc_c(c, &c->ws_iter, 4), "iters");  // [0] iterations, [1] grid barrier
   const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
-  CKC(dalloc(&c->sort_count, ncell + 2), "sort");
-  CKC(dalloc(&c->sort_fill, ncell + 2), "sort");
+  CKC(dalloc_c(c, &c->sort_count, ncell + 2), "sort");
+  CKC(dalloc_c(c, &c->sort_fill, ncell + 2), "sort");
   c->sort_cap = maxcap;
-  CKC(dalloc(&c->sort_perm, 2 * (size_t)maxcap), "sort");
+  CKC(dalloc_c(c, &c->sort_perm, 2 * (size_t)maxcap), "sort");
   c->blk_cap = maxcap / 2048 + 4;
-  CKC(dalloc(&c->blk_count, (size_t)c->blk_cap), "compact");
+  CKC(dalloc_c(c, &c->blk_count, (size_t)c->blk_cap), "compact");
   c->scan_tiles_cap = (int)(std::max<int64_t>((int64_t)ncell + 2, c->blk_cap) / 1024 + 4);
-  CKC(dalloc(&c->scan_tiles, (size_t)c->scan_tiles_cap), "scan");
+  CKC(dalloc_c(c, &c->scan_tiles, (size_t)c->scan_tiles_cap), "scan");
   c->red_blocks = c->num_cu * 4;
-  CKC(dalloc(&c->red_part, (size_t)8 * c->red_blocks), "reduce");
-  CKC(dalloc(&c->red_out, 8), "reduce");
-  CKC(dalloc(&c->errc, EPIC_NERR), "errc");
+  CKC(dalloc_c(c, &c->red_part, (size_t)8 * c->red_blocks), "reduce");
+  CKC(dalloc_c(c, &c->red_out, 8), "reduce");
+  CKC(dalloc_c(c, &c->errc, EPIC_NERR), "errc");
 #undef CKC
   *out = h;
+  return EPIC_OK;
+}
+
+}  // namespace epic
+
+extern "C" {
+
+int epic_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, epic_ctx** out) {
+  return ctx_create(dom, sp, nsp, nullptr, out);
+}
+
+int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int64_t* bytes) {
+  if (!dom || !sp || !bytes || nsp < 1 || nsp > 16) return EPIC_ERR_ARG;
+  int dev = 0, ncu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    ncu = prop.multiProcessorCount;
+  *bytes = (int64_t)ctx_bytes_estimate(dom, sp, nsp, ncu);
+  return EPIC_OK;
+}
+
+int epic_device_bytes(epic_ctx* h, int64_t* bytes) {
+  if (!h || !bytes) return EPIC_ERR_ARG;
+  *bytes = (int64_t)h->c.bytes;
   return EPIC_OK;
 }
 
@@ -413,19 +482,22 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   if (d->ntab < 2 || !d->sigma || !(d->e_min > 0) || !(d->e_max > d->e_min)) return fail(c, EPIC_ERR_ARG, "bad table");
   if (d->electron_sp >= c->nsp || d->ion_sp >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
   MccState& M = c->mcc;
+  // a repeated setup replaces the table and the event list
+  if (M.sigma) c->bytes -= (size_t)EPIC_MCC_NPROC * M.d.ntab * sizeof(double);
+  if (M.events) c->bytes -= (size_t)M.event_cap * 8 * sizeof(double);
   M.d = *d;
   std::vector<double> sig(d->sigma, d->sigma + (size_t)EPIC_MCC_NPROC * d->ntab);
   if (M.sigma) hipFree(M.sigma);
   M.sigma = nullptr;
-  CK(dalloc(&M.sigma, sig.size()), "mcc table");
+  CK(dalloc_c(c, &M.sigma, sig.size()), "mcc table");
   CK(hipMemcpy(M.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice), "mcc table");
   M.d.sigma = nullptr;
   M.event_cap = d->event_cap > 0 ? d->event_cap : 16384;
-  if (!M.counts) CK(dalloc(&M.counts, EPIC_MCC_NPROC + 2), "mcc counts");
+  if (!M.counts) CK(dalloc_c(c, &M.counts, EPIC_MCC_NPROC + 2), "mcc counts");
   if (M.events) hipFree(M.events);
   M.events = nullptr;
-  CK(dalloc(&M.events, (size_t)M.event_cap * 8), "mcc events");
-  if (!M.event_count) CK(dalloc(&M.event_count, 1), "mcc events");
+  CK(dalloc_c(c, &M.events, (size_t)M.event_cap * 8), "mcc events");
+  if (!M.event_count) CK(dalloc_c(c, &M.event_count, 1), "mcc events");
   M.nu_max_e = d->electron_sp >= 0 ? mcc_nu_max(M.d, sig, ME, true) : 0.0;
   M.nu_max_i = d->ion_sp >= 0 ? mcc_nu_max(M.d, sig, c->sp[d->ion_sp].desc.mass, false) : 0.0;
   M.on = true;

@@ -8,6 +8,7 @@
 //   rho = 0; rho += den_s * (charge_s * (1/QE))             domain.cpp:194-202
 #include "device_common.hpp"
 #include "epic_internal.hpp"
+#include "node_ops.hpp"
 
 namespace epic {
 
@@ -31,32 +32,13 @@ __global__ void rho_finalize_kernel(FinArgs a) {
   const int64_t nn = (int64_t)a.nx * a.ny;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nn; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(k / a.ny), j = (int)(k % a.ny);
-    const bool iedge = (i == 0 || i == a.nx - 1), jedge = (j == 0 || j == a.ny - 1);
     double rho = 0;
     for (int s = 0; s < a.nsp; s++) {
       double den;
       if (a.mask & (1u << s)) {
-        const unsigned long long* Q = a.Q[s];
-        unsigned long long q;
-        if (a.bc == EPIC_BC_PBC) {
-          const int ia0 = iedge ? 0 : i, ia1 = iedge ? a.nx - 1 : i;
-          const int ja0 = jedge ? 0 : j, ja1 = jedge ? a.ny - 1 : j;
-          q = Q[(size_t)ia0 * a.ny + ja0];
-          if (ja1 != ja0) q += Q[(size_t)ia0 * a.ny + ja1];
-          if (ia1 != ia0) {
-            q += Q[(size_t)ia1 * a.ny + ja0];
-            if (ja1 != ja0) q += Q[(size_t)ia1 * a.ny + ja1];
-          }
-        } else {
-          q = Q[k];
-        }
-        const long long qs = (long long)q;
+        const long long qs = (long long)fold_q(a.Q[s], i, j, a.nx, a.ny, a.bc);
         if (qs < 0) atomicAdd(&a.errc[EPIC_ERRC_FIXED_OVF], 1ull);
-        den = ((double)qs * a.scale[s]) / a.norm;
-        if (a.bc != EPIC_BC_PBC) {
-          if (iedge) den *= 0.5;
-          if (jedge) den *= 0.5;
-        }
+        den = den_from_q(qs, a.scale[s], a.norm, i, j, a.nx, a.ny, a.bc);
         a.den[s][k] = den;
       } else {
         den = a.den[s][k];

@@ -31,6 +31,8 @@ constexpr int FIX_BITS_FP32 = 28;
 constexpr int DEP_WINDOW = 8192;
 constexpr int PART_BLOCK = 256;  // threads per workgroup in particle kernels
 
+struct PushF32;
+
 struct SpeciesDev {
   epic_species_desc desc;
   int64_t cap = 0;
@@ -73,6 +75,8 @@ struct Ctx {
   int nsp = 0;
   std::vector<SpeciesDev> sp;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false for members of a batch, which share its stream
+  size_t bytes = 0;        // device memory allocated at creation
   int device = 0;
   int num_cu = 0;
   int fix_bits = FIX_BITS_FP64;
@@ -163,11 +167,26 @@ hipError_t launch_field_energy(Ctx* c, double* sum_e2);
 hipError_t launch_mcc(Ctx* c, int step);
 double mcc_nu_max(const epic_mcc_desc& d, const std::vector<double>& sig, double m, bool electron);
 
+// batch support
+PushF32 make_push_f32(Ctx* c, int s, double kv, double kx);
+void push_f32_consts(Ctx* c, int s, double* kv, double* kx);
+size_t mcc_args_bytes();
+size_t append_args_bytes();
+// appends the FP32 collision passes and the birth append of context c
+// (step counter index slot) as raw argument records
+void mcc_batch_collect(Ctx* c, int slot, std::vector<unsigned char>& args, std::vector<unsigned char>& app);
+hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int nargs, const void* d_app, int napp,
+                            const int* d_steps);
+
 // helpers (capi.hip)
 int64_t live_count(Ctx* c, int s, hipError_t* e);
 hipError_t sync_counts(Ctx* c);
 void stage_begin(Ctx* c);
 void stage_end(Ctx* c, int stage);
+// context creation on an existing stream (ext non-null: not owned)
+int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, hipStream_t ext, epic_ctx** out);
+Ctx* ctx_of(epic_ctx* h);
+size_t ctx_bytes_estimate(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int num_cu);
 
 }  // namespace epic
 

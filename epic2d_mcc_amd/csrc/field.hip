@@ -23,6 +23,7 @@
 
 #include "device_common.hpp"
 #include "epic_internal.hpp"
+#include "node_ops.hpp"
 
 namespace epic {
 
@@ -44,43 +45,6 @@ struct PcgArgs {
   int relative;         // tol relative to ||b||
   int compute_error;    // ||A x - b|| / n (field.cpp:258-259)
   unsigned* bar;        // grid barrier counter
-};
-
-template <int OP>
-struct Stencil {
-  int nx, ny;
-  // unknown flag for node k
-  __device__ __forceinline__ bool unknown(int k) const {
-    if (OP == EPIC_OP_REF_GHOST) return true;
-    const int i = k / ny, j = k % ny;
-    return i > 0 && i < nx - 1 && j > 0 && j < ny - 1;
-  }
-  // Row k of the reference's dense A (field.cpp:223-232) applied to vec(),
-  // non-zero entries in increasing column order; matrix-free.
-  template <typename F>
-  __device__ __forceinline__ double apply(int k, double self, F vec) const {
-    if (OP == EPIC_OP_REF_GHOST) {
-      const int row = k / nx, col = k % nx;
-      const int n = nx * ny;
-      double sum = 0;
-      if (row > 0) sum += vec(k - nx);
-      if (col > 0) sum += vec(k - 1);
-      sum += -4.0 * self;
-      if (col < nx - 1) sum += vec(k + 1);
-      if (row < ny - 1) sum += vec(k + nx);
-      (void)n;
-      return sum;
-    } else {
-      // interior node (boundary neighbours hold phi = 0)
-      double sum = 0;
-      sum += vec(k - ny);
-      sum += vec(k - 1);
-      sum += -4.0 * self;
-      sum += vec(k + 1);
-      sum += vec(k + ny);
-      return sum;
-    }
-  }
 };
 
 template <typename T>
@@ -336,11 +300,8 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
 // zero on every boundary node.
 __global__ void rhs_kernel(const double* rho, double* b, int nx, int ny, double dx2, double s) {
   const int n = nx * ny;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int i = k / ny, j = k % ny;
-    const bool bnd = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
-    b[k] = bnd ? 0.0 : -rho[k] * dx2 * s;
-  }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    b[k] = rhs_at(rho[k], k / ny, k % ny, nx, ny, dx2, s);
 }
 
 // r = b - A phi (FP64 true residual) and its squared norm partials.
@@ -388,27 +349,8 @@ __global__ void efield_kernel(const double* phi, double* efx, double* efy, P* E2
                               double dy, int bc) {
   const int n = nx * ny;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int i = k / ny, j = k % ny;
     double ex, ey;
-    auto ph = [&](int ii, int jj) { return phi[(size_t)ii * ny + jj]; };
-    if (i > 0 && i < nx - 1) {
-      ex = -(ph(i + 1, j) - ph(i - 1, j)) / (2 * dx);
-    } else if (bc == EPIC_BC_PBC) {
-      ex = -(ph(1, j) - ph(nx - 2, j)) / (2 * dx);
-    } else if (i == 0) {
-      ex = -(ph(1, j) - ph(0, j)) / dx;
-    } else {
-      ex = -(ph(nx - 1, j) - ph(nx - 2, j)) / dx;
-    }
-    if (j > 0 && j < ny - 1) {
-      ey = -(ph(i, j + 1) - ph(i, j - 1)) / (2 * dy);
-    } else if (bc == EPIC_BC_PBC) {
-      ey = -(ph(i, 1) - ph(i, ny - 2)) / (2 * dy);
-    } else if (j == 0) {
-      ey = -(ph(i, 1) - ph(i, 0)) / dy;
-    } else {
-      ey = -(ph(i, ny - 1) - ph(i, ny - 2)) / dy;
-    }
+    efield_at(phi, k / ny, k % ny, nx, ny, dx, dy, bc, &ex, &ey);
     efx[k] = ex;
     efy[k] = ey;
     E2[2 * (size_t)k] = (P)ex;

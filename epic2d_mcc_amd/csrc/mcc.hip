@@ -68,6 +68,7 @@ struct MccArgs {
   double* events;
   int* event_count;
   int event_cap;
+  int slot;  // batched runs: index of the simulation's step counter
 };
 
 __device__ __forceinline__ void rotate(double g, double chi, double eta, double st, double ct, double sp, double cp,
@@ -85,21 +86,22 @@ __device__ __forceinline__ double gauss(uint32_t a, uint32_t b, uint32_t c, uint
   return m * cos(2.0 * M_PI * u2);
 }
 
+// One colliding species: workgroup bid of nblk walks its particles.
 template <typename P>
-__global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
+__device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
   const int64_t n = a.d_n[0];
   unsigned long long cnt[EPIC_MCC_NPROC] = {0, 0, 0, 0, 0};
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t p = bid * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)nblk * blockDim.x) {
     const uint32_t pid = a.id[p];
-    const u32x4 r = philox4x32_10({pid, a.step, RNG_MCC, a.sp}, a.k0, a.k1);
+    const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
     const double R0 = u01_53(r.a, r.b);
     if (R0 >= a.p_null) continue;
     const P xp = a.x[p];
     if (isnan(xp)) continue;
     double vxe = a.vx[p], vye = a.vy[p], vze = a.vz[p];
     const double R1 = u01_53(r.c, r.d);
-    const u32x4 r2 = philox4x32_10({pid, a.step, RNG_MCC2, a.sp}, a.k0, a.k1);
-    const u32x4 r3 = philox4x32_10({pid, a.step, RNG_MCC3, a.sp}, a.k0, a.k1);
+    const u32x4 r2 = philox4x32_10({pid, step, RNG_MCC2, a.sp}, a.k0, a.k1);
+    const u32x4 r3 = philox4x32_10({pid, step, RNG_MCC3, a.sp}, a.k0, a.k1);
     if (a.kind == 0) {
       // electron - neutral (neutral at rest)
       const double v_sqr = (vxe * vxe + vye * vye + vze * vze) * a.vel_norm * a.vel_norm;
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
       // real collision: process by frequency share
       const double R = (R1 * a.p_null / pc) * nu;
       if (R < nu_el) {
-        const u32x4 r4 = philox4x32_10({pid, a.step, RNG_MCC_ION_SCATTER, a.sp}, a.k0, a.k1);
+        const u32x4 r4 = philox4x32_10({pid, step, RNG_MCC_ION_SCATTER, a.sp}, a.k0, a.k1);
         const double M = a.gas_mass, m = a.mass;
         const double cx = (m * vxe + M * vnx) / (m + M), cy = (m * vye + M * vny) / (m + M),
                      cz = (m * vze + M * vnz) / (m + M);
@@ -219,6 +221,19 @@ __global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
   }
 }
 
+template <typename P>
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
+  mcc_body(a, a.step, blockIdx.x, gridDim.x);
+}
+
+// Batched: blockIdx.y selects the (simulation, species) pass; the step comes
+// from the simulation's device step counter.
+template <typename P>
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_batch_kernel(const MccArgs<P>* args, const int* steps) {
+  const MccArgs<P>& a = args[blockIdx.y];
+  mcc_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x);
+}
+
 // Append ionisation births in parent-slot order: one workgroup ranks the
 // events (slots are unique) and writes the new electron at n_e + rank and the
 // new ion at n_i + rank, ids next_id + rank, and deposits both.
@@ -226,13 +241,51 @@ constexpr int APPEND_BLOCK = 1024;
 constexpr int APPEND_LDS = 16384;
 
 template <typename P>
-__global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_kernel(
-    const double* events, int* event_count, int cap_events, P* ex, P* ey, P* evx, P* evy, P* evz, uint32_t* eid,
-    int64_t* e_n, int64_t e_cap, P* ix, P* iy, P* ivx, P* ivy, P* ivz, uint32_t* iid, int64_t* i_n, int64_t i_cap,
-    double ion_vth, uint32_t step, uint32_t k0, uint32_t k1, unsigned long long* Qe, unsigned long long* Qi, int nx,
-    int ny, double x0, double y0, double dx, double dy, int fix_bits, unsigned long long* counts,
-    unsigned long long* errc) {
-  __shared__ double slots[APPEND_LDS];
+struct AppendArgs {
+  const double* events;
+  int* event_count;
+  int cap_events;
+  P *ex, *ey, *evx, *evy, *evz;
+  uint32_t* eid;
+  int64_t* e_n;
+  int64_t e_cap;
+  P *ix, *iy, *ivx, *ivy, *ivz;
+  uint32_t* iid;
+  int64_t* i_n;
+  int64_t i_cap;
+  double ion_vth;
+  uint32_t step, k0, k1;
+  unsigned long long* Qe;  // null: births are not deposited
+  unsigned long long* Qi;
+  int nx, ny;
+  double x0, y0, dx, dy;
+  int fix_bits;
+  unsigned long long* counts;
+  unsigned long long* errc;
+  int slot;  // batched runs: index of the simulation's step counter
+};
+
+template <typename P>
+__device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t step, double* slots) {
+  const double* events = A.events;
+  int* event_count = A.event_count;
+  const int cap_events = A.cap_events;
+  P *ex = A.ex, *ey = A.ey, *evx = A.evx, *evy = A.evy, *evz = A.evz;
+  uint32_t* eid = A.eid;
+  int64_t* e_n = A.e_n;
+  const int64_t e_cap = A.e_cap;
+  P *ix = A.ix, *iy = A.iy, *ivx = A.ivx, *ivy = A.ivy, *ivz = A.ivz;
+  uint32_t* iid = A.iid;
+  int64_t* i_n = A.i_n;
+  const int64_t i_cap = A.i_cap;
+  const double ion_vth = A.ion_vth;
+  const uint32_t k0 = A.k0, k1 = A.k1;
+  unsigned long long *Qe = A.Qe, *Qi = A.Qi;
+  const int nx = A.nx, ny = A.ny;
+  const double x0 = A.x0, y0 = A.y0, dx = A.dx, dy = A.dy;
+  const int fix_bits = A.fix_bits;
+  unsigned long long* counts = A.counts;
+  unsigned long long* errc = A.errc;
   const int E = min(*event_count, cap_events);
   for (int k = threadIdx.x; k < E; k += blockDim.x) slots[k] = events[(size_t)k * EVW];
   __syncthreads();
@@ -302,7 +355,22 @@ __global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_kernel(
 }
 
 template <typename P>
-static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
+__global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_kernel(AppendArgs<P> A) {
+  __shared__ double slots[APPEND_LDS];
+  mcc_append_body(A, A.step, slots);
+}
+
+// Batched: one workgroup per simulation with ionisation.
+template <typename P>
+__global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_batch_kernel(const AppendArgs<P>* args, const int* steps) {
+  __shared__ double slots[APPEND_LDS];
+  const AppendArgs<P>& A = args[blockIdx.x];
+  mcc_append_body(A, (uint32_t)steps[A.slot], slots);
+}
+
+// Arguments of one species' collision pass (kind 0 electron, 1 ion).
+template <typename P>
+static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   MccState& M = c->mcc;
   SpeciesDev& S = c->sp[s];
   const epic_domain_desc& d = c->dom;
@@ -337,20 +405,96 @@ static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
   a.events = M.events;
   a.event_count = M.event_count;
   a.event_cap = M.event_cap;
+  return a;
+}
+
+// Arguments of the ionisation-birth append (electron species -> ion species).
+template <typename P>
+static AppendArgs<P> append_args(Ctx* c, int step) {
+  MccState& M = c->mcc;
+  const epic_domain_desc& d = c->dom;
+  SpeciesDev& S = c->sp[M.d.electron_sp];
+  SpeciesDev& I = c->sp[M.d.ion_sp];
+  AppendArgs<P> A{};
+  A.events = M.events;
+  A.event_count = M.event_count;
+  A.cap_events = std::min(M.event_cap, APPEND_LDS);
+  A.ex = (P*)S.x; A.ey = (P*)S.y; A.evx = (P*)S.vx; A.evy = (P*)S.vy; A.evz = (P*)S.vz;
+  A.eid = S.id; A.e_n = S.d_n; A.e_cap = S.cap;
+  A.ix = (P*)I.x; A.iy = (P*)I.y; A.ivx = (P*)I.vx; A.ivy = (P*)I.vy; A.ivz = (P*)I.vz;
+  A.iid = I.id; A.i_n = I.d_n; A.i_cap = I.cap;
+  A.ion_vth = sqrt(I.desc.temp * QE / I.desc.mass) / d.vel_norm;
+  A.step = (uint32_t)step;
+  A.k0 = (uint32_t)d.seed;
+  A.k1 = (uint32_t)(d.seed >> 32);
+  A.Qe = S.Q;
+  A.Qi = I.Q;
+  A.nx = d.nx; A.ny = d.ny;
+  A.x0 = d.x0; A.y0 = d.y0; A.dx = d.dx; A.dy = d.dy;
+  A.fix_bits = c->fix_bits;
+  A.counts = M.counts;
+  A.errc = c->errc;
+  return A;
+}
+
+template <typename P>
+static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
+  const MccArgs<P> a = mcc_args<P>(c, s, kind, step);
   if (a.nu_max <= 0) return hipSuccess;
   const int grid = c->num_cu * 8;
   hipLaunchKernelGGL(mcc_kernel<P>, dim3(grid), dim3(MCC_BLOCK), 0, c->stream, a);
   EPIC_HIP_TRY(hipGetLastError());
-  if (kind == 0 && (en & 4)) {
-    SpeciesDev& I = c->sp[M.d.ion_sp];
-    const double ion_vth = sqrt(I.desc.temp * QE / I.desc.mass) / d.vel_norm;
-    unsigned long long* Qe = c->deposit_valid ? S.Q : nullptr;
-    unsigned long long* Qi = c->deposit_valid ? I.Q : nullptr;
-    hipLaunchKernelGGL(mcc_append_kernel<P>, dim3(1), dim3(APPEND_BLOCK), 0, c->stream, M.events, M.event_count,
-                       std::min(M.event_cap, APPEND_LDS), (P*)S.x, (P*)S.y, (P*)S.vx, (P*)S.vy, (P*)S.vz, S.id,
-                       S.d_n, S.cap, (P*)I.x, (P*)I.y, (P*)I.vx, (P*)I.vy, (P*)I.vz, I.id, I.d_n, I.cap, ion_vth,
-                       (uint32_t)step, a.k0, a.k1, Qe, Qi, d.nx, d.ny, d.x0, d.y0, d.dx, d.dy, c->fix_bits,
-                       M.counts, c->errc);
+  if (kind == 0 && (a.enable & 4)) {
+    AppendArgs<P> A = append_args<P>(c, step);
+    if (!c->deposit_valid) A.Qe = A.Qi = nullptr;
+    hipLaunchKernelGGL(mcc_append_kernel<P>, dim3(1), dim3(APPEND_BLOCK), 0, c->stream, A);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+size_t mcc_args_bytes() { return sizeof(MccArgs<float>); }
+size_t append_args_bytes() { return sizeof(AppendArgs<float>); }
+
+void mcc_batch_collect(Ctx* c, int slot, std::vector<unsigned char>& args, std::vector<unsigned char>& app) {
+  if (!c->mcc.on || c->dom.precision != EPIC_FP32) return;
+  const epic_mcc_desc& d = c->mcc.d;
+  auto put = [](std::vector<unsigned char>& v, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    v.insert(v.end(), b, b + n);
+  };
+  if (d.electron_sp >= 0) {
+    MccArgs<float> a = mcc_args<float>(c, d.electron_sp, 0, 0);
+    a.slot = slot;
+    if (a.nu_max > 0) {
+      put(args, &a, sizeof(a));
+      if (a.enable & 4) {
+        AppendArgs<float> A = append_args<float>(c, 0);
+        A.slot = slot;
+        put(app, &A, sizeof(A));
+      }
+    }
+  }
+  if (d.ion_sp >= 0 && (d.enable[EPIC_MCC_I_ELASTIC] || d.enable[EPIC_MCC_I_CX])) {
+    MccArgs<float> a = mcc_args<float>(c, d.ion_sp, 1, 0);
+    a.slot = slot;
+    if (a.nu_max > 0) put(args, &a, sizeof(a));
+  }
+}
+
+// All collision passes of a batch in one launch (blockIdx.y = pass), then
+// every simulation's birth append in one launch (one workgroup each).
+hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int nargs, const void* d_app, int napp,
+                            const int* d_steps) {
+  if (nargs > 0) {
+    const int gx = std::max(4, std::min(num_cu * 8, (num_cu * 32 + nargs - 1) / nargs));
+    hipLaunchKernelGGL(mcc_batch_kernel<float>, dim3(gx, nargs), dim3(MCC_BLOCK), 0, st,
+                       (const MccArgs<float>*)d_args, d_steps);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  if (napp > 0) {
+    hipLaunchKernelGGL(mcc_append_batch_kernel<float>, dim3(napp), dim3(APPEND_BLOCK), 0, st,
+                       (const AppendArgs<float>*)d_app, d_steps);
     EPIC_HIP_TRY(hipGetLastError());
   }
   return hipSuccess;

@@ -1,0 +1,145 @@
+// Per-node arithmetic of the field stages, shared by the single-simulation
+// kernels (deposit.hip, field.hip, spectral.hip) and the batched ensemble
+// kernels (batch.hip), so both compute every node value with the same
+// operations in the same order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/epic2d.h"
+
+namespace epic {
+
+// Fixed-point density of node (i, j) from one species' accumulator Q
+// (species.cpp:188-225): pbc takes the sum over the node's periodic images on
+// the integers (exact); Returns the raw sum; the caller scales and applies
+// the open/reflective edge factors with den_from_q.
+__device__ __forceinline__ unsigned long long fold_q(const unsigned long long* Q, int i, int j, int nx, int ny,
+                                                     int bc) {
+  if (bc != EPIC_BC_PBC) return Q[(size_t)i * ny + j];
+  const bool iedge = (i == 0 || i == nx - 1), jedge = (j == 0 || j == ny - 1);
+  const int ia0 = iedge ? 0 : i, ia1 = iedge ? nx - 1 : i;
+  const int ja0 = jedge ? 0 : j, ja1 = jedge ? ny - 1 : j;
+  unsigned long long q = Q[(size_t)ia0 * ny + ja0];
+  if (ja1 != ja0) q += Q[(size_t)ia0 * ny + ja1];
+  if (ia1 != ia0) {
+    q += Q[(size_t)ia1 * ny + ja0];
+    if (ja1 != ja0) q += Q[(size_t)ia1 * ny + ja1];
+  }
+  return q;
+}
+
+// den = q * 2^-B * spwt / (dx dy L^2 n0), edges x1/2 and corners x1/4 for
+// open / reflective walls (species.cpp:188-190, 206-225).
+__device__ __forceinline__ double den_from_q(long long q, double scale, double norm, int i, int j, int nx, int ny,
+                                             int bc) {
+  double den = ((double)q * scale) / norm;
+  if (bc != EPIC_BC_PBC) {
+    if (i == 0 || i == nx - 1) den *= 0.5;
+    if (j == 0 || j == ny - 1) den *= 0.5;
+  }
+  return den;
+}
+
+// Right-hand side of pcgsolver (field.cpp:234-251): -rho dx^2 (L/LDe)^2,
+// zero on every boundary node.
+__device__ __forceinline__ double rhs_at(double rho, int i, int j, int nx, int ny, double dx2, double s) {
+  const bool bnd = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+  return bnd ? 0.0 : -rho * dx2 * s;
+}
+
+// The 5-point operator of the Poisson solvers.
+//   EPIC_OP_REF_GHOST: pcgsolver's dense A (field.cpp:223-232): flat index
+//     k = row*nx + col over all nodes, -4 on the diagonal, neighbours outside
+//     the block are zero (ghost ring);
+//   EPIC_OP_DIRICHLET_WALL: interior nodes (i*ny + j) with phi = 0 on the
+//     boundary (gssolver's problem, field.cpp:89-98).
+template <int OP>
+struct Stencil {
+  int nx, ny;
+  __device__ __forceinline__ bool unknown(int k) const {
+    if (OP == EPIC_OP_REF_GHOST) return true;
+    const int i = k / ny, j = k % ny;
+    return i > 0 && i < nx - 1 && j > 0 && j < ny - 1;
+  }
+  // Row k of A applied to vec(), non-zero entries in increasing column order.
+  template <typename F>
+  __device__ __forceinline__ double apply(int k, double self, F vec) const {
+    if (OP == EPIC_OP_REF_GHOST) {
+      const int row = k / nx, col = k % nx;
+      double sum = 0;
+      if (row > 0) sum += vec(k - nx);
+      if (col > 0) sum += vec(k - 1);
+      sum += -4.0 * self;
+      if (col < nx - 1) sum += vec(k + 1);
+      if (row < ny - 1) sum += vec(k + nx);
+      return sum;
+    } else {
+      double sum = 0;
+      sum += vec(k - ny);
+      sum += vec(k - 1);
+      sum += -4.0 * self;
+      sum += vec(k + 1);
+      sum += vec(k + ny);
+      return sum;
+    }
+  }
+};
+
+// r_k = b_k - (A phi)_k over the unknowns, 0 elsewhere.
+__device__ __forceinline__ double residual_at(int op, const double* b, const double* phi, int k, int nx, int ny) {
+  auto pv = [&](int m) -> double { return phi[m]; };
+  if (op == EPIC_OP_REF_GHOST) {
+    const Stencil<EPIC_OP_REF_GHOST> S{nx, ny};
+    return b[k] - S.apply(k, phi[k], pv);
+  }
+  const Stencil<EPIC_OP_DIRICHLET_WALL> S{nx, ny};
+  if (!S.unknown(k)) return 0.0;
+  return b[k] - S.apply(k, phi[k], pv);
+}
+
+// FieldSolve::CalculateEfield (field.cpp:276-331) at node (i, j): central
+// differences inside; pbc edges (phi_1 - phi_{n-2}) / 2dx; open/reflective
+// edges one-sided.
+__device__ __forceinline__ void efield_at(const double* phi, int i, int j, int nx, int ny, double dx, double dy,
+                                          int bc, double* ex_out, double* ey_out) {
+  auto ph = [&](int ii, int jj) { return phi[(size_t)ii * ny + jj]; };
+  double ex, ey;
+  if (i > 0 && i < nx - 1) {
+    ex = -(ph(i + 1, j) - ph(i - 1, j)) / (2 * dx);
+  } else if (bc == EPIC_BC_PBC) {
+    ex = -(ph(1, j) - ph(nx - 2, j)) / (2 * dx);
+  } else if (i == 0) {
+    ex = -(ph(1, j) - ph(0, j)) / dx;
+  } else {
+    ex = -(ph(nx - 1, j) - ph(nx - 2, j)) / dx;
+  }
+  if (j > 0 && j < ny - 1) {
+    ey = -(ph(i, j + 1) - ph(i, j - 1)) / (2 * dy);
+  } else if (bc == EPIC_BC_PBC) {
+    ey = -(ph(i, 1) - ph(i, ny - 2)) / (2 * dy);
+  } else if (j == 0) {
+    ey = -(ph(i, 1) - ph(i, 0)) / dy;
+  } else {
+    ey = -(ph(i, ny - 1) - ph(i, ny - 2)) / dy;
+  }
+  *ex_out = ex;
+  *ey_out = ey;
+}
+
+// Column-major p x q block of unknowns inside the node array (sine-transform
+// solver): element (a, b) is node off + a + b*ld.
+struct SpecLayout {
+  int p, q, ld, off;
+};
+
+// Index of node k inside the block, or -1 when k is not an unknown.
+__device__ __forceinline__ int spec_index(const SpecLayout& L, int k) {
+  const int t = k - L.off;
+  if (t < 0) return -1;
+  const int a = t % L.ld, b = t / L.ld;
+  if (a >= L.p || b >= L.q) return -1;
+  return a + b * L.p;
+}
+
+}  // namespace epic

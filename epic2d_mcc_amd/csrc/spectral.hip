@@ -23,12 +23,9 @@
 
 #include "device_common.hpp"
 #include "epic_internal.hpp"
+#include "node_ops.hpp"
 
 namespace epic {
-
-struct SpecLayout {
-  int p, q, ld, off;  // column-major p x q block of the node array
-};
 
 static SpecLayout spec_layout(const Ctx* c) {
   const epic_domain_desc& d = c->dom;
@@ -72,23 +69,7 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
   const int n = nx * ny;
   double acc = 0;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    double rk = 0;
-    if (OP == EPIC_OP_REF_GHOST) {
-      const int row = k / nx, col = k % nx;
-      double s = 0;
-      if (row > 0) s += phi[k - nx];
-      if (col > 0) s += phi[k - 1];
-      s += -4.0 * phi[k];
-      if (col < nx - 1) s += phi[k + 1];
-      if (row < ny - 1) s += phi[k + nx];
-      rk = b[k] - s;
-    } else {
-      const int i = k / ny, j = k % ny;
-      if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
-        double s = phi[k - ny] + phi[k - 1] - 4.0 * phi[k] + phi[k + 1] + phi[k + ny];
-        rk = b[k] - s;
-      }
-    }
+    const double rk = residual_at(OP, b, phi, k, nx, ny);
     r[k] = rk;
     acc += rk * rk;
   }

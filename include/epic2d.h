@@ -214,6 +214,71 @@ int epic_clear_error_counters(epic_ctx* ctx);
 int epic_timing_enable(epic_ctx* ctx, int on);
 int epic_timing_read(epic_ctx* ctx, double ms[EPIC_NSTAGE], int64_t calls[EPIC_NSTAGE]);
 
+/* ---- memory ------------------------------------------------------------- */
+/* Device bytes epic_create allocates for this description (MCC tables and
+ * the sine-transform solver's matrices come on first use), and what a live
+ * context holds.  No reference counterpart: the reference allocates per
+ * object on the host. */
+int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int64_t* bytes);
+int epic_device_bytes(epic_ctx* ctx, int64_t* bytes);
+
+/* ---- batched ensemble --------------------------------------------------- */
+/* No reference counterpart: the reference runs one simulation per process
+ * (main.cpp:24-36).  A batch owns member simulations (FP32, CIC, mixed
+ * sine-transform Poisson solve) on one stream and advances all of them with
+ * one launch per stage; the step can be captured in a HIP graph and
+ * replayed.  epic_batch_member returns a member's context for the
+ * per-simulation calls (load/upload, epic_initial_field, epic_mcc_setup,
+ * epic_add_emitter, diagnostics, download); members are stepped only through
+ * epic_batch_step and destroyed only through epic_batch_remove or
+ * epic_batch_destroy.  Each member keeps its own step counter (the collision
+ * random-stream key) and its own sort schedule. */
+typedef struct epic_batch epic_batch;
+
+#define EPIC_BATCH_NPUSH 5 /* push shapes: 0 (8 particles/thread, 1 wave/SIMD target), 1 (4,1), 2 (16,1), 3 (8,2), 4 (4,2) */
+#define EPIC_BATCH_NNODE 4 /* node-stage workgroup sizes: 64, 128, 256, 512 */
+
+typedef struct {
+  int refine;       /* FP32 sine-transform corrections per Poisson solve, each followed by an FP64 residual (0: 2) */
+  int use_graph;    /* 1: capture the step in a HIP graph once per layout and replay it */
+  int push_variant; /* -1: 0 until tuned; else a fixed push shape */
+  int node_block;   /* 0: 256 until tuned; else 64, 128, 256 or 512 */
+} epic_batch_opts;
+
+typedef struct {
+  int push_variant;                  /* chosen push shape */
+  int node_block;                    /* chosen node-stage workgroup size */
+  double push_ms[EPIC_BATCH_NPUSH];  /* mean push-stage time per step, per shape */
+  double node_ms[EPIC_BATCH_NNODE];  /* mean node-stage (rho, solve, E) time per step, per size */
+  int steps_used;                    /* real steps the measurement advanced every member by */
+} epic_batch_tuning;
+
+typedef struct {
+  int members, groups, tiles, segments, chunks, mcc_passes;
+  int push_variant, node_block;
+  int64_t steps, graph_replays, graph_captures;
+  int64_t device_bytes;  /* members' contexts + transform workspaces */
+  double relres_max;     /* largest final ||b - A phi|| / ||b|| of the last step */
+  double ms_total;       /* step time accumulated while timing is on */
+  double stage_ms[4];    /* eager steps: rho+solve, efield, push, collide */
+} epic_batch_stats;
+
+int epic_batch_create(const epic_batch_opts* opts, epic_batch** out);
+int epic_batch_destroy(epic_batch* b);
+const char* epic_batch_last_error(const epic_batch* b);
+int epic_batch_add(epic_batch* b, const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int* id);
+int epic_batch_member(epic_batch* b, int id, epic_ctx** ctx);
+int epic_batch_remove(epic_batch* b, int id);
+int epic_batch_step(epic_batch* b, int nsteps);
+int epic_batch_sync(epic_batch* b);
+/* Times every push shape and node workgroup size on real steps (results do
+ * not depend on them) and keeps the fastest. */
+int epic_batch_tune(epic_batch* b, int steps_per_candidate, epic_batch_tuning* out);
+int epic_batch_relres(epic_batch* b, int id, double* relres);
+int epic_batch_member_steps(epic_batch* b, int id, int64_t* steps);
+int epic_batch_get_stats(epic_batch* b, epic_batch_stats* out);
+int epic_batch_timing_enable(epic_batch* b, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,9 @@ def test_host_layer_built():
 
 @pytest.mark.parametrize("name,cls", [("epic_domain_desc", _lib.DomainDesc), ("epic_species_desc", _lib.SpeciesDesc),
                                       ("epic_emitter_desc", _lib.EmitterDesc), ("epic_mcc_desc", _lib.MccDesc),
-                                      ("epic_mcc_stats", _lib.MccStats)])
+                                      ("epic_mcc_stats", _lib.MccStats), ("epic_batch_opts", _lib.BatchOpts),
+                                      ("epic_batch_tuning", _lib.BatchTuning),
+                                      ("epic_batch_stats", _lib.BatchStats)])
 def test_struct_layout_matches_c(tmp_path, name, cls):
     fields = [f[0] for f in cls._fields_]
     src = tmp_path / "probe.c"
@@ -51,3 +53,49 @@ def test_context_creation_fails_cleanly_without_gpu_or_bad_args():
     h = ctypes.c_void_p()
     assert L.epic_create(ctypes.byref(d), sp, 1, ctypes.byref(h)) == -1
     assert L.epic_create(None, sp, 1, ctypes.byref(h)) == -1
+
+
+def test_batch_entry_points_reject_bad_args_without_device_work():
+    L = e.lib()
+    assert L.epic_batch_create(None, None) == -1
+    o = _lib.BatchOpts(refine=99, use_graph=1, push_variant=-1, node_block=0)
+    h = ctypes.c_void_p()
+    assert L.epic_batch_create(ctypes.byref(o), ctypes.byref(h)) == -1
+    o = _lib.BatchOpts(refine=2, use_graph=1, push_variant=-1, node_block=100)
+    assert L.epic_batch_create(ctypes.byref(o), ctypes.byref(h)) == -1
+    assert L.epic_batch_step(None, 1) == -1
+    assert L.epic_batch_add(None, None, None, 0, None) == -1
+
+
+def test_estimate_bytes_counts_every_creation_allocation():
+    """The admission estimate is the sum of epic_create's allocations: a
+    hand count for a small two-species FP32 description."""
+    L = e.lib()
+    d = _lib.DomainDesc(nx=33, ny=17, precision=_lib.FP32)
+    sp = (_lib.SpeciesDesc * 2)()
+    sp[0].numparticle = 1000
+    sp[1].numparticle = 10
+    sp[1].capacity = 64
+    v = ctypes.c_int64()
+    assert L.epic_estimate_bytes(ctypes.byref(d), sp, 2, ctypes.byref(v)) == 0
+    nn, ncell = 33 * 17, 32 * 16
+    ncu = 256
+    hip_ok = True
+    try:
+        buf = ctypes.create_string_buffer(256)
+        hip_ok = L.epic_device_name(buf, 256) == 0
+        if hip_ok:
+            ncu = int(buf.value.decode().split(",")[-1].split()[0])
+    except Exception:
+        hip_ok = False
+    want = 0
+    for cap in (1000, 64):
+        pad = (cap + 7) // 8 * 8
+        want += 10 * pad * 4 + 2 * pad * 4 + 2 * nn * 8 + 16
+    want += 4 * nn * 8 + 2 * nn * 4 + 2 * nn * 8 + 2 * nn * 8 + 2 * nn * 8
+    want += 16 * max(ncu, 1024) * 8 + 64 + 16
+    want += 2 * (ncell + 2) * 4 + 2 * 1000 * 4
+    blk = 1000 // 2048 + 4
+    want += blk * 4 + (max(ncell + 2, blk) // 1024 + 4) * 8
+    want += 8 * ncu * 4 * 8 + 64 + 8 * 8
+    assert v.value == want
