@@ -152,6 +152,124 @@ def copy_calibration(nbytes=1 << 30, reps=5):
             "read_only_ms": ms_read, "read_only_GB/s": nbytes / ms_read / 1e6}
 
 
+def ensemble_run(sims, steps, warmup, tune_steps=2, eager_steps=5):
+    """The batched ensemble: every member in one batch, kernel shapes tuned on
+    real steps, then `steps` timed graph replays."""
+    B = _lib.Batch()
+    ids = []
+    t0 = time.perf_counter()
+    for st in sims:
+        mid, ctx = B.add(st.domain, st.species)
+        for s, sp in enumerate(st.species):
+            ctx.load(s, sp.numparticle)
+            ctx.sort(s)
+        ctx.initial_field()
+        ids.append(mid)
+    B.sync()
+    t_setup = time.perf_counter() - t0
+    tuning = B.tune(tune_steps)
+    B.step(warmup)
+    B.sync()
+
+    def count():
+        return sum(B.members[m].count(s) for m, st in zip(ids, sims) for s in range(len(st.species)))
+
+    n0 = count()
+    B.timing(True)
+    t0 = time.perf_counter()
+    B.step(steps)
+    B.sync()
+    t1 = time.perf_counter()
+    stats = B.stats()
+    n1 = count()
+    # stage breakdown from a few eager steps (a graph replay is one event pair)
+    B.configure(use_graph=False)
+    B.timing(True)
+    B.step(eager_steps)
+    B.sync()
+    eager = B.stats()
+    stage = {k: v / eager_steps for k, v in zip(("rho_solve", "efield", "push", "collide"), eager["stage_ms"])}
+    errs = np.zeros(_lib.NERR, dtype=np.int64)
+    for m in ids:
+        errs += B.members[m].errors()
+    B.close()
+    return dict(t=t1 - t0, n0=n0, n1=n1, stats=stats, tuning=tuning, t_setup=t_setup, errors=errs.tolist(),
+                stage_ms=stage)
+
+
+def sequential_run(sims, steps, precision, solver):
+    """The same simulations one at a time, each on its own context (the
+    plain single-simulation path), `steps` timed steps each."""
+    part_steps = 0.0
+    t_tot = 0.0
+    iters = []
+    for st in sims:
+        dom = _lib.DomainDesc.from_buffer_copy(st.domain)
+        dom.precision = precision
+        dom.solver = solver
+        ctx = _lib.Context(dom, st.species)
+        for s, sp in enumerate(st.species):
+            ctx.load(s, sp.numparticle)
+            ctx.sort(s)
+        ctx.initial_field()
+        ctx.step(0)
+        ctx.sync()
+        n = sum(ctx.count(s) for s in range(len(st.species)))
+        t0 = time.perf_counter()
+        for k in range(steps):
+            iters.append(ctx.step(k + 1))
+        ctx.sync()
+        t_tot += time.perf_counter() - t0
+        part_steps += n * steps
+        ctx.close()
+    return dict(value=part_steps / t_tot, t=t_tot, iters_mean=float(np.mean(iters)) if iters else 0.0)
+
+
+def main_ensemble(args):
+    sims = workloads.ensemble_64(n=args.members)
+    r = ensemble_run(sims, args.steps, args.warmup)
+    npart = 0.5 * (r["n0"] + r["n1"])
+    value = npart * args.steps / r["t"]
+    tu = r["tuning"]
+    push_ms = r["stage_ms"]["push"]
+    achieved = 32 * npart / (push_ms / 1e3) / 1e9
+    grids = {}
+    for st in sims:
+        grids[st.info["grid"]] = grids.get(st.info["grid"], 0) + 1
+    out = {
+        "metric": "particle-steps/s, batched ensemble (whole batch, one GPU)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["t"] / args.steps,
+        "higher_is_better": True,
+        "dtype": "fp32 particles, fp64-refined Poisson",
+        "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
+        "config": {"workload": "%d periodic electron-ion sims, grids %s (cells: count), 20-200 ppc, Te 0.5-5 eV, "
+                               "two-stream +-(0-5) v_th,e on odd members, sort every 20 steps, step in a HIP graph"
+                               % (len(sims), grids),
+                   "particles": int(npart), "tuning": tu, "graph_replays": r["stats"]["graph_replays"],
+                   "relres_max": r["stats"]["relres_max"], "setup_s": r["t_setup"],
+                   "eager_stage_ms_per_step": r["stage_ms"],
+                   "device_bytes": r["stats"]["device_bytes"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "push_batch_kernel (all members' species)",
+                     "bytes_per_particle_step": 32, "push_ms_per_step": push_ms},
+        "device_errors": r["errors"],
+    }
+    if not args.no_baseline:
+        seq32 = sequential_run(sims, args.baseline_steps, _lib.FP32, _lib.SOLVER_PCG_MIXED)
+        seq64 = sequential_run(sims, args.baseline_steps, _lib.FP64, _lib.SOLVER_PCG)
+        out["sequential_fp32"] = {"value": seq32["value"], "steps_per_sim": args.baseline_steps,
+                                  "speedup_of_batch": value / seq32["value"]}
+        out["sequential_fp64_pcg"] = {"value": seq64["value"], "steps_per_sim": args.baseline_steps,
+                                      "pcg_iterations_mean": seq64["iters_mean"],
+                                      "speedup_of_batch": value / seq64["value"]}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +280,10 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="bounded512", choices=["bounded512", "ensemble64"])
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--baseline-steps", type=int, default=3)
+    ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time a plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -169,6 +291,9 @@ def main():
         print("bench.py: this build runs on one GPU (the per-step loop of one simulation); "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble64":
+        main_ensemble(args)
+        return
     solver = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
               "gs": _lib.SOLVER_GS}[args.solver]
     copy_cal = copy_calibration() if args.copy_calibration else None

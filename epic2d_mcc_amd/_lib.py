@@ -166,6 +166,7 @@ def lib():
         "epic_batch_member_steps": [P, ctypes.c_int, ctypes.POINTER(I64)],
         "epic_batch_get_stats": [P, ctypes.POINTER(BatchStats)],
         "epic_batch_timing_enable": [P, ctypes.c_int],
+        "epic_batch_configure": [P, ctypes.POINTER(BatchOpts)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -451,3 +452,7 @@ class Batch:
 
     def timing(self, on=True):
         self._c(self.L.epic_batch_timing_enable(self.h, 1 if on else 0), "batch_timing")
+
+    def configure(self, refine=0, use_graph=True, push_variant=-1, node_block=0):
+        o = BatchOpts(refine, 1 if use_graph else 0, push_variant, node_block)
+        self._c(self.L.epic_batch_configure(self.h, ctypes.byref(o)), "batch_configure")

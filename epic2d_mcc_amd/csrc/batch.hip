@@ -953,6 +953,23 @@ int epic_batch_get_stats(epic_batch* h, epic_batch_stats* out) {
   return EPIC_OK;
 }
 
+int epic_batch_configure(epic_batch* h, const epic_batch_opts* opts) {
+  if (!h || !opts) return EPIC_ERR_ARG;
+  Batch* B = &h->b;
+  if (opts->refine < 0 || opts->refine > 8 || opts->push_variant >= EPIC_BATCH_NPUSH) return EPIC_ERR_ARG;
+  if (opts->node_block != 0 && opts->node_block != 64 && opts->node_block != 128 && opts->node_block != 256 &&
+      opts->node_block != 512)
+    return EPIC_ERR_ARG;
+  B->opts.refine = opts->refine > 0 ? opts->refine : 2;
+  B->opts.use_graph = opts->use_graph;
+  if (opts->push_variant >= 0) B->push_variant = opts->push_variant;
+  if (opts->node_block) B->node_block = opts->node_block;
+  B->opts.push_variant = opts->push_variant;
+  B->opts.node_block = opts->node_block;
+  B->graph_ok = false;  // launch shapes may have changed
+  return EPIC_OK;
+}
+
 int epic_batch_timing_enable(epic_batch* h, int on) {
   if (!h) return EPIC_ERR_ARG;
   Batch* B = &h->b;

@@ -170,6 +170,12 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
           else if (px >= a.xhi) px = px - a.Lx;
           if (py < a.y0) py = py + a.Ly;
           else if (py >= a.yhi) py = py - a.Ly;
+          if (sizeof(P) == 4) {
+            // FP32: x0 - tiny + Lx can round up to xhi itself; that position
+            // is the periodic image of x0
+            if (px >= a.xhi) px = px - a.Lx;
+            if (py >= a.yhi) py = py - a.Ly;
+          }
         } else if (a.bc == EPIC_BC_OPEN) {
           if (px < a.x0 || px >= a.xhi || py < a.y0 || py >= a.yhi) {
             px = p_nan<P>();

@@ -182,6 +182,9 @@ __device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int
     if (a.bc == EPIC_BC_PBC) {
       px = px < a.x0 ? px + a.Lx : (px >= a.xhi ? px - a.Lx : px);
       py = py < a.y0 ? py + a.Ly : (py >= a.yhi ? py - a.Ly : py);
+      // x0 - tiny + Lx can round up to xhi itself: the periodic image of x0
+      px = px >= a.xhi ? px - a.Lx : px;
+      py = py >= a.yhi ? py - a.Ly : py;
     } else if (a.bc == EPIC_BC_OPEN) {
       alive = !(px < a.x0 || px >= a.xhi || py < a.y0 || py >= a.yhi);
       if (!alive) px = __builtin_nanf("");

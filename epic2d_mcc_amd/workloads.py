@@ -52,3 +52,40 @@ def mcc_256(ppc=200, precision=_lib.FP32, seed=3, precond=_lib.PRECOND_SPECTRAL)
                        precision=precision, sort_interval=20, seed=seed, capacity_factor=1.05, precond=precond)
     st.info["steps"] = 2000
     return st
+
+
+ENSEMBLE_GRIDS = (32, 64, 128, 256)
+
+
+def ensemble_member(i, rng, sort_interval=20, precision=_lib.FP32, solver=_lib.SOLVER_PCG_MIXED, seed0=4000):
+    """Member i of the batched ensemble: a periodic electron-ion plasma on a
+    grid drawn from 32^2, 64^2, 128^2, 256^2 cells, 20 to 200 particles/cell,
+    Te 0.5 to 5 eV, Ti 0.026 eV, 1e13 m^-3.  Odd members are two-stream runs:
+    the electrons are two counter-drifting halves at +-vd, vd drawn from 0 to
+    5 v_th,e (the species drift is in units of L*W = v_th,e)."""
+    g = int(rng.choice(ENSEMBLE_GRIDS))
+    ppc = int(rng.integers(20, 201))
+    te = float(rng.uniform(0.5, 5.0))
+    vd = float(rng.uniform(0.0, 5.0))
+    two_stream = i % 2 == 1
+    n = ppc * g * g
+    if two_stream:
+        specs = [config.SpeciesSpec("beam_plus", ME, n // 2, te, -1, 1.0, vd, 0.0),
+                 config.SpeciesSpec("ion", M_ION, n, 0.026, 1, 0.0),
+                 config.SpeciesSpec("beam_minus", ME, n - n // 2, te, -1, 1.0, -vd, 0.0)]
+    else:
+        specs = [config.SpeciesSpec("electron", ME, n, te, -1, 1.0),
+                 config.SpeciesSpec("ion", M_ION, n, 0.026, 1, 0.0)]
+    st = config.derive(g, g, 0.0, specs, 1e13, 0.05, bc="pbc", solver=solver, tolerance=1e-10, tol_relative=1,
+                       max_iteration=20000, precision=precision, sort_interval=sort_interval, seed=seed0 + i,
+                       precond=_lib.PRECOND_SPECTRAL)
+    st.info.update({"grid": g, "ppc": ppc, "Te": te, "two_stream": two_stream, "vd": vd if two_stream else 0.0})
+    return st
+
+
+def ensemble_64(n=64, seed=4, **kw):
+    """The batched ensemble: n independent periodic sims with uneven grids
+    and particle counts, everything drawn from `seed`."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [ensemble_member(i, rng, **kw) for i in range(n)]

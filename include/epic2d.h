@@ -277,6 +277,9 @@ int epic_batch_tune(epic_batch* b, int steps_per_candidate, epic_batch_tuning* o
 int epic_batch_relres(epic_batch* b, int id, double* relres);
 int epic_batch_member_steps(epic_batch* b, int id, int64_t* steps);
 int epic_batch_get_stats(epic_batch* b, epic_batch_stats* out);
+/* Changes the options of a live batch (push_variant -1 / node_block 0 keep
+ * the current shapes); the next step re-captures its graph. */
+int epic_batch_configure(epic_batch* b, const epic_batch_opts* opts);
 int epic_batch_timing_enable(epic_batch* b, int on);
 
 #ifdef __cplusplus
