@@ -160,6 +160,7 @@ def lib():
         "epic_batch_member": [P, ctypes.c_int, ctypes.POINTER(P)],
         "epic_batch_remove": [P, ctypes.c_int],
         "epic_batch_step": [P, ctypes.c_int],
+        "epic_batch_init": [P, ctypes.c_int],
         "epic_batch_sync": [P],
         "epic_batch_tune": [P, ctypes.c_int, ctypes.POINTER(BatchTuning)],
         "epic_batch_relres": [P, ctypes.c_int, D],
@@ -424,6 +425,10 @@ class Batch:
     def step(self, n=1):
         self._c(self.L.epic_batch_step(self.h, n), "batch_step")
 
+    def init(self, mid):
+        """The member's initial field and rewind happen in the next step."""
+        self._c(self.L.epic_batch_init(self.h, mid), "batch_init")
+
     def sync(self):
         self._c(self.L.epic_batch_sync(self.h), "batch_sync")
 
@@ -456,3 +461,118 @@ class Batch:
     def configure(self, refine=0, use_graph=True, push_variant=-1, node_block=0):
         o = BatchOpts(refine, 1 if use_graph else 0, push_variant, node_block)
         self._c(self.L.epic_batch_configure(self.h, ctypes.byref(o)), "batch_configure")
+
+
+# ---- request loop (libepic2d_host.so, include/epic2d_serve.h) ---------------
+HOST_LIB_PATH = os.path.join(HERE, "libepic2d_host.so")
+SERVE_MAX_GRIDS = 8
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("rate_hz", ctypes.c_double), ("duration_s", ctypes.c_double),
+                ("max_jobs", ctypes.c_int32), ("n_grids", ctypes.c_int32),
+                ("grid_sizes", ctypes.c_int32 * SERVE_MAX_GRIDS), ("ppc_min", ctypes.c_int32),
+                ("ppc_max", ctypes.c_int32), ("steps_min", ctypes.c_int32), ("steps_max", ctypes.c_int32),
+                ("te_min", ctypes.c_double), ("te_max", ctypes.c_double), ("ti_min", ctypes.c_double),
+                ("ti_max", ctypes.c_double), ("vd_max", ctypes.c_double), ("two_stream_fraction", ctypes.c_double),
+                ("collisional_fraction", ctypes.c_double), ("density", ctypes.c_double),
+                ("gas_density", ctypes.c_double), ("dt_coeff", ctypes.c_double), ("sort_interval", ctypes.c_int32),
+                ("budget_bytes", ctypes.c_double), ("max_members", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("tune_steps", ctypes.c_int32), ("refine", ctypes.c_int32),
+                ("mcc_template", ctypes.POINTER(MccDesc))]
+
+
+class ServeJob(ctypes.Structure):
+    _fields_ = [("index", ctypes.c_int32), ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("grid", ctypes.c_int32), ("ppc", ctypes.c_int32),
+                ("steps", ctypes.c_int32), ("collisional", ctypes.c_int32), ("two_stream", ctypes.c_int32),
+                ("te", ctypes.c_double), ("ti", ctypes.c_double), ("vd", ctypes.c_double),
+                ("particles", ctypes.c_int64), ("bytes", ctypes.c_int64), ("ke", ctypes.c_double * 3),
+                ("phi_l2", ctypes.c_double), ("collisions", ctypes.c_int64)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("jobs", ctypes.c_int32), ("jobs_done", ctypes.c_int32), ("wall_s", ctypes.c_double),
+                ("busy_s", ctypes.c_double), ("latency_p50_s", ctypes.c_double), ("latency_p99_s", ctypes.c_double),
+                ("latency_mean_s", ctypes.c_double), ("latency_max_s", ctypes.c_double),
+                ("wait_p50_s", ctypes.c_double), ("wait_p99_s", ctypes.c_double),
+                ("particle_steps", ctypes.c_double), ("throughput", ctypes.c_double),
+                ("max_members", ctypes.c_int32), ("peak_bytes", ctypes.c_double), ("batch_steps", ctypes.c_int64),
+                ("graph_captures", ctypes.c_int64), ("push_variant", ctypes.c_int32), ("node_block", ctypes.c_int32)]
+
+
+_host = None
+
+
+def host_lib():
+    global _host
+    if _host is not None:
+        return _host
+    lib()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise EpicError(f"{HOST_LIB_PATH} is not built: run __graft_entry__.build()")
+    H = ctypes.CDLL(HOST_LIB_PATH)
+    H.epic_serve_make_jobs.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeJob), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int)]
+    H.epic_serve_job_setup.argtypes = [ctypes.POINTER(ServeJob), ctypes.POINTER(ServeConfig), ctypes.c_int,
+                                       ctypes.c_int, ctypes.POINTER(DomainDesc), ctypes.POINTER(SpeciesDesc),
+                                       ctypes.POINTER(ctypes.c_int)]
+    H.epic_serve_run.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), ctypes.POINTER(ServeJob),
+                                 ctypes.c_int]
+    for f in (H.epic_serve_make_jobs, H.epic_serve_job_setup, H.epic_serve_run):
+        f.restype = ctypes.c_int
+    H.epic_serve_last_error.argtypes = []
+    H.epic_serve_last_error.restype = ctypes.c_char_p
+    _host = H
+    return H
+
+
+def serve_config(**kw):
+    """ServeConfig with the request stream of the serving workload
+    (grids 64^2..512^2, 50-400 ppc, 100-2000 steps, 20 jobs/s for 30 s,
+    200 GB budget); keyword arguments override fields."""
+    grids = kw.pop("grids", (64, 128, 256, 512))
+    c = ServeConfig(seed=5, rate_hz=20.0, duration_s=30.0, max_jobs=0, n_grids=len(grids), ppc_min=50, ppc_max=400,
+                    steps_min=100, steps_max=2000, te_min=0.5, te_max=5.0, ti_min=0.02, ti_max=0.1, vd_max=5.0,
+                    two_stream_fraction=0.5, collisional_fraction=0.5, density=1e13, gas_density=1e20,
+                    dt_coeff=0.05, sort_interval=20, budget_bytes=200e9, max_members=512, use_graph=1,
+                    tune_steps=2, refine=2)
+    for k, g in enumerate(grids):
+        c.grid_sizes[k] = g
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _serve_check(rc, what):
+    if rc != EPIC_OK:
+        raise EpicError(f"{what} failed: {ERR_NAMES.get(rc, rc)} {host_lib().epic_serve_last_error().decode()}")
+
+
+def serve_make_jobs(cfg):
+    H = host_lib()
+    n = ctypes.c_int()
+    _serve_check(H.epic_serve_make_jobs(ctypes.byref(cfg), None, 0, ctypes.byref(n)), "serve_make_jobs")
+    arr = (ServeJob * max(n.value, 1))()
+    _serve_check(H.epic_serve_make_jobs(ctypes.byref(cfg), arr, n.value, ctypes.byref(n)), "serve_make_jobs")
+    return list(arr)[:n.value]
+
+
+def serve_job_setup(job, cfg, precision=FP32, solver=SOLVER_PCG_MIXED):
+    H = host_lib()
+    dom = DomainDesc()
+    sp = (SpeciesDesc * 3)()
+    nsp = ctypes.c_int()
+    _serve_check(H.epic_serve_job_setup(ctypes.byref(job), ctypes.byref(cfg), precision, solver, ctypes.byref(dom),
+                                        sp, ctypes.byref(nsp)), "serve_job_setup")
+    return dom, [SpeciesDesc.from_buffer_copy(sp[k]) for k in range(nsp.value)]
+
+
+def serve_run(cfg):
+    """Runs the stream in real time; returns (report dict, job records)."""
+    H = host_lib()
+    jobs = serve_make_jobs(cfg)
+    out = (ServeJob * max(len(jobs), 1))()
+    rep = ServeReport()
+    _serve_check(H.epic_serve_run(ctypes.byref(cfg), ctypes.byref(rep), out, len(jobs)), "serve_run")
+    return {k: getattr(rep, k) for k, _ in ServeReport._fields_}, list(out)[:len(jobs)]

@@ -266,7 +266,7 @@ struct Member {
   epic_ctx* h = nullptr;
   int id = -1;
   int64_t steps = 0;
-  double relres = 0;
+  bool init_pending = false;  // next step is the initial field + half-step rewind
 };
 
 struct Batch {
@@ -432,7 +432,15 @@ static hipError_t build_descriptors(Batch* B, bool full) {
       SpeciesDev& P = c->sp[s];
       B->h_sps.push_back(BSpecies{P.Q, P.den, ldexp(1.0, -c->fix_bits) * P.desc.spwt, P.desc.charge * (1 / QE)});
       double kv = 0, kx = 0;
-      if (P.desc.pushed) push_f32_consts(c, s, &kv, &kx);
+      if (B->members[m].init_pending) {
+        // Rewind_species (species.cpp:229-246, every species): v -= 1/2 (q/m) E DT,
+        // positions kept (and deposited again for the first step)
+        push_f32_consts(c, s, &kv, &kx);
+        kv = -0.5 * kv;
+        kx = 0.0;
+      } else if (P.desc.pushed) {
+        push_f32_consts(c, s, &kv, &kx);
+      }
       const int seg = (int)B->h_segs.size();
       B->h_segs.push_back(make_push_f32(c, s, kv, kx));
       if (full) {
@@ -440,7 +448,7 @@ static hipError_t build_descriptors(Batch* B, bool full) {
         for (int64_t k = 0; k < nch; k++) B->h_chunks.push_back(make_int2(seg, (int)k));
       }
     }
-    mcc_batch_collect(c, (int)m, B->h_mcc, B->h_app);
+    if (!B->members[m].init_pending) mcc_batch_collect(c, (int)m, B->h_mcc, B->h_app);
   }
   B->n_mcc = (int)(B->h_mcc.size() / mcc_args_bytes());
   B->n_app = (int)(B->h_app.size() / append_args_bytes());
@@ -496,7 +504,9 @@ static hipError_t refresh(Batch* B) {
   }
   if (B->layout_dirty) {
     B->h_steps.clear();
-    for (const Member& m : B->members) B->h_steps.push_back(member_ctx(m)->step_count);
+    // the step's last kernel advances every counter; an initial-field step
+    // must leave its member's counter where it was
+    for (const Member& m : B->members) B->h_steps.push_back(member_ctx(m)->step_count - (m.init_pending ? 1 : 0));
     EPIC_HIP_TRY(upload(B->d_steps, B->h_steps, B->stream));
   }
   EPIC_HIP_TRY(B->d_part_b.ensure(std::max<size_t>(B->h_tiles.size(), 1) * sizeof(double)));
@@ -637,6 +647,10 @@ static hipError_t post_step(Batch* B) {
   for (Member& m : B->members) {
     Ctx* c = member_ctx(m);
     c->deposit_valid = true;
+    if (m.init_pending) {
+      m.init_pending = false;
+      continue;
+    }
     c->step_count++;
     m.steps++;
     if (c->dom.sort_interval > 0 && c->step_count % c->dom.sort_interval == 0) {
@@ -837,6 +851,19 @@ int epic_batch_remove(epic_batch* h, int id) {
   return EPIC_OK;
 }
 
+int epic_batch_init(epic_batch* h, int id) {
+  if (!h) return EPIC_ERR_ARG;
+  Batch* B = &h->b;
+  const int m = find_member(B, id);
+  if (m < 0) return bfail(B, EPIC_ERR_ARG, "no such member");
+  if (member_ctx(B->members[m])->step_count != 0 || B->members[m].steps != 0)
+    return bfail(B, EPIC_ERR_STATE, "member has already stepped");
+  B->members[m].init_pending = true;
+  member_ctx(B->members[m])->deposit_valid = false;
+  B->layout_dirty = true;
+  return EPIC_OK;
+}
+
 int epic_batch_step(epic_batch* h, int nsteps) {
   if (!h || nsteps < 0) return EPIC_ERR_ARG;
   Batch* B = &h->b;
@@ -855,6 +882,8 @@ int epic_batch_tune(epic_batch* h, int steps_per_candidate, epic_batch_tuning* o
   if (!h || steps_per_candidate < 1) return EPIC_ERR_ARG;
   Batch* B = &h->b;
   if (B->members.empty()) return bfail(B, EPIC_ERR_STATE, "empty batch");
+  for (const Member& m : B->members)
+    if (m.init_pending) BCK(one_step(B, true, nullptr, nullptr), "initial field");
   epic_batch_tuning t{};
   for (int v = 0; v < EPIC_BATCH_NPUSH; v++) t.push_ms[v] = -1;
   for (int v = 0; v < EPIC_BATCH_NNODE; v++) t.node_ms[v] = -1;

@@ -269,6 +269,12 @@ const char* epic_batch_last_error(const epic_batch* b);
 int epic_batch_add(epic_batch* b, const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int* id);
 int epic_batch_member(epic_batch* b, int id, epic_ctx** ctx);
 int epic_batch_remove(epic_batch* b, int id);
+/* Marks a freshly loaded member to take its initial field inside the next
+ * batched step (main.cpp:366-382: deposit, rho, solve, E, then
+ * Rewind_species for every species) instead of a push; that step does not
+ * count as one of the member's steps.  The alternative is epic_initial_field
+ * on the member's context before its first batched step. */
+int epic_batch_init(epic_batch* b, int id);
 int epic_batch_step(epic_batch* b, int nsteps);
 int epic_batch_sync(epic_batch* b);
 /* Times every push shape and node workgroup size on real steps (results do
