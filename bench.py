@@ -270,6 +270,81 @@ def main_ensemble(args):
     print(json.dumps(out))
 
 
+def fp64_sequential_jobs(cfg, jobs, tmpl, steps):
+    """The plain path: each sampled job alone on an FP64 context with the
+    reference's PCG (Jacobi, FP64), `steps` timed steps after its initial
+    field; returns particle-steps/s over the sample."""
+    from epic2d_mcc_amd import xsec
+    part_steps = 0.0
+    t_tot = 0.0
+    for j in jobs:
+        dom, sp = _lib.serve_job_setup(j, cfg, precision=_lib.FP64, solver=_lib.SOLVER_PCG)
+        ctx = _lib.Context(dom, sp)
+        for s, d in enumerate(sp):
+            ctx.load(s, d.numparticle)
+            ctx.sort(s)
+        if j.collisional:
+            m = _lib.MccDesc.from_buffer_copy(tmpl)
+            m.electron_sp, m.ion_sp = 0, 1
+            m.gas_density = cfg.gas_density
+            m.gas_mass = xsec.M_AR
+            ctx.mcc_setup(m)
+        ctx.initial_field()
+        ctx.step(0)
+        ctx.sync()
+        n = sum(ctx.count(s) for s in range(len(sp)))
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.step(k + 1)
+        ctx.sync()
+        t_tot += time.perf_counter() - t0
+        part_steps += n * steps
+        ctx.close()
+        print(f"[bench] fp64 sample job {j.index}: {n} particles", file=sys.stderr, flush=True)
+    return part_steps / t_tot
+
+
+def main_serve(args):
+    import ctypes
+    from epic2d_mcc_amd import xsec
+    tmpl = xsec.mcc_desc(xsec.synthetic_ar(), gas_density=1e21, gas_temp_K=300.0, enable=(1, 1, 1, 1, 1))
+    sc = args.serve_steps_scale
+    cfg = _lib.serve_config(duration_s=args.serve_duration, steps_min=max(1, round(100 * sc)),
+                            steps_max=max(1, round(2000 * sc)), mcc_template=ctypes.pointer(tmpl), progress_s=20.0)
+    t0 = time.perf_counter()
+    rep, jobs = _lib.serve_run(cfg)
+    t_run = time.perf_counter() - t0
+    sample = jobs[::max(1, len(jobs) // args.serve_baseline_jobs)][:args.serve_baseline_jobs]
+    base = fp64_sequential_jobs(cfg, sample, tmpl, args.baseline_steps) if not args.no_baseline else None
+    grids = {}
+    for j in jobs:
+        grids[j.grid] = grids.get(j.grid, 0) + 1
+    out = {
+        "metric": "aggregate particle-steps/s of the request loop (batched FP32 + FP64-refined) and p50/p99 "
+                  "time to completion",
+        "value": rep["throughput"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "fp32 particles, fp64-refined Poisson",
+        "data": "synthetic request stream (seeded)",
+        "config": {"workload": "Poisson arrivals %.0f/s for %.0f s (%d jobs), grids %s, %d-%d ppc, %d-%d steps, "
+                               "half collisional (argon, synthetic tables), half of the rest two-stream; "
+                               "budget %.0f GB" % (cfg.rate_hz, cfg.duration_s, rep["jobs"], grids, cfg.ppc_min,
+                                                   cfg.ppc_max, cfg.steps_min, cfg.steps_max,
+                                                   cfg.budget_bytes / 1e9),
+                   "steps_scale": sc},
+        "latency_p50_s": rep["latency_p50_s"],
+        "latency_p99_s": rep["latency_p99_s"],
+        "report": rep,
+        "run_s": t_run,
+    }
+    if base:
+        out["fp64_sequential"] = {"value": base, "jobs_sampled": len(sample), "steps_per_job": args.baseline_steps,
+                                  "speedup_of_loop": rep["throughput"] / base}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,7 +355,11 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="bounded512", choices=["bounded512", "ensemble64"])
+    ap.add_argument("--workload", default="bounded512", choices=["bounded512", "ensemble64", "serve"])
+    ap.add_argument("--serve-steps-scale", type=float, default=1.0,
+                    help="multiplies the jobs' 100-2000 step range")
+    ap.add_argument("--serve-duration", type=float, default=30.0)
+    ap.add_argument("--serve-baseline-jobs", type=int, default=12)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--baseline-steps", type=int, default=3)
     ap.add_argument("--no-baseline", action="store_true")
@@ -293,6 +372,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble64":
         main_ensemble(args)
+        return
+    if args.workload == "serve":
+        main_serve(args)
         return
     solver = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
               "gs": _lib.SOLVER_GS}[args.solver]

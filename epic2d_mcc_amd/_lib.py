@@ -479,7 +479,7 @@ class ServeConfig(ctypes.Structure):
                 ("gas_density", ctypes.c_double), ("dt_coeff", ctypes.c_double), ("sort_interval", ctypes.c_int32),
                 ("budget_bytes", ctypes.c_double), ("max_members", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("tune_steps", ctypes.c_int32), ("refine", ctypes.c_int32),
-                ("mcc_template", ctypes.POINTER(MccDesc))]
+                ("mcc_template", ctypes.POINTER(MccDesc)), ("progress_s", ctypes.c_double)]
 
 
 class ServeJob(ctypes.Structure):
@@ -534,7 +534,7 @@ def serve_config(**kw):
     grids = kw.pop("grids", (64, 128, 256, 512))
     c = ServeConfig(seed=5, rate_hz=20.0, duration_s=30.0, max_jobs=0, n_grids=len(grids), ppc_min=50, ppc_max=400,
                     steps_min=100, steps_max=2000, te_min=0.5, te_max=5.0, ti_min=0.02, ti_max=0.1, vd_max=5.0,
-                    two_stream_fraction=0.5, collisional_fraction=0.5, density=1e13, gas_density=1e20,
+                    two_stream_fraction=0.5, collisional_fraction=0.5, density=1e13, gas_density=1e21,
                     dt_coeff=0.05, sort_interval=20, budget_bytes=200e9, max_members=512, use_graph=1,
                     tune_steps=2, refine=2)
     for k, g in enumerate(grids):

@@ -311,7 +311,7 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
   std::vector<Active> active;
   int next = 0, done = 0, max_members = 0;
   int64_t used = 0;
-  double peak = 0, busy = 0;
+  double peak = 0, busy = 0, last_print = 0;
   const auto t0 = Clock::now();
   auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   while (done < n) {
@@ -359,6 +359,13 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
     }
     (void)any_done;
     busy += now() - ts;
+    if (c->progress_s > 0 && now() - last_print >= c->progress_s) {
+      last_print = now();
+      std::fprintf(stderr, "[serve] t=%.1fs arrived %d admitted %d done %d active %d queued %d used %.2f GB\n",
+                   last_print, next, next - (int)queue.size(), done, (int)active.size(), (int)queue.size(),
+                   (double)used / 1e9);
+      std::fflush(stderr);
+    }
   }
   epic_batch_sync(B);
   epic_batch_stats bs{};

@@ -55,6 +55,7 @@ typedef struct {
   int32_t tune_steps;       /* > 0: tune kernel shapes on the first jobs before the clock starts */
   int32_t refine;           /* Poisson corrections per step (0: 2) */
   const epic_mcc_desc* mcc_template; /* tables for collisional jobs (species indices are set per job) */
+  double progress_s;        /* > 0: a progress line on stderr at this interval */
 } epic_serve_config;
 
 typedef struct {

@@ -31,12 +31,14 @@ def test_host_layer_built():
                                       ("epic_emitter_desc", _lib.EmitterDesc), ("epic_mcc_desc", _lib.MccDesc),
                                       ("epic_mcc_stats", _lib.MccStats), ("epic_batch_opts", _lib.BatchOpts),
                                       ("epic_batch_tuning", _lib.BatchTuning),
-                                      ("epic_batch_stats", _lib.BatchStats)])
+                                      ("epic_batch_stats", _lib.BatchStats),
+                                      ("epic_serve_config", _lib.ServeConfig), ("epic_serve_job", _lib.ServeJob),
+                                      ("epic_serve_report", _lib.ServeReport)])
 def test_struct_layout_matches_c(tmp_path, name, cls):
     fields = [f[0] for f in cls._fields_]
     src = tmp_path / "probe.c"
     body = "".join(f'printf("%zu ", offsetof({name}, {f}));' for f in fields)
-    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "epic2d.h"\n'
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "epic2d_serve.h"\n'
                    f'int main(void){{printf("%zu ", sizeof({name}));{body}return 0;}}\n')
     exe = tmp_path / "probe"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -99,3 +101,12 @@ def test_estimate_bytes_counts_every_creation_allocation():
     want += blk * 4 + (max(ncell + 2, blk) // 1024 + 4) * 8
     want += 8 * ncu * 4 * 8 + 64 + 8 * 8
     assert v.value == want
+
+
+def test_host_library_exports_serving_entry_points():
+    import re
+    text = open(os.path.join(ROOT, "include", "epic2d_serve.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(epic_\w+)\s*\(", text, re.M)))
+    assert len(names) == 4
+    H = _lib.host_lib()
+    assert [n for n in names if not hasattr(H, n)] == []
