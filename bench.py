@@ -27,6 +27,12 @@ from epic2d_mcc_amd import _lib, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes the push kernel moves per particle, from rocprofv3 --pmc on this
+# workload (profiles/push_f32_kernel_bounded512_pmc.md): FETCH_SIZE x 2 +
+# WRITE_SIZE x 1 per dispatch (the factors a plain 1 GiB device copy reads
+# back as), divided by the 26,214,400 particles of a dispatch.
+PUSH_TRAFFIC_BYTES_PER_PARTICLE = (214624.42 * 2 + 430505.95) * 1024 / 26214400
+
 
 def gpu_run(st, steps, warmup, solver_name):
     ctx = _lib.Context(st.domain, st.species)
@@ -139,17 +145,9 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     moved = 2 * nbytes
-    # a read-only pass of known size (reduction) for the read counter
-    e0.record()
-    for _ in range(reps):
-        tot = src.sum()
-    e1.record()
-    torch.cuda.synchronize()
-    ms_read = e0.elapsed_time(e1) / reps
-    del src, dst, tot
+    del src, dst
     torch.cuda.empty_cache()
-    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": moved / ms / 1e6,
-            "read_only_ms": ms_read, "read_only_GB/s": nbytes / ms_read / 1e6}
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": moved / ms / 1e6}
 
 
 def ensemble_run(sims, steps, warmup, tune_steps=2, eager_steps=5):
@@ -411,8 +409,9 @@ def main():
                    "poisson_iterations_mean": float(np.mean(r["iters"])) if r["iters"] else 0,
                    "stage_ms_per_step": stage},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "push_f32_kernel<true,8,1> (CIC gather + leapfrog push + walls + deposit)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": PUSH_TRAFFIC_BYTES_PER_PARTICLE * parts_per_launch,
+                     "kernel": "push_f32_kernel<true,4,1> (CIC gather + leapfrog push + walls + deposit)",
                      "bytes_per_particle_step": bytes_per_ps, "avg_launch_ms": avg_launch_s * 1e3,
                      "particles_per_launch": parts_per_launch},
         "device_errors": r["errors"],

@@ -261,7 +261,9 @@ struct LaneRun {
 };
 __device__ __forceinline__ LaneRun lane_run(int key) {
   const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1, 64);
+  // the previous lane's key by a DPP whole-wave shift (wave_shr:1); lane 0
+  // receives a value different from its own key
+  const int prev = __builtin_amdgcn_update_dpp(~key, key, 0x138, 0xf, 0xf, false);
   const bool h = lane == 0 || key != prev;
   const unsigned long long heads = __ballot(h);
   const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
@@ -272,17 +274,28 @@ __device__ __forceinline__ LaneRun lane_run(int key) {
   return r;
 }
 
+// Particles per thread per pass of the sort kernels: the loads of SORT_U
+// particles are in flight together.
+constexpr int SORT_U = 4;
+
 template <typename P>
 __global__ void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0, P y0, P dx, P dy,
                                  int ncx, int ncy) {
   const int64_t n = *d_n;
   const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += stride) {
-    const int64_t p = base + threadIdx.x;
-    const int key = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
-    const LaneRun r = lane_run(key);
-    if (r.head == lane && key >= 0) atomicAdd(&count[key], r.len);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
+    int key[SORT_U];
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const LaneRun r = lane_run(key[u]);
+      if (r.head == lane && key[u] >= 0) atomicAdd(&count[key[u]], r.len);
+    }
   }
 }
 
@@ -295,15 +308,23 @@ __global__ void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, con
   const int64_t n = *d_n;
   const int dead = ncx * ncy;
   const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n; base += stride) {
-    const int64_t p = base + threadIdx.x;
-    const int key = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
-    const LaneRun r = lane_run(key);
-    int b = 0;
-    if (r.head == lane && key >= 0 && key != dead) b = atomicAdd(&fill[key], r.len);
-    b = __shfl(b, r.head, 64);
-    if (key >= 0 && key != dead) perm[start[key] + b + (lane - r.head)] = (uint32_t)p;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
+    int key[SORT_U];
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+      const LaneRun r = lane_run(key[u]);
+      int b = 0;
+      if (r.head == lane && key[u] >= 0 && key[u] != dead) b = atomicAdd(&fill[key[u]], r.len);
+      b = __shfl(b, r.head, 64);
+      if (key[u] >= 0 && key[u] != dead) perm[start[key[u]] + b + (lane - r.head)] = (uint32_t)p;
+    }
   }
 }
 
@@ -388,9 +409,30 @@ __global__ void sort_gather_kernel(const uint32_t* perm, const int* total, const
                                    const P* vy, const P* vz, const uint32_t* id, P* x2, P* y2, P* vx2, P* vy2,
                                    P* vz2, uint32_t* id2) {
   const int64_t n = *total;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t p = perm[i];
-    x2[i] = x[p]; y2[i] = y[p]; vx2[i] = vx[p]; vy2[i] = vy[p]; vz2[i] = vz[p]; id2[i] = id[p];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
+    uint32_t p[SORT_U];
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+      p[u] = i < n ? perm[i] : 0u;
+    }
+    P a[SORT_U], b[SORT_U], c[SORT_U], d[SORT_U], e[SORT_U];
+    uint32_t f[SORT_U];
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+      if (i < n) {
+        a[u] = x[p[u]]; b[u] = y[p[u]]; c[u] = vx[p[u]]; d[u] = vy[p[u]]; e[u] = vz[p[u]]; f[u] = id[p[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) {
+      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+      if (i < n) {
+        x2[i] = a[u]; y2[i] = b[u]; vx2[i] = c[u]; vy2[i] = d[u]; vz2[i] = e[u]; id2[i] = f[u];
+      }
+    }
   }
 }
 
@@ -406,7 +448,7 @@ static hipError_t sort_t(Ctx* c, int s) {
   const int64_t n = S.n_host;
   if (n == 0) return hipSuccess;
   const P x0 = (P)d.x0, y0 = (P)d.y0, dx = (P)d.dx, dy = (P)d.dy;
-  const int g = std::max(1, (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->num_cu * 16));
+  const int g = std::max(1, (int)std::min<int64_t>((n + 256 * SORT_U - 1) / (256 * SORT_U), (int64_t)c->num_cu * 16));
   const int gc = std::max(1, std::min((ncell + 2 + 255) / 256, 4096));
   hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_count, (int64_t)ncell + 2);
   hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_fill, (int64_t)ncell + 2);

@@ -51,6 +51,7 @@ struct BSim {
   double* efy;
   double* b;
   float2* E2;
+  float4* Ec;      // FP32 gather table (cell_coeffs)
   float* X;        // this member's p*q block of the group's transform workspace
   const float* G;  // group's spectral scaling c / (lam_k + mu_l)
   SpecLayout L;
@@ -169,6 +170,18 @@ __global__ void befield_kernel(const BSim* sims, const BSpecies* sps, const BTil
   for (int s = S.sp0; s < S.sp0 + S.nsp; s++) sps[s].Q[k] = 0ull;
 }
 
+// The FP32 gather table of every member (the node tile's cells; the last
+// node row and column own no cell).
+__global__ void bcell_kernel(const BSim* sims, const BTile* tiles) {
+  const BTile t = tiles[blockIdx.x];
+  const BSim& S = sims[t.sim];
+  const int k = t.k0 + threadIdx.x;
+  if (k >= S.nx * S.ny) return;
+  const int i = k / S.ny, j = k % S.ny;
+  if (i >= S.nx - 1 || j >= S.ny - 1) return;
+  cell_coeffs(S.E2, i, j, S.ny, S.Ec + 2 * (size_t)(i * (S.ny - 1) + j));
+}
+
 __global__ void bstep_kernel(int* steps, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) steps[i] += 1;
 }
@@ -186,21 +199,45 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const Pus
   constexpr int CH = PART_BLOCK * PPT;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned long long* wl = lds + wid * WAVE_WIN;
-  int64_t cb, ce, cs;
-  xcd_chunk_range(nchunks, &cb, &ce, &cs);
-  for (int64_t c = cb; c < ce; c += cs) {
-    const int2 ch = chunks[c];
-    const PushF32 a = segs[ch.x];
-    const int64_t n = *a.d_n;
-    const int64_t wb = (int64_t)ch.y * CH + (int64_t)wid * CHW;
-    if (wb < n) push_f32_chunk<true, PPT>(a, wb, n, lane, wl);
+  // round-robin walk, software-pipelined as the single-simulation kernel:
+  // the next item's particles (full chunks) are loaded before this item's
+  // deposit
+  int c = blockIdx.x;
+  if (c >= nchunks) return;
+  int2 ch = chunks[c];
+  PushF32 a = segs[ch.x];
+  int64_t n = *a.d_n;
+  int64_t wb = (int64_t)ch.y * CH + (int64_t)wid * CHW;
+  ChunkRegs<PPT> r;
+  if (wb < n) load_chunk<PPT>(a, wb, n, lane, r);
+  for (; c < nchunks; c += gridDim.x) {
+    ChunkDep<PPT> dp;
+    const bool active = wb < n;
+    if (active) push_chunk_regs<true, PPT>(a, wb, n, lane, r, dp);
+    const int cn = c + (int)gridDim.x;
+    PushF32 an = a;
+    int64_t nn = 0, wbn = 0;
+    bool full = false;
+    if (cn < nchunks) {
+      const int2 chn = chunks[cn];
+      an = segs[chn.x];
+      nn = *an.d_n;
+      wbn = (int64_t)chn.y * CH + (int64_t)wid * CHW;
+      full = wbn + 64 * PPT <= nn;
+      if (full) load_chunk_full<PPT>(an, wbn, lane, r);
+    }
+    if (active) deposit_chunk<PPT>(a, lane, wl, dp);
+    if (cn < nchunks && !full && wbn < nn) load_chunk<PPT>(an, wbn, nn, lane, r);
+    a = an;
+    n = nn;
+    wb = wbn;
   }
 }
 
 struct PushShape {
   int ppt, waves;
 };
-static const PushShape kPushShapes[EPIC_BATCH_NPUSH] = {{8, 1}, {4, 1}, {16, 1}, {8, 2}, {4, 2}};
+static const PushShape kPushShapes[EPIC_BATCH_NPUSH] = {{4, 1}, {8, 1}, {16, 1}, {8, 2}, {4, 2}};
 static const int kNodeBlocks[EPIC_BATCH_NNODE] = {64, 128, 256, 512};
 
 template <int PPT, int W>
@@ -220,11 +257,11 @@ static hipError_t launch_push_batch(int variant, hipStream_t st, int num_cu, con
                                     int nchunks) {
   if (nchunks <= 0) return hipSuccess;
   switch (variant) {
-    case 1: return launch_push_batch_t<4, 1>(st, num_cu, segs, chunks, nchunks);
+    case 1: return launch_push_batch_t<8, 1>(st, num_cu, segs, chunks, nchunks);
     case 2: return launch_push_batch_t<16, 1>(st, num_cu, segs, chunks, nchunks);
     case 3: return launch_push_batch_t<8, 2>(st, num_cu, segs, chunks, nchunks);
     case 4: return launch_push_batch_t<4, 2>(st, num_cu, segs, chunks, nchunks);
-    default: return launch_push_batch_t<8, 1>(st, num_cu, segs, chunks, nchunks);
+    default: return launch_push_batch_t<4, 1>(st, num_cu, segs, chunks, nchunks);
   }
 }
 
@@ -408,6 +445,7 @@ static hipError_t build_descriptors(Batch* B, bool full) {
     BSim S{};
     S.rho = c->rho; S.phi = c->phi; S.efx = c->efx; S.efy = c->efy; S.b = c->b64;
     S.E2 = (float2*)c->E2;
+    S.Ec = (float4*)c->Ec;
     S.X = (float*)g.X.p + (size_t)sig[m] * g.p * g.q;
     S.G = g.G;
     S.L = layout_of(d);
@@ -583,6 +621,8 @@ static hipError_t enqueue_step(Batch* B, bool events) {
     EPIC_HIP_TRY(hipGetLastError());
     if (ev) rec(B, 1);
     hipLaunchKernelGGL(befield_kernel, dim3(nt), dim3(NB), 0, B->stream, sims, sps, tiles);
+    EPIC_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bcell_kernel, dim3(nt), dim3(NB), 0, B->stream, sims, tiles);
     EPIC_HIP_TRY(hipGetLastError());
   }
   if (ev) rec(B, 2);
@@ -818,6 +858,7 @@ int epic_batch_add(epic_batch* h, const epic_domain_desc* dom, const epic_specie
     return bfail(B, EPIC_ERR_ARG, "batch members solve with the mixed-precision sine-transform solver");
   if (dom->poisson_op == EPIC_OP_DIRICHLET_WALL && (dom->nx < 4 || dom->ny < 4))
     return bfail(B, EPIC_ERR_ARG, "grid too small");
+  if (dom->nx > 65535 || dom->ny > 65535) return bfail(B, EPIC_ERR_ARG, "grid sides up to 65535 nodes");
   epic_ctx* m = nullptr;
   const int rc = ctx_create(dom, sp, nsp, B->stream, &m);
   if (rc != EPIC_OK) return bfail(B, rc, "member creation failed");

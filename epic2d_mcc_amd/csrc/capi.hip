@@ -99,7 +99,7 @@ static void free_all(Ctx* c) {
     for (void* p : ps)
       if (p) hipFree(p);
   }
-  void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
+  void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->Ec, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
                 c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
                 c->mcc.event_count, c->sp_S, c->sp_T, c->sp_lam};
@@ -129,7 +129,9 @@ size_t ctx_bytes_estimate(const epic_domain_desc* dom, const epic_species_desc* 
     const size_t cap_pad = (size_t)((cap + 7) / 8 * 8);
     tot += 10 * b(cap_pad * ps, 1) + 2 * b(cap_pad, 4) + 2 * b(nn, 8) + b(2, 8);
   }
-  tot += 4 * b(nn, 8) + b(2 * nn * ps, 1) + 2 * b(nn, 8) + b(2 * nn, 8) + 2 * b(nn, 8);
+  tot += 4 * b(nn, 8) + b(2 * nn * ps, 1);
+  if (dom->precision == EPIC_FP32) tot += b((size_t)(dom->nx - 1) * (dom->ny - 1) * 8 * sizeof(float), 1);
+  tot += 2 * b(nn, 8) + b(2 * nn, 8) + 2 * b(nn, 8);
   tot += b((size_t)16 * std::max(num_cu, 1024), 8) + b(8, 8) + b(4, 4);
   const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
   tot += 2 * b(ncell + 2, 4) + b(2 * (size_t)maxcap, 4);
@@ -193,6 +195,8 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   CKC(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
   c->num_cu = prop.multiProcessorCount;
   if (const char* v = getenv("EPIC_PUSH_VARIANT")) c->push_variant = atoi(v);
+  if (const char* v = getenv("EPIC_PUSH_GRID")) c->push_grid_per_cu = atoi(v);
+  if (const char* v = getenv("EPIC_PUSH_MAP")) c->push_map = atoi(v);
   c->max_solver_blocks = c->num_cu;
   if (ext) {
     c->stream = ext;
@@ -232,6 +236,8 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   CKC(dalloc_c(c, &c->efx, nn), "efx");
   CKC(dalloc_c(c, &c->efy, nn), "efy");
   CKC(dalloc_c(c, (char**)&c->E2, 2 * nn * ps), "E2");
+  if (dom->precision == EPIC_FP32)
+    CKC(dalloc_c(c, (char**)&c->Ec, (size_t)(dom->nx - 1) * (dom->ny - 1) * 8 * sizeof(float)), "Ec");
   CKC(dalloc_c(c, &c->b64, nn), "b");
   CKC(dalloc_c(c, &c->r64, nn), "r");
   CKC(dalloc_c(c, (double**)&c->ws_v, 2 * nn), "v");

@@ -77,6 +77,25 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// Wave-uniform min / max of an int over the 64 lanes with DPP moves: an
+// inclusive scan inside each 16-lane row (row_shr 1, 2, 4, 8), the row
+// results carried across rows (row_bcast 15, 31), lane 63 read back.  Lanes
+// whose DPP source is outside the row or masked off contribute the identity.
+template <bool MIN>
+__device__ __forceinline__ int wave_ext_dpp(int v) {
+  const int id = MIN ? 0x7fffffff : (int)0x80000000;
+  auto op = [](int a, int b) { return MIN ? min(a, b) : max(a, b); };
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_dpp(int v) { return wave_ext_dpp<true>(v); }
+__device__ __forceinline__ int wave_max_dpp(int v) { return wave_ext_dpp<false>(v); }
+
 // Block sum of NV doubles; result valid in every thread.  `sh` holds
 // NV * (blockDim/64) doubles.  Deterministic (fixed order).
 template <int NV>

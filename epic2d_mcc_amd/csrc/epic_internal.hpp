@@ -87,6 +87,7 @@ struct Ctx {
   double* efx = nullptr;
   double* efy = nullptr;
   void* E2 = nullptr;  // interleaved (ex, ey) per node in particle precision
+  void* Ec = nullptr;  // FP32: per-cell bilinear coefficients of (ex, ey), 8 floats
 
   // Poisson workspace
   double* b64 = nullptr;
@@ -127,6 +128,8 @@ struct Ctx {
   MccState mcc;
 
   int push_variant = 0;        // FP32 push kernel shape (see particles.hip)
+  int push_grid_per_cu = 0;    // FP32 push workgroups per CU (0: co-resident maximum)
+  int push_map = 1;            // FP32 push chunk walk: 0 per-XCD ranges, 1 round-robin
   bool deposit_valid = false;  // Q holds the deposit of the current positions
   int step_count = 0;
 

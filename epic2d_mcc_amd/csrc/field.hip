@@ -24,6 +24,7 @@
 #include "device_common.hpp"
 #include "epic_internal.hpp"
 #include "node_ops.hpp"
+#include "push_f32.hpp"
 
 namespace epic {
 
@@ -358,6 +359,23 @@ __global__ void efield_kernel(const double* phi, double* efx, double* efy, P* E2
   }
 }
 
+// FP32 gather table: bilinear coefficients of every cell from E2.
+__global__ void cell_table_kernel(const float2* E2, float4* Ec, int nx, int ny) {
+  const int nc = (nx - 1) * (ny - 1);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    cell_coeffs(E2, c / (ny - 1), c % (ny - 1), ny, Ec + 2 * (size_t)c);
+}
+
+static hipError_t launch_cell_table(Ctx* c) {
+  const epic_domain_desc& d = c->dom;
+  if (d.precision != EPIC_FP32 || !c->Ec) return hipSuccess;
+  const int nc = (d.nx - 1) * (d.ny - 1);
+  const int grid = std::min((nc + 255) / 256, 4096);
+  hipLaunchKernelGGL(cell_table_kernel, dim3(grid), dim3(256), 0, c->stream, (const float2*)c->E2, (float4*)c->Ec,
+                     d.nx, d.ny);
+  return hipGetLastError();
+}
+
 hipError_t launch_efield(Ctx* c) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
@@ -368,7 +386,8 @@ hipError_t launch_efield(Ctx* c) {
   else
     hipLaunchKernelGGL(efield_kernel<float>, dim3(grid), dim3(256), 0, c->stream, c->phi, c->efx, c->efy,
                        (float*)c->E2, d.nx, d.ny, d.dx, d.dy, d.bc);
-  return hipGetLastError();
+  EPIC_HIP_TRY(hipGetLastError());
+  return launch_cell_table(c);
 }
 
 template <typename P>
@@ -386,7 +405,8 @@ hipError_t launch_pack_E2(Ctx* c) {
     hipLaunchKernelGGL(pack_E2_kernel<double>, dim3(grid), dim3(256), 0, c->stream, c->efx, c->efy, (double*)c->E2, n);
   else
     hipLaunchKernelGGL(pack_E2_kernel<float>, dim3(grid), dim3(256), 0, c->stream, c->efx, c->efy, (float*)c->E2, n);
-  return hipGetLastError();
+  EPIC_HIP_TRY(hipGetLastError());
+  return launch_cell_table(c);
 }
 
 // ---------------------------------------------------------------------------

@@ -298,8 +298,11 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
   }
 }
 
+// rr = 0: each XCD walks its own contiguous range of chunks; rr = 1: all
+// workgroups walk the chunks round-robin (the whole device streams through
+// one region at a time).
 template <bool DEP, int PPT, int WAVES>
-__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a) {
+__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, int rr) {
   __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
   constexpr int CHW = 64 * PPT;
   constexpr int CH = PART_BLOCK * PPT;
@@ -308,9 +311,28 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a) 
   const int64_t n = *a.d_n;
   const int64_t nchunks = (n + CH - 1) / CH;
   int64_t cb, ce_, cs;
-  xcd_chunk_range(nchunks, &cb, &ce_, &cs);
-  for (int64_t chunk = cb; chunk < ce_; chunk += cs)
-    push_f32_chunk<DEP, PPT>(a, chunk * CH + (int64_t)wid * CHW, n, lane, wl);
+  if (rr) {
+    cb = blockIdx.x;
+    ce_ = nchunks;
+    cs = gridDim.x;
+  } else {
+    xcd_chunk_range(nchunks, &cb, &ce_, &cs);
+  }
+  // software pipeline: the next chunk's loads go out (into the registers the
+  // stores just freed) before this chunk's deposit
+  // (full chunks only: a partial last chunk is loaded after the deposit)
+  ChunkRegs<PPT> r;
+  if (cb < ce_) load_chunk<PPT>(a, cb * CH + (int64_t)wid * CHW, n, lane, r);
+  for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
+    ChunkDep<PPT> dp;
+    push_chunk_regs<DEP, PPT>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp);
+    const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
+    const bool more = chunk + cs < ce_;
+    const bool full = more && nb + 64 * PPT <= n;
+    if (full) load_chunk_full<PPT>(a, nb, lane, r);
+    if (DEP) deposit_chunk<PPT>(a, lane, wl, dp);
+    if (more && !full) load_chunk<PPT>(a, nb, n, lane, r);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -363,6 +385,14 @@ static int particle_grid(Ctx* c, K kernel) {
   return c->num_cu * per_cu;
 }
 
+// Persistent grid of the FP32 push: push_grid_per_cu workgroups per CU when
+// set, else as many as are co-resident.
+template <typename K>
+static int push_grid(Ctx* c, K kernel) {
+  if (c->push_grid_per_cu > 0) return c->num_cu * c->push_grid_per_cu;
+  return particle_grid(c, kernel);
+}
+
 template <typename P, int PPT>
 static hipError_t launch_particles_t(Ctx* c, int s, int mode, bool dep, double c1, double c2, double c3) {
   PartArgs<P> a = make_args<P>(c, s);
@@ -406,7 +436,7 @@ PushF32 make_push_f32(Ctx* c, int s, double kv, double kx) {
   PushF32 a{};
   a.x = (float*)S.x; a.y = (float*)S.y; a.vx = (float*)S.vx; a.vy = (float*)S.vy;
   a.d_n = S.d_n;
-  a.E = (const float2*)c->E2;
+  a.Ec = (const float4*)c->Ec;
   a.x0 = (float)d.x0; a.y0 = (float)d.y0;
   a.idx = (float)(1.0 / d.dx); a.idy = (float)(1.0 / d.dy);
   a.xhi = (float)(d.x0 + Lx); a.yhi = (float)(d.y0 + Ly);
@@ -437,27 +467,27 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   push_consts(c, s, sub_cycle, false, &c1, &c2, &c3);
   const epic_domain_desc& d = c->dom;
   const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
-  if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole) {
+  if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
     const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2);
     // kernel shape: particles per thread x occupancy target (waves per SIMD);
     // EPIC_PUSH_VARIANT at context creation
     switch (c->push_variant) {
-#define EPIC_V(ID, PPT, WV)                                                               \
-  case ID:                                                                                \
-    if (deposit) {                                                                        \
-      auto k = push_f32_kernel<true, PPT, WV>;                                            \
-      hipLaunchKernelGGL(k, dim3(particle_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a); \
-    } else {                                                                              \
-      auto k = push_f32_kernel<false, PPT, WV>;                                           \
-      hipLaunchKernelGGL(k, dim3(particle_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a); \
-    }                                                                                     \
+#define EPIC_V(ID, PPT, WV)                                                                        \
+  case ID:                                                                                         \
+    if (deposit) {                                                                                 \
+      auto k = push_f32_kernel<true, PPT, WV>;                                                     \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map); \
+    } else {                                                                                       \
+      auto k = push_f32_kernel<false, PPT, WV>;                                                    \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map); \
+    }                                                                                              \
     break;
       EPIC_V(1, 8, 5)
-      EPIC_V(2, 4, 1)
+      EPIC_V(2, 8, 1)
       EPIC_V(3, 4, 8)
       EPIC_V(4, 16, 1)
       default:
-      EPIC_V(0, 8, 1)
+      EPIC_V(0, 4, 1)
 #undef EPIC_V
     }
     return hipGetLastError();

@@ -112,7 +112,8 @@ constexpr int WAVE_WIN = 512;
 // particle needs:
 //   logical coordinates by multiplication with 1/dx (no division),
 //   v += (c1 c2) E and x += (c3 c2) v as fused multiply-adds,
-//   bilinear gather as two lerps per component,
+//   bilinear gather from per-cell coefficients (cell_coeffs): one 32-byte
+//   fetch when the cell changes, three FMAs per component,
 //   integer CIC weights from 14-bit offsets dI = rint(di 2^14):
 //     w00 = (2^14-dI)(2^14-dJ), w10 = dI(2^14-dJ), w01 = (2^14-dI)dJ, w11 = dI dJ
 //   which sum to exactly 2^28 per particle (FIX_BITS_FP32).
@@ -124,7 +125,7 @@ struct PushF32 {
   float* vx;
   float* vy;
   const int64_t* d_n;
-  const float2* E;
+  const float4* Ec;  // per cell c = i*(ny-1) + j: bilinear coefficients of ex, then of ey
   float x0, y0, idx, idy, xhi, yhi, Lx, Ly;
   float kv, kx;  // c1*c2, c3*c2
   int nx, ny, bc;
@@ -132,52 +133,123 @@ struct PushF32 {
   unsigned long long* errc;
 };
 
-__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
+// Bilinear interpolation of node values e00, e10 (i+1), e01 (j+1), e11 over
+// a cell as a0 + a1 di + dj (a2 + a3 di).
+__device__ __forceinline__ float4 bilinear_coeffs(float e00, float e10, float e01, float e11) {
+  const float d10 = e10 - e00, d01 = e01 - e00;
+  return make_float4(e00, d10, d01, (e11 - e10) - d01);
+}
 
-// One wavefront's chunk of 64*PPT particles starting at wb (n = particles in
-// the array): push, walls, store, and (DEP) deposit of the new positions
-// through the wavefront's LDS window wl.  Shared by the single-simulation
-// kernel and the batched one, so both do identical arithmetic.
-template <bool DEP, int PPT>
-__device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int64_t n, int lane,
-                                               unsigned long long* wl) {
-  const int ny = a.ny;
-  const float fmax_i = (float)(a.nx - 1), fmax_j = (float)(a.ny - 1);
+// Cell table of the FP32 gather from the interleaved node field E2.
+__device__ __forceinline__ void cell_coeffs(const float2* E2, int i, int j, int ny, float4* out2) {
+  const float2 e00 = E2[(size_t)i * ny + j], e10 = E2[(size_t)(i + 1) * ny + j];
+  const float2 e01 = E2[(size_t)i * ny + j + 1], e11 = E2[(size_t)(i + 1) * ny + j + 1];
+  out2[0] = bilinear_coeffs(e00.x, e10.x, e01.x, e11.x);
+  out2[1] = bilinear_coeffs(e00.y, e10.y, e01.y, e11.y);
+}
+
+constexpr unsigned NO_DEPOSIT = 0xffffffffu;
+
+// The particle registers of one wavefront chunk (64*PPT particles).
+template <int PPT>
+struct ChunkRegs {
   float x[PPT], y[PPT], vx[PPT], vy[PPT];
-  ld_run<float, PPT>(a.x, wb, lane, n, x);
-  ld_run<float, PPT>(a.y, wb, lane, n, y);
-  ld_run<float, PPT>(a.vx, wb, lane, n, vx);
-  ld_run<float, PPT>(a.vy, wb, lane, n, vy);
-  unsigned live = 0;
-  int imin = 1 << 30, imax = -1, jmin = 1 << 30, jmax = -1;
-  int ckey = -1;
-  float c00x = 0, c10x = 0, c01x = 0, c11x = 0, c00y = 0, c10y = 0, c01y = 0, c11y = 0;
+};
+
+// Deposit information of a pushed chunk: packed cells (i << 16 | j) and
+// 14-bit offsets (dI << 16 | dJ) of the new positions (grids up to 65535
+// nodes per side), and the lane's bounding box of cells.
+template <int PPT>
+struct ChunkDep {
+  unsigned cp[PPT], wp[PPT];
+  int imin, imax, jmin, jmax;
+};
+
+template <int PPT>
+__device__ __forceinline__ void load_chunk(const PushF32& a, int64_t wb, int64_t n, int lane, ChunkRegs<PPT>& r) {
+  ld_run<float, PPT>(a.x, wb, lane, n, r.x);
+  ld_run<float, PPT>(a.y, wb, lane, n, r.y);
+  ld_run<float, PPT>(a.vx, wb, lane, n, r.vx);
+  ld_run<float, PPT>(a.vy, wb, lane, n, r.vy);
+}
+
+// A chunk known to be full (wb + 64*PPT <= n): the vector loads only.
+template <int PPT>
+__device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, int lane, ChunkRegs<PPT>& r) {
+  constexpr int V = 4;
+  float* const arr[4] = {a.x, a.y, a.vx, a.vy};
+  float* const dst[4] = {r.x, r.y, r.vx, r.vy};
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int g = 0; g < PPT / V; g++) {
+      const v4u32 t = __builtin_nontemporal_load(
+          reinterpret_cast<const v4u32*>(arr[q] + wb + (int64_t)g * 64 * V + (int64_t)lane * V));
+      __builtin_memcpy(&dst[q][g * V], &t, 16);
+    }
+}
+
+// Gather, push, walls and store of a loaded chunk; fills the deposit
+// information of the new positions (DEP).
+template <bool DEP, int PPT>
+__device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, int64_t n, int lane,
+                                                ChunkRegs<PPT>& r, ChunkDep<PPT>& dp) {
+  constexpr int V = 4;
+  const int nyc = a.ny - 1;
+  const float fmax_i = (float)(a.nx - 1), fmax_j = (float)(a.ny - 1);
+  const int rem = (int)(n - wb < (int64_t)(64 * PPT) ? n - wb : (int64_t)(64 * PPT));
+  dp.imin = 1 << 30;
+  dp.imax = -1;
+  dp.jmin = 1 << 30;
+  dp.jmax = -1;
+  // the cell of each run's first particle: its coefficients are fetched for
+  // all runs at once, so the fetches overlap
+  constexpr int G = PPT / V;
+  int gkey[G];
+  float4 gx[G], gy[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    gkey[g] = -1;
+    gx[g] = gy[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int off = g * (64 * V) + lane * V;
+    const float lx = (r.x[g * V] - a.x0) * a.idx;
+    const float ly = (r.y[g * V] - a.y0) * a.idy;
+    if (off < rem && lx >= 0.f && ly >= 0.f && lx <= fmax_i && ly <= fmax_j) {
+      gkey[g] = min((int)lx, a.nx - 2) * nyc + min((int)ly, a.ny - 2);
+      gx[g] = a.Ec[2 * (size_t)gkey[g]];
+      gy[g] = a.Ec[2 * (size_t)gkey[g] + 1];
+    }
+  }
   int bad = 0;
 #pragma unroll
   for (int k = 0; k < PPT; k++) {
-    if (!(part_index<float, PPT>(wb, lane, k) < n) || isnan(x[k])) continue;
-    const float lx = (x[k] - a.x0) * a.idx;
-    const float ly = (y[k] - a.y0) * a.idy;
-    const int i = (int)lx, j = (int)ly;
+    dp.cp[k] = NO_DEPOSIT;
+    dp.wp[k] = 0;
+    const int off = (k / V) * (64 * V) + lane * V + (k % V);
+    if (off >= rem || isnan(r.x[k])) continue;
+    const float lx = (r.x[k] - a.x0) * a.idx;
+    const float ly = (r.y[k] - a.y0) * a.idy;
     float ex = 0.f, ey = 0.f;
-    if (i >= 0 && i + 1 < a.nx && j >= 0 && j + 1 < a.ny) {
-      const int key = i * ny + j;
-      if (key != ckey) {
-        const float2 e00 = a.E[key], e01 = a.E[key + 1], e10 = a.E[key + ny], e11 = a.E[key + ny + 1];
-        c00x = e00.x; c01x = e01.x; c10x = e10.x; c11x = e11.x;
-        c00y = e00.y; c01y = e01.y; c10y = e10.y; c11y = e11.y;
-        ckey = key;
+    if (lx >= 0.f && ly >= 0.f && lx <= fmax_i && ly <= fmax_j) {
+      const int i = min((int)lx, a.nx - 2), j = min((int)ly, a.ny - 2);
+      const int key = i * nyc + j;
+      float4& cx = gx[k / V];
+      float4& cy = gy[k / V];
+      if (key != gkey[k / V]) {
+        cx = a.Ec[2 * (size_t)key];
+        cy = a.Ec[2 * (size_t)key + 1];
+        gkey[k / V] = key;
       }
       const float di = lx - (float)i, dj = ly - (float)j;
-      ex = lerpf(lerpf(c00x, c10x, di), lerpf(c01x, c11x, di), dj);
-      ey = lerpf(lerpf(c00y, c10y, di), lerpf(c01y, c11y, di), dj);
+      ex = fmaf(dj, fmaf(di, cx.w, cx.z), fmaf(di, cx.y, cx.x));
+      ey = fmaf(dj, fmaf(di, cy.w, cy.z), fmaf(di, cy.y, cy.x));
     } else {
       bad++;
     }
-    vx[k] = fmaf(a.kv, ex, vx[k]);
-    vy[k] = fmaf(a.kv, ey, vy[k]);
-    float px = fmaf(a.kx, vx[k], x[k]);
-    float py = fmaf(a.kx, vy[k], y[k]);
+    r.vx[k] = fmaf(a.kv, ex, r.vx[k]);
+    r.vy[k] = fmaf(a.kv, ey, r.vy[k]);
+    float px = fmaf(a.kx, r.vx[k], r.x[k]);
+    float py = fmaf(a.kx, r.vy[k], r.y[k]);
     bool alive = true;
     if (a.bc == EPIC_BC_PBC) {
       px = px < a.x0 ? px + a.Lx : (px >= a.xhi ? px - a.Lx : px);
@@ -189,37 +261,53 @@ __device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int
       alive = !(px < a.x0 || px >= a.xhi || py < a.y0 || py >= a.yhi);
       if (!alive) px = __builtin_nanf("");
     } else {
-      if (px < a.x0) { px = a.x0 + (a.x0 - px); vx[k] = -vx[k]; }
-      else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); vx[k] = -vx[k]; }
-      if (py < a.y0) { py = a.y0 + (a.y0 - py); vy[k] = -vy[k]; }
-      else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); vy[k] = -vy[k]; }
+      if (px < a.x0) { px = a.x0 + (a.x0 - px); r.vx[k] = -r.vx[k]; }
+      else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); r.vx[k] = -r.vx[k]; }
+      if (py < a.y0) { py = a.y0 + (a.y0 - py); r.vy[k] = -r.vy[k]; }
+      else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); r.vy[k] = -r.vy[k]; }
     }
-    x[k] = px;
-    y[k] = py;
+    r.x[k] = px;
+    r.y[k] = py;
     if (DEP && alive) {
       const float nlx = (px - a.x0) * a.idx, nly = (py - a.y0) * a.idy;
       // a wrapped position can round onto the upper boundary node: it
       // deposits into the last cell
       if (nlx >= 0.f && nly >= 0.f && nlx <= fmax_i && nly <= fmax_j) {
-        live |= 1u << k;
         const int ni = min((int)nlx, a.nx - 2), nj = min((int)nly, a.ny - 2);
-        imin = min(imin, ni); imax = max(imax, ni);
-        jmin = min(jmin, nj); jmax = max(jmax, nj);
+        const int dI = min((int)rintf((nlx - (float)ni) * 16384.0f), 16384);
+        const int dJ = min((int)rintf((nly - (float)nj) * 16384.0f), 16384);
+        dp.cp[k] = ((unsigned)ni << 16) | (unsigned)nj;
+        dp.wp[k] = ((unsigned)dI << 16) | (unsigned)dJ;
+        dp.imin = min(dp.imin, ni); dp.imax = max(dp.imax, ni);
+        dp.jmin = min(dp.jmin, nj); dp.jmax = max(dp.jmax, nj);
       } else {
         bad++;
       }
     }
   }
-  st_run<float, PPT>(a.vx, wb, lane, n, vx);
-  st_run<float, PPT>(a.vy, wb, lane, n, vy);
-  st_run<float, PPT>(a.x, wb, lane, n, x);
-  st_run<float, PPT>(a.y, wb, lane, n, y);
+  // Every gather fetch has been consumed here, so this wait costs nothing;
+  // it tells the compiler that no fetch is outstanding, which otherwise
+  // (its analysis is path-insensitive) drains all vector memory -- the next
+  // chunk's loads included -- before registers are reused in the deposit.
+  // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15)
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  st_run<float, PPT>(a.vx, wb, lane, n, r.vx);
+  st_run<float, PPT>(a.vy, wb, lane, n, r.vy);
+  st_run<float, PPT>(a.x, wb, lane, n, r.x);
+  st_run<float, PPT>(a.y, wb, lane, n, r.y);
   if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
-  if (!DEP) return;
+}
 
-  const int gi0 = wave_min(imin), gi1 = wave_max(imax);
+// Deposit of a pushed chunk through the wavefront's LDS window wl: each
+// lane sums runs of equal cells in registers, runs go to the window (or
+// straight to Q outside it), the window goes to Q.
+template <int PPT>
+__device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsigned long long* wl,
+                                              const ChunkDep<PPT>& dp) {
+  const int ny = a.ny;
+  const int gi0 = wave_min_dpp(dp.imin), gi1 = wave_max_dpp(dp.imax);
   if (gi1 < 0) return;
-  const int gj0 = wave_min(jmin), gj1 = wave_max(jmax);
+  const int gj0 = wave_min_dpp(dp.jmin), gj1 = wave_max_dpp(dp.jmax);
   const int wi = gi1 - gi0 + 2, wj = gj1 - gj0 + 2;
   Window win;
   if (wi * wj <= WAVE_WIN) {
@@ -231,33 +319,29 @@ __device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int
   for (int k = lane; k < win.wi * win.wj; k += 64) wl[k] = 0ull;
   wave_fence();
 
-  int rk_i = -1, rk_j = -1;
+  unsigned rk = NO_DEPOSIT;
   unsigned acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
 #pragma unroll
   for (int k = 0; k < PPT; k++) {
-    if (!(live & (1u << k))) continue;
-    const float nlx = (x[k] - a.x0) * a.idx, nly = (y[k] - a.y0) * a.idy;
-    const int i = min((int)nlx, a.nx - 2), j = min((int)nly, a.ny - 2);
-    const int dI = min((int)rintf((nlx - (float)i) * 16384.0f), 16384);
-    const int dJ = min((int)rintf((nly - (float)j) * 16384.0f), 16384);
-    if (i != rk_i || j != rk_j) {
-      if (rk_i >= 0) {
+    if (dp.cp[k] == NO_DEPOSIT) continue;
+    if (dp.cp[k] != rk) {
+      if (rk != NO_DEPOSIT) {
         const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
-        flush_block(rk_i, rk_j, ny, acc, 4, win, wl, a.Q);
+        flush_block((int)(rk >> 16), (int)(rk & 0xffffu), ny, acc, 4, win, wl, a.Q);
       }
-      rk_i = i;
-      rk_j = j;
+      rk = dp.cp[k];
       acc0 = acc1 = acc2 = acc3 = 0;
     }
-    const unsigned oI = 16384u - (unsigned)dI, oJ = 16384u - (unsigned)dJ;
+    const unsigned dI = dp.wp[k] >> 16, dJ = dp.wp[k] & 0xffffu;
+    const unsigned oI = 16384u - dI, oJ = 16384u - dJ;
     acc0 += oI * oJ;
-    acc1 += (unsigned)dI * oJ;
-    acc2 += oI * (unsigned)dJ;
-    acc3 += (unsigned)dI * (unsigned)dJ;
+    acc1 += dI * oJ;
+    acc2 += oI * dJ;
+    acc3 += dI * dJ;
   }
-  if (rk_i >= 0) {
+  if (rk != NO_DEPOSIT) {
     const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
-    flush_block(rk_i, rk_j, ny, acc, 4, win, wl, a.Q);
+    flush_block((int)(rk >> 16), (int)(rk & 0xffffu), ny, acc, 4, win, wl, a.Q);
   }
   wave_fence();
   const int W = win.wi * win.wj;
@@ -265,10 +349,24 @@ __device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int
     const unsigned long long v = wl[k];
     if (v) {
       const int gi = win.i0 + k / win.wj, gj = win.j0 + k % win.wj;
-      if (gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
+      if (gi >= 0 && gj >= 0 && gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
     }
   }
   wave_fence();
+}
+
+// One wavefront's chunk of 64*PPT particles starting at wb (n = particles in
+// the array): push, walls, store, and (DEP) deposit of the new positions.
+// Shared by the single-simulation kernel and the batched one, so both do
+// identical arithmetic.
+template <bool DEP, int PPT>
+__device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int64_t n, int lane,
+                                               unsigned long long* wl) {
+  ChunkRegs<PPT> r;
+  ChunkDep<PPT> dp;
+  load_chunk<PPT>(a, wb, n, lane, r);
+  push_chunk_regs<DEP, PPT>(a, wb, n, lane, r, dp);
+  if (DEP) deposit_chunk<PPT>(a, lane, wl, dp);
 }
 
 }  // namespace epic
