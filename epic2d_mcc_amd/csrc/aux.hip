@@ -3,10 +3,13 @@
 //
 // Compaction keeps the order of the live particles, which is what the
 // reference's vector::erase removal does (species.cpp:95-121).  The counting
-// sort orders particles by cell key i*(ny-1)+j and, inside a cell, by their
-// previous storage index: a stable sort, equal element for element to
-// std::stable_sort by key on the CPU.  Absorbed particles get the key ncell and
-// are dropped, so a sort also compacts.
+// sort orders particles by a tile-major cell key (16 x 16-cell tiles in
+// row-major order, cells row-major inside a tile; sort_key_space) and, inside
+// a cell, by their previous storage index: a stable sort, equal element for
+// element to std::stable_sort by key on the CPU.  Tiles keep a particle that
+// moved a cell or two since the previous sort within a few thousand slots of
+// where it was, so the permutation is local.  Absorbed particles get the key
+// K (the key space size) and are dropped, so a sort also compacts.
 #include <algorithm>
 
 #include "device_common.hpp"
@@ -244,13 +247,34 @@ hipError_t launch_compact(Ctx* c, int s) {
 }
 
 // ---- counting sort by cell ----------------------------------------------------
+constexpr int SORT_TILE = 16;
+
+struct KeyMap {
+  int ncx, ncy;  // cells
+  int nty;       // tiles along y
+  int K;         // key space: tiles * SORT_TILE^2; K is the key of absorbed particles
+};
+
+static KeyMap key_map(int ncx, int ncy) {
+  KeyMap m;
+  m.ncx = ncx;
+  m.ncy = ncy;
+  const int ntx = (ncx + SORT_TILE - 1) / SORT_TILE;
+  m.nty = (ncy + SORT_TILE - 1) / SORT_TILE;
+  m.K = ntx * m.nty * SORT_TILE * SORT_TILE;
+  return m;
+}
+
+int64_t sort_key_space(int nx, int ny) { return key_map(nx - 1, ny - 1).K; }
+
 template <typename P>
-__device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, int ncx, int ncy) {
-  if (isnan(x)) return ncx * ncy;
+__device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, const KeyMap& m) {
+  if (isnan(x)) return m.K;
   const P lx = (x - x0) / dx, ly = (y - y0) / dy;
   const int i = (int)lx, j = (int)ly;
-  if (!(lx >= 0) || !(ly >= 0) || i >= ncx || j >= ncy) return ncx * ncy;
-  return i * ncy + j;
+  if (!(lx >= 0) || !(ly >= 0) || i >= m.ncx || j >= m.ncy) return m.K;
+  return ((i / SORT_TILE) * m.nty + j / SORT_TILE) * (SORT_TILE * SORT_TILE) + (i % SORT_TILE) * SORT_TILE +
+         j % SORT_TILE;
 }
 
 // Runs of equal keys among consecutive lanes (particles are nearly sorted
@@ -278,9 +302,46 @@ __device__ __forceinline__ LaneRun lane_run(int key) {
 // particles are in flight together.
 constexpr int SORT_U = 4;
 
+// Lane-xor exchange: ds_swizzle's xor mode inside 32-lane halves, a
+// permute across them.
+template <int J>
+__device__ __forceinline__ int xor_lane(int v) {
+  if (J < 32) return __builtin_amdgcn_ds_swizzle(v, (J << 10) | 0x1F);
+  return __shfl_xor(v, J, 64);
+}
+
+// Ascending bitonic sort of one int per lane across the 64-lane wavefront.
+__device__ __forceinline__ int wave_sort(int v) {
+  const int lane = threadIdx.x & 63;
+#define EPIC_BSTEP(K, J)                                             \
+  {                                                                  \
+    const int o = xor_lane<J>(v);                                    \
+    const bool asc = (lane & K) == 0, lower = (lane & J) == 0;       \
+    v = (lower == asc) ? min(v, o) : max(v, o);                      \
+  }
+  EPIC_BSTEP(2, 1)
+  EPIC_BSTEP(4, 2) EPIC_BSTEP(4, 1)
+  EPIC_BSTEP(8, 4) EPIC_BSTEP(8, 2) EPIC_BSTEP(8, 1)
+  EPIC_BSTEP(16, 8) EPIC_BSTEP(16, 4) EPIC_BSTEP(16, 2) EPIC_BSTEP(16, 1)
+  EPIC_BSTEP(32, 16) EPIC_BSTEP(32, 8) EPIC_BSTEP(32, 4) EPIC_BSTEP(32, 2) EPIC_BSTEP(32, 1)
+  EPIC_BSTEP(64, 32) EPIC_BSTEP(64, 16) EPIC_BSTEP(64, 8) EPIC_BSTEP(64, 4) EPIC_BSTEP(64, 2) EPIC_BSTEP(64, 1)
+#undef EPIC_BSTEP
+  return v;
+}
+
+// Histogram of keys.  Each workgroup counts its chunk of 256*SORT_U
+// particles into an LDS window of HWIN keys centred on the chunk's first key
+// (runs of equal keys in a wavefront add once), then adds the window's
+// non-zero counts to the global histogram: particles that moved less than a
+// tile since the previous sort cost no global atomic.  Keys outside the
+// window go straight to the global histogram.
+constexpr int HWIN = 4096;
+
 template <typename P>
-__global__ void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0, P y0, P dx, P dy,
-                                 int ncx, int ncy) {
+__global__ __launch_bounds__(256) void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0,
+                                                        P y0, P dx, P dy, KeyMap m) {
+  __shared__ int win[HWIN];
+  __shared__ int wbase_s;
   const int64_t n = *d_n;
   const int lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
@@ -289,24 +350,42 @@ __global__ void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int
 #pragma unroll
     for (int u = 0; u < SORT_U; u++) {
       const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
+      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
     }
+    if (threadIdx.x == 0) wbase_s = key[0] >= 0 && key[0] < m.K ? max(0, key[0] - HWIN / 2) : 0;
+    for (int t = threadIdx.x; t < HWIN; t += blockDim.x) win[t] = 0;
+    __syncthreads();
+    const int wbase = wbase_s;
 #pragma unroll
     for (int u = 0; u < SORT_U; u++) {
-      const LaneRun r = lane_run(key[u]);
-      if (r.head == lane && key[u] >= 0) atomicAdd(&count[key[u]], r.len);
+      // sorted in the wavefront, equal keys form one run each
+      const int ks = wave_sort(key[u]);
+      const LaneRun r = lane_run(ks);
+      if (r.head == lane && ks >= 0) {
+        const int loc = ks - wbase;
+        if (loc >= 0 && loc < HWIN) atomicAdd(&win[loc], r.len);
+        else atomicAdd(&count[ks], r.len);
+      }
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < HWIN; t += blockDim.x) {
+      const int v = win[t];
+      if (v) atomicAdd(&count[wbase + t], v);
+    }
+    __syncthreads();
   }
 }
 
-// Slots: a run's first lane reserves len consecutive slots with one atomic,
-// so particles keep their order within a wavefront; the order between
-// wavefronts is fixed afterwards by sort_fix_kernel.
+// Slots: the wavefront sorts its (key, lane) pairs, so equal keys form one
+// run in lane (= index) order; the run's first lane reserves len consecutive
+// slots with one atomic.  Particles keep their order within a wavefront; the
+// order between wavefronts is fixed afterwards by sort_fix_kernel.  The pair
+// is packed as key * 64 + lane (key spaces below 2^25).
 template <typename P>
 __global__ void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, const int* start, int* fill,
-                                 uint32_t* perm, P x0, P y0, P dx, P dy, int ncx, int ncy) {
+                                 uint32_t* perm, P x0, P y0, P dx, P dy, KeyMap m) {
   const int64_t n = *d_n;
-  const int dead = ncx * ncy;
+  const int dead = m.K;
   const int lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
   for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
@@ -314,16 +393,21 @@ __global__ void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, con
 #pragma unroll
     for (int u = 0; u < SORT_U; u++) {
       const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, ncx, ncy) : -1;
+      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
     }
 #pragma unroll
     for (int u = 0; u < SORT_U; u++) {
-      const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-      const LaneRun r = lane_run(key[u]);
+      const bool place = key[u] >= 0 && key[u] != dead;
+      const int pk = wave_sort(place ? key[u] * 64 + lane : 0x7fffffff);
+      const int ks = pk == 0x7fffffff ? -1 : pk >> 6;
+      const LaneRun r = lane_run(ks);
       int b = 0;
-      if (r.head == lane && key[u] >= 0 && key[u] != dead) b = atomicAdd(&fill[key[u]], r.len);
+      if (r.head == lane && ks >= 0) b = atomicAdd(&fill[ks], r.len);
       b = __shfl(b, r.head, 64);
-      if (key[u] >= 0 && key[u] != dead) perm[start[key[u]] + b + (lane - r.head)] = (uint32_t)p;
+      if (ks >= 0) {
+        const int64_t p = base + (int64_t)u * blockDim.x + (threadIdx.x & ~63) + (pk & 63);
+        perm[start[ks] + b + (lane - r.head)] = (uint32_t)p;
+      }
     }
   }
 }
@@ -409,8 +493,13 @@ __global__ void sort_gather_kernel(const uint32_t* perm, const int* total, const
                                    const P* vy, const P* vz, const uint32_t* id, P* x2, P* y2, P* vx2, P* vy2,
                                    P* vz2, uint32_t* id2) {
   const int64_t n = *total;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
+  // each XCD walks its own contiguous range of chunks: the source slots of
+  // neighbouring chunks overlap and stay in that XCD's L2
+  const int64_t CHK = (int64_t)blockDim.x * SORT_U;
+  int64_t cb, ce, cs;
+  xcd_chunk_range((n + CHK - 1) / CHK, &cb, &ce, &cs);
+  for (int64_t chunk = cb; chunk < ce; chunk += cs) {
+    const int64_t base = chunk * CHK;
     uint32_t p[SORT_U];
 #pragma unroll
     for (int u = 0; u < SORT_U; u++) {
@@ -444,7 +533,8 @@ template <typename P>
 static hipError_t sort_t(Ctx* c, int s) {
   SpeciesDev& S = c->sp[s];
   const epic_domain_desc& d = c->dom;
-  const int ncx = d.nx - 1, ncy = d.ny - 1, ncell = ncx * ncy;
+  const KeyMap km = key_map(d.nx - 1, d.ny - 1);
+  const int ncell = km.K;  // key space (tile-padded cells)
   const int64_t n = S.n_host;
   if (n == 0) return hipSuccess;
   const P x0 = (P)d.x0, y0 = (P)d.y0, dx = (P)d.dx, dy = (P)d.dy;
@@ -453,12 +543,12 @@ static hipError_t sort_t(Ctx* c, int s) {
   hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_count, (int64_t)ncell + 2);
   hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_fill, (int64_t)ncell + 2);
   hipLaunchKernelGGL(sort_hist_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
-                     c->sort_count, x0, y0, dx, dy, ncx, ncy);
+                     c->sort_count, x0, y0, dx, dy, km);
   EPIC_HIP_TRY(hipGetLastError());
   // start[k] = exclusive scan; start[ncell] = live total; start[ncell+1] = all
   EPIC_HIP_TRY(scan_ints(c, c->sort_count, (int64_t)ncell + 1));
   hipLaunchKernelGGL(sort_fill_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
-                     c->sort_count, c->sort_fill, c->sort_perm, x0, y0, dx, dy, ncx, ncy);
+                     c->sort_count, c->sort_fill, c->sort_perm, x0, y0, dx, dy, km);
   EPIC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(sort_fix_kernel, dim3(std::min((ncell + 3) / 4, 65536)), dim3(256), 0, c->stream,
                      c->sort_count, c->sort_perm, c->sort_perm + c->sort_cap, ncell);

@@ -133,7 +133,7 @@ size_t ctx_bytes_estimate(const epic_domain_desc* dom, const epic_species_desc* 
   if (dom->precision == EPIC_FP32) tot += b((size_t)(dom->nx - 1) * (dom->ny - 1) * 8 * sizeof(float), 1);
   tot += 2 * b(nn, 8) + b(2 * nn, 8) + 2 * b(nn, 8);
   tot += b((size_t)16 * std::max(num_cu, 1024), 8) + b(8, 8) + b(4, 4);
-  const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
+  const size_t ncell = (size_t)sort_key_space(dom->nx, dom->ny);
   tot += 2 * b(ncell + 2, 4) + b(2 * (size_t)maxcap, 4);
   const int64_t blk_cap = maxcap / 2048 + 4;
   tot += b((size_t)blk_cap, 4);
@@ -170,6 +170,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (dom->bc < EPIC_BC_PBC || dom->bc > EPIC_BC_REFLECTIVE) return EPIC_ERR_ARG;
   if (dom->shape != EPIC_SHAPE_CIC && dom->shape != EPIC_SHAPE_NGP) return EPIC_ERR_ARG;
   if ((int64_t)dom->nx * dom->ny > (1ll << 30)) return EPIC_ERR_ARG;
+  if (sort_key_space(dom->nx, dom->ny) >= (1ll << 25)) return EPIC_ERR_ARG;  // sort packs key * 64 + lane
   epic_ctx* h = new (std::nothrow) epic_ctx();
   if (!h) return EPIC_ERR_CAPACITY;
   Ctx* c = &h->c;
@@ -246,7 +247,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   CKC(dalloc_c(c, &c->ws_part, (size_t)16 * std::max(c->max_solver_blocks, 1024)), "partials");
   CKC(dalloc_c(c, &c->ws_scalar, 8), "scalars");
   CKC(dalloc_c(c, &c->ws_iter, 4), "iters");  // [0] iterations, [1] grid barrier
-  const size_t ncell = (size_t)(dom->nx - 1) * (dom->ny - 1);
+  const size_t ncell = (size_t)sort_key_space(dom->nx, dom->ny);
   CKC(dalloc_c(c, &c->sort_count, ncell + 2), "sort");
   CKC(dalloc_c(c, &c->sort_fill, ncell + 2), "sort");
   c->sort_cap = maxcap;

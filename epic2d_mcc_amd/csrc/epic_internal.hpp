@@ -159,6 +159,7 @@ hipError_t launch_load(Ctx* c, int s, int64_t n, int vz_thermal);
 hipError_t launch_compact(Ctx* c, int s);
 hipError_t launch_sort(Ctx* c, int s);
 hipError_t launch_inject(Ctx* c, int step);
+int64_t sort_key_space(int nx, int ny);  // size of the tile-major cell key space
 // field.hip
 hipError_t launch_solve(Ctx* c, int* iters, double* err);
 hipError_t launch_efield(Ctx* c);

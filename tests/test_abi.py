@@ -81,6 +81,7 @@ def test_estimate_bytes_counts_every_creation_allocation():
     v = ctypes.c_int64()
     assert L.epic_estimate_bytes(ctypes.byref(d), sp, 2, ctypes.byref(v)) == 0
     nn, ncell = 33 * 17, 32 * 16
+    nkey = 2 * 1 * 256  # sort keys: 16x16-cell tiles, 2 x 1 of them
     ncu = 256
     hip_ok = True
     try:
@@ -96,9 +97,9 @@ def test_estimate_bytes_counts_every_creation_allocation():
         want += 10 * pad * 4 + 2 * pad * 4 + 2 * nn * 8 + 16
     want += 4 * nn * 8 + 2 * nn * 4 + ncell * 8 * 4 + 2 * nn * 8 + 2 * nn * 8 + 2 * nn * 8
     want += 16 * max(ncu, 1024) * 8 + 64 + 16
-    want += 2 * (ncell + 2) * 4 + 2 * 1000 * 4
+    want += 2 * (nkey + 2) * 4 + 2 * 1000 * 4
     blk = 1000 // 2048 + 4
-    want += blk * 4 + (max(ncell + 2, blk) // 1024 + 4) * 8
+    want += blk * 4 + (max(nkey + 2, blk) // 1024 + 4) * 8
     want += 8 * ncu * 4 * 8 + 64 + 8 * 8
     assert v.value == want
 

@@ -13,13 +13,16 @@ from oracle_api import oracle
 pytestmark = pytest.mark.gpu
 
 
-def _cell_keys(x, y, d):
+def _cell_keys(x, y, d, T=16):
+    """Tile-major cell key: T x T-cell tiles in row-major order, cells
+    row-major inside a tile."""
     lx = (x - d.x0) / d.dx
     ly = (y - d.y0) / d.dy
-    ncx, ncy = d.nx - 1, d.ny - 1
+    ncy = d.ny - 1
+    nty = (ncy + T - 1) // T
     i = np.floor(lx).astype(np.int64)
     j = np.floor(ly).astype(np.int64)
-    return i * ncy + j
+    return ((i // T) * nty + j // T) * (T * T) + (i % T) * T + j % T
 
 
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32])
