@@ -1,19 +1,11 @@
 // Species storage kernels: native load, stable compaction, stable counting
-// sort by cell, emitter injection.
+// sort by cell (sort_ops.hpp), emitter injection.
 //
 // Compaction keeps the order of the live particles, which is what the
-// reference's vector::erase removal does (species.cpp:95-121).  The counting
-// sort orders particles by a tile-major cell key (16 x 16-cell tiles in
-// row-major order, cells row-major inside a tile; sort_key_space) and, inside
-// a cell, by their previous storage index: a stable sort, equal element for
-// element to std::stable_sort by key on the CPU.  Tiles keep a particle that
-// moved a cell or two since the previous sort within a few thousand slots of
-// where it was, so the permutation is local.  Absorbed particles get the key
-// K (the key space size) and are dropped, so a sort also compacts.
+// reference's vector::erase removal does (species.cpp:95-121).
 #include <algorithm>
 
-#include "device_common.hpp"
-#include "epic_internal.hpp"
+#include "sort_ops.hpp"
 
 namespace epic {
 
@@ -246,96 +238,8 @@ hipError_t launch_compact(Ctx* c, int s) {
   return compact_t<float>(c, s);
 }
 
-// ---- counting sort by cell ----------------------------------------------------
-constexpr int SORT_TILE = 16;
-
-struct KeyMap {
-  int ncx, ncy;  // cells
-  int nty;       // tiles along y
-  int K;         // key space: tiles * SORT_TILE^2; K is the key of absorbed particles
-};
-
-static KeyMap key_map(int ncx, int ncy) {
-  KeyMap m;
-  m.ncx = ncx;
-  m.ncy = ncy;
-  const int ntx = (ncx + SORT_TILE - 1) / SORT_TILE;
-  m.nty = (ncy + SORT_TILE - 1) / SORT_TILE;
-  m.K = ntx * m.nty * SORT_TILE * SORT_TILE;
-  return m;
-}
-
+// ---- counting sort by cell (sort_ops.hpp) ------------------------------------
 int64_t sort_key_space(int nx, int ny) { return key_map(nx - 1, ny - 1).K; }
-
-template <typename P>
-__device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, const KeyMap& m) {
-  if (isnan(x)) return m.K;
-  const P lx = (x - x0) / dx, ly = (y - y0) / dy;
-  const int i = (int)lx, j = (int)ly;
-  if (!(lx >= 0) || !(ly >= 0) || i >= m.ncx || j >= m.ncy) return m.K;
-  return ((i / SORT_TILE) * m.nty + j / SORT_TILE) * (SORT_TILE * SORT_TILE) + (i % SORT_TILE) * SORT_TILE +
-         j % SORT_TILE;
-}
-
-// Runs of equal keys among consecutive lanes (particles are nearly sorted
-// between sorts): the run's first lane does one atomic for the whole run.
-struct LaneRun {
-  int head;  // lane that starts this lane's run
-  int len;   // run length (valid in the head lane)
-};
-__device__ __forceinline__ LaneRun lane_run(int key) {
-  const int lane = threadIdx.x & 63;
-  // the previous lane's key by a DPP whole-wave shift (wave_shr:1); lane 0
-  // receives a value different from its own key
-  const int prev = __builtin_amdgcn_update_dpp(~key, key, 0x138, 0xf, 0xf, false);
-  const bool h = lane == 0 || key != prev;
-  const unsigned long long heads = __ballot(h);
-  const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-  LaneRun r;
-  r.head = 63 - __builtin_clzll(upto);
-  const unsigned long long after = lane == 63 ? 0ull : (heads >> (lane + 1));
-  r.len = (after ? __builtin_ctzll(after) + 1 : 64 - lane);
-  return r;
-}
-
-// Particles per thread per pass of the sort kernels: the loads of SORT_U
-// particles are in flight together.
-constexpr int SORT_U = 4;
-
-// Lane-xor exchange: ds_swizzle's xor mode inside 32-lane halves, a
-// permute across them.
-template <int J>
-__device__ __forceinline__ int xor_lane(int v) {
-  if (J < 32) return __builtin_amdgcn_ds_swizzle(v, (J << 10) | 0x1F);
-  return __shfl_xor(v, J, 64);
-}
-
-// Ascending bitonic sort of one int per lane across the 64-lane wavefront.
-__device__ __forceinline__ int wave_sort(int v) {
-  const int lane = threadIdx.x & 63;
-#define EPIC_BSTEP(K, J)                                             \
-  {                                                                  \
-    const int o = xor_lane<J>(v);                                    \
-    const bool asc = (lane & K) == 0, lower = (lane & J) == 0;       \
-    v = (lower == asc) ? min(v, o) : max(v, o);                      \
-  }
-  EPIC_BSTEP(2, 1)
-  EPIC_BSTEP(4, 2) EPIC_BSTEP(4, 1)
-  EPIC_BSTEP(8, 4) EPIC_BSTEP(8, 2) EPIC_BSTEP(8, 1)
-  EPIC_BSTEP(16, 8) EPIC_BSTEP(16, 4) EPIC_BSTEP(16, 2) EPIC_BSTEP(16, 1)
-  EPIC_BSTEP(32, 16) EPIC_BSTEP(32, 8) EPIC_BSTEP(32, 4) EPIC_BSTEP(32, 2) EPIC_BSTEP(32, 1)
-  EPIC_BSTEP(64, 32) EPIC_BSTEP(64, 16) EPIC_BSTEP(64, 8) EPIC_BSTEP(64, 4) EPIC_BSTEP(64, 2) EPIC_BSTEP(64, 1)
-#undef EPIC_BSTEP
-  return v;
-}
-
-// Histogram of keys.  Each workgroup counts its chunk of 256*SORT_U
-// particles into an LDS window of HWIN keys centred on the chunk's first key
-// (runs of equal keys in a wavefront add once), then adds the window's
-// non-zero counts to the global histogram: particles that moved less than a
-// tile since the previous sort cost no global atomic.  Keys outside the
-// window go straight to the global histogram.
-constexpr int HWIN = 4096;
 
 template <typename P>
 __global__ __launch_bounds__(256) void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0,
@@ -343,186 +247,40 @@ __global__ __launch_bounds__(256) void sort_hist_kernel(const P* x, const P* y, 
   __shared__ int win[HWIN];
   __shared__ int wbase_s;
   const int64_t n = *d_n;
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
-    int key[SORT_U];
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
-    }
-    if (threadIdx.x == 0) wbase_s = key[0] >= 0 && key[0] < m.K ? max(0, key[0] - HWIN / 2) : 0;
-    for (int t = threadIdx.x; t < HWIN; t += blockDim.x) win[t] = 0;
-    __syncthreads();
-    const int wbase = wbase_s;
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      // sorted in the wavefront, equal keys form one run each
-      const int ks = wave_sort(key[u]);
-      const LaneRun r = lane_run(ks);
-      if (r.head == lane && ks >= 0) {
-        const int loc = ks - wbase;
-        if (loc >= 0 && loc < HWIN) atomicAdd(&win[loc], r.len);
-        else atomicAdd(&count[ks], r.len);
-      }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < HWIN; t += blockDim.x) {
-      const int v = win[t];
-      if (v) atomicAdd(&count[wbase + t], v);
-    }
-    __syncthreads();
-  }
+  const int64_t stride = (int64_t)gridDim.x * SORT_CHUNK;
+  for (int64_t base = blockIdx.x * (int64_t)SORT_CHUNK; base < n; base += stride)
+    hist_chunk(x, y, n, base, count, x0, y0, dx, dy, m, win, &wbase_s);
 }
 
-// Slots: the wavefront sorts its (key, lane) pairs, so equal keys form one
-// run in lane (= index) order; the run's first lane reserves len consecutive
-// slots with one atomic.  Particles keep their order within a wavefront; the
-// order between wavefronts is fixed afterwards by sort_fix_kernel.  The pair
-// is packed as key * 64 + lane (key spaces below 2^25).
 template <typename P>
-__global__ void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, const int* start, int* fill,
-                                 uint32_t* perm, P x0, P y0, P dx, P dy, KeyMap m) {
+__global__ __launch_bounds__(256) void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, const int* start,
+                                                        int* fill, uint32_t* perm, P x0, P y0, P dx, P dy, KeyMap m) {
   const int64_t n = *d_n;
-  const int dead = m.K;
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x * SORT_U;
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x * SORT_U; base < n; base += stride) {
-    int key[SORT_U];
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-      key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      const bool place = key[u] >= 0 && key[u] != dead;
-      const int pk = wave_sort(place ? key[u] * 64 + lane : 0x7fffffff);
-      const int ks = pk == 0x7fffffff ? -1 : pk >> 6;
-      const LaneRun r = lane_run(ks);
-      int b = 0;
-      if (r.head == lane && ks >= 0) b = atomicAdd(&fill[ks], r.len);
-      b = __shfl(b, r.head, 64);
-      if (ks >= 0) {
-        const int64_t p = base + (int64_t)u * blockDim.x + (threadIdx.x & ~63) + (pk & 63);
-        perm[start[ks] + b + (lane - r.head)] = (uint32_t)p;
-      }
-    }
-  }
+  const int64_t stride = (int64_t)gridDim.x * SORT_CHUNK;
+  for (int64_t base = blockIdx.x * (int64_t)SORT_CHUNK; base < n; base += stride)
+    fill_chunk(x, y, n, base, start, fill, perm, x0, y0, dx, dy, m);
 }
 
-// One wavefront per cell: order the cell's particle indices ascending by
-// rank counting (rank = number of smaller indices; indices are unique).
-// Cells of up to FIX_LDS particles are staged in LDS; larger ones are staged
-// in `scratch` (same offsets as perm).
-constexpr int FIX_LDS = 1024;
-constexpr int FIX_Q = FIX_LDS / 64;
+// One wavefront per cell.
 __global__ __launch_bounds__(256) void sort_fix_kernel(const int* start, uint32_t* perm, uint32_t* scratch,
                                                         int ncell) {
   __shared__ uint32_t buf[4][FIX_LDS];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int cell = blockIdx.x * 4 + w; cell < ncell; cell += gridDim.x * 4) {
-    const int b = start[cell], e = start[cell + 1], m = e - b;
-    if (m <= 1) continue;
-    // already in order?  (common: the cell's particles came from one wavefront)
-    bool ordered = true;
-    for (int k0 = 0; k0 < m - 1; k0 += 64) {
-      const int k = k0 + lane;
-      const bool ok = k >= m - 1 || perm[b + k] < perm[b + k + 1];
-      if (__ballot(!ok)) {
-        ordered = false;
-        break;
-      }
-    }
-    if (ordered) continue;
-    if (m <= FIX_LDS) {
-      for (int k = lane; k < m; k += 64) buf[w][k] = perm[b + k];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int cnt = (m + 63) / 64;
-      uint32_t mine[FIX_Q];
-      int rank[FIX_Q];
-#pragma unroll
-      for (int q = 0; q < FIX_Q; q++) {
-        const int k = lane + q * 64;
-        mine[q] = (q < cnt && k < m) ? buf[w][k] : 0xFFFFFFFFu;
-        rank[q] = 0;
-      }
-      if (cnt <= 2) {
-        for (int t = 0; t < m; t++) {
-          const uint32_t o = buf[w][t];
-          rank[0] += o < mine[0] ? 1 : 0;
-          rank[1] += o < mine[1] ? 1 : 0;
-        }
-      } else {
-        for (int t = 0; t < m; t++) {
-          const uint32_t o = buf[w][t];
-#pragma unroll
-          for (int q = 0; q < FIX_Q; q++) rank[q] += (q < cnt && o < mine[q]) ? 1 : 0;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < FIX_Q; q++) {
-        const int k = lane + q * 64;
-        if (q < cnt && k < m) perm[b + rank[q]] = mine[q];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-      for (int k = lane; k < m; k += 64) scratch[b + k] = perm[b + k];
-      __threadfence();
-      __builtin_amdgcn_wave_barrier();
-      for (int k0 = 0; k0 < m; k0 += 64) {
-        const int k = k0 + lane;
-        const uint32_t me = k < m ? __builtin_nontemporal_load(&scratch[b + k]) : 0xFFFFFFFFu;
-        int rk = 0;
-        for (int t = 0; t < m; t++) rk += (__builtin_nontemporal_load(&scratch[b + t]) < me) ? 1 : 0;
-        if (k < m) perm[b + rk] = me;
-      }
-      __threadfence();
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
+  const int w = threadIdx.x >> 6;
+  for (int cell = blockIdx.x * 4 + w; cell < ncell; cell += gridDim.x * 4) fix_cell(start, perm, scratch, cell, buf[w]);
 }
 
+// Each XCD walks its own contiguous range of chunks: the source slots of
+// neighbouring chunks overlap and stay in that XCD's L2.
 template <typename P>
-__global__ void sort_gather_kernel(const uint32_t* perm, const int* total, const P* x, const P* y, const P* vx,
-                                   const P* vy, const P* vz, const uint32_t* id, P* x2, P* y2, P* vx2, P* vy2,
-                                   P* vz2, uint32_t* id2) {
+__global__ __launch_bounds__(256) void sort_gather_kernel(const uint32_t* perm, const int* total, const P* x,
+                                                          const P* y, const P* vx, const P* vy, const P* vz,
+                                                          const uint32_t* id, P* x2, P* y2, P* vx2, P* vy2, P* vz2,
+                                                          uint32_t* id2) {
   const int64_t n = *total;
-  // each XCD walks its own contiguous range of chunks: the source slots of
-  // neighbouring chunks overlap and stay in that XCD's L2
-  const int64_t CHK = (int64_t)blockDim.x * SORT_U;
   int64_t cb, ce, cs;
-  xcd_chunk_range((n + CHK - 1) / CHK, &cb, &ce, &cs);
-  for (int64_t chunk = cb; chunk < ce; chunk += cs) {
-    const int64_t base = chunk * CHK;
-    uint32_t p[SORT_U];
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      p[u] = i < n ? perm[i] : 0u;
-    }
-    P a[SORT_U], b[SORT_U], c[SORT_U], d[SORT_U], e[SORT_U];
-    uint32_t f[SORT_U];
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (i < n) {
-        a[u] = x[p[u]]; b[u] = y[p[u]]; c[u] = vx[p[u]]; d[u] = vy[p[u]]; e[u] = vz[p[u]]; f[u] = id[p[u]];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < SORT_U; u++) {
-      const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-      if (i < n) {
-        x2[i] = a[u]; y2[i] = b[u]; vx2[i] = c[u]; vy2[i] = d[u]; vz2[i] = e[u]; id2[i] = f[u];
-      }
-    }
-  }
+  xcd_chunk_range((n + SORT_CHUNK - 1) / SORT_CHUNK, &cb, &ce, &cs);
+  for (int64_t chunk = cb; chunk < ce; chunk += cs)
+    gather_chunk(perm, n, chunk * SORT_CHUNK, x, y, vx, vy, vz, id, x2, y2, vx2, vy2, vz2, id2);
 }
 
 __global__ void zero_int_kernel(int* p, int64_t n) {

@@ -41,6 +41,7 @@
 
 #include "node_ops.hpp"
 #include "push_f32.hpp"
+#include "sort_ops.hpp"
 
 namespace epic {
 
@@ -234,6 +235,135 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const Pus
   }
 }
 
+// ---- batched counting sort ------------------------------------------------------
+// One species of each member whose sort is due, all sorted together with the
+// routines of the single-simulation sort (sort_ops.hpp); members' own sort
+// workspaces, so species go in rounds (species 0 of every due member, then
+// species 1, ...).  Work items (particle chunks of SORT_CHUNK over capacity,
+// groups of 4 cells) find their segment by binary search over per-segment
+// prefix offsets, so no host synchronisation sizes any launch.
+struct SortSeg {
+  const float *x, *y, *vx, *vy, *vz;
+  const uint32_t* id;
+  float *x2, *y2, *vx2, *vy2, *vz2;
+  uint32_t* id2;
+  int64_t* d_n;
+  int* count;  // K + 2: histogram, then start
+  int* fill;   // K + 2
+  uint32_t* perm;
+  uint32_t* scratch;
+  float x0, y0, dx, dy;
+  KeyMap km;
+  int64_t chunk0;  // first particle chunk of this segment
+  int64_t item0;   // first 4-cell group of this segment
+};
+
+__device__ __forceinline__ int find_seg(const SortSeg* segs, int nseg, int64_t v, bool cells) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t b = cells ? segs[mid].item0 : segs[mid].chunk0;
+    if (b <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void bsort_zero_kernel(const SortSeg* segs) {
+  const SortSeg& S = segs[blockIdx.y];
+  const int n = S.km.K + 2;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    S.count[k] = 0;
+    S.fill[k] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void bsort_hist_kernel(const SortSeg* segs, int nseg, int64_t nchunks) {
+  __shared__ int win[HWIN];
+  __shared__ int wbase_s;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
+    const int64_t n = *S.d_n, base = (c - S.chunk0) * SORT_CHUNK;
+    if (base < n) hist_chunk(S.x, S.y, n, base, S.count, S.x0, S.y0, S.dx, S.dy, S.km, win, &wbase_s);
+  }
+}
+
+// Exclusive scan of count[0..K] in place, one workgroup per segment:
+// count[k] = start of key k, count[K] = live total, count[K+1] = all.
+__global__ __launch_bounds__(1024) void bsort_scan_kernel(const SortSeg* segs) {
+  __shared__ int sh[16];
+  __shared__ int carry_s;
+  const SortSeg& S = segs[blockIdx.x];
+  const int m = S.km.K + 1;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (int t0 = 0; t0 < m; t0 += 4 * 1024) {
+    int v[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = t0 + threadIdx.x * 4 + q;
+      v[q] = k < m ? S.count[k] : 0;
+      sum += v[q];
+    }
+    int inc = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += o;
+    }
+    if (lane == 63) sh[wid] = inc;
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int w = 0; w < 16; w++) {
+      if (w < wid) wb += sh[w];
+      tot += sh[w];
+    }
+    int run = carry_s + wb + inc - sum;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = t0 + threadIdx.x * 4 + q;
+      if (k < m) S.count[k] = run;
+      run += v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S.count[m] = carry_s;
+}
+
+__global__ __launch_bounds__(256) void bsort_fill_kernel(const SortSeg* segs, int nseg, int64_t nchunks) {
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
+    const int64_t n = *S.d_n, base = (c - S.chunk0) * SORT_CHUNK;
+    if (base < n) fill_chunk(S.x, S.y, n, base, S.count, S.fill, S.perm, S.x0, S.y0, S.dx, S.dy, S.km);
+  }
+}
+
+__global__ __launch_bounds__(256) void bsort_fix_kernel(const SortSeg* segs, int nseg, int64_t nitems) {
+  __shared__ uint32_t buf[4][FIX_LDS];
+  const int w = threadIdx.x >> 6;
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const SortSeg& S = segs[find_seg(segs, nseg, it, true)];
+    const int cell = (int)(it - S.item0) * 4 + w;
+    if (cell < S.km.K) fix_cell(S.count, S.perm, S.scratch, cell, buf[w]);
+  }
+}
+
+__global__ __launch_bounds__(256) void bsort_gather_kernel(const SortSeg* segs, int nseg, int64_t nchunks) {
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
+    const int64_t n = S.count[S.km.K], base = (c - S.chunk0) * SORT_CHUNK;
+    if (base < n)
+      gather_chunk(S.perm, n, base, S.x, S.y, S.vx, S.vy, S.vz, S.id, S.x2, S.y2, S.vx2, S.vy2, S.vz2, S.id2);
+  }
+}
+
+__global__ void bsort_setn_kernel(const SortSeg* segs, int nseg) {
+  for (int k = threadIdx.x; k < nseg; k += blockDim.x) segs[k].d_n[0] = (int64_t)segs[k].count[segs[k].km.K];
+}
+
 struct PushShape {
   int ppt, waves;
 };
@@ -332,6 +462,9 @@ struct Batch {
   bool layout_dirty = true;
   int built_variant = -1, built_nb = -1;  // shapes the tile and chunk lists were built for
   std::vector<long long> shape_key;  // launch dimensions the graph was captured with
+
+  DevBuf d_sortsegs[4];                    // one per species round
+  std::vector<SortSeg> h_sortsegs[4];
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -683,7 +816,69 @@ static hipError_t pre_step(Batch* B) {
   return hipSuccess;
 }
 
+// Sorts species `round` of the due members' (one launch per sort stage).
+static hipError_t batch_sort_round(Batch* B, const std::vector<Ctx*>& due, int round) {
+  std::vector<SortSeg>& hs = B->h_sortsegs[round];
+  hs.clear();
+  int64_t chunks = 0, items = 0;
+  int kmax = 0;
+  for (Ctx* c : due) {
+    if (round >= c->nsp) continue;
+    SpeciesDev& P = c->sp[round];
+    const epic_domain_desc& d = c->dom;
+    SortSeg g{};
+    g.x = (const float*)P.x; g.y = (const float*)P.y; g.vx = (const float*)P.vx; g.vy = (const float*)P.vy;
+    g.vz = (const float*)P.vz; g.id = P.id;
+    g.x2 = (float*)P.x2; g.y2 = (float*)P.y2; g.vx2 = (float*)P.vx2; g.vy2 = (float*)P.vy2;
+    g.vz2 = (float*)P.vz2; g.id2 = P.id2;
+    g.d_n = P.d_n;
+    g.count = c->sort_count;
+    g.fill = c->sort_fill;
+    g.perm = c->sort_perm;
+    g.scratch = c->sort_perm + c->sort_cap;
+    g.x0 = (float)d.x0; g.y0 = (float)d.y0; g.dx = (float)d.dx; g.dy = (float)d.dy;
+    g.km = key_map(d.nx - 1, d.ny - 1);
+    g.chunk0 = chunks;
+    g.item0 = items;
+    chunks += (P.cap + SORT_CHUNK - 1) / SORT_CHUNK;
+    items += (g.km.K + 3) / 4;
+    kmax = std::max(kmax, g.km.K);
+    hs.push_back(g);
+  }
+  const int nseg = (int)hs.size();
+  if (nseg == 0) return hipSuccess;
+  DevBuf& db = B->d_sortsegs[round];
+  EPIC_HIP_TRY(db.ensure(hs.size() * sizeof(SortSeg)));
+  EPIC_HIP_TRY(hipMemcpyAsync(db.p, hs.data(), hs.size() * sizeof(SortSeg), hipMemcpyHostToDevice, B->stream));
+  const SortSeg* S = (const SortSeg*)db.p;
+  const int gpart = (int)std::min<int64_t>(chunks, (int64_t)B->num_cu * 16);
+  hipLaunchKernelGGL(bsort_zero_kernel, dim3(std::min((kmax + 2 + 255) / 256, 256), nseg), dim3(256), 0, B->stream, S);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bsort_hist_kernel, dim3(gpart), dim3(256), 0, B->stream, S, nseg, chunks);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bsort_scan_kernel, dim3(nseg), dim3(1024), 0, B->stream, S);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bsort_fill_kernel, dim3(gpart), dim3(256), 0, B->stream, S, nseg, chunks);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bsort_fix_kernel, dim3((int)std::min<int64_t>(items, 65536)), dim3(256), 0, B->stream, S, nseg,
+                     items);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bsort_gather_kernel, dim3(gpart), dim3(256), 0, B->stream, S, nseg, chunks);
+  EPIC_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bsort_setn_kernel, dim3(1), dim3(256), 0, B->stream, S, nseg);
+  EPIC_HIP_TRY(hipGetLastError());
+  for (Ctx* c : due) {
+    if (round >= c->nsp) continue;
+    SpeciesDev& P = c->sp[round];
+    std::swap(P.x, P.x2); std::swap(P.y, P.y2); std::swap(P.vx, P.vx2);
+    std::swap(P.vy, P.vy2); std::swap(P.vz, P.vz2); std::swap(P.id, P.id2);
+  }
+  return hipSuccess;
+}
+
 static hipError_t post_step(Batch* B) {
+  std::vector<Ctx*> due;
+  int nsp_max = 0;
   for (Member& m : B->members) {
     Ctx* c = member_ctx(m);
     c->deposit_valid = true;
@@ -694,9 +889,19 @@ static hipError_t post_step(Batch* B) {
     c->step_count++;
     m.steps++;
     if (c->dom.sort_interval > 0 && c->step_count % c->dom.sort_interval == 0) {
-      EPIC_HIP_TRY(sync_counts(c));
-      for (int s = 0; s < c->nsp; s++) EPIC_HIP_TRY(launch_sort(c, s));
+      due.push_back(c);
+      nsp_max = std::max(nsp_max, c->nsp);
     }
+  }
+  if (!due.empty()) {
+    // the previous round's host records may still be read by a pending copy
+    EPIC_HIP_TRY(hipStreamSynchronize(B->stream));
+    for (int r = 0; r < nsp_max && r < 4; r++) EPIC_HIP_TRY(batch_sort_round(B, due, r));
+    for (Ctx* c : due)
+      if (c->nsp > 4) {  // beyond four species: one by one
+        EPIC_HIP_TRY(sync_counts(c));
+        for (int r = 4; r < c->nsp; r++) EPIC_HIP_TRY(launch_sort(c, r));
+      }
   }
   B->steps++;
   return hipSuccess;
@@ -838,6 +1043,7 @@ int epic_batch_destroy(epic_batch* h) {
   DevBuf* bufs[] = {&B->d_sims, &B->d_sps, &B->d_tiles, &B->d_segs, &B->d_chunks, &B->d_mcc,
                     &B->d_app, &B->d_steps, &B->d_part_b, &B->d_part_r, &B->d_relres};
   for (DevBuf* d : bufs) d->release();
+  for (DevBuf& d : B->d_sortsegs) d.release();
   if (B->blas) rocblas_destroy_handle(B->blas);
   B->blas_ws.release();
   for (auto& e : B->ev)

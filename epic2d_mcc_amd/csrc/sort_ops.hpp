@@ -1,0 +1,273 @@
+// Counting sort by cell: the per-chunk and per-cell routines, shared by the
+// single-simulation kernels (aux.hip) and the batched ones (batch.hip).
+//
+// The sort orders particles by a tile-major cell key (16 x 16-cell tiles in
+// row-major order, cells row-major inside a tile) and, inside a cell, by
+// their previous storage index: a stable sort, equal element for element to
+// std::stable_sort by key on the CPU.  Tiles keep a particle that moved a
+// cell or two since the previous sort within a few thousand slots of where it
+// was, so the permutation is local.  Absorbed particles get the key K (the
+// key space size) and are dropped, so a sort also compacts.
+//
+// Stages: histogram of keys -> exclusive scan (start) -> slot reservation
+// (perm: destination slot -> source index; order inside a wavefront kept,
+// order between wavefronts arbitrary) -> per-cell fix (each cell's sources
+// ascending) -> gather into the second buffer.
+#pragma once
+
+#include "device_common.hpp"
+#include "epic_internal.hpp"
+
+namespace epic {
+
+constexpr int SORT_TILE = 16;
+// Particles per thread per pass of the sort kernels: the loads of SORT_U
+// particles are in flight together; a workgroup chunk is 256 * SORT_U.
+constexpr int SORT_U = 4;
+constexpr int SORT_CHUNK = 256 * SORT_U;
+
+struct KeyMap {
+  int ncx, ncy;  // cells
+  int nty;       // tiles along y
+  int K;         // key space: tiles * SORT_TILE^2; K is the key of absorbed particles
+};
+
+inline KeyMap key_map(int ncx, int ncy) {
+  KeyMap m;
+  m.ncx = ncx;
+  m.ncy = ncy;
+  const int ntx = (ncx + SORT_TILE - 1) / SORT_TILE;
+  m.nty = (ncy + SORT_TILE - 1) / SORT_TILE;
+  m.K = ntx * m.nty * SORT_TILE * SORT_TILE;
+  return m;
+}
+
+template <typename P>
+__device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, const KeyMap& m) {
+  if (isnan(x)) return m.K;
+  const P lx = (x - x0) / dx, ly = (y - y0) / dy;
+  const int i = (int)lx, j = (int)ly;
+  if (!(lx >= 0) || !(ly >= 0) || i >= m.ncx || j >= m.ncy) return m.K;
+  return ((i / SORT_TILE) * m.nty + j / SORT_TILE) * (SORT_TILE * SORT_TILE) + (i % SORT_TILE) * SORT_TILE +
+         j % SORT_TILE;
+}
+
+// Runs of equal keys among consecutive lanes: the run's first lane does one
+// atomic for the whole run.
+struct LaneRun {
+  int head;  // lane that starts this lane's run
+  int len;   // run length (valid in the head lane)
+};
+__device__ __forceinline__ LaneRun lane_run(int key) {
+  const int lane = threadIdx.x & 63;
+  // the previous lane's key by a DPP whole-wave shift (wave_shr:1); lane 0
+  // receives a value different from its own key
+  const int prev = __builtin_amdgcn_update_dpp(~key, key, 0x138, 0xf, 0xf, false);
+  const bool h = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(h);
+  const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+  LaneRun r;
+  r.head = 63 - __builtin_clzll(upto);
+  const unsigned long long after = lane == 63 ? 0ull : (heads >> (lane + 1));
+  r.len = (after ? __builtin_ctzll(after) + 1 : 64 - lane);
+  return r;
+}
+
+// Lane-xor exchange: ds_swizzle's xor mode inside 32-lane halves, a
+// permute across them.
+template <int J>
+__device__ __forceinline__ int xor_lane(int v) {
+  if (J < 32) return __builtin_amdgcn_ds_swizzle(v, (J << 10) | 0x1F);
+  return __shfl_xor(v, J, 64);
+}
+
+// Ascending bitonic sort of one int per lane across the 64-lane wavefront.
+__device__ __forceinline__ int wave_sort(int v) {
+  const int lane = threadIdx.x & 63;
+#define EPIC_BSTEP(K, J)                                       \
+  {                                                            \
+    const int o = xor_lane<J>(v);                              \
+    const bool asc = (lane & K) == 0, lower = (lane & J) == 0; \
+    v = (lower == asc) ? min(v, o) : max(v, o);                \
+  }
+  EPIC_BSTEP(2, 1)
+  EPIC_BSTEP(4, 2) EPIC_BSTEP(4, 1)
+  EPIC_BSTEP(8, 4) EPIC_BSTEP(8, 2) EPIC_BSTEP(8, 1)
+  EPIC_BSTEP(16, 8) EPIC_BSTEP(16, 4) EPIC_BSTEP(16, 2) EPIC_BSTEP(16, 1)
+  EPIC_BSTEP(32, 16) EPIC_BSTEP(32, 8) EPIC_BSTEP(32, 4) EPIC_BSTEP(32, 2) EPIC_BSTEP(32, 1)
+  EPIC_BSTEP(64, 32) EPIC_BSTEP(64, 16) EPIC_BSTEP(64, 8) EPIC_BSTEP(64, 4) EPIC_BSTEP(64, 2) EPIC_BSTEP(64, 1)
+#undef EPIC_BSTEP
+  return v;
+}
+
+// Histogram of one workgroup chunk (256*SORT_U particles from base;
+// block-uniform call).  The chunk counts into an LDS window of HWIN keys
+// centred on its first key (after a wavefront sort, equal keys add once),
+// then adds the window's non-zero counts to the global histogram: particles
+// that moved less than a tile since the previous sort cost no global atomic.
+// Keys outside the window go straight to the global histogram.
+constexpr int HWIN = 4096;
+
+template <typename P>
+__device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, int64_t base, int* count, P x0, P y0,
+                                           P dx, P dy, const KeyMap& m, int* win, int* wbase_s) {
+  const int lane = threadIdx.x & 63;
+  int key[SORT_U];
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+    key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
+  }
+  if (threadIdx.x == 0) *wbase_s = key[0] >= 0 && key[0] < m.K ? max(0, key[0] - HWIN / 2) : 0;
+  for (int t = threadIdx.x; t < HWIN; t += blockDim.x) win[t] = 0;
+  __syncthreads();
+  const int wbase = *wbase_s;
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int ks = wave_sort(key[u]);
+    const LaneRun r = lane_run(ks);
+    if (r.head == lane && ks >= 0) {
+      const int loc = ks - wbase;
+      if (loc >= 0 && loc < HWIN) atomicAdd(&win[loc], r.len);
+      else atomicAdd(&count[ks], r.len);
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < HWIN; t += blockDim.x) {
+    const int v = win[t];
+    if (v) atomicAdd(&count[wbase + t], v);
+  }
+  __syncthreads();
+}
+
+// Slots of one chunk: the wavefront sorts its (key, lane) pairs (packed as
+// key * 64 + lane; key spaces below 2^25), so equal keys form one run in lane
+// (= index) order; the run's first lane reserves len consecutive slots with
+// one atomic.
+template <typename P>
+__device__ __forceinline__ void fill_chunk(const P* x, const P* y, int64_t n, int64_t base, const int* start,
+                                           int* fill, uint32_t* perm, P x0, P y0, P dx, P dy, const KeyMap& m) {
+  const int dead = m.K;
+  const int lane = threadIdx.x & 63;
+  int key[SORT_U];
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+    key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const bool place = key[u] >= 0 && key[u] != dead;
+    const int pk = wave_sort(place ? key[u] * 64 + lane : 0x7fffffff);
+    const int ks = pk == 0x7fffffff ? -1 : pk >> 6;
+    const LaneRun r = lane_run(ks);
+    int b = 0;
+    if (r.head == lane && ks >= 0) b = atomicAdd(&fill[ks], r.len);
+    b = __shfl(b, r.head, 64);
+    if (ks >= 0) {
+      const int64_t p = base + (int64_t)u * blockDim.x + (threadIdx.x & ~63) + (pk & 63);
+      perm[start[ks] + b + (lane - r.head)] = (uint32_t)p;
+    }
+  }
+}
+
+// One cell's sources ascending (one wavefront): rank counting (rank = number
+// of smaller indices; indices are unique).  Cells of up to FIX_LDS particles
+// are staged in LDS (buf), larger ones in scratch (same offsets as perm).
+constexpr int FIX_LDS = 1024;
+constexpr int FIX_Q = FIX_LDS / 64;
+
+__device__ __forceinline__ void fix_cell(const int* start, uint32_t* perm, uint32_t* scratch, int cell,
+                                         uint32_t* buf) {
+  const int lane = threadIdx.x & 63;
+  const int b = start[cell], e = start[cell + 1], m = e - b;
+  if (m <= 1) return;
+  // already in order?  (common: the cell's particles came from one wavefront)
+  for (int k0 = 0; k0 < m - 1; k0 += 64) {
+    const int k = k0 + lane;
+    const bool ok = k >= m - 1 || perm[b + k] < perm[b + k + 1];
+    if (__ballot(!ok)) goto unordered;
+  }
+  return;
+unordered:
+  if (m <= FIX_LDS) {
+    for (int k = lane; k < m; k += 64) buf[k] = perm[b + k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int cnt = (m + 63) / 64;
+    uint32_t mine[FIX_Q];
+    int rank[FIX_Q];
+#pragma unroll
+    for (int q = 0; q < FIX_Q; q++) {
+      const int k = lane + q * 64;
+      mine[q] = (q < cnt && k < m) ? buf[k] : 0xFFFFFFFFu;
+      rank[q] = 0;
+    }
+    if (cnt <= 2) {
+      for (int t = 0; t < m; t++) {
+        const uint32_t o = buf[t];
+        rank[0] += o < mine[0] ? 1 : 0;
+        rank[1] += o < mine[1] ? 1 : 0;
+      }
+    } else {
+      for (int t = 0; t < m; t++) {
+        const uint32_t o = buf[t];
+#pragma unroll
+        for (int q = 0; q < FIX_Q; q++) rank[q] += (q < cnt && o < mine[q]) ? 1 : 0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < FIX_Q; q++) {
+      const int k = lane + q * 64;
+      if (q < cnt && k < m) perm[b + rank[q]] = mine[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    for (int k = lane; k < m; k += 64) scratch[b + k] = perm[b + k];
+    __threadfence();
+    __builtin_amdgcn_wave_barrier();
+    for (int k0 = 0; k0 < m; k0 += 64) {
+      const int k = k0 + lane;
+      const uint32_t me = k < m ? __builtin_nontemporal_load(&scratch[b + k]) : 0xFFFFFFFFu;
+      int rk = 0;
+      for (int t = 0; t < m; t++) rk += (__builtin_nontemporal_load(&scratch[b + t]) < me) ? 1 : 0;
+      if (k < m) perm[b + rk] = me;
+    }
+    __threadfence();
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Gather of one chunk into the second buffer (n = live total).
+template <typename P>
+__device__ __forceinline__ void gather_chunk(const uint32_t* perm, int64_t n, int64_t base, const P* x, const P* y,
+                                             const P* vx, const P* vy, const P* vz, const uint32_t* id, P* x2,
+                                             P* y2, P* vx2, P* vy2, P* vz2, uint32_t* id2) {
+  uint32_t p[SORT_U];
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+    p[u] = i < n ? perm[i] : 0u;
+  }
+  P a[SORT_U], b[SORT_U], c[SORT_U], d[SORT_U], e[SORT_U];
+  uint32_t f[SORT_U];
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+    if (i < n) {
+      a[u] = x[p[u]]; b[u] = y[p[u]]; c[u] = vx[p[u]]; d[u] = vy[p[u]]; e[u] = vz[p[u]]; f[u] = id[p[u]];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+    if (i < n) {
+      x2[i] = a[u]; y2[i] = b[u]; vx2[i] = c[u]; vy2[i] = d[u]; vz2[i] = e[u]; id2[i] = f[u];
+    }
+  }
+}
+
+}  // namespace epic
