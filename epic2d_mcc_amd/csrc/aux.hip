@@ -261,12 +261,13 @@ __global__ __launch_bounds__(256) void sort_fill_kernel(const P* x, const P* y, 
     fill_chunk(x, y, n, base, start, fill, perm, x0, y0, dx, dy, m);
 }
 
-// One wavefront per cell.
+// One wavefront per FIX_GROUP cells.
 __global__ __launch_bounds__(256) void sort_fix_kernel(const int* start, uint32_t* perm, uint32_t* scratch,
                                                         int ncell) {
-  __shared__ uint32_t buf[4][FIX_LDS];
+  __shared__ __attribute__((aligned(16))) uint32_t buf[4][FIX_LDS + 4];
   const int w = threadIdx.x >> 6;
-  for (int cell = blockIdx.x * 4 + w; cell < ncell; cell += gridDim.x * 4) fix_cell(start, perm, scratch, cell, buf[w]);
+  for (int c0 = (blockIdx.x * 4 + w) * FIX_GROUP; c0 < ncell; c0 += gridDim.x * 4 * FIX_GROUP)
+    fix_cells(start, perm, scratch, c0, ncell, buf[w]);
 }
 
 // Each XCD walks its own contiguous range of chunks: the source slots of
@@ -308,7 +309,8 @@ static hipError_t sort_t(Ctx* c, int s) {
   hipLaunchKernelGGL(sort_fill_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
                      c->sort_count, c->sort_fill, c->sort_perm, x0, y0, dx, dy, km);
   EPIC_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(sort_fix_kernel, dim3(std::min((ncell + 3) / 4, 65536)), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(sort_fix_kernel, dim3(std::min((ncell + 4 * FIX_GROUP - 1) / (4 * FIX_GROUP), 65536)), dim3(256),
+                     0, c->stream,
                      c->sort_count, c->sort_perm, c->sort_perm + c->sort_cap, ncell);
   EPIC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(sort_gather_kernel<P>, dim3(g), dim3(256), 0, c->stream, c->sort_perm, c->sort_count + ncell,

@@ -255,7 +255,7 @@ struct SortSeg {
   float x0, y0, dx, dy;
   KeyMap km;
   int64_t chunk0;  // first particle chunk of this segment
-  int64_t item0;   // first 4-cell group of this segment
+  int64_t item0;   // first group of 4 * FIX_GROUP cells of this segment
 };
 
 __device__ __forceinline__ int find_seg(const SortSeg* segs, int nseg, int64_t v, bool cells) {
@@ -342,12 +342,12 @@ __global__ __launch_bounds__(256) void bsort_fill_kernel(const SortSeg* segs, in
 }
 
 __global__ __launch_bounds__(256) void bsort_fix_kernel(const SortSeg* segs, int nseg, int64_t nitems) {
-  __shared__ uint32_t buf[4][FIX_LDS];
+  __shared__ __attribute__((aligned(16))) uint32_t buf[4][FIX_LDS + 4];
   const int w = threadIdx.x >> 6;
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
     const SortSeg& S = segs[find_seg(segs, nseg, it, true)];
-    const int cell = (int)(it - S.item0) * 4 + w;
-    if (cell < S.km.K) fix_cell(S.count, S.perm, S.scratch, cell, buf[w]);
+    const int c0 = ((int)(it - S.item0) * 4 + w) * FIX_GROUP;
+    if (c0 < S.km.K) fix_cells(S.count, S.perm, S.scratch, c0, S.km.K, buf[w]);
   }
 }
 
@@ -841,7 +841,7 @@ static hipError_t batch_sort_round(Batch* B, const std::vector<Ctx*>& due, int r
     g.chunk0 = chunks;
     g.item0 = items;
     chunks += (P.cap + SORT_CHUNK - 1) / SORT_CHUNK;
-    items += (g.km.K + 3) / 4;
+    items += (g.km.K + 4 * FIX_GROUP - 1) / (4 * FIX_GROUP);
     kmax = std::max(kmax, g.km.K);
     hs.push_back(g);
   }

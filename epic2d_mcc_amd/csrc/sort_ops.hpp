@@ -177,51 +177,49 @@ __device__ __forceinline__ void fill_chunk(const P* x, const P* y, int64_t n, in
 constexpr int FIX_LDS = 1024;
 constexpr int FIX_Q = FIX_LDS / 64;
 
-__device__ __forceinline__ void fix_cell(const int* start, uint32_t* perm, uint32_t* scratch, int cell,
-                                         uint32_t* buf) {
+// Ranks of a staged cell of m <= 64 * CNT indices; four staged indices per
+// LDS read (the staging pads to a multiple of 4 with all-ones, which is
+// never smaller).
+template <int CNT>
+__device__ __forceinline__ void fix_rank_lds(uint32_t* perm, int b, int m, const uint32_t* buf) {
   const int lane = threadIdx.x & 63;
-  const int b = start[cell], e = start[cell + 1], m = e - b;
-  if (m <= 1) return;
-  // already in order?  (common: the cell's particles came from one wavefront)
-  for (int k0 = 0; k0 < m - 1; k0 += 64) {
-    const int k = k0 + lane;
-    const bool ok = k >= m - 1 || perm[b + k] < perm[b + k + 1];
-    if (__ballot(!ok)) goto unordered;
+  uint32_t mine[CNT];
+  int rank[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; q++) {
+    const int k = lane + q * 64;
+    mine[q] = k < m ? buf[k] : 0xFFFFFFFFu;
+    rank[q] = 0;
   }
-  return;
-unordered:
+  for (int t = 0; t < m; t += 4) {
+    const uint4 o = *reinterpret_cast<const uint4*>(buf + t);
+#pragma unroll
+    for (int q = 0; q < CNT; q++)
+      rank[q] += (o.x < mine[q] ? 1 : 0) + (o.y < mine[q] ? 1 : 0) + (o.z < mine[q] ? 1 : 0) + (o.w < mine[q] ? 1 : 0);
+  }
+#pragma unroll
+  for (int q = 0; q < CNT; q++) {
+    const int k = lane + q * 64;
+    if (k < m) perm[b + rank[q]] = mine[q];
+  }
+}
+
+// Sort one cell known to be out of order (buf: FIX_LDS + 4 entries, 16-byte
+// aligned).
+__device__ __forceinline__ void fix_unordered(uint32_t* perm, uint32_t* scratch, int b, int m, uint32_t* buf) {
+  const int lane = threadIdx.x & 63;
   if (m <= FIX_LDS) {
     for (int k = lane; k < m; k += 64) buf[k] = perm[b + k];
+    if (lane < 4) buf[m + lane] = 0xFFFFFFFFu;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cnt = (m + 63) / 64;
-    uint32_t mine[FIX_Q];
-    int rank[FIX_Q];
-#pragma unroll
-    for (int q = 0; q < FIX_Q; q++) {
-      const int k = lane + q * 64;
-      mine[q] = (q < cnt && k < m) ? buf[k] : 0xFFFFFFFFu;
-      rank[q] = 0;
-    }
-    if (cnt <= 2) {
-      for (int t = 0; t < m; t++) {
-        const uint32_t o = buf[t];
-        rank[0] += o < mine[0] ? 1 : 0;
-        rank[1] += o < mine[1] ? 1 : 0;
-      }
-    } else {
-      for (int t = 0; t < m; t++) {
-        const uint32_t o = buf[t];
-#pragma unroll
-        for (int q = 0; q < FIX_Q; q++) rank[q] += (q < cnt && o < mine[q]) ? 1 : 0;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < FIX_Q; q++) {
-      const int k = lane + q * 64;
-      if (q < cnt && k < m) perm[b + rank[q]] = mine[q];
-    }
+    if (cnt == 1) fix_rank_lds<1>(perm, b, m, buf);
+    else if (cnt == 2) fix_rank_lds<2>(perm, b, m, buf);
+    else if (cnt <= 4) fix_rank_lds<4>(perm, b, m, buf);
+    else if (cnt <= 8) fix_rank_lds<8>(perm, b, m, buf);
+    else fix_rank_lds<FIX_Q>(perm, b, m, buf);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -238,6 +236,36 @@ unordered:
     }
     __threadfence();
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// FIX_GROUP consecutive cells per wavefront: their FIX_GROUP + 1 starts
+// are read at once and all their slots checked in one strided pass (a
+// descent between two slots of one cell marks the cell), so the dependent
+// loads of the check are shared by the group; marked cells are then sorted.
+constexpr int FIX_GROUP = 8;
+
+__device__ __forceinline__ void fix_cells(const int* start, uint32_t* perm, uint32_t* scratch, int c0, int ncell,
+                                          uint32_t* buf) {
+  const int lane = threadIdx.x & 63;
+  const int nc = min(FIX_GROUP, ncell - c0);
+  const int sv = lane <= nc ? start[c0 + lane] : 0;
+  int st[FIX_GROUP + 1];
+#pragma unroll
+  for (int q = 0; q <= FIX_GROUP; q++) st[q] = __builtin_amdgcn_readlane(sv, q);
+  const int b0 = st[0], e0 = st[nc];
+  unsigned mark = 0;
+  for (int t = b0 + lane; t + 1 < e0; t += 64) {
+    if (perm[t] > perm[t + 1]) {
+      // the cell of slot t, when slot t + 1 belongs to it too
+#pragma unroll
+      for (int q = 0; q < FIX_GROUP; q++)
+        if (q < nc && t >= st[q] && t + 1 < st[q + 1]) mark |= 1u << q;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < FIX_GROUP; q++) {
+    if (q < nc && __ballot((mark >> q) & 1u)) fix_unordered(perm, scratch, st[q], st[q + 1] - st[q], buf);
   }
 }
 
