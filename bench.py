@@ -34,13 +34,15 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 PUSH_TRAFFIC_BYTES_PER_PARTICLE = (214624.42 * 2 + 430505.95) * 1024 / 26214400
 
 
-def gpu_run(st, steps, warmup, solver_name):
+def gpu_run(st, steps, warmup, solver_name, mcc=None):
     ctx = _lib.Context(st.domain, st.species)
     nsp = len(st.species)
     for s, sp in enumerate(st.species):
         ctx.load(s, sp.numparticle, vz_thermal=False)
     for s in range(nsp):
         ctx.sort(s)
+    if mcc is not None:
+        ctx.mcc_setup(mcc)
     ctx.initial_field()
     ts = 0
     for _ in range(warmup):
@@ -62,12 +64,13 @@ def gpu_run(st, steps, warmup, solver_name):
     n1 = sum(ctx.count(s) for s in range(nsp))
     per_sp1 = [ctx.count(s) for s in range(nsp)]
     errs = ctx.errors()
+    coll = [int(v) for v in ctx.mcc_stats()["count"]] if mcc is not None else None
     ctx.close()
     return dict(t=t1 - t0, n0=n0, n1=n1, per_sp0=per_sp0, per_sp1=per_sp1, timing=tm, iters=iters,
-                errors=errs.tolist())
+                errors=errs.tolist(), collisions=coll)
 
 
-def cpu_baseline(st, sample_frac=1.0, steps=2):
+def cpu_baseline(st, sample_frac=1.0, steps=2, gas=None):
     """The oracle (FP64 CPU restatement of the reference loop, one thread) on a
     bounded sample of the same workload: the full 512x512 grid and one step,
     with `sample_frac` of the particles.  Particle stages (scatter, push) are
@@ -81,7 +84,10 @@ def cpu_baseline(st, sample_frac=1.0, steps=2):
     d = st.domain
     dom = _lib.DomainDesc.from_buffer_copy(d)
     dom.solver = _lib.SOLVER_PCG
-    o = OracleSim(dom, st.species)
+    o = OracleSim(dom, st.species, gas=gas)
+    if gas is not None:
+        from oracle_api import oracle
+        oracle().orc_mcc_build_tables(o.h)
     rng = np.random.default_rng(7)
     Lx = (d.nx - 1) * d.dx
     Ly = (d.ny - 1) * d.dy
@@ -115,6 +121,12 @@ def cpu_baseline(st, sample_frac=1.0, steps=2):
         t2 = t()
         for s in range(nsp):
             o.push(s)
+        if gas is not None:
+            import ctypes
+            from oracle_api import oracle
+            counts = (ctypes.c_long * 3)()
+            eloss = ctypes.c_double()
+            oracle().orc_collide_direct(o.h, 0, 1, counts, ctypes.byref(eloss))
         t3 = t()
         tp += (t1 - t0) + (t3 - t2)
         tf += t2 - t1
@@ -343,6 +355,51 @@ def main_serve(args):
     print(json.dumps(out))
 
 
+def main_single(args, st, label, dtype, bytes_ps, kernel, mcc=None, gas=None, cpu_note=""):
+    """One simulation of the given set-up: whole-step particle-steps/s, the
+    fused push's roofline, and the oracle on the host beside it."""
+    r = gpu_run(st, args.steps, args.warmup, label, mcc=mcc)
+    nsp = len(st.species)
+    value = 0.5 * (r["n0"] + r["n1"]) * args.steps / r["t"]
+    push_ms, push_calls = r["timing"]["push"]
+    avg_launch_s = push_ms / max(push_calls, 1) / 1e3
+    parts_per_launch = 0.5 * (sum(r["per_sp0"]) + sum(r["per_sp1"])) / nsp
+    achieved = bytes_ps * parts_per_launch / avg_launch_s / 1e9
+    stage = {k: round(v[0] / max(args.steps, 1), 4) for k, v in r["timing"].items() if v[1]}
+    out = {
+        "metric": "particle-steps/s per GPU (+% HBM roofline)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["t"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
+        "config": {"workload": label, "particles_start": r["n0"], "particles_end": r["n1"],
+                   "poisson_iterations_mean": float(np.mean(r["iters"])) if r["iters"] else 0,
+                   "stage_ms_per_step": stage, "collisions": r["collisions"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "bytes_per_particle_step": bytes_ps, "avg_launch_ms": avg_launch_s * 1e3,
+                     "particles_per_launch": parts_per_launch},
+        "device_errors": r["errors"],
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps, gas=gas)
+        out["cpu_baseline"] = {
+            "value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of the same workload by the FP64 oracle on one thread, %d of the %d particles "
+                      "(particle stages scaled x%.2f), %.2f s particle + %.2f s field stages per step, Poisson by "
+                      "the matrix-free pcg() restatement (%s iterations)%s; %.1f s of CPU work" % (
+                          cb["steps"], cb["nsamp"], cb["npart"], cb["npart"] / cb["nsamp"], cb["t_particles"],
+                          cb["t_fields"], cb["iters"], cpu_note, cb["t"])}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,7 +410,10 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="bounded512", choices=["bounded512", "ensemble64", "serve"])
+    ap.add_argument("--workload", default="bounded512",
+                    choices=["bounded512", "periodic64", "mcc256", "ensemble64", "serve"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"],
+                    help="bounded512: fp64 runs the plain FP64 path (FP64 particles, the reference PCG)")
     ap.add_argument("--serve-steps-scale", type=float, default=1.0,
                     help="multiplies the jobs' 100-2000 step range")
     ap.add_argument("--serve-duration", type=float, default=30.0)
@@ -373,6 +433,28 @@ def main():
         return
     if args.workload == "serve":
         main_serve(args)
+        return
+    if args.workload == "periodic64":
+        st = workloads.periodic_64()
+        main_single(args, st, "periodic 64x64 cells, electron + ion (409,600 each), FP64 particles, reference PCG "
+                              "(Jacobi, FP64) to 1e-8 relative", "fp64", 64,
+                    "particle_kernel<double,2,PUSH,whole-grid LDS,deposit> (gather + push + wrap + deposit)")
+        return
+    if args.workload == "mcc256":
+        from epic2d_mcc_amd import xsec
+        st = workloads.mcc_256()
+        mcc = xsec.mcc_desc(xsec.synthetic_ar(), gas_density=1e21, gas_temp_K=300.0, enable=(1, 1, 1, 1, 1))
+        main_single(args, st, "periodic 256x256 cells, electron + Ar+ at 1e15 m^-3, 200 ppc each (26,214,400), "
+                              "FP32, Ar neutrals 1e21 m^-3 at 300 K (null-collision MCC, e-Ar elastic/excitation/"
+                              "ionisation, Ar+-Ar elastic/CX), mixed Poisson 1e-10, sort every 20 steps", "fp32", 32,
+                    "push_f32_kernel<true,4,1> (gather + push + wrap + deposit)", mcc=mcc, gas=(1, 1, 1, 1e21),
+                    cpu_note=", electron collisions by the reference's direct method")
+        return
+    if args.precision == "fp64":
+        st = workloads.bounded_512(ppc=args.ppc, precision=_lib.FP64, solver=_lib.SOLVER_PCG)
+        main_single(args, st, "bounded 512x512, 4 species x %d ppc, FP64 particles (the plain path), reference PCG "
+                              "(Jacobi, FP64) on the Dirichlet-wall operator to 1e-10 relative" % args.ppc, "fp64", 64,
+                    "push_f64_kernel<2,deposit> (gather + push + walls + deposit)")
         return
     solver = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
               "gs": _lib.SOLVER_GS}[args.solver]

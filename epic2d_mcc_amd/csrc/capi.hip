@@ -198,6 +198,9 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PUSH_VARIANT")) c->push_variant = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_GRID")) c->push_grid_per_cu = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MAP")) c->push_map = atoi(v);
+  if (const char* v = getenv("EPIC_PART_MAP")) c->part_map = atoi(v);
+  if (const char* v = getenv("EPIC_PART_PPT")) c->part_ppt = atoi(v);
+  if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
   c->max_solver_blocks = c->num_cu;
   if (ext) {
     c->stream = ext;

@@ -128,8 +128,11 @@ struct Ctx {
   MccState mcc;
 
   int push_variant = 0;        // FP32 push kernel shape (see particles.hip)
-  int push_grid_per_cu = 0;    // FP32 push workgroups per CU (0: co-resident maximum)
-  int push_map = 1;            // FP32 push chunk walk: 0 per-XCD ranges, 1 round-robin
+  int push_grid_per_cu = 0;    // FP32 / FP64 push workgroups per CU (0: co-resident maximum)
+  int push_map = 1;            // push chunk walk: 0 per-XCD ranges, 1 round-robin (FP32 push)
+  int part_map = 0;            // same, generic particle kernel
+  int part_ppt = 2;            // FP64 particle kernels: particles per thread (2 or 4; generic also 8)
+  int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   bool deposit_valid = false;  // Q holds the deposit of the current positions
   int step_count = 0;
 

@@ -37,6 +37,8 @@ struct PartArgs {
   unsigned long long* Q;
   int fix_bits;
   unsigned long long* errc;
+  int rr;              // chunk walk: 0 per-XCD contiguous ranges, 1 round-robin
+  int fastdiv;         // FP64: dx, dy in [2^-400, 2^400], so x_to_l may divide by reciprocal + correction
 };
 
 template <typename P>
@@ -50,8 +52,50 @@ __device__ __forceinline__ float p_nan<float>() {
   return __builtin_nanf("");
 }
 
-__device__ __forceinline__ long long fix_round(double w, double scale) { return (long long)rint(w * scale); }
+// rint(w * scale) as an integer.  For |w * scale| < 2^51 the sum
+// w*scale + 1.5*2^52 lies in [2^52, 2^53), where doubles are exactly the
+// integers: the addition rounds w*scale to the nearest integer, ties to even
+// (1.5*2^52 is even), and the bit pattern minus that of 1.5*2^52 is it.
+// Deposit weights are below 4 in magnitude (bounds-checked cells) and the
+// scale is 2^44.
+static_assert(FIX_BITS_FP64 <= 48, "fix_round needs |w| * 2^fix_bits < 2^51");
+__device__ __forceinline__ long long fix_round(double w, double scale) {
+  const double t = w * scale + 0x1.8p52;
+  return __double_as_longlong(t) - 0x4338000000000000ll;
+}
 __device__ __forceinline__ long long fix_round(float w, float scale) { return (long long)rintf(w * scale); }
+
+// (x - x0) / dx, the reference's XtoL/YtoL (domain.cpp:257-264), correctly
+// rounded.  FP64: q = RN(a * r) with r = RN(1/dx), then two correction steps
+// q = RN(q + RN(a - q*dx) * r) (fused multiply-adds).  The first brings q
+// within one ulp of a/dx; the second is then exactly RN(a/dx) by Markstein's
+// theorem (P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990: r within half
+// an ulp of 1/b and q within one ulp of a/b give RN(q + (a - bq) r) =
+// RN(a/b)).  The theorem needs the remainder and products to stay normal:
+// |a| and dx in [2^-400, 2^400]; anything else (a == 0, infinities, NaN)
+// takes the division instruction sequence.
+template <typename P>
+__device__ __forceinline__ P x_to_l(const PartArgs<P>& a, P x, P x0, P d, P rd) {
+  return (x - x0) / d;
+}
+template <>
+__device__ __forceinline__ double x_to_l<double>(const PartArgs<double>& a, double x, double x0, double d,
+                                                 double rd) {
+  const double n = x - x0;
+  const double m = fabs(n);
+  double q;
+  if (a.fastdiv && m >= 0x1p-400 && m <= 0x1p400) {
+    q = n * rd;
+    double r = fma(-q, d, n);
+    q = fma(r, rd, q);
+    r = fma(-q, d, n);
+    q = fma(r, rd, q);
+  } else {
+    q = n / d;
+  }
+  return q;
+}
 
 // One gather of (ex, ey) at logical position (lx, ly), following
 // Domain::Interpolate (domain.cpp:231-255) operation for operation.
@@ -88,8 +132,8 @@ __device__ __forceinline__ bool gather(const PartArgs<P>& a, P lx, P ly, int& ck
 template <typename P>
 __device__ __forceinline__ bool dep_cell(const PartArgs<P>& a, P x, P y, int& i, int& j, P& di, P& dj) {
   // FP64: the reference's XtoL division; FP32: the hot kernel's reciprocal
-  const P lx = sizeof(P) == 8 ? (x - a.x0) / a.dx : (x - a.x0) * a.idx;
-  const P ly = sizeof(P) == 8 ? (y - a.y0) / a.dy : (y - a.y0) * a.idy;
+  const P lx = sizeof(P) == 8 ? x_to_l(a, x, a.x0, a.dx, a.idx) : (x - a.x0) * a.idx;
+  const P ly = sizeof(P) == 8 ? x_to_l(a, y, a.y0, a.dy, a.idy) : (y - a.y0) * a.idy;
   if (a.shape == EPIC_SHAPE_NGP) {
     i = (int)round(lx);
     j = (int)round(ly);
@@ -103,11 +147,149 @@ __device__ __forceinline__ bool dep_cell(const PartArgs<P>& a, P x, P y, int& i,
   return i >= 0 && i + 1 < a.nx && j >= 0 && j + 1 < a.ny;
 }
 
+// A chunk known to be full: the vector loads only.
+template <typename P, int PPT>
+__device__ __forceinline__ void ld_run_full(const P* __restrict__ a, int64_t wbase, int lane, P (&o)[PPT]) {
+  constexpr int V = 16 / (int)sizeof(P);
+#pragma unroll
+  for (int g = 0; g < PPT / V; g++) {
+    const v4u32 t = __builtin_nontemporal_load(
+        reinterpret_cast<const v4u32*>(a + wbase + (int64_t)g * 64 * V + (int64_t)lane * V));
+    __builtin_memcpy(&o[g * V], &t, 16);
+  }
+}
+
+// Deposit cells of a thread's particles: base node (i, j), offsets (di, dj),
+// -1 for dead or out-of-grid particles, and the thread's bounding box.
+template <typename P, int PPT>
+struct DepCells {
+  int ci[PPT], cj[PPT];
+  P di[PPT], dj[PPT];
+  int imin, imax, jmin, jmax;
+  int bad;  // live particles outside the grid (the reference's scatter throws)
+};
+
+template <typename P, int PPT>
+__device__ __forceinline__ void dep_cells(const PartArgs<P>& a, const bool (&live)[PPT], const P (&x)[PPT],
+                                          const P (&y)[PPT], DepCells<P, PPT>& d) {
+  d.imin = 1 << 30; d.imax = -1; d.jmin = 1 << 30; d.jmax = -1;
+  d.bad = 0;
+#pragma unroll
+  for (int k = 0; k < PPT; k++) {
+    d.ci[k] = -1;
+    d.cj[k] = 0;
+    d.di[k] = d.dj[k] = 0;
+    if (!live[k]) continue;
+    int i, j;
+    P di, dj;
+    if (!dep_cell(a, x[k], y[k], i, j, di, dj)) {
+      d.bad++;
+      continue;
+    }
+    d.ci[k] = i;
+    d.cj[k] = j;
+    d.di[k] = di;
+    d.dj[k] = dj;
+    d.imin = min(d.imin, i); d.imax = max(d.imax, i);
+    d.jmin = min(d.jmin, j); d.jmax = max(d.jmax, j);
+  }
+}
+
+// Deposit of one chunk's cells.  WHOLE: into the workgroup's whole-grid LDS
+// histogram.  Otherwise the wavefront's LDS window is placed on the bounding
+// box of its cells, filled, and flushed to the global accumulator.
+template <typename P, int PPT, bool WHOLE>
+__device__ __forceinline__ void deposit_cells(const PartArgs<P>& a, int lane, unsigned long long* wl, Window& win,
+                                              const DepCells<P, PPT>& d) {
+  const int ny = a.ny;
+  const int ncorner = a.shape == EPIC_SHAPE_NGP ? 1 : 4;
+  const P scale = (P)ldexp(1.0, a.fix_bits);
+  const long long one = 1ll << a.fix_bits;
+  bool any = true;
+  if (!WHOLE) {
+    const int gi0 = wave_min_dpp(d.imin), gi1 = wave_max_dpp(d.imax);
+    any = gi1 >= 0;  // uniform
+    if (any) {
+      const int gj0 = wave_min_dpp(d.jmin), gj1 = wave_max_dpp(d.jmax);
+      const int wi = gi1 - gi0 + 2, wj = gj1 - gj0 + 2;
+      if (wi * wj <= WAVE_WIN) {
+        win = Window{gi0, gj0, wi, wj};
+      } else {
+        // too wide (drifted outliers): a window on the wavefront's first
+        // cells, the rest goes straight to global atomics
+        const int wjc = min(wj, 32);
+        win = Window{gi0, gj0, WAVE_WIN / wjc, wjc};
+      }
+      for (int k = lane; k < win.wi * win.wj; k += 64) wl[k] = 0ull;
+      wave_fence();
+    }
+  }
+  if (any) {
+    // run-length accumulation over this thread's consecutive particles
+    int rk_i = -1, rk_j = -1;
+    unsigned long long acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      if (d.ci[k] < 0) continue;
+      const int i = d.ci[k], j = d.cj[k];
+      const P di = d.di[k], dj = d.dj[k];
+      if (i != rk_i || j != rk_j) {
+        if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, wl, a.Q);
+        rk_i = i;
+        rk_j = j;
+        acc[0] = acc[1] = acc[2] = acc[3] = 0;
+      }
+      if (ncorner == 1) {
+        acc[0] += (unsigned long long)one;
+      } else if (sizeof(P) == 4) {
+        // FP32 storage: 14-bit offsets, weights sum to 2^28 exactly
+        const unsigned dI = (unsigned)min((int)rintf((float)di * 16384.0f), 16384);
+        const unsigned dJ = (unsigned)min((int)rintf((float)dj * 16384.0f), 16384);
+        const unsigned oI = 16384u - dI, oJ = 16384u - dJ;
+        acc[0] += oI * oJ;
+        acc[1] += dI * oJ;
+        acc[2] += oI * dJ;
+        acc[3] += dI * dJ;
+      } else {
+        const P omdi = (P)1 - di, omdj = (P)1 - dj;
+        const long long q00 = fix_round(omdi * omdj, scale);
+        const long long q10 = fix_round(di * omdj, scale);
+        const long long q01 = fix_round(omdi * dj, scale);
+        const long long q11 = one - q00 - q10 - q01;
+        acc[0] += (unsigned long long)q00;
+        acc[1] += (unsigned long long)q10;
+        acc[2] += (unsigned long long)q01;
+        acc[3] += (unsigned long long)q11;
+      }
+    }
+    if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, wl, a.Q);
+
+    if (!WHOLE) {
+      wave_fence();
+      const int W = win.wi * win.wj;
+      for (int k = lane; k < W; k += 64) {
+        const unsigned long long v = wl[k];
+        if (v) {
+          const int gi = win.i0 + k / win.wj, gj = win.j0 + k % win.wj;
+          if (gi >= 0 && gj >= 0 && gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
+        }
+      }
+      wave_fence();
+    }
+  }
+  if (d.bad) atomicAdd(&a.errc[EPIC_ERRC_DEPOSIT_OOB], (unsigned long long)d.bad);
+}
+
 // WHOLE: the whole node grid is one LDS histogram per workgroup, flushed once
 // at the end (small grids).  Otherwise each wavefront owns an LDS window of
 // WAVE_WIN node accumulators placed on the bounding box of the cells its
 // 64*PPT particles deposit to; deposits outside go to global atomics.  A
 // wavefront never waits on the rest of its workgroup.
+//
+// Each particle's deposit cell and offsets are computed once, after its push,
+// and kept for the deposit; the next chunk's particles are loaded before this
+// chunk's deposit (software pipeline).  The arithmetic of every particle is
+// the reference's, operation for operation, whatever the schedule.
 template <typename P, int PPT, int MODE, bool WHOLE, bool DEP>
 __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
   constexpr int LDS_N = DEP ? (WHOLE ? DEP_WINDOW : (PART_BLOCK / 64) * WAVE_WIN) : 1;
@@ -119,11 +301,6 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
   unsigned long long* wl = WHOLE ? lds : lds + wid * WAVE_WIN;
   const int64_t n = a.n_fixed >= 0 ? a.n_fixed : *a.d_n;
   const int64_t nchunks = (n + CH - 1) / CH;
-  const int ny = a.ny;
-  const int ncorner = a.shape == EPIC_SHAPE_NGP ? 1 : 4;
-  const P scale = (P)ldexp(1.0, a.fix_bits);
-  const long long one = 1ll << a.fix_bits;
-
   Window win{0, 0, a.nx, a.ny};
   if (DEP && WHOLE) {
     for (int k = threadIdx.x; k < a.nx * a.ny; k += blockDim.x) lds[k] = 0ull;
@@ -131,16 +308,25 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
   }
 
   int64_t cb, ce_, cs;
-  xcd_chunk_range(nchunks, &cb, &ce_, &cs);
+  if (a.rr) {
+    cb = blockIdx.x;
+    ce_ = nchunks;
+    cs = gridDim.x;
+  } else {
+    xcd_chunk_range(nchunks, &cb, &ce_, &cs);
+  }
+  P x[PPT], y[PPT], vx[PPT], vy[PPT];
+  if (cb < ce_) {
+    const int64_t wb0 = cb * CH + (int64_t)wid * CHW;
+    ld_run<P, PPT>(a.x, wb0, lane, n, x);
+    ld_run<P, PPT>(a.y, wb0, lane, n, y);
+    if (MODE != MODE_DEPOSIT) {
+      ld_run<P, PPT>(a.vx, wb0, lane, n, vx);
+      ld_run<P, PPT>(a.vy, wb0, lane, n, vy);
+    }
+  }
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
     const int64_t wb = chunk * CH + (int64_t)wid * CHW;
-    P x[PPT], y[PPT], vx[PPT], vy[PPT];
-    ld_run<P, PPT>(a.x, wb, lane, n, x);
-    ld_run<P, PPT>(a.y, wb, lane, n, y);
-    if (MODE != MODE_DEPOSIT) {
-      ld_run<P, PPT>(a.vx, wb, lane, n, vx);
-      ld_run<P, PPT>(a.vy, wb, lane, n, vy);
-    }
     bool live[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) live[k] = (part_index<P, PPT>(wb, lane, k) < n) && !isnan(x[k]);
@@ -152,8 +338,8 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
 #pragma unroll
       for (int k = 0; k < PPT; k++) {
         if (!live[k]) continue;
-        const P lx = (x[k] - a.x0) / a.dx;
-        const P ly = (y[k] - a.y0) / a.dy;
+        const P lx = x_to_l(a, x[k], a.x0, a.dx, a.idx);
+        const P ly = x_to_l(a, y[k], a.y0, a.dy, a.idy);
         P ex = 0, ey = 0;
         if (!gather(a, lx, ly, ckey, cev, ex, ey)) bad++;
         if (MODE == MODE_REWIND) {
@@ -191,6 +377,16 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
         y[k] = py;
       }
       if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
+    }
+
+    // deposit cells and offsets of the (new) positions, computed once
+    DepCells<P, PPT> dc;
+    if (DEP) dep_cells<P, PPT>(a, live, x, y, dc);
+    // every gather fetch has been consumed: nothing outstanding but the
+    // stores below and the next chunk's loads (this keeps the compiler from
+    // draining those loads before the deposit)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (MODE != MODE_DEPOSIT) {
       st_run<P, PPT>(a.vx, wb, lane, n, vx);
       st_run<P, PPT>(a.vy, wb, lane, n, vy);
       if (MODE == MODE_PUSH) {
@@ -199,92 +395,28 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
       }
     }
 
-    if (!DEP) continue;
-
-    // ---- deposit the (new) positions ------------------------------------
-    int bad = 0;
-    if (!WHOLE) {
-      int imin = 1 << 30, imax = -1, jmin = 1 << 30, jmax = -1;
-#pragma unroll
-      for (int k = 0; k < PPT; k++) {
-        if (!live[k]) continue;
-        int i, j;
-        P di, dj;
-        if (!dep_cell(a, x[k], y[k], i, j, di, dj)) continue;
-        imin = min(imin, i); imax = max(imax, i);
-        jmin = min(jmin, j); jmax = max(jmax, j);
-      }
-      const int gi0 = wave_min(imin), gi1 = wave_max(imax);
-      if (gi1 < 0) continue;  // nothing to deposit in this wavefront chunk (uniform)
-      const int gj0 = wave_min(jmin), gj1 = wave_max(jmax);
-      const int wi = gi1 - gi0 + 2, wj = gj1 - gj0 + 2;
-      if (wi * wj <= WAVE_WIN) {
-        win = Window{gi0, gj0, wi, wj};
-      } else {
-        // too wide (drifted outliers): a window on the wavefront's first
-        // cells, the rest goes straight to global atomics
-        const int wjc = min(wj, 32);
-        win = Window{gi0, gj0, WAVE_WIN / wjc, wjc};
-      }
-      for (int k = lane; k < win.wi * win.wj; k += 64) wl[k] = 0ull;
-      wave_fence();
-    }
-
-    // run-length accumulation over this thread's consecutive particles
-    int rk_i = -1, rk_j = -1;
-    unsigned long long acc[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      if (!live[k]) continue;
-      int i, j;
-      P di, dj;
-      if (!dep_cell(a, x[k], y[k], i, j, di, dj)) {
-        bad++;
-        continue;
-      }
-      if (i != rk_i || j != rk_j) {
-        if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, wl, a.Q);
-        rk_i = i;
-        rk_j = j;
-        acc[0] = acc[1] = acc[2] = acc[3] = 0;
-      }
-      if (ncorner == 1) {
-        acc[0] += (unsigned long long)one;
-      } else if (sizeof(P) == 4) {
-        // FP32 storage: 14-bit offsets, weights sum to 2^28 exactly
-        const unsigned dI = (unsigned)min((int)rintf((float)di * 16384.0f), 16384);
-        const unsigned dJ = (unsigned)min((int)rintf((float)dj * 16384.0f), 16384);
-        const unsigned oI = 16384u - dI, oJ = 16384u - dJ;
-        acc[0] += oI * oJ;
-        acc[1] += dI * oJ;
-        acc[2] += oI * dJ;
-        acc[3] += dI * dJ;
-      } else {
-        const P omdi = (P)1 - di, omdj = (P)1 - dj;
-        const long long q00 = fix_round(omdi * omdj, scale);
-        const long long q10 = fix_round(di * omdj, scale);
-        const long long q01 = fix_round(omdi * dj, scale);
-        const long long q11 = one - q00 - q10 - q01;
-        acc[0] += (unsigned long long)q00;
-        acc[1] += (unsigned long long)q10;
-        acc[2] += (unsigned long long)q01;
-        acc[3] += (unsigned long long)q11;
+    // next chunk's particles (full chunks now, a partial one after the deposit)
+    const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
+    const bool more = chunk + cs < ce_;
+    const bool full = more && nb + CHW <= n;
+    if (full) {
+      ld_run_full<P, PPT>(a.x, nb, lane, x);
+      ld_run_full<P, PPT>(a.y, nb, lane, y);
+      if (MODE != MODE_DEPOSIT) {
+        ld_run_full<P, PPT>(a.vx, nb, lane, vx);
+        ld_run_full<P, PPT>(a.vy, nb, lane, vy);
       }
     }
-    if (rk_i >= 0) flush_block(rk_i, rk_j, ny, acc, ncorner, win, wl, a.Q);
-    if (bad) atomicAdd(&a.errc[EPIC_ERRC_DEPOSIT_OOB], (unsigned long long)bad);
 
-    if (!WHOLE) {
-      wave_fence();
-      const int W = win.wi * win.wj;
-      for (int k = lane; k < W; k += 64) {
-        const unsigned long long v = wl[k];
-        if (v) {
-          const int gi = win.i0 + k / win.wj, gj = win.j0 + k % win.wj;
-          if (gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
-        }
+    if (DEP) deposit_cells<P, PPT, WHOLE>(a, lane, wl, win, dc);
+
+    if (more && !full) {
+      ld_run<P, PPT>(a.x, nb, lane, n, x);
+      ld_run<P, PPT>(a.y, nb, lane, n, y);
+      if (MODE != MODE_DEPOSIT) {
+        ld_run<P, PPT>(a.vx, nb, lane, n, vx);
+        ld_run<P, PPT>(a.vy, nb, lane, n, vy);
       }
-      wave_fence();
     }
   }
 
@@ -294,6 +426,171 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
     for (int k = threadIdx.x; k < W; k += blockDim.x) {
       const unsigned long long v = lds[k];
       if (v) atomicAdd(&a.Q[k], v);
+    }
+  }
+}
+
+// Buffer resource over one wavefront chunk of an FP64 particle array: the
+// chw elements from wb, of which those below n are live.  Buffer loads past
+// the range return zeros without a memory access, so every chunk (the last,
+// partial one and one past the end included) is loaded by the same
+// instructions.  wb and n are wavefront-uniform.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t chunk_rsrc(const double* p, int64_t wb, int64_t n, int chw) {
+  const int64_t left = n - wb;
+  const int recs = left <= 0 ? 0 : (int)((left < chw ? left : (int64_t)chw) * (int64_t)sizeof(double));
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(p + (left <= 0 ? 0 : wb)), (short)0, recs, 0x00020000);
+}
+
+template <int PPT>
+__device__ __forceinline__ void ld_chunk_buf(__amdgpu_buffer_rsrc_t r, int lane, double (&o)[PPT]) {
+#pragma unroll
+  for (int g = 0; g < PPT / 2; g++) {
+    // cache policy 2: non-temporal (streamed once)
+    const v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(r, (g * 128 + lane * 2) * 8, 0, 2);
+    __builtin_memcpy(&o[2 * g], &t, 16);
+  }
+}
+
+// FP64 CIC push + deposit over a windowed grid (the plain path's hot
+// kernel).  Per chunk of 64*PPT particles a wavefront
+//   1. computes the cells of the current positions (Domain::Interpolate's
+//      logical coordinates, bounds and offsets),
+//   2. issues the field loads of all its particles' cells at once,
+//   3. issues the next chunk's particle loads (buffer loads, so the last,
+//      partial chunk needs no separate path),
+//   4. pushes (waiting only for the field loads: the next chunk stays in
+//      flight), stores, and deposits the new positions.
+// Every particle's arithmetic is that of particle_kernel<double> and of the
+// reference, operation for operation.
+template <int PPT, bool DEP>
+__global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a) {
+  using P = double;
+  __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
+  constexpr int CHW = 64 * PPT;
+  constexpr int CH = PART_BLOCK * PPT;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned long long* wl = lds + (DEP ? wid * WAVE_WIN : 0);
+  const int64_t n = a.n_fixed >= 0 ? a.n_fixed : *a.d_n;
+  const int64_t nchunks = (n + CH - 1) / CH;
+  const int ny = a.ny;
+  Window win{0, 0, a.nx, a.ny};
+  // the (ex, ey) node pairs, 16 bytes each (the host keeps nx*ny*16 < 2^31)
+  const __amdgpu_buffer_rsrc_t erc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.E, (short)0, a.nx * ny * 16, 0x00020000);
+
+  int64_t cb, ce_, cs;
+  if (a.rr) {
+    cb = blockIdx.x;
+    ce_ = nchunks;
+    cs = gridDim.x;
+  } else {
+    xcd_chunk_range(nchunks, &cb, &ce_, &cs);
+  }
+  P x[PPT], y[PPT], vx[PPT], vy[PPT];
+  {
+    const int64_t wb0 = cb * CH + (int64_t)wid * CHW;
+    ld_chunk_buf<PPT>(chunk_rsrc(a.x, wb0, n, CHW), lane, x);
+    ld_chunk_buf<PPT>(chunk_rsrc(a.y, wb0, n, CHW), lane, y);
+    ld_chunk_buf<PPT>(chunk_rsrc(a.vx, wb0, n, CHW), lane, vx);
+    ld_chunk_buf<PPT>(chunk_rsrc(a.vy, wb0, n, CHW), lane, vy);
+  }
+  for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
+    const int64_t wb = chunk * CH + (int64_t)wid * CHW;
+    bool live[PPT], ok[PPT];
+    int key[PPT];
+    P gdi[PPT], gdj[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      live[k] = (part_index<P, PPT>(wb, lane, k) < n) && !isnan(x[k]);
+      const P lx = live[k] ? x_to_l(a, x[k], a.x0, a.dx, a.idx) : 0.0;
+      const P ly = live[k] ? x_to_l(a, y[k], a.y0, a.dy, a.idy) : 0.0;
+      const int i = (int)lx, j = (int)ly;
+      ok[k] = live[k] && i >= 0 && i + 1 < a.nx && j >= 0 && j + 1 < ny;
+      key[k] = ok[k] ? i * ny + j : 0;
+      gdi[k] = lx - (P)i;
+      gdj[k] = ly - (P)j;
+    }
+    Vec2<P> e00[PPT], e10[PPT], e01[PPT], e11[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      const int o = key[k] * 16;
+      v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(erc, o, 0, 0);
+      __builtin_memcpy(&e00[k], &t, 16);
+      t = __builtin_amdgcn_raw_buffer_load_b128(erc, o + ny * 16, 0, 0);
+      __builtin_memcpy(&e10[k], &t, 16);
+      t = __builtin_amdgcn_raw_buffer_load_b128(erc, o + 16, 0, 0);
+      __builtin_memcpy(&e01[k], &t, 16);
+      t = __builtin_amdgcn_raw_buffer_load_b128(erc, o + ny * 16 + 16, 0, 0);
+      __builtin_memcpy(&e11[k], &t, 16);
+    }
+    P xn[PPT], yn[PPT], vxn[PPT], vyn[PPT];
+    {
+      const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
+      const int64_t nlim = chunk + cs < ce_ ? n : 0;
+      ld_chunk_buf<PPT>(chunk_rsrc(a.x, nb, nlim, CHW), lane, xn);
+      ld_chunk_buf<PPT>(chunk_rsrc(a.y, nb, nlim, CHW), lane, yn);
+      ld_chunk_buf<PPT>(chunk_rsrc(a.vx, nb, nlim, CHW), lane, vxn);
+      ld_chunk_buf<PPT>(chunk_rsrc(a.vy, nb, nlim, CHW), lane, vyn);
+    }
+    // Wait here for the field values (and only for them: the next chunk's
+    // loads were issued after them and stay in flight), and keep the
+    // compiler from sinking any field load into the branches below.
+#pragma unroll
+    for (int k = 0; k < PPT; k++)
+      asm volatile("" ::"v"(e00[k].a), "v"(e00[k].b), "v"(e10[k].a), "v"(e10[k].b), "v"(e01[k].a), "v"(e01[k].b),
+                   "v"(e11[k].a), "v"(e11[k].b));
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      if (!live[k]) continue;
+      P ex = 0, ey = 0;
+      if (ok[k]) {
+        // Domain::Interpolate (domain.cpp:231-255), as gather()
+        const P di = gdi[k], dj = gdj[k];
+        const P omdi = (P)1 - di, omdj = (P)1 - dj;
+        ex = e00[k].a * omdi * omdj + e10[k].a * (di)*omdj + e01[k].a * omdi * (dj) + e11[k].a * (di) * (dj);
+        ey = e00[k].b * omdi * omdj + e10[k].b * (di)*omdj + e01[k].b * omdi * (dj) + e11[k].b * (di) * (dj);
+      } else {
+        bad++;
+      }
+      vx[k] = vx[k] + (a.c1 * ex) * a.c2;
+      vy[k] = vy[k] + (a.c1 * ey) * a.c2;
+      P px = x[k] + (a.c3 * vx[k]) * a.c2;
+      P py = y[k] + (a.c3 * vy[k]) * a.c2;
+      if (a.bc == EPIC_BC_PBC) {
+        if (px < a.x0) px = px + a.Lx;
+        else if (px >= a.xhi) px = px - a.Lx;
+        if (py < a.y0) py = py + a.Ly;
+        else if (py >= a.yhi) py = py - a.Ly;
+      } else if (a.bc == EPIC_BC_OPEN) {
+        if (px < a.x0 || px >= a.xhi || py < a.y0 || py >= a.yhi) {
+          px = p_nan<P>();
+          live[k] = false;
+        }
+      } else {
+        if (px < a.x0) { px = a.x0 + (a.x0 - px); vx[k] = -vx[k]; }
+        else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); vx[k] = -vx[k]; }
+        if (py < a.y0) { py = a.y0 + (a.y0 - py); vy[k] = -vy[k]; }
+        else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); vy[k] = -vy[k]; }
+      }
+      x[k] = px;
+      y[k] = py;
+    }
+    if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
+    DepCells<P, PPT> dc;
+    if (DEP) dep_cells<P, PPT>(a, live, x, y, dc);
+    st_run<P, PPT>(a.vx, wb, lane, n, vx);
+    st_run<P, PPT>(a.vy, wb, lane, n, vy);
+    st_run<P, PPT>(a.x, wb, lane, n, x);
+    st_run<P, PPT>(a.y, wb, lane, n, y);
+    if (DEP) deposit_cells<P, PPT, false>(a, lane, wl, win, dc);
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      x[k] = xn[k];
+      y[k] = yn[k];
+      vx[k] = vxn[k];
+      vy[k] = vyn[k];
     }
   }
 }
@@ -355,6 +652,8 @@ static PartArgs<P> make_args(Ctx* c, int s) {
   a.Q = S.Q;
   a.fix_bits = c->fix_bits;
   a.errc = c->errc;
+  a.rr = c->part_map;
+  a.fastdiv = sizeof(P) == 8 && d.dx >= 0x1p-400 && d.dx <= 0x1p400 && d.dy >= 0x1p-400 && d.dy <= 0x1p400;
   return a;
 }
 
@@ -423,7 +722,12 @@ static hipError_t launch_particles_t(Ctx* c, int s, int mode, bool dep, double c
 }
 
 static hipError_t launch_particles(Ctx* c, int s, int mode, bool dep, double c1, double c2, double c3) {
-  if (c->dom.precision == EPIC_FP64) return launch_particles_t<double, 4>(c, s, mode, dep, c1, c2, c3);
+  if (c->dom.precision == EPIC_FP64) {
+    // particles per thread: EPIC_PART_PPT at context creation (2, 4 or 8)
+    if (c->part_ppt == 4) return launch_particles_t<double, 4>(c, s, mode, dep, c1, c2, c3);
+    if (c->part_ppt == 8) return launch_particles_t<double, 8>(c, s, mode, dep, c1, c2, c3);
+    return launch_particles_t<double, 2>(c, s, mode, dep, c1, c2, c3);
+  }
   return launch_particles_t<float, 8>(c, s, mode, dep, c1, c2, c3);
 }
 
@@ -490,6 +794,25 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
       EPIC_V(0, 4, 1)
 #undef EPIC_V
     }
+    return hipGetLastError();
+  }
+  if (d.precision == EPIC_FP64 && d.shape == EPIC_SHAPE_CIC && !whole && c->push64) {
+    PartArgs<double> a = make_args<double>(c, s);
+    a.c1 = c1; a.c2 = c2; a.c3 = c3;
+#define EPIC_L(PPT)                                                                              \
+    if (deposit) {                                                                               \
+      auto k = push_f64_kernel<PPT, true>;                                                       \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a);          \
+    } else {                                                                                     \
+      auto k = push_f64_kernel<PPT, false>;                                                      \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a);          \
+    }
+    if (c->part_ppt == 4) {
+      EPIC_L(4)
+    } else {
+      EPIC_L(2)
+    }
+#undef EPIC_L
     return hipGetLastError();
   }
   return launch_particles(c, s, MODE_PUSH, deposit, c1, c2, c3);

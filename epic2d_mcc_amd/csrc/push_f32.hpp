@@ -72,6 +72,20 @@ struct Window {
   int i0, j0, wi, wj;
 };
 
+// Atomic adds with the address space spelled out.  Through a generic
+// pointer the compiler may merge an LDS and a global atomic into one FLAT
+// atomic, which counts against both the vector-memory and the LDS counters
+// and makes every later wait drain all outstanding loads.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef __attribute__((address_space(1))) unsigned long long global_u64;
+
+__device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_add((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void global_add(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_add((global_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void flush_block(int bi, int bj, int ny, const unsigned long long (&acc)[4],
                                             int ncorner, const Window& w, unsigned long long* lds,
                                             unsigned long long* Q) {
@@ -79,19 +93,19 @@ __device__ __forceinline__ void flush_block(int bi, int bj, int ny, const unsign
   const bool in = li >= 0 && lj >= 0 && li + (ncorner > 1 ? 1 : 0) < w.wi && lj + (ncorner > 1 ? 1 : 0) < w.wj;
   if (in) {
     unsigned long long* b = lds + li * w.wj + lj;
-    if (acc[0]) atomicAdd(b, acc[0]);
+    if (acc[0]) lds_add(b, acc[0]);
     if (ncorner > 1) {
-      if (acc[1]) atomicAdd(b + w.wj, acc[1]);
-      if (acc[2]) atomicAdd(b + 1, acc[2]);
-      if (acc[3]) atomicAdd(b + w.wj + 1, acc[3]);
+      if (acc[1]) lds_add(b + w.wj, acc[1]);
+      if (acc[2]) lds_add(b + 1, acc[2]);
+      if (acc[3]) lds_add(b + w.wj + 1, acc[3]);
     }
   } else {
     unsigned long long* b = Q + (size_t)bi * ny + bj;
-    if (acc[0]) atomicAdd(b, acc[0]);
+    if (acc[0]) global_add(b, acc[0]);
     if (ncorner > 1) {
-      if (acc[1]) atomicAdd(b + ny, acc[1]);
-      if (acc[2]) atomicAdd(b + 1, acc[2]);
-      if (acc[3]) atomicAdd(b + ny + 1, acc[3]);
+      if (acc[1]) global_add(b + ny, acc[1]);
+      if (acc[2]) global_add(b + 1, acc[2]);
+      if (acc[3]) global_add(b + ny + 1, acc[3]);
     }
   }
 }
