@@ -153,6 +153,8 @@ def lib():
         "epic_estimate_bytes": [ctypes.POINTER(DomainDesc), ctypes.POINTER(SpeciesDesc), ctypes.c_int,
                                 ctypes.POINTER(I64)],
         "epic_device_bytes": [P, ctypes.POINTER(I64)],
+        "epic_mcc_estimate_bytes": [ctypes.POINTER(MccDesc), ctypes.POINTER(SpeciesDesc), ctypes.c_int,
+                                    ctypes.POINTER(I64)],
         "epic_batch_create": [ctypes.POINTER(BatchOpts), ctypes.POINTER(P)],
         "epic_batch_destroy": [P],
         "epic_batch_add": [P, ctypes.POINTER(DomainDesc), ctypes.POINTER(SpeciesDesc), ctypes.c_int,
@@ -369,6 +371,16 @@ def estimate_bytes(domain: DomainDesc, species):
     rc = lib().epic_estimate_bytes(ctypes.byref(domain), arr, len(species), ctypes.byref(v))
     if rc != EPIC_OK:
         raise EpicError(f"epic_estimate_bytes failed: {ERR_NAMES.get(rc, rc)}")
+    return v.value
+
+
+def mcc_estimate_bytes(d: MccDesc, species):
+    """Device bytes epic_mcc_setup adds to a context of these species."""
+    arr = (SpeciesDesc * len(species))(*species)
+    v = ctypes.c_int64()
+    rc = lib().epic_mcc_estimate_bytes(ctypes.byref(d), arr, len(species), ctypes.byref(v))
+    if rc != EPIC_OK:
+        raise EpicError(f"epic_mcc_estimate_bytes failed: {ERR_NAMES.get(rc, rc)}")
     return v.value
 
 

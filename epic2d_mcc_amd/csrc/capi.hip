@@ -102,7 +102,7 @@ static void free_all(Ctx* c) {
   void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->Ec, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
                 c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
-                c->mcc.event_count, c->sp_S, c->sp_T, c->sp_lam};
+                c->mcc.event_count, c->mcc.cand, c->mcc.cand_n, c->sp_S, c->sp_T, c->sp_lam};
   for (void* p : gs)
     if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -201,6 +201,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PART_MAP")) c->part_map = atoi(v);
   if (const char* v = getenv("EPIC_PART_PPT")) c->part_ppt = atoi(v);
   if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
+  if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
   c->max_solver_blocks = c->num_cu;
   if (ext) {
     c->stream = ext;
@@ -283,6 +284,28 @@ int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
     ncu = prop.multiProcessorCount;
   *bytes = (int64_t)ctx_bytes_estimate(dom, sp, nsp, ncu);
+  return EPIC_OK;
+}
+
+// What epic_mcc_setup allocates: table, counters, event list, candidate
+// lists (16-bit offsets, capacity rounded up to MCC_CHUNK) and their
+// per-chunk counts.
+static size_t mcc_bytes_of(int ntab, int event_cap, int64_t len) {
+  const int cap = event_cap > 0 ? event_cap : 16384;
+  return (size_t)EPIC_MCC_NPROC * std::max(ntab, 1) * sizeof(double) + (EPIC_MCC_NPROC + 2) * sizeof(uint64_t) +
+         (size_t)cap * 8 * sizeof(double) + sizeof(int) + (size_t)std::max<int64_t>(len, 1) * sizeof(uint16_t) +
+         (size_t)std::max<int64_t>(len / MCC_CHUNK, 1) * sizeof(int);
+}
+
+static int64_t mcc_list_len(const epic_species_desc* sp, int s) {
+  if (s < 0) return 0;
+  const int64_t cap = sp[s].capacity > 0 ? sp[s].capacity : std::max<int64_t>(sp[s].numparticle, 1);
+  return (cap + MCC_CHUNK - 1) / MCC_CHUNK * MCC_CHUNK;
+}
+
+int epic_mcc_estimate_bytes(const epic_mcc_desc* d, const epic_species_desc* sp, int nsp, int64_t* bytes) {
+  if (!d || !sp || !bytes || nsp < 1 || d->electron_sp >= nsp || d->ion_sp >= nsp) return EPIC_ERR_ARG;
+  *bytes = (int64_t)mcc_bytes_of(d->ntab, d->event_cap, mcc_list_len(sp, d->electron_sp) + mcc_list_len(sp, d->ion_sp));
   return EPIC_OK;
 }
 
@@ -508,6 +531,27 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   M.events = nullptr;
   CK(dalloc_c(c, &M.events, (size_t)M.event_cap * 8), "mcc events");
   if (!M.event_count) CK(dalloc_c(c, &M.event_count, 1), "mcc events");
+  // candidate lists (16-bit offsets) and per-chunk counts of the colliding
+  // species, sized by their capacities
+  auto cand_bytes = [](int64_t len) -> size_t {
+    return (size_t)std::max<int64_t>(len, 1) * sizeof(uint16_t) +
+           (size_t)std::max<int64_t>(len / MCC_CHUNK, 1) * sizeof(int);
+  };
+  if (M.cand) {
+    c->bytes -= cand_bytes(M.cand_len_e + M.cand_len_i);
+    hipFree(M.cand);
+    hipFree(M.cand_n);
+    M.cand = nullptr;
+    M.cand_n = nullptr;
+  }
+  auto list_len = [&](int s) -> int64_t {
+    return s >= 0 ? (c->sp[s].cap + MCC_CHUNK - 1) / MCC_CHUNK * MCC_CHUNK : 0;
+  };
+  M.cand_len_e = list_len(d->electron_sp);
+  M.cand_len_i = list_len(d->ion_sp);
+  const int64_t len = M.cand_len_e + M.cand_len_i;
+  CK(dalloc_c(c, &M.cand, (size_t)len), "mcc candidates");
+  CK(dalloc_c(c, &M.cand_n, (size_t)(len / MCC_CHUNK)), "mcc candidates");
   M.nu_max_e = d->electron_sp >= 0 ? mcc_nu_max(M.d, sig, ME, true) : 0.0;
   M.nu_max_i = d->ion_sp >= 0 ? mcc_nu_max(M.d, sig, c->sp[d->ion_sp].desc.mass, false) : 0.0;
   M.on = true;

@@ -29,6 +29,7 @@ constexpr int FIX_BITS_FP32 = 28;
 
 // LDS deposit window: 8192 int64 node accumulators (64 KiB) per workgroup.
 constexpr int DEP_WINDOW = 8192;
+constexpr int MCC_CHUNK = 2048;  // particle slots per MCC selection chunk
 constexpr int PART_BLOCK = 256;  // threads per workgroup in particle kernels
 
 struct PushF32;
@@ -68,6 +69,11 @@ struct MccState {
   double* events = nullptr;              // device event_cap * 8
   int* event_count = nullptr;            // device
   int event_cap = 0;
+  // candidate lists of the two-pass null-collision step: electron list
+  // (cand_len_e offsets) then ion list, and a count per MCC_CHUNK slots
+  uint16_t* cand = nullptr;
+  int* cand_n = nullptr;
+  int64_t cand_len_e = 0, cand_len_i = 0;
 };
 
 struct Ctx {
@@ -133,6 +139,7 @@ struct Ctx {
   int part_map = 0;            // same, generic particle kernel
   int part_ppt = 2;            // FP64 particle kernels: particles per thread (2 or 4; generic also 8)
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
+  int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
   bool deposit_valid = false;  // Q holds the deposit of the current positions
   int step_count = 0;
 

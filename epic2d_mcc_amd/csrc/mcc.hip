@@ -69,6 +69,8 @@ struct MccArgs {
   int* event_count;
   int event_cap;
   int slot;  // batched runs: index of the simulation's step counter
+  uint16_t* cand;  // candidate offsets, MCC_CHUNK per selection chunk
+  int* cand_n;     // candidates per selection chunk
 };
 
 __device__ __forceinline__ void rotate(double g, double chi, double eta, double st, double ct, double sp, double cp,
@@ -86,16 +88,14 @@ __device__ __forceinline__ double gauss(uint32_t a, uint32_t b, uint32_t c, uint
   return m * cos(2.0 * M_PI * u2);
 }
 
-// One colliding species: workgroup bid of nblk walks its particles.
+// One candidate of the null-collision method (R0 < P_null already drawn):
+// the real/null decision, the process and the kinematics.
 template <typename P>
-__device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
-  const int64_t n = a.d_n[0];
-  unsigned long long cnt[EPIC_MCC_NPROC] = {0, 0, 0, 0, 0};
-  for (int64_t p = bid * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)nblk * blockDim.x) {
+__device__ __forceinline__ void mcc_candidate(const MccArgs<P>& a, uint32_t step, int64_t p,
+                                              unsigned long long (&cnt)[EPIC_MCC_NPROC]) {
+  do {
     const uint32_t pid = a.id[p];
     const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
-    const double R0 = u01_53(r.a, r.b);
-    if (R0 >= a.p_null) continue;
     const P xp = a.x[p];
     if (isnan(xp)) continue;
     double vxe = a.vx[p], vye = a.vy[p], vze = a.vz[p];
@@ -211,6 +211,66 @@ __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int
         cnt[4]++;
       }
     }
+  } while (false);
+}
+
+// A selection chunk (MCC_CHUNK consecutive slots) is one wavefront's work
+// in both passes.
+static_assert(MCC_CHUNK <= 65536, "candidate offsets are 16-bit");
+
+// Pass 1 of a colliding species: every particle draws R0 from its (id, step)
+// stream; those with R0 < P_null (a fraction ~nu_max dt) are candidates.
+// Chunk c's candidate offsets go to cand[c*MCC_CHUNK + 0 .. cand_n[c]), in
+// slot order.  Wavefront w of the grid takes chunks w, w + W, ...
+template <typename P>
+__device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
+  const int64_t n = a.d_n[0];
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const int64_t nch = (n + MCC_CHUNK - 1) / MCC_CHUNK;
+  for (int64_t c = (int64_t)bid * wpb + (threadIdx.x >> 6); c < nch; c += (int64_t)nblk * wpb) {
+    const int64_t base = c * MCC_CHUNK;
+    int cnt = 0;
+    for (int k0 = 0; k0 < MCC_CHUNK / 64; k0 += 8) {
+      uint32_t pid[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int64_t p = base + (k0 + j) * 64 + lane;
+        pid[j] = p < n ? a.id[p] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int off = (k0 + j) * 64 + lane;
+        bool cand = false;
+        if (base + off < n) {
+          const u32x4 r = philox4x32_10({pid[j], step, RNG_MCC, a.sp}, a.k0, a.k1);
+          cand = u01_53(r.a, r.b) < a.p_null;
+        }
+        const unsigned long long m = __ballot(cand);
+        const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (cand) a.cand[base + cnt + rank] = (uint16_t)off;
+        cnt += (int)__popcll(m);
+      }
+    }
+    if (lane == 0) a.cand_n[c] = cnt;
+  }
+}
+
+// Pass 2: the candidates, one per lane, so the collision arithmetic runs on
+// full wavefronts instead of on the few lanes of each wavefront that drew a
+// candidate.  Each particle's draws are keyed by (id, step): the order of
+// processing changes nothing.
+template <typename P>
+__device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
+  const int64_t n = a.d_n[0];
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const int64_t nch = (n + MCC_CHUNK - 1) / MCC_CHUNK;
+  unsigned long long cnt[EPIC_MCC_NPROC] = {0, 0, 0, 0, 0};
+  for (int64_t c = (int64_t)bid * wpb + (threadIdx.x >> 6); c < nch; c += (int64_t)nblk * wpb) {
+    const int64_t base = c * MCC_CHUNK;
+    const int nc = a.cand_n[c];
+    for (int t = lane; t < nc; t += 64) mcc_candidate(a, step, base + a.cand[base + t], cnt);
   }
 #pragma unroll
   for (int q = 0; q < EPIC_MCC_NPROC; q++) {
@@ -222,14 +282,26 @@ __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int
 }
 
 template <typename P>
-__global__ __launch_bounds__(MCC_BLOCK) void mcc_kernel(MccArgs<P> a) {
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_select_kernel(MccArgs<P> a) {
+  mcc_select_body(a, a.step, blockIdx.x, gridDim.x);
+}
+
+// OCC: workgroups per CU the register allocation is held to (1: none)
+template <typename P, int OCC>
+__global__ __launch_bounds__(MCC_BLOCK, OCC) void mcc_kernel(MccArgs<P> a) {
   mcc_body(a, a.step, blockIdx.x, gridDim.x);
 }
 
 // Batched: blockIdx.y selects the (simulation, species) pass; the step comes
 // from the simulation's device step counter.
 template <typename P>
-__global__ __launch_bounds__(MCC_BLOCK) void mcc_batch_kernel(const MccArgs<P>* args, const int* steps) {
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_select_batch_kernel(const MccArgs<P>* args, const int* steps) {
+  const MccArgs<P>& a = args[blockIdx.y];
+  mcc_select_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x);
+}
+
+template <typename P>
+__global__ __launch_bounds__(MCC_BLOCK, 2) void mcc_batch_kernel(const MccArgs<P>* args, const int* steps) {
   const MccArgs<P>& a = args[blockIdx.y];
   mcc_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x);
 }
@@ -405,6 +477,8 @@ static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   a.events = M.events;
   a.event_count = M.event_count;
   a.event_cap = M.event_cap;
+  a.cand = kind == 0 ? M.cand : M.cand + M.cand_len_e;
+  a.cand_n = kind == 0 ? M.cand_n : M.cand_n + M.cand_len_e / MCC_CHUNK;
   return a;
 }
 
@@ -441,8 +515,16 @@ template <typename P>
 static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
   const MccArgs<P> a = mcc_args<P>(c, s, kind, step);
   if (a.nu_max <= 0) return hipSuccess;
-  const int grid = c->num_cu * 8;
-  hipLaunchKernelGGL(mcc_kernel<P>, dim3(grid), dim3(MCC_BLOCK), 0, c->stream, a);
+  // one wavefront per selection chunk in pass 1; pass 2 runs at the
+  // occupancy of the collision arithmetic
+  const int64_t nch = (c->sp[s].cap + MCC_CHUNK - 1) / MCC_CHUNK;
+  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(c->num_cu * 16, (nch + 3) / 4));
+  hipLaunchKernelGGL(mcc_select_kernel<P>, dim3(g1), dim3(MCC_BLOCK), 0, c->stream, a);
+  EPIC_HIP_TRY(hipGetLastError());
+  if (c->mcc_occ == 1)
+    hipLaunchKernelGGL((mcc_kernel<P, 1>), dim3(c->num_cu * 4), dim3(MCC_BLOCK), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((mcc_kernel<P, 2>), dim3(c->num_cu * 8), dim3(MCC_BLOCK), 0, c->stream, a);
   EPIC_HIP_TRY(hipGetLastError());
   if (kind == 0 && (a.enable & 4)) {
     AppendArgs<P> A = append_args<P>(c, step);
@@ -488,7 +570,11 @@ hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int 
                             const int* d_steps) {
   if (nargs > 0) {
     const int gx = std::max(4, std::min(num_cu * 8, (num_cu * 32 + nargs - 1) / nargs));
-    hipLaunchKernelGGL(mcc_batch_kernel<float>, dim3(gx, nargs), dim3(MCC_BLOCK), 0, st,
+    hipLaunchKernelGGL(mcc_select_batch_kernel<float>, dim3(gx, nargs), dim3(MCC_BLOCK), 0, st,
+                       (const MccArgs<float>*)d_args, d_steps);
+    EPIC_HIP_TRY(hipGetLastError());
+    const int g2 = std::max(1, std::min(num_cu * 8, (num_cu * 16 + nargs - 1) / nargs));
+    hipLaunchKernelGGL(mcc_batch_kernel<float>, dim3(g2, nargs), dim3(MCC_BLOCK), 0, st,
                        (const MccArgs<float>*)d_args, d_steps);
     EPIC_HIP_TRY(hipGetLastError());
   }

@@ -186,18 +186,20 @@ double pct(std::vector<double> v, double p) {
   return v[lo] + (pos - (double)lo) * (v[hi] - v[lo]);
 }
 
-int64_t mcc_bytes(const epic_mcc_desc* m) {
-  const int64_t cap = m->event_cap > 0 ? m->event_cap : 16384;
-  return (int64_t)EPIC_MCC_NPROC * m->ntab * 8 + (EPIC_MCC_NPROC + 2) * 8 + cap * 64 + 4;
-}
-
 // Device bytes a job holds while admitted: its context, MCC tables and its
 // share of the batch's FP32 transform workspace (X and Y, p*q each).
 int64_t job_bytes(const epic_domain_desc& d, const epic_species_desc* sp, int nsp, const epic_serve_job& j,
                   const epic_serve_config* c) {
   int64_t b = 0;
   if (epic_estimate_bytes(&d, sp, nsp, &b) != EPIC_OK) return -1;
-  if (j.collisional && c->mcc_template) b += mcc_bytes(c->mcc_template);
+  if (j.collisional && c->mcc_template) {
+    epic_mcc_desc m = *c->mcc_template;
+    m.electron_sp = 0;
+    m.ion_sp = 1;
+    int64_t mb = 0;
+    if (epic_mcc_estimate_bytes(&m, sp, nsp, &mb) != EPIC_OK) return -1;
+    b += mb;
+  }
   b += 2ll * d.nx * d.ny * 4 * 3 / 2;  // X, Y (the workspace grows by 1.5x)
   return b;
 }

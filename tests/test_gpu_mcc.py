@@ -157,3 +157,49 @@ def test_ion_neutral_rates_match_monte_carlo_expectation():
     print(f"  gpu elastic {g[3]} cx {g[4]}; expected {exp_el:.0f} {exp_cx:.0f}")
     assert abs(g[3] - exp_el) < 4 * np.sqrt(exp_el) + 0.02 * exp_el
     assert abs(g[4] - exp_cx) < 4 * np.sqrt(exp_cx) + 0.02 * exp_cx
+
+
+def test_mcc_device_bytes_match_estimate():
+    """epic_mcc_setup adds exactly what epic_mcc_estimate_bytes states
+    (tables, event list, candidate lists sized by species capacity); a
+    repeated setup replaces rather than adds."""
+    st = _setup(5000, 8.0)
+    ctx = _lib.Context(st.domain, st.species)
+    base = ctx.device_bytes()
+    m = xsec.mcc_desc(xsec.reference_e_ar(), gas_density=GAS, gas_temp_K=0.0, enable=(1, 1, 1, 0, 0))
+    ctx.mcc_setup(m)
+    want = _lib.mcc_estimate_bytes(m, st.species)
+    assert ctx.device_bytes() - base == want
+    ctx.mcc_setup(m)
+    assert ctx.device_bytes() - base == want
+    ctx.close()
+
+
+def test_candidate_lists_independent_of_layout():
+    """The two-pass collision step (candidate selection per 2048-slot chunk,
+    then one candidate per lane) gives each particle the outcome of its own
+    (id, step) draws: the same particles, reordered in memory by the cell
+    sort, end with the same velocities."""
+    print("[mcc] layout independence", flush=True)
+    N = 70001  # a partial last chunk
+    st = _setup(N, 8.0)
+    e = _electrons(st, N, 8.0, 3)
+    m = xsec.mcc_desc(xsec.reference_e_ar(), gas_density=GAS, gas_temp_K=0.0, enable=(1, 1, 0, 0, 0))
+    outs = []
+    for sort in (False, True):
+        ctx = _lib.Context(st.domain, st.species)
+        ctx.upload(0, *e)
+        ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+        if sort:
+            ctx.sort(0)  # reorders the slots by cell; ids travel with the particles
+        ctx.mcc_setup(m)
+        for k in range(20):
+            ctx.collide(k)
+        x, y, vx, vy, vz, ids = ctx.download(0)
+        srt = np.argsort(ids)
+        outs.append((vx[srt], vy[srt], vz[srt], ctx.mcc_stats()["count"].copy()))
+        ctx.close()
+    for k in range(3):
+        assert np.array_equal(outs[0][k], outs[1][k])
+    assert np.array_equal(outs[0][3], outs[1][3])
+    assert outs[0][3][0] > 0
