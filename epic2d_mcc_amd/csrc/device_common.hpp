@@ -96,13 +96,39 @@ __device__ __forceinline__ int wave_ext_dpp(int v) {
 __device__ __forceinline__ int wave_min_dpp(int v) { return wave_ext_dpp<true>(v); }
 __device__ __forceinline__ int wave_max_dpp(int v) { return wave_ext_dpp<false>(v); }
 
+// One DPP move of a double (two 32-bit halves); lanes without a source in
+// the pattern receive +0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Wave sum of a double with DPP moves (the scan of wave_ext_dpp: row_shr 1,
+// 2, 4, 8 inside each 16-lane row, row_bcast 15 and 31 across rows), read
+// back from lane 63: uniform, fixed order, no LDS traffic.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, 63);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Block sum of NV doubles; result valid in every thread.  `sh` holds
 // NV * (blockDim/64) doubles.  Deterministic (fixed order).
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  for (int k = 0; k < NV; k++) v[k] = wave_sum_dpp(v[k]);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NV; k++) sh[k * nw + wid] = v[k];
@@ -115,6 +141,29 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
     v[k] = s;
   }
   __syncthreads();
+}
+
+// Block sum without the closing barrier: `sh` (NV * (blockDim/64) doubles)
+// may be written again only after one more workgroup barrier, so callers
+// alternate two regions between consecutive sums.
+template <int NV, int NW>
+__device__ __forceinline__ void block_sum_open(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int nw = NW;  // wavefronts per workgroup
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum_dpp(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < nw; w++) s += sh[k * nw + w];
+    v[k] = s;
+  }
 }
 
 __device__ __forceinline__ void atomic_add_u64(unsigned long long* p, unsigned long long v) {

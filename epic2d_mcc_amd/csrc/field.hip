@@ -107,9 +107,14 @@ __device__ __forceinline__ void gsync(GridBar& grid) {
   }
 }
 
-template <typename T, int OP, int NPT, bool COOP>
+// LDSV (single workgroup): the search direction lives in LDS (n values of
+// T, dynamic shared memory) instead of the global double buffer, so the
+// stencil reads its neighbours from LDS.
+template <typename T, int OP, int NPT, bool COOP, bool LDSV>
 __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
   __shared__ double sh[64];
+  extern __shared__ double vs_raw[];
+  T* const vs = reinterpret_cast<T*>(vs_raw);
   GridBar grid{a.bar, 0u};
   const int n = a.nx * a.ny;
   const int TT = gridDim.x * blockDim.x;
@@ -118,12 +123,14 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
 
   T x[NPT], r[NPT], w[NPT], vold[NPT];
   bool own[NPT];
+  unsigned nbm[NPT];  // the stencil's neighbour mask per owned node
   double d2[2] = {0, 0};
   double bb = 0;
 #pragma unroll
   for (int m = 0; m < NPT; m++) {
     const int k = g0 + m * TT;
     own[m] = k < n && S.unknown(k);
+    nbm[m] = own[m] ? S.mask(k) : 0u;
     const T bk = own[m] ? (T)a.b[k] : (T)0;
     x[m] = 0;
     // r = b - A*0 (solvers.h:75): A*0 = 0 exactly
@@ -154,25 +161,47 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
     const T sT = (T)s;
     auto vec = [&](int j) -> double { return (double)(a.wg[j] + vo[j] * sT); };
     double dvu = 0;
+    if (LDSV) {
+      // every thread has finished the previous stencil (the reductions
+      // since then end in workgroup barriers)
 #pragma unroll
-    for (int m = 0; m < NPT; m++) {
-      const int k = g0 + m * TT;
-      v[m] = w[m] + vold[m] * sT;
-      if (own[m]) {
-        u[m] = (T)S.apply(k, (double)v[m], vec);
-      } else {
-        u[m] = 0;
-        v[m] = 0;
+      for (int m = 0; m < NPT; m++) {
+        const int k = g0 + m * TT;
+        v[m] = own[m] ? w[m] + vold[m] * sT : (T)0;
+        if (k < n) vs[k] = v[m];
       }
-      dvu += (double)v[m] * (double)u[m];
-    }
+      __syncthreads();
+      auto vl = [&](int j) -> double { return (double)vs[j]; };
 #pragma unroll
-    for (int m = 0; m < NPT; m++) {
-      const int k = g0 + m * TT;
-      if (k < n) a.vb[(size_t)(cur ^ 1) * n + k] = v[m];
+      for (int m = 0; m < NPT; m++) {
+        const int k = g0 + m * TT;
+        u[m] = own[m] ? (T)S.apply_mask(k, nbm[m], (double)v[m], vl) : (T)0;
+        dvu += (double)v[m] * (double)u[m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < NPT; m++) {
+        const int k = g0 + m * TT;
+        v[m] = w[m] + vold[m] * sT;
+        if (own[m]) {
+          u[m] = (T)S.apply_mask(k, nbm[m], (double)v[m], vec);
+        } else {
+          u[m] = 0;
+          v[m] = 0;
+        }
+        dvu += (double)v[m] * (double)u[m];
+      }
+#pragma unroll
+      for (int m = 0; m < NPT; m++) {
+        const int k = g0 + m * TT;
+        if (k < n) a.vb[(size_t)(cur ^ 1) * n + k] = v[m];
+      }
     }
     double d1[1] = {dvu};
-    grid_sum<1, COOP>(d1, a.part, 1, sh, grid);
+    if (LDSV)
+      block_sum_open<1, SOLVER_BLOCK / 64>(d1, sh);  // region 0..15; the next sum uses 32..63
+    else
+      grid_sum<1, COOP>(d1, a.part, 1, sh, grid);
     const double t = alpha / d1[0];
     const T tT = (T)t;
     double db[2] = {0, 0};
@@ -182,18 +211,21 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
       x[m] = x[m] + v[m] * tT;
       r[m] = r[m] - u[m] * tT;
       w[m] = (T)(1.0 / -4.0) * r[m];
-      if (k < n) a.wg[k] = w[m];
+      if (!LDSV && k < n) a.wg[k] = w[m];
       db[0] += (double)w[m] * (double)r[m];
       db[1] += (double)r[m] * (double)r[m];
       vold[m] = v[m];
     }
-    grid_sum<2, COOP>(db, a.part, 2, sh, grid);
+    if (LDSV)
+      block_sum_open<2, SOLVER_BLOCK / 64>(db, sh + 32);
+    else
+      grid_sum<2, COOP>(db, a.part, 2, sh, grid);
     const double beta = db[0];
     s = beta / alpha;
     alpha = beta;
     rr = db[1];
     cur ^= 1;
-    if (!COOP) __syncthreads();
+    if (!COOP && !LDSV) __syncthreads();
   }
   // write x
 #pragma unroll
@@ -412,11 +444,11 @@ hipError_t launch_pack_E2(Ctx* c) {
 // ---------------------------------------------------------------------------
 // launch helpers
 static int solver_blocks(Ctx* c, int n, int* npt) {
-  // one workgroup while 8 nodes per thread suffice; otherwise one workgroup
-  // per CU (co-resident for the cooperative launch), 1-8 nodes per thread
+  // one workgroup while 8 nodes per thread suffice (exactly ceil(n/1024) per
+  // thread); otherwise one workgroup per CU (co-resident for the cooperative
+  // launch), 1, 2, 4 or 8 nodes per thread
   if (n <= SOLVER_BLOCK * 8) {
-    const int p = (n + SOLVER_BLOCK - 1) / SOLVER_BLOCK;
-    *npt = p <= 1 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : 8;
+    *npt = std::max(1, (n + SOLVER_BLOCK - 1) / SOLVER_BLOCK);
     return 1;
   }
   int G = std::min(c->max_solver_blocks, (n + SOLVER_BLOCK - 1) / SOLVER_BLOCK);
@@ -427,20 +459,48 @@ static int solver_blocks(Ctx* c, int n, int* npt) {
 
 __global__ void zero_bar_kernel(unsigned* b) { *b = 0u; }
 
+// LDS for the single-workgroup solve's search direction: up to 60 KiB
+constexpr size_t PCG_LDS_MAX = 60 * 1024;
+
 template <typename T, int OP, int NPT>
 static hipError_t run_pcg_t(Ctx* c, int G, PcgArgs<T>& a) {
-  hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
   if (G == 1) {
-    hipLaunchKernelGGL((pcg_kernel<T, OP, NPT, false>), dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
+    hipLaunchKernelGGL((pcg_kernel<T, OP, NPT, false, false>), dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
     return hipGetLastError();
   }
+  hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
   void* args[] = {&a};
-  return hipLaunchCooperativeKernel((const void*)pcg_kernel<T, OP, NPT, true>, dim3(G), dim3(SOLVER_BLOCK), args, 0,
-                                    c->stream);
+  return hipLaunchCooperativeKernel((const void*)pcg_kernel<T, OP, NPT, true, false>, dim3(G), dim3(SOLVER_BLOCK),
+                                    args, 0, c->stream);
+}
+
+// Single workgroup, NPT nodes per thread: the search direction in LDS when
+// it fits, else the global double buffer (NPT rounded up to 4 or 8).
+template <typename T, int OP, int NPT>
+static hipError_t run_pcg_single(Ctx* c, PcgArgs<T>& a) {
+  const size_t lds = (size_t)a.nx * a.ny * sizeof(T);
+  if (lds <= PCG_LDS_MAX) {
+    hipLaunchKernelGGL((pcg_kernel<T, OP, NPT, false, true>), dim3(1), dim3(SOLVER_BLOCK), lds, c->stream, a);
+    return hipGetLastError();
+  }
+  return run_pcg_t<T, OP, (NPT <= 4 ? 4 : 8)>(c, 1, a);
 }
 
 template <typename T, int OP>
 static hipError_t run_pcg_op(Ctx* c, int G, int npt, PcgArgs<T>& a) {
+  if (G == 1) {
+    hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
+    switch (npt) {
+      case 1: return run_pcg_single<T, OP, 1>(c, a);
+      case 2: return run_pcg_single<T, OP, 2>(c, a);
+      case 3: return run_pcg_single<T, OP, 3>(c, a);
+      case 4: return run_pcg_single<T, OP, 4>(c, a);
+      case 5: return run_pcg_single<T, OP, 5>(c, a);
+      case 6: return run_pcg_single<T, OP, 6>(c, a);
+      case 7: return run_pcg_single<T, OP, 7>(c, a);
+      default: return run_pcg_single<T, OP, 8>(c, a);
+    }
+  }
   switch (npt) {
     case 1: return run_pcg_t<T, OP, 1>(c, G, a);
     case 2: return run_pcg_t<T, OP, 2>(c, G, a);

@@ -84,6 +84,26 @@ struct Stencil {
       return sum;
     }
   }
+  // Which of row k's off-diagonal entries exist (bit 0: k - stride, 1: k - 1,
+  // 2: k + 1, 3: k + stride), for apply_mask: the index arithmetic of apply
+  // done once instead of per product.
+  __device__ __forceinline__ unsigned mask(int k) const {
+    if (OP != EPIC_OP_REF_GHOST) return 0xfu;
+    const int row = k / nx, col = k % nx;
+    return (row > 0 ? 1u : 0u) | (col > 0 ? 2u : 0u) | (col < nx - 1 ? 4u : 0u) | (row < ny - 1 ? 8u : 0u);
+  }
+  // apply() with the entries given by mask(k): the same terms in the same order.
+  template <typename F>
+  __device__ __forceinline__ double apply_mask(int k, unsigned m, double self, F vec) const {
+    const int st = OP == EPIC_OP_REF_GHOST ? nx : ny;
+    double sum = 0;
+    if (m & 1u) sum += vec(k - st);
+    if (m & 2u) sum += vec(k - 1);
+    sum += -4.0 * self;
+    if (m & 4u) sum += vec(k + 1);
+    if (m & 8u) sum += vec(k + st);
+    return sum;
+  }
 };
 
 // r_k = b_k - (A phi)_k over the unknowns, 0 elsewhere.
