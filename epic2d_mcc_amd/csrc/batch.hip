@@ -888,7 +888,9 @@ static hipError_t post_step(Batch* B) {
     }
     c->step_count++;
     m.steps++;
-    if (c->dom.sort_interval > 0 && c->step_count % c->dom.sort_interval == 0) {
+    bool any = false;
+    for (int s = 0; s < c->nsp; s++) any = any || c->sort_due(s);
+    if (any) {
       due.push_back(c);
       nsp_max = std::max(nsp_max, c->nsp);
     }
@@ -896,11 +898,18 @@ static hipError_t post_step(Batch* B) {
   if (!due.empty()) {
     // the previous round's host records may still be read by a pending copy
     EPIC_HIP_TRY(hipStreamSynchronize(B->stream));
-    for (int r = 0; r < nsp_max && r < 4; r++) EPIC_HIP_TRY(batch_sort_round(B, due, r));
+    std::vector<Ctx*> due_r;
+    for (int r = 0; r < nsp_max && r < 4; r++) {
+      due_r.clear();
+      for (Ctx* c : due)
+        if (r < c->nsp && c->sort_due(r)) due_r.push_back(c);
+      EPIC_HIP_TRY(batch_sort_round(B, due_r, r));
+    }
     for (Ctx* c : due)
       if (c->nsp > 4) {  // beyond four species: one by one
         EPIC_HIP_TRY(sync_counts(c));
-        for (int r = 4; r < c->nsp; r++) EPIC_HIP_TRY(launch_sort(c, r));
+        for (int r = 4; r < c->nsp; r++)
+          if (c->sort_due(r)) EPIC_HIP_TRY(launch_sort(c, r));
       }
   }
   B->steps++;

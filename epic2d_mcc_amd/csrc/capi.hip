@@ -215,9 +215,22 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   const size_t ps = psize(c);
   int64_t maxcap = 1;
   c->sp.resize(nsp);
+  // Sort schedule: the sort keeps a wavefront's particles in few cells, and
+  // particles leave their cells at their thermal speed (a drift moves a
+  // species' particles together and does not disorder them).  A species
+  // whose thermal speed is k times below the fastest one's is sorted k times
+  // less often (k up to 64), so cold ions are not re-sorted every interval.
+  // Sorting changes no result.  EPIC_SORT_SCALE=0: every species every
+  // interval.
+  double vmax = 0;
+  for (int s = 0; s < nsp; s++) vmax = std::max(vmax, std::sqrt(std::max(sp[s].temp, 1e-30) / sp[s].mass));
+  const char* scale_env = getenv("EPIC_SORT_SCALE");
+  const bool scale = !(scale_env && atoi(scale_env) == 0);
   for (int s = 0; s < nsp; s++) {
     SpeciesDev& S = c->sp[s];
     S.desc = sp[s];
+    const double v = std::sqrt(std::max(sp[s].temp, 1e-30) / sp[s].mass);
+    S.sort_mult = scale ? (int)std::min(64.0, std::max(1.0, std::floor(vmax / v))) : 1;
     S.cap = sp[s].capacity > 0 ? sp[s].capacity : std::max<int64_t>(sp[s].numparticle, 1);
     if (S.cap > (int64_t)0x7fffffff) {
       free_all(c);
@@ -642,10 +655,13 @@ int epic_step(epic_ctx* h, int ts, int* iters) {
     stage_end(c, 5);
   }
   c->step_count++;
-  if (c->dom.sort_interval > 0 && c->step_count % c->dom.sort_interval == 0) {
+  bool any_due = false;
+  for (int s = 0; s < c->nsp; s++) any_due = any_due || c->sort_due(s);
+  if (any_due) {
     stage_begin(c);
     CK(sync_counts(c), "counts");
-    for (int s = 0; s < c->nsp; s++) CK(launch_sort(c, s), "sort");
+    for (int s = 0; s < c->nsp; s++)
+      if (c->sort_due(s)) CK(launch_sort(c, s), "sort");
     stage_end(c, 6);
   }
   return EPIC_OK;

@@ -37,6 +37,7 @@ struct PushF32;
 struct SpeciesDev {
   epic_species_desc desc;
   int64_t cap = 0;
+  int sort_mult = 1;        // sorted every dom.sort_interval * sort_mult steps
   int64_t n_host = 0;       // host copy of the live+dead slot count (authoritative unless births)
   void* x = nullptr;        // P*  (P = double | float)
   void* y = nullptr;
@@ -140,6 +141,12 @@ struct Ctx {
   int part_ppt = 2;            // FP64 particle kernels: particles per thread (2 or 4; generic also 8)
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
+
+  // species s is due for its counting sort after this step
+  bool sort_due(int s) const {
+    const long long k = (long long)dom.sort_interval * sp[s].sort_mult;
+    return k > 0 && step_count % k == 0;
+  }
   bool deposit_valid = false;  // Q holds the deposit of the current positions
   int step_count = 0;
 
