@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # workload (profiles/push_f32_kernel_bounded512_pmc.md): FETCH_SIZE x 2 +
 # WRITE_SIZE x 1 per dispatch (the factors a plain 1 GiB device copy reads
 # back as), divided by the 26,214,400 particles of a dispatch.
-PUSH_TRAFFIC_BYTES_PER_PARTICLE = (214624.42 * 2 + 430505.95) * 1024 / 26214400
+PUSH_TRAFFIC_BYTES_PER_PARTICLE = (214532.78 * 2 + 429676.73) * 1024 / 26214400
 
 
 def gpu_run(st, steps, warmup, solver_name, mcc=None):
