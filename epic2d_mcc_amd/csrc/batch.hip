@@ -485,8 +485,8 @@ struct Batch {
 static Ctx* member_ctx(const Member& m) { return ctx_of(m.h); }
 
 static SpecLayout layout_of(const epic_domain_desc& d) {
-  if (d.poisson_op == EPIC_OP_DIRICHLET_WALL) return SpecLayout{d.ny - 2, d.nx - 2, d.ny, d.ny + 1};
-  return SpecLayout{d.nx, d.ny, d.nx, 0};
+  if (d.poisson_op == EPIC_OP_DIRICHLET_WALL) return SpecLayout{d.ny - 2, d.nx - 2, d.ny, d.ny + 1, d.ny - 2, d.nx - 2};
+  return SpecLayout{d.nx, d.ny, d.nx, 0, d.nx, d.ny};
 }
 
 // FP32 sine matrices and scaling, computed in FP64 and rounded (the FP32

@@ -149,8 +149,12 @@ __device__ __forceinline__ void efield_at(const double* phi, int i, int j, int n
 
 // Column-major p x q block of unknowns inside the node array (sine-transform
 // solver): element (a, b) is node off + a + b*ld.
+// The p x q block of unknowns: node (a, b) of the block is grid node
+// off + a + b*ld; in the transform matrices it is entry a + b*pp (pp >= p:
+// rows padded with zeros so the GEMM dimensions are multiples of the tile).
 struct SpecLayout {
   int p, q, ld, off;
+  int pp, qq;  // padded transform dimensions (>= p, >= q)
 };
 
 // Index of node k inside the block, or -1 when k is not an unknown.
@@ -159,7 +163,7 @@ __device__ __forceinline__ int spec_index(const SpecLayout& L, int k) {
   if (t < 0) return -1;
   const int a = t % L.ld, b = t / L.ld;
   if (a >= L.p || b >= L.q) return -1;
-  return a + b * L.p;
+  return a + b * L.pp;
 }
 
 }  // namespace epic

@@ -27,11 +27,18 @@
 
 namespace epic {
 
+// Transform dimensions padded to a multiple of 64 with zero rows and
+// columns (S, G and the work matrices are zero there, so the padding stays
+// zero through every product): rocBLAS runs 512^3 in 7.3 us and 511^3 in
+// 11.8 us on this device.
+static int spec_pad(int p) { return p >= 64 ? (p + 63) / 64 * 64 : p; }
+
 static SpecLayout spec_layout(const Ctx* c) {
   const epic_domain_desc& d = c->dom;
-  if (d.poisson_op == EPIC_OP_DIRICHLET_WALL) return SpecLayout{d.ny - 2, d.nx - 2, d.ny, d.ny + 1};
+  if (d.poisson_op == EPIC_OP_DIRICHLET_WALL)
+    return SpecLayout{d.ny - 2, d.nx - 2, d.ny, d.ny + 1, spec_pad(d.ny - 2), spec_pad(d.nx - 2)};
   // pcgsolver's A: flat index row*nx + col, all nodes unknown
-  return SpecLayout{d.nx, d.ny, d.nx, 0};
+  return SpecLayout{d.nx, d.ny, d.nx, 0, spec_pad(d.nx), spec_pad(d.ny)};
 }
 
 template <typename T>
@@ -39,7 +46,7 @@ __global__ void pack_kernel(const double* src, T* X, SpecLayout L) {
   const int n = L.p * L.q;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int a = k % L.p, b = k / L.p;
-    X[k] = (T)src[L.off + a + (size_t)b * L.ld];
+    X[a + (size_t)b * L.pp] = (T)src[L.off + a + (size_t)b * L.ld];
   }
 }
 
@@ -49,7 +56,8 @@ __global__ void unpack_add_kernel(const T* U, double* phi, SpecLayout L, int ove
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int a = k % L.p, b = k / L.p;
     double* d = phi + L.off + a + (size_t)b * L.ld;
-    *d = overwrite ? (double)U[k] : *d + (double)U[k];
+    const double u = (double)U[a + (size_t)b * L.pp];
+    *d = overwrite ? u : *d + u;
   }
 }
 
@@ -97,17 +105,17 @@ static hipError_t spec_init(Ctx* c) {
   if (c->blas) return hipSuccess;
   SpecState* S = new SpecState();
   S->L = spec_layout(c);
-  const int p = S->L.p, q = S->L.q;
-  std::vector<double> sp((size_t)p * p), sq((size_t)q * q), g((size_t)p * q);
+  const int p = S->L.p, q = S->L.q, P = S->L.pp, Q = S->L.qq;
+  std::vector<double> sp((size_t)P * P, 0.0), sq((size_t)Q * Q, 0.0), g((size_t)P * Q, 0.0);
   for (int a = 0; a < p; a++)
-    for (int k = 0; k < p; k++) sp[a + (size_t)k * p] = sin(M_PI * (double)(a + 1) * (k + 1) / (p + 1));
+    for (int k = 0; k < p; k++) sp[a + (size_t)k * P] = sin(M_PI * (double)(a + 1) * (k + 1) / (p + 1));
   for (int a = 0; a < q; a++)
-    for (int k = 0; k < q; k++) sq[a + (size_t)k * q] = sin(M_PI * (double)(a + 1) * (k + 1) / (q + 1));
+    for (int k = 0; k < q; k++) sq[a + (size_t)k * Q] = sin(M_PI * (double)(a + 1) * (k + 1) / (q + 1));
   const double cc = (2.0 / (p + 1)) * (2.0 / (q + 1));
   for (int k = 0; k < p; k++)
     for (int l = 0; l < q; l++) {
       const double lam = 2 * cos(M_PI * (k + 1) / (p + 1)) - 2 + 2 * cos(M_PI * (l + 1) / (q + 1)) - 2;
-      g[k + (size_t)l * p] = cc / lam;
+      g[k + (size_t)l * P] = cc / lam;
     }
   std::vector<float> sp32(sp.begin(), sp.end()), sq32(sq.begin(), sq.end()), g32(g.begin(), g.end());
   EPIC_HIP_TRY(hipMalloc(&S->Sp64, sp.size() * 8));
@@ -126,6 +134,11 @@ static hipError_t spec_init(Ctx* c) {
   EPIC_HIP_TRY(hipMemcpy(S->Sp32, sp32.data(), sp32.size() * 4, hipMemcpyHostToDevice));
   EPIC_HIP_TRY(hipMemcpy(S->Sq32, sq32.data(), sq32.size() * 4, hipMemcpyHostToDevice));
   EPIC_HIP_TRY(hipMemcpy(S->G32, g32.data(), g32.size() * 4, hipMemcpyHostToDevice));
+  // the work matrices' padding must start (and so stays) zero
+  EPIC_HIP_TRY(hipMemset(S->X64, 0, g.size() * 8));
+  EPIC_HIP_TRY(hipMemset(S->Y64, 0, g.size() * 8));
+  EPIC_HIP_TRY(hipMemset(S->X32, 0, g.size() * 4));
+  EPIC_HIP_TRY(hipMemset(S->Y32, 0, g.size() * 4));
   if (rocblas_create_handle(&S->h) != rocblas_status_success) return hipErrorUnknown;
   if (rocblas_set_stream(S->h, c->stream) != rocblas_status_success) return hipErrorUnknown;
   c->blas = S;
@@ -163,15 +176,17 @@ static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwr
   SpecState* S = (SpecState*)c->blas;
   const SpecLayout L = S->L;
   const int n = L.p * L.q;
+  const int P = L.pp, Q = L.qq;
   const int grid = std::min((n + 255) / 256, 4096);
+  const int gridP = std::min((P * Q + 255) / 256, 4096);
   hipLaunchKernelGGL(pack_kernel<T>, dim3(grid), dim3(256), 0, c->stream, src, X, L);
   EPIC_HIP_TRY(hipGetLastError());
-  if (gemm(S->h, L.p, L.q, L.p, Sp, L.p, X, L.p, Y, L.p) != rocblas_status_success) return hipErrorUnknown;
-  if (gemm(S->h, L.p, L.q, L.q, Y, L.p, Sq, L.q, X, L.p) != rocblas_status_success) return hipErrorUnknown;
-  hipLaunchKernelGGL(hadamard_kernel<T>, dim3(grid), dim3(256), 0, c->stream, X, G, n);
+  if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
+  if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
+  hipLaunchKernelGGL(hadamard_kernel<T>, dim3(gridP), dim3(256), 0, c->stream, X, G, P * Q);
   EPIC_HIP_TRY(hipGetLastError());
-  if (gemm(S->h, L.p, L.q, L.p, Sp, L.p, X, L.p, Y, L.p) != rocblas_status_success) return hipErrorUnknown;
-  if (gemm(S->h, L.p, L.q, L.q, Y, L.p, Sq, L.q, X, L.p) != rocblas_status_success) return hipErrorUnknown;
+  if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
+  if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
   hipLaunchKernelGGL(unpack_add_kernel<T>, dim3(grid), dim3(256), 0, c->stream, X, phi, L, overwrite ? 1 : 0);
   return hipGetLastError();
 }
