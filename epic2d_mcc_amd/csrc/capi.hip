@@ -202,6 +202,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PART_PPT")) c->part_ppt = atoi(v);
   if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
   if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
+  if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
   c->max_solver_blocks = c->num_cu;
   if (ext) {
     c->stream = ext;
@@ -642,7 +643,15 @@ int epic_step(epic_ctx* h, int ts, int* iters) {
   stage_begin(c);
   CK(launch_efield(c), "efield");
   stage_end(c, 3);
+  // every pushed species in one launch where the FP32 hot kernel applies
+  const bool all = push_all_ok(c);
+  if (all) {
+    stage_begin(c);
+    CK(launch_push_all(c), "push");
+    stage_end(c, 4);
+  }
   for (int s = 0; s < c->nsp; s++) {
+    if (all && c->sp[s].desc.pushed) continue;
     stage_begin(c);
     if (c->sp[s].desc.pushed) CK(launch_push(c, s, 1, true, 0.0), "push");
     else CK(launch_deposit(c, s), "deposit");

@@ -141,6 +141,7 @@ struct Ctx {
   int part_ppt = 2;            // FP64 particle kernels: particles per thread (2 or 4; generic also 8)
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
+  int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
 
   // species s is due for its counting sort after this step
   bool sort_due(int s) const {
@@ -172,6 +173,8 @@ hipError_t launch_den_only(Ctx* c, int s);
 hipError_t launch_zero_Q(Ctx* c);
 // particles.hip
 hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind_factor);
+bool push_all_ok(const Ctx* c);
+hipError_t launch_push_all(Ctx* c);
 hipError_t launch_load(Ctx* c, int s, int64_t n, int vz_thermal);
 hipError_t launch_compact(Ctx* c, int s);
 hipError_t launch_sort(Ctx* c, int s);

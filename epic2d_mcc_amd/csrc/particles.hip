@@ -632,6 +632,78 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
   }
 }
 
+// All pushed FP32 species of a context in one launch: the species' chunk
+// ranges are concatenated (species s owns global chunks [cbeg_s, cend_s)),
+// and the workgroups walk the global chunk index round-robin with the
+// software pipeline of push_f32_kernel, the next chunk possibly of the next
+// species.  Per-particle arithmetic is push_f32_kernel's.
+constexpr int PUSH_MAXSP = 8;
+struct PushF32Set {
+  PushF32 sp[PUSH_MAXSP];
+  int nsp;
+};
+
+template <int PPT>
+__global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S) {
+  __shared__ unsigned long long lds[(PART_BLOCK / 64) * WAVE_WIN];
+  constexpr int CHW = 64 * PPT;
+  constexpr int CH = PART_BLOCK * PPT;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long* wl = lds + wid * WAVE_WIN;
+  // slot counts per species (LDS: indexed by a runtime species number)
+  __shared__ int64_t ns[PUSH_MAXSP];
+  if (threadIdx.x < PUSH_MAXSP) ns[threadIdx.x] = (int)threadIdx.x < S.nsp ? *S.sp[threadIdx.x].d_n : 0;
+  __syncthreads();
+  int64_t cend[PUSH_MAXSP];
+  int64_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < PUSH_MAXSP; k++) {
+    tot += (ns[k] + CH - 1) / CH;
+    cend[k] = tot;
+  }
+  // species and first slot of global chunk g (wavefront-uniform)
+  auto locate = [&](int64_t g, int& sp, int64_t& wb, int64_t& n) {
+    int s = 0;
+    int64_t beg = 0;
+#pragma unroll
+    for (int k = 0; k + 1 < PUSH_MAXSP; k++)
+      if (g >= cend[k]) {
+        s = k + 1;
+        beg = cend[k];
+      }
+    sp = s;
+    n = ns[s];
+    wb = (g - beg) * CH + (int64_t)wid * CHW;
+  };
+  int64_t g = blockIdx.x;
+  if (g >= tot) return;
+  int s;
+  int64_t wb, n;
+  locate(g, s, wb, n);
+  PushF32 a = S.sp[s];
+  ChunkRegs<PPT> r;
+  if (wb < n) load_chunk<PPT>(a, wb, n, lane, r);
+  for (; g < tot; g += gridDim.x) {
+    ChunkDep<PPT> dp;
+    const bool active = wb < n;
+    if (active) push_chunk_regs<true, PPT>(a, wb, n, lane, r, dp);
+    const int64_t gn = g + gridDim.x;
+    const bool more = gn < tot;
+    int sn = s;
+    int64_t wbn = 0, nn = 0;
+    if (more) locate(gn, sn, wbn, nn);
+    const PushF32 an = S.sp[sn];
+    const bool full = more && wbn + CHW <= nn;
+    if (full) load_chunk_full<PPT>(an, wbn, lane, r);
+    if (active) deposit_chunk<PPT>(a, lane, wl, dp);
+    if (more && !full && wbn < nn) load_chunk<PPT>(an, wbn, nn, lane, r);
+    a = an;
+    s = sn;
+    n = nn;
+    wb = wbn;
+  }
+}
+
 // ---------------------------------------------------------------------------
 template <typename P>
 static PartArgs<P> make_args(Ctx* c, int s) {
@@ -819,5 +891,32 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
 }
 
 hipError_t launch_deposit(Ctx* c, int s) { return launch_particles(c, s, MODE_DEPOSIT, true, 0, 0, 0); }
+
+// Whether one launch can push (with deposit) every species whose
+// desc.pushed is set: the FP32 hot kernel applies to all of them (default
+// shape) and there are 1 to PUSH_MAXSP of them.
+bool push_all_ok(const Ctx* c) {
+  const epic_domain_desc& d = c->dom;
+  const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
+  if (!c->push_multi || d.precision != EPIC_FP32 || d.shape != EPIC_SHAPE_CIC || whole || d.nx > 65535 ||
+      d.ny > 65535 || c->push_variant != 0)
+    return false;
+  int np = 0;
+  for (int s = 0; s < c->nsp; s++) np += c->sp[s].desc.pushed ? 1 : 0;
+  return np >= 1 && np <= PUSH_MAXSP;
+}
+
+hipError_t launch_push_all(Ctx* c) {
+  PushF32Set S{};
+  for (int s = 0; s < c->nsp; s++) {
+    if (!c->sp[s].desc.pushed) continue;
+    double c1, c2, c3;
+    push_consts(c, s, 1, false, &c1, &c2, &c3);
+    S.sp[S.nsp++] = make_push_f32(c, s, c1 * c2, c3 * c2);
+  }
+  auto k = push_f32_multi_kernel<4>;
+  hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, S);
+  return hipGetLastError();
+}
 
 }  // namespace epic
