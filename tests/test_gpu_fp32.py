@@ -73,3 +73,48 @@ def test_fp32_tracks_fp64_oracle():
         print(f"  species {s}: x {errs[0]:.2e} y {errs[1]:.2e} vx {errs[2]:.2e} vy {errs[3]:.2e}")
         assert errs[0] < 1e-5 and errs[1] < 1e-5
         assert errs[2] < 2e-3 and errs[3] < 2e-3
+
+
+def _run_env(st, parts, env, steps=3):
+    """A context created under the given kernel-shape environment (read at
+    epic_create), after `steps` fused steps: particles and rho."""
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        ctx = _lib.Context(st.domain, st.species)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    for s, p in enumerate(parts):
+        ctx.upload(s, *p)
+    ctx.initial_field()
+    for ts in range(steps):
+        ctx.step(ts)
+    out = [ctx.download(s)[:4] for s in range(len(parts))], ctx.get_field(0).copy()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("env", [{"EPIC_PUSH_MULTI": "0"}, {"EPIC_PUSH_VARIANT": "1"}, {"EPIC_PUSH_VARIANT": "2"},
+                                 {"EPIC_PUSH_VARIANT": "3"}, {"EPIC_PUSH_VARIANT": "4"}, {"EPIC_PUSH_MAP": "0"},
+                                 {"EPIC_PUSH_GRID": "2"}])
+def test_fp32_kernel_shapes_identical(env):
+    """Every FP32 push shape (one launch for all species or one per species,
+    particles per thread, occupancy target, chunk walk, grid size) gives the
+    default's particles and rho bit for bit."""
+    print(f"[fp32] shape {env}", flush=True)
+    st = small_setup(N=200000, Nx=128, Ny=128, precision=_lib.FP32, bc="open", vsx_beam=3.0,
+                     solver=_lib.SOLVER_PCG_MIXED, poisson_op=_lib.OP_DIRICHLET_WALL, tol=1e-10)
+    st.domain.tol_relative = 1
+    st.domain.precond = _lib.PRECOND_SPECTRAL
+    o, parts = reference_load(st)
+    ref_parts, ref_rho = _run_env(st, parts, {})
+    got_parts, got_rho = _run_env(st, parts, env)
+    for s in range(2):
+        for k in range(4):
+            assert np.array_equal(ref_parts[s][k], got_parts[s][k], equal_nan=True), (s, k)
+    assert np.array_equal(ref_rho, got_rho)

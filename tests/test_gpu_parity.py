@@ -174,3 +174,37 @@ def test_windowed_fused_step_matches_oracle():
     _compare_fields(o, ctx, "after 5 fused steps")
     _compare_particles(o, ctx, 2, "after 5 fused steps")
     assert not ctx.errors().any()
+
+
+@pytest.mark.parametrize("env", [{"EPIC_PUSH64": "0"}, {"EPIC_PART_PPT": "4"}, {"EPIC_PART_MAP": "1"}])
+def test_windowed_fp64_shapes_identical(env):
+    """The FP64 windowed push under other kernel shapes (the generic kernel,
+    4 particles per thread, round-robin walk) reproduces the default's
+    particles and rho bit for bit."""
+    import os
+    print(f"[parity] fp64 shape {env}", flush=True)
+    st = small_setup(N=60000, Nx=128, Ny=128, bc="reflective", tol=1e-10, vsx_beam=2.0)
+    o, parts = reference_load(st)
+    outs = []
+    for e in ({}, env):
+        old = {k: os.environ.get(k) for k in e}
+        os.environ.update(e)
+        try:
+            ctx = _lib.Context(st.domain, st.species)
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        for s, p in enumerate(parts):
+            ctx.upload(s, *p)
+        ctx.initial_field()
+        for ts in range(3):
+            ctx.step(ts)
+        outs.append(([ctx.download(s)[:4] for s in range(2)], ctx.get_field(0).copy()))
+        ctx.close()
+    for s in range(2):
+        for k in range(4):
+            assert np.array_equal(outs[0][0][s][k], outs[1][0][s][k]), (s, k)
+    assert np.array_equal(outs[0][1], outs[1][1])
