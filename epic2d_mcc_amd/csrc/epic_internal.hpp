@@ -142,6 +142,7 @@ struct Ctx {
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
   int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
+  int spec_mfma = 1;           // FP32 sine transforms on the hand-written MFMA GEMM (EPIC_SPEC_MFMA=0: rocBLAS)
 
   // species s is due for its counting sort after this step
   bool sort_due(int s) const {

@@ -156,6 +156,93 @@ void spec_destroy(Ctx* c) {
   c->blas = nullptr;
 }
 
+// C = A B (times G elementwise when G is given), column-major FP32, on the
+// matrix cores: M, N multiples of 32, K a multiple of 64 up to 512.  One workgroup per
+// 32 x 32 tile of C; its four wavefronts take a quarter of K each, with a
+// 2 x 2 block of v_mfma_f32_16x16x4f32 accumulators; the four partial tiles
+// are summed in LDS in wavefront order, so the result does not depend on
+// scheduling.  Operand layout of the 16x16x4 MFMA: lane l holds A[l%16][k]
+// and B[k][l%16] for its k = l/16, and D[4(l/16)+r][l%16] in register r.
+// The contraction index is free to permute, so lane group g takes the four
+// consecutive k = kb + 4g + e over the four MFMAs e of a 16-deep step: the B
+// operand (contiguous along k) is then one 16-byte load per lane.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// (one or two wavefronts per SIMD: the register budget lets the scheduler
+// keep every operand load of the K slice in flight at once)
+template <int NSTEP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void sgemm_tile32_kernel(const float* __restrict__ A, int lda,
+                                                           const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                                           int ldc, const float* __restrict__ G) {
+  __shared__ float part[4][32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tm = blockIdx.x * 32, tn = blockIdx.y * 32;
+  const int r16 = lane & 15, g = lane >> 4;
+  const int k0 = w * NSTEP * 16;  // this wavefront's quarter of K = 64 NSTEP
+  // the whole K slice's operands are loaded before the first product (one
+  // wavefront per SIMD: nothing else would hide the latency)
+  v4f b[NSTEP][2];
+  float a[NSTEP][2][4];
+#pragma unroll
+  for (int st = 0; st < NSTEP; st++) {
+    const int kb = k0 + st * 16;
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+      b[st][cb] = *reinterpret_cast<const v4f*>(B + (size_t)(tn + cb * 16 + r16) * ldb + kb + 4 * g);
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) a[st][rb][e] = A[(size_t)(tm + rb * 16 + r16) + (size_t)(kb + 4 * g + e) * lda];
+  }
+  // the scheduler may not move anything across: every load above is in
+  // flight before the first product waits for its own operands
+  __builtin_amdgcn_sched_barrier(0);
+  v4f acc[2][2];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = 0; st < NSTEP; st++)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st][rb][e], b[st][cb][e], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) part[w][(rb * 16 + 4 * g + r) + (cb * 16 + r16) * 32] = acc[rb][cb][r];
+  __syncthreads();
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    float v = part[0][t] + part[1][t];
+    v += part[2][t];
+    v += part[3][t];
+    const size_t o = (size_t)(tm + (t & 31)) + (size_t)(tn + (t >> 5)) * ldc;
+    if (G) v *= G[o];
+    C[o] = v;
+  }
+}
+
+// C (m x n) = A (m x k) B (k x n) (times G), k = 64 NSTEP
+static void sgemm_tile32(hipStream_t st, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
+                         float* C, int ldc, const float* G) {
+  const dim3 grid(m / 32, n / 32);
+  switch (k / 64) {
+#define EPIC_G(N) \
+  case N: hipLaunchKernelGGL(sgemm_tile32_kernel<N>, grid, dim3(256), 0, st, A, lda, B, ldb, C, ldc, G); break;
+    EPIC_G(1) EPIC_G(2) EPIC_G(3) EPIC_G(4) EPIC_G(5) EPIC_G(6) EPIC_G(7) EPIC_G(8)
+#undef EPIC_G
+    default: break;
+  }
+}
+
+static bool tile32_ok(int m, int n, int k) { return m % 32 == 0 && n % 32 == 0 && k % 64 == 0 && k <= 512; }
+
 static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                            double* C, int ldc) {
   const double one = 1.0, zero = 0.0;
@@ -181,12 +268,26 @@ static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwr
   const int gridP = std::min((P * Q + 255) / 256, 4096);
   hipLaunchKernelGGL(pack_kernel<T>, dim3(grid), dim3(256), 0, c->stream, src, X, L);
   EPIC_HIP_TRY(hipGetLastError());
-  if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
-  if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
-  hipLaunchKernelGGL(hadamard_kernel<T>, dim3(gridP), dim3(256), 0, c->stream, X, G, P * Q);
-  EPIC_HIP_TRY(hipGetLastError());
-  if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
-  if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
+  if (sizeof(T) == 4 && c->spec_mfma && tile32_ok(P, Q, P) && tile32_ok(P, Q, Q)) {
+    // FP32 on the hand-written MFMA kernel; the Hadamard scaling by G in
+    // the second product's epilogue
+    const float* Spf = (const float*)Sp;
+    const float* Sqf = (const float*)Sq;
+    float* Xf = (float*)X;
+    float* Yf = (float*)Y;
+    sgemm_tile32(c->stream, P, Q, P, Spf, P, Xf, P, Yf, P, nullptr);
+    sgemm_tile32(c->stream, P, Q, Q, Yf, P, Sqf, Q, Xf, P, (const float*)G);
+    sgemm_tile32(c->stream, P, Q, P, Spf, P, Xf, P, Yf, P, nullptr);
+    sgemm_tile32(c->stream, P, Q, Q, Yf, P, Sqf, Q, Xf, P, nullptr);
+    EPIC_HIP_TRY(hipGetLastError());
+  } else {
+    if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
+    if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
+    hipLaunchKernelGGL(hadamard_kernel<T>, dim3(gridP), dim3(256), 0, c->stream, X, G, P * Q);
+    EPIC_HIP_TRY(hipGetLastError());
+    if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
+    if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
+  }
   hipLaunchKernelGGL(unpack_add_kernel<T>, dim3(grid), dim3(256), 0, c->stream, X, phi, L, overwrite ? 1 : 0);
   return hipGetLastError();
 }
