@@ -70,9 +70,11 @@ __global__ void zero_kernel(double* p, int n) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0.0;
 }
 
-// residual r = b - A phi over the unknowns (FP64) and ||r||^2 partials
+// residual r = b - A phi over the unknowns (FP64) and ||r||^2 partials;
+// X (when given): r packed in FP32 as the transform's operand
 template <int OP>
-__global__ void spec_residual_kernel(const double* b, const double* phi, double* r, int nx, int ny, double* part) {
+__global__ void spec_residual_kernel(const double* b, const double* phi, double* r, int nx, int ny, double* part,
+                                     float* X, SpecLayout L) {
   __shared__ double sh[64];
   const int n = nx * ny;
   double acc = 0;
@@ -80,6 +82,10 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
     const double rk = residual_at(OP, b, phi, k, nx, ny);
     r[k] = rk;
     acc += rk * rk;
+    if (X) {
+      const int x = spec_index(L, k);
+      if (x >= 0) X[x] = (float)rk;
+    }
   }
   double v[1] = {acc};
   block_sum<1>(v, sh);
@@ -168,31 +174,46 @@ void spec_destroy(Ctx* c) {
 // operand (contiguous along k) is then one 16-byte load per lane.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Operands of one tile GEMM.  dst: the product goes to dst through the
+// layout L (overwritten or added) instead of to C.  G: the product is scaled
+// by G elementwise.
+struct TileGemm {
+  const float* A;
+  int lda;
+  const float* B;
+  int ldb;
+  float* C;
+  int ldc;
+  const float* G;
+  double* dst;
+  SpecLayout L;
+  int overwrite;
+};
+
 // (one or two wavefronts per SIMD: the register budget lets the scheduler
 // keep every operand load of the K slice in flight at once)
 template <int NSTEP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void sgemm_tile32_kernel(const float* __restrict__ A, int lda,
-                                                           const float* __restrict__ B, int ldb, float* __restrict__ C,
-                                                           int ldc, const float* __restrict__ G) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void sgemm_tile32_kernel(TileGemm t) {
   __shared__ float part[4][32 * 32];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int tm = blockIdx.x * 32, tn = blockIdx.y * 32;
   const int r16 = lane & 15, g = lane >> 4;
   const int k0 = w * NSTEP * 16;  // this wavefront's quarter of K = 64 NSTEP
-  // the whole K slice's operands are loaded before the first product (one
-  // wavefront per SIMD: nothing else would hide the latency)
   v4f b[NSTEP][2];
   float a[NSTEP][2][4];
 #pragma unroll
   for (int st = 0; st < NSTEP; st++) {
     const int kb = k0 + st * 16;
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++)
-      b[st][cb] = *reinterpret_cast<const v4f*>(B + (size_t)(tn + cb * 16 + r16) * ldb + kb + 4 * g);
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = tn + cb * 16 + r16;
+      b[st][cb] = *reinterpret_cast<const v4f*>(t.B + (size_t)col * t.ldb + kb + 4 * g);
+    }
 #pragma unroll
     for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-      for (int e = 0; e < 4; e++) a[st][rb][e] = A[(size_t)(tm + rb * 16 + r16) + (size_t)(kb + 4 * g + e) * lda];
+      for (int e = 0; e < 4; e++)
+        a[st][rb][e] = t.A[(size_t)(tm + rb * 16 + r16) + (size_t)(kb + 4 * g + e) * t.lda];
   }
   // the scheduler may not move anything across: every load above is in
   // flight before the first product waits for its own operands
@@ -218,23 +239,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 #pragma unroll
       for (int r = 0; r < 4; r++) part[w][(rb * 16 + 4 * g + r) + (cb * 16 + r16) * 32] = acc[rb][cb][r];
   __syncthreads();
-  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
-    float v = part[0][t] + part[1][t];
-    v += part[2][t];
-    v += part[3][t];
-    const size_t o = (size_t)(tm + (t & 31)) + (size_t)(tn + (t >> 5)) * ldc;
-    if (G) v *= G[o];
-    C[o] = v;
+  for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+    float v = part[0][q] + part[1][q];
+    v += part[2][q];
+    v += part[3][q];
+    const int i = tm + (q & 31), j = tn + (q >> 5);
+    const size_t o = (size_t)i + (size_t)j * t.ldc;
+    if (t.G) v *= t.G[o];
+    if (t.dst) {
+      if (i < t.L.p && j < t.L.q) {
+        double* d = t.dst + t.L.off + i + (size_t)j * t.L.ld;
+        *d = t.overwrite ? (double)v : *d + (double)v;
+      }
+    } else {
+      t.C[o] = v;
+    }
   }
 }
 
 // C (m x n) = A (m x k) B (k x n) (times G), k = 64 NSTEP
-static void sgemm_tile32(hipStream_t st, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
-                         float* C, int ldc, const float* G) {
+static void sgemm_tile32(hipStream_t st, int m, int n, int k, const TileGemm& t) {
   const dim3 grid(m / 32, n / 32);
   switch (k / 64) {
 #define EPIC_G(N) \
-  case N: hipLaunchKernelGGL(sgemm_tile32_kernel<N>, grid, dim3(256), 0, st, A, lda, B, ldb, C, ldc, G); break;
+  case N: hipLaunchKernelGGL(sgemm_tile32_kernel<N>, grid, dim3(256), 0, st, t); break;
     EPIC_G(1) EPIC_G(2) EPIC_G(3) EPIC_G(4) EPIC_G(5) EPIC_G(6) EPIC_G(7) EPIC_G(8)
 #undef EPIC_G
     default: break;
@@ -242,6 +270,14 @@ static void sgemm_tile32(hipStream_t st, int m, int n, int k, const float* A, in
 }
 
 static bool tile32_ok(int m, int n, int k) { return m % 32 == 0 && n % 32 == 0 && k % 64 == 0 && k <= 512; }
+
+// The FP32 transform runs on sgemm_tile32 (the residual kernel then writes
+// its packed FP32 operand).
+static bool spec_mfma_ok(const Ctx* c) {
+  const SpecState* S = (const SpecState*)c->blas;
+  const int P = S->L.pp, Q = S->L.qq;
+  return c->spec_mfma && tile32_ok(P, Q, P) && tile32_ok(P, Q, Q);
+}
 
 static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                            double* C, int ldc) {
@@ -266,21 +302,26 @@ static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwr
   const int P = L.pp, Q = L.qq;
   const int grid = std::min((n + 255) / 256, 4096);
   const int gridP = std::min((P * Q + 255) / 256, 4096);
-  hipLaunchKernelGGL(pack_kernel<T>, dim3(grid), dim3(256), 0, c->stream, src, X, L);
-  EPIC_HIP_TRY(hipGetLastError());
-  if (sizeof(T) == 4 && c->spec_mfma && tile32_ok(P, Q, P) && tile32_ok(P, Q, Q)) {
-    // FP32 on the hand-written MFMA kernel; the Hadamard scaling by G in
-    // the second product's epilogue
+  if (sizeof(T) == 4 && spec_mfma_ok(c)) {
+    // FP32 on the hand-written MFMA kernel: the Hadamard scaling by G in
+    // the second product's epilogue, the unpacking into phi in the fourth's
     const float* Spf = (const float*)Sp;
     const float* Sqf = (const float*)Sq;
     float* Xf = (float*)X;
     float* Yf = (float*)Y;
-    sgemm_tile32(c->stream, P, Q, P, Spf, P, Xf, P, Yf, P, nullptr);
-    sgemm_tile32(c->stream, P, Q, Q, Yf, P, Sqf, Q, Xf, P, (const float*)G);
-    sgemm_tile32(c->stream, P, Q, P, Spf, P, Xf, P, Yf, P, nullptr);
-    sgemm_tile32(c->stream, P, Q, Q, Yf, P, Sqf, Q, Xf, P, nullptr);
-    EPIC_HIP_TRY(hipGetLastError());
+    // X holds the packed FP32 residual, written by the residual kernel
+    TileGemm t1{Spf, P, Xf, P, Yf, P, nullptr, nullptr, L, 0};
+    TileGemm t2{Yf, P, Sqf, Q, Xf, P, (const float*)G, nullptr, L, 0};
+    TileGemm t3{Spf, P, Xf, P, Yf, P, nullptr, nullptr, L, 0};
+    TileGemm t4{Yf, P, Sqf, Q, Xf, P, nullptr, phi, L, overwrite ? 1 : 0};
+    sgemm_tile32(c->stream, P, Q, P, t1);
+    sgemm_tile32(c->stream, P, Q, Q, t2);
+    sgemm_tile32(c->stream, P, Q, P, t3);
+    sgemm_tile32(c->stream, P, Q, Q, t4);
+    return hipGetLastError();
   } else {
+    hipLaunchKernelGGL(pack_kernel<T>, dim3(grid), dim3(256), 0, c->stream, src, X, L);
+    EPIC_HIP_TRY(hipGetLastError());
     if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
     if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
     hipLaunchKernelGGL(hadamard_kernel<T>, dim3(gridP), dim3(256), 0, c->stream, X, G, P * Q);
@@ -292,16 +333,18 @@ static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwr
   return hipGetLastError();
 }
 
-static hipError_t residual_norm2(Ctx* c, const double* b, const double* phi, double* r, double* out_dev) {
+static hipError_t residual_norm2(Ctx* c, const double* b, const double* phi, double* r, double* out_dev,
+                                 float* X = nullptr) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
   const int g = std::min((n + 255) / 256, 1024);
+  const SpecLayout L = c->blas ? ((SpecState*)c->blas)->L : SpecLayout{};
   if (d.poisson_op == EPIC_OP_REF_GHOST)
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_REF_GHOST>, dim3(g), dim3(256), 0, c->stream, b, phi, r, d.nx,
-                       d.ny, c->ws_part);
+                       d.ny, c->ws_part, X, L);
   else
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_DIRICHLET_WALL>, dim3(g), dim3(256), 0, c->stream, b, phi, r,
-                       d.nx, d.ny, c->ws_part);
+                       d.nx, d.ny, c->ws_part, X, L);
   EPIC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(64), 0, c->stream, c->ws_part, g, out_dev);
   return hipGetLastError();
@@ -320,7 +363,8 @@ hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32
   hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(256), 0, c->stream, phi, n);
   EPIC_HIP_TRY(hipGetLastError());
   double* dscal = c->ws_scalar + 4;  // [4] ||b||^2, [5] ||r||^2
-  EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal));  // phi = 0: r = b
+  float* Xpack = fp32_inner && spec_mfma_ok(c) ? S->X32 : nullptr;
+  EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal, Xpack));  // phi = 0: r = b
   int k = 0;
   double host[2] = {0, 0};
   for (; k < max_ref; k++) {
@@ -328,7 +372,7 @@ hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32
       EPIC_HIP_TRY(spec_apply<float>(c, c->r64, phi, k == 0, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32));
     else
       EPIC_HIP_TRY(spec_apply<double>(c, c->r64, phi, k == 0, S->Sp64, S->Sq64, S->G64, S->X64, S->Y64));
-    EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal + 1));
+    EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal + 1, Xpack));
     // the FP32 correction gains ~6 digits: two applications reach 1e-10 and
     // more; check the true residual after the second
     if (k + 1 >= (fp32_inner ? 2 : 1)) {
