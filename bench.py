@@ -28,10 +28,10 @@ from epic2d_mcc_amd import _lib, workloads  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes the push kernel moves per particle, from rocprofv3 --pmc on this
-# workload (profiles/push_f32_kernel_bounded512_pmc.md): FETCH_SIZE x 2 +
-# WRITE_SIZE x 1 per dispatch (the factors a plain 1 GiB device copy reads
-# back as), divided by the 26,214,400 particles of a dispatch.
-PUSH_TRAFFIC_BYTES_PER_PARTICLE = (214532.78 * 2 + 429676.73) * 1024 / 26214400
+# workload (profiles/push_f32_multi_kernel_bounded512_pmc.md): FETCH_SIZE x 2
+# + WRITE_SIZE x 1 per dispatch (the factors a plain 1 GiB device copy reads
+# back as), divided by the 104,857,600 particles of a dispatch.
+PUSH_TRAFFIC_BYTES_PER_PARTICLE = (856875.02 * 2 + 1718744.54) * 1024 / 104857600
 
 
 def gpu_run(st, steps, warmup, solver_name, mcc=None):
