@@ -42,12 +42,16 @@ inline KeyMap key_map(int ncx, int ncy) {
   return m;
 }
 
+// A live position maps to a cell; one whose logical coordinate rounds onto
+// the upper boundary node belongs to the last cell (as in the push's
+// deposit), so the sort never drops it.  Positions outside the grid and dead
+// (NaN) ones get the key K and are dropped.
 template <typename P>
 __device__ __forceinline__ int cell_key(P x, P y, P x0, P y0, P dx, P dy, const KeyMap& m) {
   if (isnan(x)) return m.K;
   const P lx = (x - x0) / dx, ly = (y - y0) / dy;
-  const int i = (int)lx, j = (int)ly;
-  if (!(lx >= 0) || !(ly >= 0) || i >= m.ncx || j >= m.ncy) return m.K;
+  if (!(lx >= 0) || !(ly >= 0) || !(lx <= (P)m.ncx) || !(ly <= (P)m.ncy)) return m.K;
+  const int i = min((int)lx, m.ncx - 1), j = min((int)ly, m.ncy - 1);
   return ((i / SORT_TILE) * m.nty + j / SORT_TILE) * (SORT_TILE * SORT_TILE) + (i % SORT_TILE) * SORT_TILE +
          j % SORT_TILE;
 }

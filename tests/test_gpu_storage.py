@@ -18,10 +18,11 @@ def _cell_keys(x, y, d, T=16):
     row-major inside a tile."""
     lx = (x - d.x0) / d.dx
     ly = (y - d.y0) / d.dy
-    ncy = d.ny - 1
+    ncx, ncy = d.nx - 1, d.ny - 1
     nty = (ncy + T - 1) // T
-    i = np.floor(lx).astype(np.int64)
-    j = np.floor(ly).astype(np.int64)
+    # a coordinate that rounds onto the upper boundary node is in the last cell
+    i = np.minimum(np.floor(lx).astype(np.int64), ncx - 1)
+    j = np.minimum(np.floor(ly).astype(np.int64), ncy - 1)
     return ((i // T) * nty + j // T) * (T * T) + (i % T) * T + j % T
 
 
@@ -133,3 +134,33 @@ def test_loader_positions_are_philox():
     # Maxwellian spread sqrt(kT/m)/vel_norm per component (= 1 for electrons at Te)
     for comp in (vx, vy, vz):
         assert abs(np.std(comp) - 1.0) < 0.05 and abs(np.mean(comp)) < 0.05
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32])
+def test_sort_keeps_particles_on_the_upper_boundary(precision):
+    """Live positions within an ulp of the upper walls (whose logical
+    coordinate can round onto the boundary node) are kept by the sort and
+    land in the last cells."""
+    st = small_setup(N=4096, Nx=32, Ny=24, precision=precision, bc="open")
+    d = st.domain
+    Lx, Ly = (d.nx - 1) * d.dx, (d.ny - 1) * d.dy
+    ftype = np.float32 if precision == _lib.FP32 else np.float64
+    xe = float(np.nextafter(ftype(d.x0 + Lx), ftype(0)))
+    ye = float(np.nextafter(ftype(d.y0 + Ly), ftype(0)))
+    n = 4096
+    rng = np.random.default_rng(3)
+    x = rng.random(n) * Lx
+    y = rng.random(n) * Ly
+    x[::4] = xe
+    y[1::4] = ye
+    x[2::8] = xe
+    y[2::8] = ye
+    z = np.zeros(n)
+    ctx = _lib.Context(st.domain, st.species)
+    ctx.upload(0, x, y, z, z, z)
+    ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    ctx.sort(0)
+    assert ctx.count(0) == n
+    gx, gy = ctx.download(0)[:2]
+    assert np.array_equal(np.sort(gx), np.sort(x.astype(ftype).astype(np.float64)))
+    ctx.close()
