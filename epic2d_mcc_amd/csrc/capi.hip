@@ -204,6 +204,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
   if (const char* v = getenv("EPIC_SPEC_MFMA")) c->spec_mfma = atoi(v);
+  if (const char* v = getenv("EPIC_PCG_BLOCK")) c->pcg_block = atoi(v);
   c->max_solver_blocks = c->num_cu;
   if (ext) {
     c->stream = ext;

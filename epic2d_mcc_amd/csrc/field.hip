@@ -110,8 +110,8 @@ __device__ __forceinline__ void gsync(GridBar& grid) {
 // LDSV (single workgroup): the search direction lives in LDS (n values of
 // T, dynamic shared memory) instead of the global double buffer, so the
 // stencil reads its neighbours from LDS.
-template <typename T, int OP, int NPT, bool COOP, bool LDSV>
-__global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
+template <typename T, int OP, int NPT, bool COOP, bool LDSV, int BS = SOLVER_BLOCK>
+__global__ __launch_bounds__(BS) void pcg_kernel(PcgArgs<T> a) {
   __shared__ double sh[64];
   extern __shared__ double vs_raw[];
   T* const vs = reinterpret_cast<T*>(vs_raw);
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
     }
     double d1[1] = {dvu};
     if (LDSV)
-      block_sum_open<1, SOLVER_BLOCK / 64>(d1, sh);  // region 0..15; the next sum uses 32..63
+      block_sum_open<1, BS / 64>(d1, sh);  // region 0..15; the next sum uses 32..63
     else
       grid_sum<1, COOP>(d1, a.part, 1, sh, grid);
     const double t = alpha / d1[0];
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void pcg_kernel(PcgArgs<T> a) {
       vold[m] = v[m];
     }
     if (LDSV)
-      block_sum_open<2, SOLVER_BLOCK / 64>(db, sh + 32);
+      block_sum_open<2, BS / 64>(db, sh + 32);
     else
       grid_sum<2, COOP>(db, a.part, 2, sh, grid);
     const double beta = db[0];
@@ -486,8 +486,33 @@ static hipError_t run_pcg_single(Ctx* c, PcgArgs<T>& a) {
   return run_pcg_t<T, OP, (NPT <= 4 ? 4 : 8)>(c, 1, a);
 }
 
+// Single workgroup of BS < 1024 threads (EPIC_PCG_BLOCK), search direction
+// in LDS: NPT >= n / BS nodes per thread.
+template <typename T, int OP, int BS, int NPT>
+static hipError_t run_pcg_small(Ctx* c, PcgArgs<T>& a) {
+  const size_t lds = (size_t)a.nx * a.ny * sizeof(T);
+  hipLaunchKernelGGL((pcg_kernel<T, OP, NPT, false, true, BS>), dim3(1), dim3(BS), lds, c->stream, a);
+  return hipGetLastError();
+}
+
 template <typename T, int OP>
 static hipError_t run_pcg_op(Ctx* c, int G, int npt, PcgArgs<T>& a) {
+  const int n = a.nx * a.ny;
+  // 512 threads (measured 2.5 % faster per iteration at 65^2) once each of
+  // 1024 threads would own more than 4 nodes
+  if (G == 1 && (size_t)n * sizeof(T) <= PCG_LDS_MAX &&
+      (c->pcg_block == 256 || (c->pcg_block == 512 && n > 4096))) {
+    hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
+    const int p = (n + c->pcg_block - 1) / c->pcg_block;
+    if (c->pcg_block == 512) {
+      if (p <= 9) return run_pcg_small<T, OP, 512, 9>(c, a);
+      if (p <= 12) return run_pcg_small<T, OP, 512, 12>(c, a);
+      return run_pcg_small<T, OP, 512, 16>(c, a);
+    }
+    if (p <= 17) return run_pcg_small<T, OP, 256, 17>(c, a);
+    if (p <= 24) return run_pcg_small<T, OP, 256, 24>(c, a);
+    return run_pcg_small<T, OP, 256, 32>(c, a);
+  }
   if (G == 1) {
     hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
     switch (npt) {
