@@ -402,6 +402,80 @@ def main_single(args, st, label, dtype, bytes_ps, kernel, mcc=None, gas=None, cp
     print(json.dumps(out))
 
 
+def main_ops(args):
+    """Device time of each operator of the loop on its own (rows the step
+    fuses or skips), on the headline workload: 512x512 cells, 4 x 26.2M FP32
+    particles; SOR and the FP64 push on the same grid in FP64.  Averages of
+    `reps` calls between synchronisations (host clock)."""
+    reps = 10
+
+    def timed(fn, n=reps):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        ctx.sync()
+        return 1e6 * (time.perf_counter() - t0) / n
+
+    st = workloads.bounded_512()
+    ctx = _lib.Context(st.domain, st.species)
+    nsp = len(st.species)
+    ops = {}
+    n0 = st.species[0].numparticle
+    t0 = time.perf_counter()
+    ctx.load(0, n0)
+    ctx.sync()
+    ops["load (Init, Philox) per 26.2M species, us"] = 1e6 * (time.perf_counter() - t0)
+    for s in range(1, nsp):
+        ctx.load(s, st.species[s].numparticle)
+    for s in range(nsp):
+        ctx.sort(s)
+    ctx.initial_field()
+    for ts in range(20):
+        ctx.step(ts)
+    ops["scatter (stand-alone deposit) per species, us"] = timed(lambda: ctx.scatter(0))
+    ops["compute_rho, us"] = timed(ctx.compute_rho)
+    ops["efield, us"] = timed(ctx.efield)
+    ops["rewind per species, us"] = timed(lambda: ctx.rewind(0))
+    ops["push without deposit per species, us"] = timed(lambda: ctx.push(0))
+    ops["sort per species (electrons, 20 steps after the last), us"] = timed(lambda: ctx.sort(0), 3)
+    ops["compact per species (nothing to remove), us"] = timed(lambda: ctx.compact(0), 3)
+    ops["kinetic energy per species, us"] = timed(lambda: ctx.ke(0, 0))
+    ops["momentum per species, us"] = timed(lambda: ctx.momentum(0, 0))
+    ops["potential energy, us"] = timed(lambda: ctx.pe(0))
+    for k in range(8):
+        e = _lib.EmitterDesc(x0=0.0, y0=0.0, x1=0.0, y1=(st.domain.ny - 1) * st.domain.dy, temp=1.0,
+                             numparticle=1000, vdx=1.0, vdy=0.0, species_idx1=0, species_idx2=1)
+        ctx.add_emitter(e)
+    step = [100]
+
+    def inj():
+        ctx.inject(step[0])
+        step[0] += 1
+    ops["inject, 8 emitters x 1000 pairs, us"] = timed(inj)
+    ctx.close()
+    # SOR (red-black, gssolver's problem) on the same grid, FP64, 1e-6
+    st64 = workloads.bounded_512(ppc=1, precision=_lib.FP64, solver=_lib.SOLVER_GS, tol=1e-6)
+    ctx = _lib.Context(st64.domain, st64.species)
+    for s, sp in enumerate(st64.species):
+        ctx.load(s, sp.numparticle)
+    for s in range(len(st64.species)):
+        ctx.scatter(s)
+    ctx.compute_rho()
+    t0 = time.perf_counter()
+    it, _ = ctx.solve()
+    ctx.sync()
+    ops["SOR solve 513^2 FP64 to 1e-6 RMS residual, us"] = 1e6 * (time.perf_counter() - t0)
+    ops["SOR iterations"] = it
+    ctx.close()
+    out = {"metric": "per-operator device time (us), headline workload", "value": None, "unit": "us",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": "bounded 512x512, 4 x 26,214,400 FP32 particles; SOR in FP64"},
+           "ops": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in ops.items()}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,7 +487,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="bounded512",
-                    choices=["bounded512", "periodic64", "mcc256", "ensemble64", "serve"])
+                    choices=["bounded512", "periodic64", "mcc256", "ensemble64", "serve", "ops"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"],
                     help="bounded512: fp64 runs the plain FP64 path (FP64 particles, the reference PCG)")
     ap.add_argument("--serve-steps-scale", type=float, default=1.0,
@@ -432,6 +506,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble64":
         main_ensemble(args)
+        return
+    if args.workload == "ops":
+        main_ops(args)
         return
     if args.workload == "serve":
         main_serve(args)
