@@ -99,18 +99,28 @@ def _run_env(st, parts, env, steps=3):
     return out
 
 
-@pytest.mark.parametrize("env", [{"EPIC_PUSH_MULTI": "0"}, {"EPIC_PUSH_VARIANT": "1"}, {"EPIC_PUSH_VARIANT": "2"},
-                                 {"EPIC_PUSH_VARIANT": "3"}, {"EPIC_PUSH_VARIANT": "4"}, {"EPIC_PUSH_MAP": "0"},
-                                 {"EPIC_PUSH_GRID": "2"}])
-def test_fp32_kernel_shapes_identical(env):
+_SHAPES = [("open", {"EPIC_PUSH_MULTI": "0"}), ("open", {"EPIC_PUSH_VARIANT": "1"}),
+           ("open", {"EPIC_PUSH_VARIANT": "2"}), ("open", {"EPIC_PUSH_VARIANT": "3"}),
+           ("open", {"EPIC_PUSH_VARIANT": "4"}), ("open", {"EPIC_PUSH_MAP": "0"}),
+           ("open", {"EPIC_PUSH_GRID": "2"}), ("pbc", {"EPIC_PUSH_MULTI": "0"}),
+           ("reflective", {"EPIC_PUSH_MULTI": "0"})]
+
+
+@pytest.mark.parametrize("bc,env", _SHAPES)
+def test_fp32_kernel_shapes_identical(bc, env):
     """Every FP32 push shape (one launch for all species or one per species,
     particles per thread, occupancy target, chunk walk, grid size) gives the
-    default's particles and rho bit for bit."""
-    print(f"[fp32] shape {env}", flush=True)
-    st = small_setup(N=200000, Nx=128, Ny=128, precision=_lib.FP32, bc="open", vsx_beam=3.0,
-                     solver=_lib.SOLVER_PCG_MIXED, poisson_op=_lib.OP_DIRICHLET_WALL, tol=1e-10)
-    st.domain.tol_relative = 1
-    st.domain.precond = _lib.PRECOND_SPECTRAL
+    default's particles and rho bit for bit; the one-launch and per-species
+    kernels also for periodic and reflecting walls."""
+    print(f"[fp32] shape {bc} {env}", flush=True)
+    if bc == "open":
+        st = small_setup(N=200000, Nx=128, Ny=128, precision=_lib.FP32, bc=bc, vsx_beam=3.0,
+                         solver=_lib.SOLVER_PCG_MIXED, poisson_op=_lib.OP_DIRICHLET_WALL, tol=1e-10)
+        st.domain.tol_relative = 1
+        st.domain.precond = _lib.PRECOND_SPECTRAL
+    else:
+        st = small_setup(N=200000, Nx=128, Ny=128, precision=_lib.FP32, bc=bc, vsx_beam=3.0,
+                         solver=_lib.SOLVER_PCG, tol=1e-8)
     o, parts = reference_load(st)
     ref_parts, ref_rho = _run_env(st, parts, {})
     got_parts, got_rho = _run_env(st, parts, env)
