@@ -56,6 +56,26 @@ class EmitterDesc(ctypes.Structure):
                 ("vdy", ctypes.c_double), ("species_idx1", ctypes.c_int32), ("species_idx2", ctypes.c_int32)]
 
 
+COND_CIRCULAR, COND_RECT = 0, 1
+
+
+class ConductorDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("x_center", ctypes.c_int32), ("y_center", ctypes.c_int32),
+                ("electrode_number", ctypes.c_int32), ("grid_radius", ctypes.c_double),
+                ("electrode_radius", ctypes.c_double), ("min_x", ctypes.c_int32), ("max_x", ctypes.c_int32),
+                ("min_y", ctypes.c_int32), ("max_y", ctypes.c_int32), ("voltage", ctypes.c_double)]
+
+
+def conductors(items):
+    """A ctypes array of ConductorDesc from dicts, e.g.
+    {"kind": COND_RECT, "min_x": 4, "max_x": 8, "min_y": 4, "max_y": 6, "voltage": 2.0}."""
+    arr = (ConductorDesc * len(items))()
+    for k, it in enumerate(items):
+        for f, v in it.items():
+            setattr(arr[k], f, v)
+    return arr
+
+
 class MccDesc(ctypes.Structure):
     _fields_ = [("electron_sp", ctypes.c_int32), ("ion_sp", ctypes.c_int32), ("gas_density", ctypes.c_double),
                 ("gas_temp_K", ctypes.c_double), ("gas_mass", ctypes.c_double), ("e_exc_th", ctypes.c_double),
@@ -134,6 +154,7 @@ def lib():
         "epic_rewind": [P, ctypes.c_int],
         "epic_inject": [P, ctypes.c_int],
         "epic_add_emitter": [P, ctypes.POINTER(EmitterDesc)],
+        "epic_set_conductors": [P, ctypes.POINTER(ConductorDesc), ctypes.c_int],
         "epic_mcc_setup": [P, ctypes.POINTER(MccDesc)],
         "epic_collide": [P, ctypes.c_int],
         "epic_mcc_get_stats": [P, ctypes.POINTER(MccStats)],
@@ -293,6 +314,10 @@ class Context:
 
     def compact(self, s):
         self._c(self.L.epic_compact(self.h, s), "compact")
+
+    def set_conductors(self, conds):
+        """conds: a ctypes array of ConductorDesc (see conductors())."""
+        self._c(self.L.epic_set_conductors(self.h, conds, len(conds)), "set_conductors")
 
     def add_emitter(self, e: EmitterDesc):
         self._c(self.L.epic_add_emitter(self.h, ctypes.byref(e)), "add_emitter")

@@ -102,7 +102,8 @@ static void free_all(Ctx* c) {
   void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->Ec, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
                 c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
-                c->mcc.event_count, c->mcc.cand, c->mcc.cand_n, c->sp_S, c->sp_T, c->sp_lam};
+                c->mcc.event_count, c->mcc.cand, c->mcc.cand_n, c->sp_S, c->sp_T, c->sp_lam,
+                c->cond.table, c->cond.node, c->cond.value};
   for (void* p : gs)
     if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -513,6 +514,108 @@ int epic_add_emitter(epic_ctx* h, const epic_emitter_desc* e) {
       e->numparticle < 0)
     return fail(c, EPIC_ERR_ARG, "bad emitter");
   c->emitters.push_back(Emitter{*e});
+  return EPIC_OK;
+}
+
+// ---- internal conductors ----------------------------------------------------
+// The absorbing shapes and the SOR's rho := voltage nodes of a conductor list,
+// computed on the host in the reference's order and arithmetic:
+// CircularGrid::initialize (grid.cpp:40-50) places electrode k at
+// (x_center + R cos t, y_center + R sin t), t = k (2 pi / n); each Electrode
+// (electrode.cpp:4-17) marks 100 points (int)(x0 + r cos u + 0.5) of its
+// circle, u = 2 m pi / 100; ReactConductorGrid::initialize (grid.cpp:68-85)
+// marks the boundary of its node rectangle.  Later writes to a node win.
+static int conductor_plan(const epic_domain_desc& d, const epic_conductor_desc* cd, int n,
+                          std::vector<CondDev>& rects, std::vector<CondDev>& elecs, std::vector<int>& node,
+                          std::vector<double>& value, std::string& why) {
+  const int nx = d.nx, ny = d.ny;
+  std::vector<double> val((size_t)nx * ny, 0.0);
+  std::vector<char> set((size_t)nx * ny, 0);
+  auto write = [&](int i, int j, double v) {
+    val[(size_t)i * ny + j] = v;
+    set[(size_t)i * ny + j] = 1;
+  };
+  for (int k = 0; k < n; k++) {
+    const epic_conductor_desc& q = cd[k];
+    if (q.kind == EPIC_COND_RECT) {
+      rects.push_back(CondDev{COND_RECT, (double)q.min_x, (double)q.max_x, (double)q.min_y, (double)q.max_y});
+      const int min_i = (int)((q.min_x - d.x0) / d.dx), max_i = (int)((q.max_x - d.x0) / d.dx);
+      const int min_j = (int)((q.min_y - d.y0) / d.dy), max_j = (int)((q.max_y - d.y0) / d.dy);
+      for (int i = min_i; i <= max_i; ++i)
+        for (int j = min_j; j <= max_j; ++j)
+          if (i == min_i || i == max_i || j == min_j || j == max_j) {
+            if (i < 0 || i >= nx || j < 0 || j >= ny) {
+              why = "conductor " + std::to_string(k) + ": rectangle boundary outside the node grid";
+              return EPIC_ERR_ARG;
+            }
+            write(i, j, q.voltage);
+          }
+    } else if (q.kind == EPIC_COND_CIRCULAR) {
+      if (q.electrode_number < 0) {
+        why = "conductor " + std::to_string(k) + ": negative electrode_number";
+        return EPIC_ERR_ARG;
+      }
+      for (int e = 0; e < q.electrode_number; ++e) {
+        const double theta = e * (2.0 * M_PI / q.electrode_number);
+        const double x0 = q.x_center + q.grid_radius * cos(theta);
+        const double y0 = q.y_center + q.grid_radius * sin(theta);
+        const double r = q.electrode_radius;
+        elecs.push_back(CondDev{COND_DISK, x0, y0, r * r, 0.0});
+        const int N = 100;
+        for (int m = 0; m < N; ++m) {
+          const double u = 2.0 * m * M_PI / N;
+          const int x = (int)(x0 + r * cos(u) + 0.5);
+          const int y = (int)(y0 + r * sin(u) + 0.5);
+          if (x >= 0 && x < nx && y >= 0 && y < ny) write(x, y, q.voltage);
+        }
+      }
+    } else {
+      why = "conductor " + std::to_string(k) + ": unknown kind";
+      return EPIC_ERR_ARG;
+    }
+  }
+  for (size_t t = 0; t < set.size(); t++)
+    if (set[t]) {
+      node.push_back((int)t);
+      value.push_back(val[t]);
+    }
+  return EPIC_OK;
+}
+
+int epic_set_conductors(epic_ctx* h, const epic_conductor_desc* cd, int n) {
+  if (!h || n < 0 || (n > 0 && !cd)) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!c->own_stream && n > 0) return fail(c, EPIC_ERR_ARG, "conductors are not supported on batch members");
+  std::vector<CondDev> rects, elecs;
+  std::vector<int> node;
+  std::vector<double> value;
+  std::string why;
+  const int rc = conductor_plan(c->dom, cd, n, rects, elecs, node, value, why);
+  if (rc != EPIC_OK) return fail(c, rc, why.c_str());
+  Conductors& q = c->cond;
+  CK(hipStreamSynchronize(c->stream), "sync");
+  if (q.table) hipFree(q.table);
+  if (q.node) hipFree(q.node);
+  if (q.value) hipFree(q.value);
+  q.table = nullptr;
+  q.node = nullptr;
+  q.value = nullptr;
+  q.desc.assign(cd, cd + n);
+  q.nrect = (int)rects.size();
+  q.nelec = (int)elecs.size();
+  q.nnode = (int)node.size();
+  std::vector<CondDev> table(rects);
+  table.insert(table.end(), elecs.begin(), elecs.end());
+  if (!table.empty()) {
+    CK(hipMalloc(&q.table, table.size() * sizeof(CondDev)), "conductors");
+    CK(hipMemcpy(q.table, table.data(), table.size() * sizeof(CondDev), hipMemcpyHostToDevice), "conductors");
+  }
+  if (q.nnode > 0) {
+    CK(hipMalloc(&q.node, node.size() * sizeof(int)), "conductors");
+    CK(hipMalloc(&q.value, value.size() * sizeof(double)), "conductors");
+    CK(hipMemcpy(q.node, node.data(), node.size() * sizeof(int), hipMemcpyHostToDevice), "conductors");
+    CK(hipMemcpy(q.value, value.data(), value.size() * sizeof(double), hipMemcpyHostToDevice), "conductors");
+  }
   return EPIC_OK;
 }
 

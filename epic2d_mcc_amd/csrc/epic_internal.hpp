@@ -77,6 +77,43 @@ struct MccState {
   int64_t cand_len_e = 0, cand_len_i = 0;
 };
 
+// One absorbing shape on the device: a disk (one electrode, a, b centre and
+// c = r*r) or a rectangle [a, b] x [c, d].  Rectangles come first in the
+// table, then the electrodes of the circular grids.
+enum { COND_DISK = 0, COND_RECT = 1 };
+struct CondDev {
+  int kind;
+  double a, b, c, d;
+};
+
+// Whether (x, y) lies in any of the first n shapes: Electrode::Iselectrode
+// (electrode.cpp:19-26), ReactConductorGrid::IsGrid (grid.cpp:60-63).
+__host__ __device__ inline bool in_conductor(const CondDev* t, int n, double x, double y) {
+  for (int k = 0; k < n; k++) {
+    const CondDev q = t[k];
+    if (q.kind == COND_DISK) {
+      const double dx = x - q.a, dy = y - q.b;
+      const double d2 = dx * dx + dy * dy;
+      if (d2 <= q.c) return true;
+    } else if (x >= q.a && x <= q.b && y >= q.c && y <= q.d) {
+      return true;
+    }
+  }
+  return false;
+}
+
+struct Conductors {
+  std::vector<epic_conductor_desc> desc;
+  CondDev* table = nullptr;  // device: rectangles, then electrodes
+  int nrect = 0, nelec = 0;
+  bool elec_live = false;    // the electrodes exist once an SOR solve has run
+  int* node = nullptr;       // device: nodes an SOR solve sets to a voltage
+  double* value = nullptr;
+  int nnode = 0;
+  bool any() const { return nrect + nelec > 0; }
+  int active() const { return nrect + (elec_live ? nelec : 0); }
+};
+
 struct Ctx {
   epic_domain_desc dom{};
   int nsp = 0;
@@ -133,6 +170,7 @@ struct Ctx {
 
   std::vector<Emitter> emitters;
   MccState mcc;
+  Conductors cond;
 
   int push_variant = 0;        // FP32 push kernel shape (see particles.hip)
   int push_grid_per_cu = 0;    // FP32 / FP64 push workgroups per CU (0: co-resident maximum)
@@ -184,6 +222,7 @@ hipError_t launch_inject(Ctx* c, int step);
 int64_t sort_key_space(int nx, int ny);  // size of the tile-major cell key space
 // field.hip
 hipError_t launch_solve(Ctx* c, int* iters, double* err);
+hipError_t launch_conductor_rho(Ctx* c);
 hipError_t launch_efield(Ctx* c);
 hipError_t launch_pack_E2(Ctx* c);
 // diag.hip

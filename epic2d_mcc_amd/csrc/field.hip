@@ -562,6 +562,26 @@ static hipError_t read_scalars(Ctx* c, int* iters, double* scal3) {
 hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32_inner, double tol, int max_ref,
                                  int* solves, double* relres);
 
+// rho := voltage on the conductors' nodes (the grids' initialize() at the
+// start of gssolver, field.cpp:100-103); the host resolved repeated nodes to
+// the last write, so the entries are distinct.
+__global__ void conductor_rho_kernel(double* rho, const int* node, const double* value, int n) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) rho[node[k]] = value[k];
+}
+
+hipError_t launch_conductor_rho(Ctx* c) {
+  Conductors& q = c->cond;
+  if (q.nnode > 0) {
+    hipLaunchKernelGGL(conductor_rho_kernel, dim3(std::min((q.nnode + 255) / 256, 1024)), dim3(256), 0, c->stream,
+                       c->rho, q.node, q.value, q.nnode);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  // CircularGrid::initialize (grid.cpp:40-50) creates the electrodes that
+  // IsGrid tests from now on
+  q.elec_live = true;
+  return hipSuccess;
+}
+
 hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
@@ -570,6 +590,7 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   const int grid = std::min((n + 255) / 256, 4096);
   const long long cap = std::min<long long>((long long)n * n, (long long)d.max_iteration);
   if (d.solver == EPIC_SOLVER_GS) {
+    if (c->cond.any()) EPIC_HIP_TRY(launch_conductor_rho(c));
     SorArgs a{};
     a.nx = d.nx; a.ny = d.ny; a.rho = c->rho; a.phi = c->phi;
     a.dx2 = d.dx * d.dx; a.dy2 = d.dy * d.dy;

@@ -39,6 +39,8 @@ struct PartArgs {
   unsigned long long* errc;
   int rr;              // chunk walk: 0 per-XCD contiguous ranges, 1 round-robin
   int fastdiv;         // FP64: dx, dy in [2^-400, 2^400], so x_to_l may divide by reciprocal + correction
+  const CondDev* cond; // absorbing conductors (the first ncond entries are active)
+  int ncond;
 };
 
 template <typename P>
@@ -372,6 +374,12 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
           else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); vx[k] = -vx[k]; }
           if (py < a.y0) { py = a.y0 + (a.y0 - py); vy[k] = -vy[k]; }
           else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); vy[k] = -vy[k]; }
+        }
+        // a conductor absorbs what the walls left (species.cpp:84-98,
+        // 111-124, 152-165)
+        if (a.ncond > 0 && live[k] && in_conductor(a.cond, a.ncond, (double)px, (double)py)) {
+          px = p_nan<P>();
+          live[k] = false;
         }
         x[k] = px;
         y[k] = py;
@@ -726,6 +734,8 @@ static PartArgs<P> make_args(Ctx* c, int s) {
   a.errc = c->errc;
   a.rr = c->part_map;
   a.fastdiv = sizeof(P) == 8 && d.dx >= 0x1p-400 && d.dx <= 0x1p400 && d.dy >= 0x1p-400 && d.dy <= 0x1p400;
+  a.cond = c->cond.table;
+  a.ncond = c->cond.active();
   return a;
 }
 
@@ -843,7 +853,10 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   push_consts(c, s, sub_cycle, false, &c1, &c2, &c3);
   const epic_domain_desc& d = c->dom;
   const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
-  if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
+  // the dedicated push kernels have no conductor test: with conductors the
+  // generic kernel pushes
+  const bool plain = !c->cond.any();
+  if (plain && d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
     const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2);
     // kernel shape: particles per thread x occupancy target (waves per SIMD);
     // EPIC_PUSH_VARIANT at context creation
@@ -868,7 +881,7 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
     }
     return hipGetLastError();
   }
-  if (d.precision == EPIC_FP64 && d.shape == EPIC_SHAPE_CIC && !whole && c->push64) {
+  if (plain && d.precision == EPIC_FP64 && d.shape == EPIC_SHAPE_CIC && !whole && c->push64) {
     PartArgs<double> a = make_args<double>(c, s);
     a.c1 = c1; a.c2 = c2; a.c3 = c3;
 #define EPIC_L(PPT)                                                                              \
@@ -899,7 +912,7 @@ bool push_all_ok(const Ctx* c) {
   const epic_domain_desc& d = c->dom;
   const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
   if (!c->push_multi || d.precision != EPIC_FP32 || d.shape != EPIC_SHAPE_CIC || whole || d.nx > 65535 ||
-      d.ny > 65535 || c->push_variant != 0)
+      d.ny > 65535 || c->push_variant != 0 || c->cond.any())
     return false;
   int np = 0;
   for (int s = 0; s < c->nsp; s++) np += c->sp[s].desc.pushed ? 1 : 0;

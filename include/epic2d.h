@@ -113,6 +113,24 @@ typedef struct epic_emitter_desc {
   int32_t species_idx1, species_idx2;
 } epic_emitter_desc;
 
+/* Internal conductors ("grids", grid.h:13-66; electrode.h:9-20), as
+ * main.cpp:287-323 builds them from an ini's [Grids] section.  Coordinates are
+ * in the domain's position units (the reference decks use node units, dx = 1).
+ *   EPIC_COND_CIRCULAR  CircularGrid: electrode_number disks of radius
+ *                       electrode_radius centred on a circle of radius
+ *                       grid_radius about (x_center, y_center).
+ *   EPIC_COND_RECT      ReactConductorGrid: [min_x, max_x] x [min_y, max_y].
+ * voltage is the normalised value main.cpp:310,321 computes. */
+#define EPIC_COND_CIRCULAR 0
+#define EPIC_COND_RECT 1
+typedef struct epic_conductor_desc {
+  int32_t kind;
+  int32_t x_center, y_center, electrode_number; /* CIRCULAR (main.cpp:305-307) */
+  double grid_radius, electrode_radius;         /* CIRCULAR (main.cpp:308-309) */
+  int32_t min_x, max_x, min_y, max_y;           /* RECT (main.cpp:317-320) */
+  double voltage;
+} epic_conductor_desc;
+
 /* Collision processes for epic_mcc_setup. */
 #define EPIC_MCC_E_ELASTIC 0
 #define EPIC_MCC_E_EXCITATION 1
@@ -185,6 +203,21 @@ int epic_add_emitter(epic_ctx* ctx, const epic_emitter_desc* e);
 int epic_mcc_setup(epic_ctx* ctx, const epic_mcc_desc* d); /* CollisionHandler(domain) + tables (collision.h:29-31) */
 int epic_collide(epic_ctx* ctx, int step);             /* CollisionHandler::handle_collisions (collision.h:33) */
 int epic_mcc_get_stats(epic_ctx* ctx, epic_mcc_stats* out);
+/* The conductors of FieldSolve(domain, grids) / Push_species_serial(.., grids)
+ * (field.h:19, species.h:70); replaces any earlier list, n = 0 clears it.
+ *  - A push removes a particle whose new position (after the walls) lies in a
+ *    conductor (species.cpp:84-98, 111-124, 152-165).  A rectangle absorbs
+ *    from the start; a circular grid's electrodes exist only once an SOR solve
+ *    has run (CircularGrid::initialize, grid.cpp:40-50, is called by
+ *    gssolver, field.cpp:100-103), so with the PCG solvers it never absorbs.
+ *  - Each SOR solve (EPIC_SOLVER_GS) first sets rho := voltage on the
+ *    conductors' nodes, in list order: 100 points of each electrode's circle
+ *    (electrode.cpp:7-16) and a rectangle's boundary nodes (grid.cpp:68-85).
+ *  - EPIC_ERR_ARG if a rectangle's boundary leaves the node grid (the
+ *    reference's bounds-checked rho(i, j) throws), or for a batch member.
+ * The device loader (epic_load_species) does not avoid conductors; the
+ * reference's loader does (init.cpp:71-96), as does the host Init. */
+int epic_set_conductors(epic_ctx* ctx, const epic_conductor_desc* c, int n);
 int epic_sort(epic_ctx* ctx, int sp);                  /* no reference counterpart: stable counting sort by cell */
 int epic_compact(epic_ctx* ctx, int sp);               /* order-preserving removal, as vector::erase (species.cpp:95-121) */
 

@@ -251,6 +251,19 @@ struct Sim {
   std::uniform_real_distribution<double> dist{0.0, 1.0};
   double rnd() { return dist(gen); }
 
+  // internal conductors (grid.h:13-66): kind 0 CircularGrid, 1
+  // ReactConductorGrid; a circular grid's electrodes (x0, y0, r) exist only
+  // after its initialize() (an SOR solve)
+  struct Grid {
+    int kind;
+    int xc, yc, ne;
+    double grad, erad;
+    int min_x, max_x, min_y, max_y;
+    double voltage;
+    std::vector<double> ex, ey, er;
+  };
+  std::vector<Grid> grids;
+
   // collision tables (collision.cpp:12-42)
   std::vector<double> sigma_tot_e, sigma_ela, sigma_exc, sigma_ionz;
   double delta_g = 0;
@@ -309,6 +322,8 @@ void orc_destroy(void* h) { delete (Sim*)h; }
 
 void orc_reseed(void* h, unsigned seed) { ((Sim*)h)->gen.seed(seed); }
 
+static bool on_any_grid(const Sim* m, double x, double y);
+
 // Init::SampleVel, init.cpp:104-108 (draws sequenced left to right).
 static double sample_vel(Sim* m, double mass, double temp) {
   double v_th = sqrt(2 * K_b * temp * EV_to_K / mass);
@@ -318,10 +333,12 @@ static double sample_vel(Sim* m, double mass, double temp) {
   return v_th * sqrt(2) * (r1 + r2 + r3 - 1.5);
 }
 
-// Init::Init, init.cpp:3-102, without conductors (grid_count = 0).  The
-// gaussian load draws from its own generator seeded with `gauss_seed`
-// (the reference seeds it from std::random_device, init.cpp:11-12).
-void orc_init_species(void* h, int s, unsigned gauss_seed) {
+// Init::Init, init.cpp:3-102.  The gaussian load draws from its own
+// generator seeded with `gauss_seed` (the reference seeds it from
+// std::random_device, init.cpp:11-12).  A position on a conductor is drawn
+// again (init.cpp:71-96); a deterministic load that lands on one would loop
+// for ever there, and fails here (-1).
+int orc_init_species(void* h, int s, unsigned gauss_seed) {
   Sim* m = (Sim*)h;
   Sim::Sp& sp = m->sp[s];
   std::mt19937 gen(gauss_seed);
@@ -331,6 +348,7 @@ void orc_init_species(void* h, int s, unsigned gauss_seed) {
   int n = sp.load_n;
   for (int p = 0; p < sp.numparticle; p++) {
     double x = 0, y = 0;
+  draw:
     if (sp.load_type == LOAD_RANDOM) {
       x = m->x0 + m->Lx * m->rnd();
       y = m->y0 + m->Ly * m->rnd();
@@ -350,6 +368,10 @@ void orc_init_species(void* h, int s, unsigned gauss_seed) {
       x = gx(gen);
       y = gy(gen);
     }
+    if (on_any_grid(m, x, y)) {
+      if (sp.load_type != LOAD_RANDOM && sp.load_type != LOAD_GAUSSIAN) return -1;
+      goto draw;
+    }
     double vx = sample_vel(m, sp.mass, sp.temp) + sp.vsx * m->vel_norm;
     double vy = sample_vel(m, sp.mass, sp.temp) + sp.vsy * m->vel_norm;
     double vz = 0;
@@ -358,6 +380,7 @@ void orc_init_species(void* h, int s, unsigned gauss_seed) {
     q.pos[0] = x; q.pos[1] = y; q.vel[0] = vx; q.vel[1] = vy; q.vel[2] = vz;
     sp.part_list.push_back(q);
   }
+  return 0;
 }
 
 void orc_set_particles(void* h, int s, const double* x, const double* y, const double* vx,
@@ -404,6 +427,62 @@ void orc_set_grid(void* h, int which, const double* in) {
 // Domain::XtoL / YtoL, domain.cpp:257-264.
 static inline double XtoL(const Sim* m, double p) { return (p - m->x0) / m->dx; }
 static inline double YtoL(const Sim* m, double p) { return (p - m->y0) / m->dy; }
+
+// Grid::IsGrid: CustomGrid/CircularGrid test their electrodes
+// (grid.cpp:12-35, Electrode::Iselectrode electrode.cpp:19-26);
+// ReactConductorGrid::IsGrid (grid.cpp:60-63).
+static bool is_grid(const Sim::Grid& g, double x, double y) {
+  if (g.kind == 1) return (x >= g.min_x && x <= g.max_x && y >= g.min_y && y <= g.max_y);
+  for (size_t e = 0; e < g.ex.size(); e++) {
+    double dx = x - g.ex[e];
+    double dy = y - g.ey[e];
+    double distance_squared = dx * dx + dy * dy;
+    double radius_squared = g.er[e] * g.er[e];
+    if (distance_squared <= radius_squared) return true;
+  }
+  return false;
+}
+
+static bool on_any_grid(const Sim* m, double x, double y) {
+  for (const Sim::Grid& g : m->grids)
+    if (is_grid(g, x, y)) return true;
+  return false;
+}
+
+// Grid::initialize: CircularGrid (grid.cpp:40-50) rebuilds its electrodes,
+// each of which writes rho = voltage on 100 points of its circle (Electrode
+// constructor, electrode.cpp:4-17); ReactConductorGrid (grid.cpp:68-85)
+// writes the boundary of its node rectangle.  Returns -1 where the bounds-
+// checked rho(i, j) would throw.
+static int grid_initialize(Sim* m, Sim::Grid& g) {
+  if (g.kind == 1) {
+    int min_i = (int)XtoL(m, g.min_x), max_i = (int)XtoL(m, g.max_x);
+    int min_j = (int)YtoL(m, g.min_y), max_j = (int)YtoL(m, g.max_y);
+    for (int i = min_i; i <= max_i; ++i)
+      for (int j = min_j; j <= max_j; ++j)
+        if (i == min_i || i == max_i || j == min_j || j == max_j) {
+          if (i < 0 || i >= m->nx || j < 0 || j >= m->ny) return -1;
+          m->rho[m->idx(i, j)] = g.voltage;
+        }
+    return 0;
+  }
+  g.ex.clear(); g.ey.clear(); g.er.clear();
+  for (int k = 0; k < g.ne; ++k) {
+    double theta = k * (2.0 * M_PI / g.ne);
+    double x0 = g.xc + g.grad * cos(theta);
+    double y0 = g.yc + g.grad * sin(theta);
+    double r = g.erad;
+    const int N = 100;
+    for (int n = 0; n < N; ++n) {
+      double t = 2.0 * n * M_PI / N;
+      int x = static_cast<int>(x0 + r * cos(t) + 0.5);
+      int y = static_cast<int>(y0 + r * sin(t) + 0.5);
+      if (x >= 0 && x < m->nx && y >= 0 && y < m->ny) m->rho[m->idx(x, y)] = g.voltage;
+    }
+    g.ex.push_back(x0); g.ey.push_back(y0); g.er.push_back(r);
+  }
+  return 0;
+}
 
 // Domain::Scatter, domain.cpp:205-228.  Returns -1 where the reference's
 // bounds-checked Matrix::operator() would throw (linalg.h:473-480).
@@ -557,11 +636,14 @@ static void pcgsolver(Sim* m) {
   m->phi = x;  // unflatten(x, nx, ny) (function.h:171-189)
 }
 
-// FieldSolve::gssolver, field.cpp:70-139 (no conductors).
-static void gssolver(Sim* m) {
+// FieldSolve::gssolver, field.cpp:70-139; the conductors' initialize()
+// writes rho first (field.cpp:100-103).
+static int gssolver(Sim* m) {
   double dx2 = m->dx * m->dx, dy2 = m->dy * m->dy;
   int nx = m->nx, ny = m->ny;
   std::fill(m->phi.begin(), m->phi.end(), 0.0);
+  for (Sim::Grid& g : m->grids)
+    if (grid_initialize(m, g)) return -1;
   double L2 = 0;
   double coeff = (2 / dx2) + (2 / dy2);
   auto phi = [&](int i, int j) -> double& { return m->phi[m->idx(i, j)]; };
@@ -592,13 +674,17 @@ static void gssolver(Sim* m) {
   }
   (void)converged;
   m->last_iters = it_done;
+  return 0;
 }
 
 // FieldSolve::PotentialSolver, field.cpp:266-272.
 int orc_solve(void* h, double* err) {
   Sim* m = (Sim*)h;
-  if (m->solver == SOLVER_GS) gssolver(m);
-  else pcgsolver(m);
+  if (m->solver == SOLVER_GS) {
+    if (gssolver(m)) return -1;
+  } else {
+    pcgsolver(m);
+  }
   if (err) *err = m->error;
   return m->last_iters;
 }
@@ -636,7 +722,8 @@ void orc_efield(void* h) {
     }
 }
 
-// Species::Push_species_serial, species.cpp:29-170 (no conductors).
+// Species::Push_species_serial, species.cpp:29-170; after the walls a
+// particle on a conductor is erased (:84-98, :111-124, :152-165).
 // Returns the number of particles whose gather would have thrown.
 int orc_push(void* h, int s, int sub_cycle) {
   Sim* m = (Sim*)h;
@@ -677,6 +764,7 @@ int orc_push(void* h, int s, int sub_cycle) {
         part.pos[1] = m->x0 + m->Ly - (part.pos[1] - (m->x0 + m->Ly)); part.vel[1] = -part.vel[1];
       }
     }
+    if (on_any_grid(m, part.pos[0], part.pos[1])) continue;
     kept.push_back(part);
   }
   pl.swap(kept);
@@ -886,6 +974,32 @@ void orc_collide_direct(void* h, int se, int st, long* counts, double* eloss_ev)
   }
   if (eloss_ev) *eloss_ev = lost;
 }
+
+// The grids main.cpp:287-323 builds (same fields as the device library's
+// conductor descriptor).
+struct orc_conductor {
+  int kind;
+  int x_center, y_center, electrode_number;
+  double grid_radius, electrode_radius;
+  int min_x, max_x, min_y, max_y;
+  double voltage;
+};
+
+void orc_set_conductors(void* h, const orc_conductor* c, int n) {
+  Sim* m = (Sim*)h;
+  m->grids.clear();
+  for (int k = 0; k < n; k++) {
+    Sim::Grid g;
+    g.kind = c[k].kind;
+    g.xc = c[k].x_center; g.yc = c[k].y_center; g.ne = c[k].electrode_number;
+    g.grad = c[k].grid_radius; g.erad = c[k].electrode_radius;
+    g.min_x = c[k].min_x; g.max_x = c[k].max_x; g.min_y = c[k].min_y; g.max_y = c[k].max_y;
+    g.voltage = c[k].voltage;
+    m->grids.push_back(g);
+  }
+}
+
+int orc_is_conductor(void* h, double x, double y) { return on_any_grid((Sim*)h, x, y) ? 1 : 0; }
 
 int orc_last_iters(void* h) { return ((Sim*)h)->last_iters; }
 

@@ -89,6 +89,10 @@ def oracle():
         o.orc_mcc_build_tables.argtypes = [ctypes.c_void_p]
         o.orc_collide_direct.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_long), D]
+        o.orc_set_conductors.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        o.orc_is_conductor.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        o.orc_init_species.restype = ctypes.c_int
+        o.orc_solve.restype = ctypes.c_int
         o.orc_last_iters.argtypes = [ctypes.c_void_p]
         o.orc_set_tolerance.argtypes = [ctypes.c_void_p, ctypes.c_double]
     return _o
@@ -131,7 +135,15 @@ class OracleSim:
         self.o.orc_set_tolerance(self.h, tol)
 
     def init_species(self, s, gauss_seed=0):
-        self.o.orc_init_species(self.h, s, gauss_seed)
+        return self.o.orc_init_species(self.h, s, gauss_seed)
+
+    def set_conductors(self, conds):
+        """conds: a ctypes array of the library's epic_conductor_desc (the
+        oracle's orc_conductor has the same layout)."""
+        self.o.orc_set_conductors(self.h, ctypes.cast(conds, ctypes.c_void_p) if len(conds) else None, len(conds))
+
+    def is_conductor(self, x, y):
+        return bool(self.o.orc_is_conductor(self.h, x, y))
 
     def set_particles(self, s, x, y, vx, vy, vz):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, vx, vy, vz)]

@@ -29,6 +29,7 @@ def test_host_layer_built():
 
 @pytest.mark.parametrize("name,cls", [("epic_domain_desc", _lib.DomainDesc), ("epic_species_desc", _lib.SpeciesDesc),
                                       ("epic_emitter_desc", _lib.EmitterDesc), ("epic_mcc_desc", _lib.MccDesc),
+                                      ("epic_conductor_desc", _lib.ConductorDesc),
                                       ("epic_mcc_stats", _lib.MccStats), ("epic_batch_opts", _lib.BatchOpts),
                                       ("epic_batch_tuning", _lib.BatchTuning),
                                       ("epic_batch_stats", _lib.BatchStats),
