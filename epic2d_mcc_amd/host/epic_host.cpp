@@ -137,6 +137,15 @@ void Domain::bind_species(std::vector<Species>& list) {
   for (auto& s : list) reg_.push_back(&s);
 }
 
+void Domain::set_grids(const std::vector<Grid*>& grids) {
+  if (grids_set_ && grids == grids_) return;
+  std::vector<epic_conductor_desc> d;
+  for (Grid* g : grids) d.push_back(g->desc());
+  check(epic_set_conductors(ctx(), d.data(), (int)d.size()), "epic_set_conductors");
+  grids_ = grids;
+  grids_set_ = true;
+}
+
 void Domain::check(int rc, const char* what) {
   if (rc != EPIC_OK) {
     std::string m = std::string(what) + ": " + (ctx_ ? epic_last_error(ctx_) : "no context");
@@ -218,7 +227,7 @@ Species::Species(std::string name, double mass, double charge, double spwt, doub
 }
 
 void Species::Push_species_serial(int sub_cycle, const std::vector<Grid*>& grids) {
-  if (!grids.empty()) throw EpicError(EPIC_ERR_ARG, "internal conductors are not part of this build");
+  domain_->set_grids(grids);
   domain_->check(epic_push(domain_->ctx(), index, sub_cycle), "epic_push");
 }
 void Species::ScatterSpecies_serial() { domain_->check(epic_scatter(domain_->ctx(), index), "epic_scatter"); }
@@ -251,7 +260,6 @@ double Init::SampleVel(Species& species, double temp) {
 }
 
 Init::Init(Species& species, Domain& domain, std::vector<Grid*>& grids) {
-  if (!grids.empty()) throw EpicError(EPIC_ERR_ARG, "internal conductors are not part of this build");
   auto lt = INIParser::loadtypeextract(species.initialization);
   const std::string& type = lt.first;
   const int n = lt.second;
@@ -261,8 +269,15 @@ Init::Init(Species& species, Domain& domain, std::vector<Grid*>& grids) {
   std::mt19937 gen(rd());
   std::normal_distribution<> gx((domain.x0 + domain.Lx) / 2.0, domain.Lx / 5);
   std::normal_distribution<> gy((domain.y0 + domain.Ly) / 2.0, domain.Ly / 5);
+  // a position on a conductor is drawn again (init.cpp:71-96)
+  auto on_grid = [&](double qx, double qy) {
+    for (Grid* g : grids)
+      if (g->IsGrid(qx, qy)) return true;
+    return false;
+  };
   for (int p = 0; p < N; p++) {
     double px = 0, py = 0;
+  draw:
     if (type == "random") {
       px = domain.x0 + domain.Lx * rnd();
       py = domain.y0 + domain.Ly * rnd();
@@ -282,6 +297,12 @@ Init::Init(Species& species, Domain& domain, std::vector<Grid*>& grids) {
     } else {
       throw EpicError(EPIC_ERR_ARG, "unknown load type " + type);
     }
+    if (on_grid(px, py)) {
+      // the reference redraws a fixed position for ever
+      if (type != "random" && type != "gaussian")
+        throw EpicError(EPIC_ERR_ARG, "load type " + type + " places a particle on a conductor");
+      goto draw;
+    }
     double pvx = SampleVel(species) + species.vsx * domain.vel_norm;
     double pvy = SampleVel(species) + species.vsy * domain.vel_norm;
     x[p] = px;
@@ -294,10 +315,86 @@ Init::Init(Species& species, Domain& domain, std::vector<Grid*>& grids) {
                "epic_upload_particles");
 }
 
+// ---- internal conductors ---------------------------------------------------------
+CircularGrid::CircularGrid(Domain& domain, int center_x, int center_y, int num_electrodes, double voltage,
+                           double electrode_rad, double grid_rad)
+    : Grid(domain), center_x(center_x), center_y(center_y), num_electrodes(num_electrodes), voltage(voltage),
+      electrode_rad(electrode_rad), grid_rad(grid_rad) {}
+
+// Electrode::Iselectrode (electrode.cpp:19-26) over the electrodes that exist
+bool CircularGrid::IsGrid(double x, double y) {
+  for (size_t k = 0; k < ex_.size(); k++) {
+    const double dx = x - ex_[k], dy = y - ey_[k];
+    if (dx * dx + dy * dy <= electrode_rad * electrode_rad) return true;
+  }
+  return false;
+}
+
+// grid.cpp:40-50 (the device writes the electrodes' voltages)
+void CircularGrid::initialize() {
+  ex_.clear();
+  ey_.clear();
+  for (int k = 0; k < num_electrodes; ++k) {
+    const double theta = k * (2.0 * M_PI / num_electrodes);
+    ex_.push_back(center_x + grid_rad * std::cos(theta));
+    ey_.push_back(center_y + grid_rad * std::sin(theta));
+  }
+}
+
+epic_conductor_desc CircularGrid::desc() const {
+  epic_conductor_desc d{};
+  d.kind = EPIC_COND_CIRCULAR;
+  d.x_center = center_x;
+  d.y_center = center_y;
+  d.electrode_number = num_electrodes;
+  d.grid_radius = grid_rad;
+  d.electrode_radius = electrode_rad;
+  d.voltage = voltage;
+  return d;
+}
+
+ReactConductorGrid::ReactConductorGrid(int min_x, int max_x, int min_y, int max_y, double voltage, Domain& domain)
+    : Grid(domain), min_x(min_x), max_x(max_x), min_y(min_y), max_y(max_y), voltage(voltage) {}
+
+bool ReactConductorGrid::IsGrid(double x, double y) {
+  return (x >= min_x && x <= max_x && y >= min_y && y <= max_y);
+}
+
+epic_conductor_desc ReactConductorGrid::desc() const {
+  epic_conductor_desc d{};
+  d.kind = EPIC_COND_RECT;
+  d.min_x = min_x;
+  d.max_x = max_x;
+  d.min_y = min_y;
+  d.max_y = max_y;
+  d.voltage = voltage;
+  return d;
+}
+
+Grid* Create_Grid(GridType type, Domain& domain, const GridParams& params, const RectangularGridParams& rparam) {
+  switch (type) {
+    case GridType::Circular:
+      return new CircularGrid(domain, params.grid_centerx, params.grid_centery, params.num_electrodes,
+                              params.voltage, params.electrode_rad, params.grid_rad);
+    case GridType::Line:
+    case GridType::ReactConductor:
+      return new ReactConductorGrid(rparam.min_x, rparam.max_x, rparam.min_y, rparam.max_y, rparam.voltage, domain);
+    default:
+      throw EpicError(EPIC_ERR_ARG, "custom grids (electrodes added one by one) are not supported");
+  }
+}
+
 // ---- FieldSolve -----------------------------------------------------------------
+FieldSolve::FieldSolve(Domain& domain, const std::vector<Grid*>& grids) : domain(domain), grids(grids) {
+  domain.set_grids(grids);
+}
+
 void FieldSolve::PotentialSolver() {
   int it = 0;
   double err = 0;
+  // gssolver calls each grid's initialize() (field.cpp:100-103)
+  if (domain.SolverType == "gs")
+    for (Grid* g : grids) g->initialize();
   domain.check(epic_solve_potential(domain.ctx(), &it, &err), "epic_solve_potential");
   domain.error = err;
   domain.last_iterations = it;

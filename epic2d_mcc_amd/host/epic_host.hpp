@@ -67,7 +67,7 @@ class INIParser {
 bool string_to_bool(const std::string& s);  // include/extrafun.h:6-17
 
 class Species;
-class Grid;  // conductors are not part of this build; the lists stay empty
+class Grid;
 
 class Domain {
  public:
@@ -123,9 +123,14 @@ class Domain {
   void check(int rc, const char* what);
   int register_species(Species* s);
   void bind_species(std::vector<Species>& list);
+  // the conductors the device pushes and solves with (epic_set_conductors);
+  // a repeated call with the same list does nothing
+  void set_grids(const std::vector<Grid*>& grids);
 
  private:
   epic_ctx* ctx_ = nullptr;
+  std::vector<Grid*> grids_;
+  bool grids_set_ = false;
   std::vector<Species*> reg_;
   std::vector<epic_emitter_desc> pending_emitters_;
   int n_emitters_ = 0;
@@ -158,6 +163,73 @@ class Species {
   Domain* domain_;
 };
 
+// ---- internal conductors (grid.h:13-88, electrode.h:9-20) ----------------------
+// The host keeps each grid's geometry, tests it for the loader (IsGrid) and
+// hands it to the device (desc()), which absorbs particles in the push and
+// writes the voltages before each SOR solve.
+enum class GridType { Custom, Circular, Line, ReactConductor };
+
+struct GridParams {  // grid.h:69-77
+  int grid_centerx;
+  int grid_centery;
+  int num_electrodes;
+  double grid_rad;
+  double electrode_rad;
+  double voltage;
+};
+
+struct RectangularGridParams {  // grid.h:79-86
+  int min_x;
+  int max_x;
+  int min_y;
+  int max_y;
+  double voltage;
+};
+
+class Grid {
+ public:
+  explicit Grid(Domain& domain) : domain(domain) {}
+  virtual ~Grid() = default;
+  virtual bool IsGrid(double x, double y) = 0;
+  virtual void initialize() = 0;
+  virtual epic_conductor_desc desc() const = 0;
+
+ protected:
+  Domain& domain;
+};
+
+// A ring of electrode disks; they exist once initialize() has run (an SOR
+// solve), as in grid.cpp:24-50.
+class CircularGrid : public Grid {
+ public:
+  CircularGrid(Domain& domain, int center_x, int center_y, int num_electrodes, double voltage,
+               double electrode_rad, double grid_rad);
+  bool IsGrid(double x, double y) override;
+  void initialize() override;
+  epic_conductor_desc desc() const override;
+
+ private:
+  int center_x, center_y, num_electrodes;
+  double voltage, electrode_rad, grid_rad;
+  std::vector<double> ex_, ey_;  // electrode centres after initialize()
+};
+
+class ReactConductorGrid : public Grid {  // grid.cpp:53-85
+ public:
+  ReactConductorGrid(int min_x, int max_x, int min_y, int max_y, double voltage, Domain& domain);
+  bool IsGrid(double x, double y) override;
+  void initialize() override {}  // the device writes the boundary voltages
+  epic_conductor_desc desc() const override;
+
+ private:
+  int min_x, max_x, min_y, max_y;
+  double voltage;
+};
+
+// grid.cpp:88-103; GridType::Custom (electrodes added one by one) is not
+// supported and throws.
+Grid* Create_Grid(GridType type, Domain& domain, const GridParams& params, const RectangularGridParams& rparam);
+
 // Init::Init (init.cpp:3-102): the reference's loader, drawing from the
 // global mt19937 stream in the reference's order, then uploading.
 class Init {
@@ -169,12 +241,13 @@ class Init {
 
 class FieldSolve {
  public:
-  FieldSolve(Domain& domain, const std::vector<Grid*>& grids) : domain(domain) { (void)grids; }
+  FieldSolve(Domain& domain, const std::vector<Grid*>& grids);
   void PotentialSolver();  // field.cpp:266-272
   void CalculateEfield();  // field.cpp:276-331
 
  private:
   Domain& domain;
+  std::vector<Grid*> grids;
 };
 
 class CollisionHandler {

@@ -155,9 +155,32 @@ int main(int argc, char* argv[]) {
     }
     domain.bind_species(species_list);
 
+    // main.cpp:287-323: conductors, voltages normalised by kT/e at 1 eV
     std::vector<Grid*> grids;
-    if (INIParser::getInt(ini.at("Grids"), "grid_count") != 0)
-      throw std::runtime_error("internal conductors (grid_count > 0) are not part of this build");
+    const auto& gs = ini.at("Grids");
+    const int grid_count = INIParser::getInt(gs, "grid_count");
+    for (int i = 0; i < grid_count; i++) {
+      const std::string p = "grid_" + std::to_string(i) + ".";
+      const std::string type = INIParser::getString(gs, p + "type");
+      GridParams gp{};
+      RectangularGridParams rp{};
+      if (type == "circular") {
+        gp.grid_centerx = INIParser::getInt(gs, p + "x_center");
+        gp.grid_centery = INIParser::getInt(gs, p + "y_center");
+        gp.num_electrodes = INIParser::getInt(gs, p + "electrode_number");
+        gp.grid_rad = INIParser::getDouble(gs, p + "grid_radius");
+        gp.electrode_rad = INIParser::getDouble(gs, p + "electrod_radius");
+        gp.voltage = (Const::eV / (Const::K_b * Const::EV_to_K)) * INIParser::getDouble(gs, p + "electrode_voltage");
+        grids.push_back(Create_Grid(GridType::Circular, domain, gp, rp));
+      } else if (type == "reactconduct") {
+        rp.min_x = INIParser::getInt(gs, p + "min_x");
+        rp.min_y = INIParser::getInt(gs, p + "min_y");
+        rp.max_x = INIParser::getInt(gs, p + "max_x");
+        rp.max_y = INIParser::getInt(gs, p + "max_y");
+        rp.voltage = (Const::eV / (Const::K_b * Const::EV_to_K)) * INIParser::getDouble(gs, p + "grid_voltage");
+        grids.push_back(Create_Grid(GridType::ReactConductor, domain, gp, rp));
+      }
+    }
     std::vector<Emitter> emitters;
     const auto& es = ini.at("Emitters");
     const int emitter_count = INIParser::getInt(es, "count");
@@ -241,6 +264,7 @@ int main(int argc, char* argv[]) {
     const auto end_time = std::chrono::high_resolution_clock::now();
     const double el = std::chrono::duration<double>(end_time - start_time).count();
     std::cout << "Elapsed time: " << el << " seconds" << std::endl;
+    for (Grid* g : grids) delete g;  // main.cpp:460-463
   } catch (const std::exception& e) {
     std::cerr << "epic2d_run: " << e.what() << std::endl;
     return EXIT_FAILURE;

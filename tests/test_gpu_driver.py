@@ -97,3 +97,52 @@ def test_driver_rejects_missing_key(tmp_path):
     ini.write_text(INI.format(out=str(tmp_path / "d")).replace("max_iteration = 10000\n", ""))
     r = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "epic2d_run" in r.stderr
+
+
+GRIDS = """[Grids]
+grid_count = 2
+grid_0.type = reactconduct
+grid_0.min_x = 3
+grid_0.max_x = 6
+grid_0.min_y = 9
+grid_0.max_y = 13
+grid_0.grid_voltage = 2
+grid_1.type = circular
+grid_1.x_center = 10
+grid_1.y_center = 6
+grid_1.electrode_number = 4
+grid_1.grid_radius = 3
+grid_1.electrod_radius = 1.5
+grid_1.electrode_voltage = -1
+"""
+
+
+def test_driver_with_conductors_matches_oracle(tmp_path):
+    """[Grids] as main.cpp:287-323 reads it: the loader avoids the rectangle,
+    the push absorbs on it (the circular grid has no electrodes under cg),
+    and the energy history follows the oracle's replay."""
+    from epic2d_mcc_amd import _lib
+    out = tmp_path / "data"
+    ini = tmp_path / "input.ini"
+    text = INI.format(out=str(out)).replace("[Grids]\ngrid_count = 0\n", GRIDS).replace("bc = pbc", "bc = open")
+    ini.write_text(text)
+    r = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    hist = np.loadtxt(out / "kinetic_energy.csv", delimiter=",")
+    st = small_setup(N=2000, Nx=16, Ny=16, bc="open", tol=1e-10, ion_mass=1.6724823e-27)
+    o = OracleSim(st.domain, st.species)
+    o.set_conductors(_lib.conductors([
+        {"kind": _lib.COND_RECT, "min_x": 3, "max_x": 6, "min_y": 9, "max_y": 13, "voltage": 2.0},
+        {"kind": _lib.COND_CIRCULAR, "x_center": 10, "y_center": 6, "electrode_number": 4, "grid_radius": 3.0,
+         "electrode_radius": 1.5, "voltage": -1.0}]))
+    for s in range(2):
+        assert o.init_species(s, 0) == 0
+    o.initial_field()
+    for ts in range(11):
+        o.step()
+        row = [ts * st.domain.DT]
+        for s in range(2):
+            row += list(o.ke(s, 0))
+        row.append(o.pe(0))
+        assert np.allclose(hist[ts], np.array(row), rtol=1e-12, atol=1e-300), (ts, hist[ts], row)
