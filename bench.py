@@ -454,6 +454,21 @@ def main_ops(args):
         ctx.inject(step[0])
         step[0] += 1
     ops["inject, 8 emitters x 1000 pairs, us"] = timed(inj)
+    # internal conductors: a 64x64-node rectangle and a ring of 8 electrodes
+    # of radius 20 (the generic particle kernel tests each new position)
+    ctx.set_conductors(_lib.conductors([
+        {"kind": _lib.COND_RECT, "min_x": 100, "max_x": 164, "min_y": 100, "max_y": 164, "voltage": 1.0},
+        {"kind": _lib.COND_CIRCULAR, "x_center": 320, "y_center": 320, "electrode_number": 8,
+         "grid_radius": 120.0, "electrode_radius": 20.0, "voltage": -1.0}]))
+    ops["push without deposit per species, 2 conductors (9 shapes), us"] = timed(lambda: ctx.push(0))
+    tsn = [200]
+
+    def step():
+        ctx.step(tsn[0])
+        tsn[0] += 1
+    ops["fused step with 2 conductors (all 4 species), us"] = timed(step, 20)
+    ctx.set_conductors(_lib.conductors([]))
+    ops["fused step without conductors (all 4 species), us"] = timed(step, 20)
     ctx.close()
     # SOR (red-black, gssolver's problem) on the same grid, FP64, 1e-6
     st64 = workloads.bounded_512(ppc=1, precision=_lib.FP64, solver=_lib.SOLVER_GS, tol=1e-6)
