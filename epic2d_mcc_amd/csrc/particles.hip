@@ -470,7 +470,7 @@ __device__ __forceinline__ void ld_chunk_buf(__amdgpu_buffer_rsrc_t r, int lane,
 //      flight), stores, and deposits the new positions.
 // Every particle's arithmetic is that of particle_kernel<double> and of the
 // reference, operation for operation.
-template <int PPT, bool DEP>
+template <int PPT, bool DEP, bool COND = false>
 __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a) {
   using P = double;
   __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
@@ -582,6 +582,10 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
         if (py < a.y0) { py = a.y0 + (a.y0 - py); vy[k] = -vy[k]; }
         else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); vy[k] = -vy[k]; }
       }
+      if (COND && live[k] && in_conductor(a.cond, a.ncond, px, py)) {
+        px = p_nan<P>();
+        live[k] = false;
+      }
       x[k] = px;
       y[k] = py;
     }
@@ -606,7 +610,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
 // rr = 0: each XCD walks its own contiguous range of chunks; rr = 1: all
 // workgroups walk the chunks round-robin (the whole device streams through
 // one region at a time).
-template <bool DEP, int PPT, int WAVES>
+template <bool DEP, int PPT, int WAVES, bool COND = false>
 __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, int rr) {
   __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
   constexpr int CHW = 64 * PPT;
@@ -630,7 +634,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
   if (cb < ce_) load_chunk<PPT>(a, cb * CH + (int64_t)wid * CHW, n, lane, r);
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
     ChunkDep<PPT> dp;
-    push_chunk_regs<DEP, PPT>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp);
+    push_chunk_regs<DEP, PPT, COND>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp);
     const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
     const bool more = chunk + cs < ce_;
     const bool full = more && nb + 64 * PPT <= n;
@@ -651,7 +655,7 @@ struct PushF32Set {
   int nsp;
 };
 
-template <int PPT>
+template <int PPT, bool COND>
 __global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S) {
   __shared__ unsigned long long lds[(PART_BLOCK / 64) * WAVE_WIN];
   constexpr int CHW = 64 * PPT;
@@ -694,7 +698,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S
   for (; g < tot; g += gridDim.x) {
     ChunkDep<PPT> dp;
     const bool active = wb < n;
-    if (active) push_chunk_regs<true, PPT>(a, wb, n, lane, r, dp);
+    if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp);
     const int64_t gn = g + gridDim.x;
     const bool more = gn < tot;
     int sn = s;
@@ -832,6 +836,8 @@ PushF32 make_push_f32(Ctx* c, int s, double kv, double kx) {
   a.nx = d.nx; a.ny = d.ny; a.bc = d.bc;
   a.Q = S.Q;
   a.errc = c->errc;
+  a.cond = c->cond.table;
+  a.ncond = c->cond.active();
   return a;
 }
 
@@ -853,11 +859,20 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   push_consts(c, s, sub_cycle, false, &c1, &c2, &c3);
   const epic_domain_desc& d = c->dom;
   const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
-  // the dedicated push kernels have no conductor test: with conductors the
-  // generic kernel pushes
-  const bool plain = !c->cond.any();
-  if (plain && d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
+  const bool cond = c->cond.any();
+  if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
     const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2);
+    if (cond) {
+      // with conductors: the default shape with the conductor test
+      if (deposit) {
+        auto k = push_f32_kernel<true, 4, 1, true>;
+        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map);
+      } else {
+        auto k = push_f32_kernel<false, 4, 1, true>;
+        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map);
+      }
+      return hipGetLastError();
+    }
     // kernel shape: particles per thread x occupancy target (waves per SIMD);
     // EPIC_PUSH_VARIANT at context creation
     switch (c->push_variant) {
@@ -881,15 +896,15 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
     }
     return hipGetLastError();
   }
-  if (plain && d.precision == EPIC_FP64 && d.shape == EPIC_SHAPE_CIC && !whole && c->push64) {
+  if (d.precision == EPIC_FP64 && d.shape == EPIC_SHAPE_CIC && !whole && c->push64) {
     PartArgs<double> a = make_args<double>(c, s);
     a.c1 = c1; a.c2 = c2; a.c3 = c3;
 #define EPIC_L(PPT)                                                                              \
     if (deposit) {                                                                               \
-      auto k = push_f64_kernel<PPT, true>;                                                       \
+      auto k = cond ? push_f64_kernel<PPT, true, true> : push_f64_kernel<PPT, true, false>;      \
       hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a);          \
     } else {                                                                                     \
-      auto k = push_f64_kernel<PPT, false>;                                                      \
+      auto k = cond ? push_f64_kernel<PPT, false, true> : push_f64_kernel<PPT, false, false>;    \
       hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a);          \
     }
     if (c->part_ppt == 4) {
@@ -912,7 +927,7 @@ bool push_all_ok(const Ctx* c) {
   const epic_domain_desc& d = c->dom;
   const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
   if (!c->push_multi || d.precision != EPIC_FP32 || d.shape != EPIC_SHAPE_CIC || whole || d.nx > 65535 ||
-      d.ny > 65535 || c->push_variant != 0 || c->cond.any())
+      d.ny > 65535 || c->push_variant != 0)
     return false;
   int np = 0;
   for (int s = 0; s < c->nsp; s++) np += c->sp[s].desc.pushed ? 1 : 0;
@@ -927,7 +942,7 @@ hipError_t launch_push_all(Ctx* c) {
     push_consts(c, s, 1, false, &c1, &c2, &c3);
     S.sp[S.nsp++] = make_push_f32(c, s, c1 * c2, c3 * c2);
   }
-  auto k = push_f32_multi_kernel<4>;
+  auto k = c->cond.any() ? push_f32_multi_kernel<4, true> : push_f32_multi_kernel<4, false>;
   hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, S);
   return hipGetLastError();
 }

@@ -145,6 +145,8 @@ struct PushF32 {
   int nx, ny, bc;
   unsigned long long* Q;
   unsigned long long* errc;
+  const CondDev* cond;  // absorbing conductors (the first ncond entries are active)
+  int ncond;
 };
 
 // Bilinear interpolation of node values e00, e10 (i+1), e01 (j+1), e11 over
@@ -204,8 +206,10 @@ __device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, in
 }
 
 // Gather, push, walls and store of a loaded chunk; fills the deposit
-// information of the new positions (DEP).
-template <bool DEP, int PPT>
+// information of the new positions (DEP).  COND: the walls are followed by
+// the conductor test (a separate instantiation, so the kernels without
+// conductors keep their code).
+template <bool DEP, int PPT, bool COND = false>
 __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, int64_t n, int lane,
                                                 ChunkRegs<PPT>& r, ChunkDep<PPT>& dp) {
   constexpr int V = 4;
@@ -279,6 +283,10 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
       else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); r.vx[k] = -r.vx[k]; }
       if (py < a.y0) { py = a.y0 + (a.y0 - py); r.vy[k] = -r.vy[k]; }
       else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); r.vy[k] = -r.vy[k]; }
+    }
+    if (COND && alive && in_conductor(a.cond, a.ncond, (double)px, (double)py)) {
+      alive = false;
+      px = __builtin_nanf("");
     }
     r.x[k] = px;
     r.y[k] = py;

@@ -132,3 +132,90 @@ def test_conductor_errors():
         ctx.set_conductors(bad)
     ctx.set_conductors(_lib.conductors([]))
     ctx.close()
+
+
+WIDE = [{"kind": _lib.COND_RECT, "min_x": 20, "max_x": 40, "min_y": 70, "max_y": 100, "voltage": 3.0},
+        {"kind": _lib.COND_CIRCULAR, "x_center": 80, "y_center": 50, "electrode_number": 6, "grid_radius": 25.0,
+         "electrode_radius": 7.0, "voltage": -2.0}]
+
+
+@pytest.mark.parametrize("bc", ["pbc", "open", "reflective"])
+def test_windowed_fp64_push_with_conductors_bitwise(bc):
+    """129 x 129 nodes (the dedicated FP64 push kernel, with its conductor
+    test): from the oracle's own E field, after an SOR solve (electrodes
+    live), every surviving particle is bit-identical to the oracle's and the
+    same particles are erased."""
+    print(f"[conductors] windowed fp64 {bc}", flush=True)
+    st = small_setup(N=60000, Nx=128, Ny=128, bc=bc, solver=_lib.SOLVER_GS, tol=1e-6, maxit=20000,
+                     vsx_beam=2.0 if bc != "pbc" else 0.0)
+    o, ctx = _pair(st, WIDE)
+    for s in range(2):
+        o.scatter(s)
+        ctx.scatter(s)
+    o.compute_rho()
+    ctx.compute_rho()
+    o.solve()
+    ctx.solve()
+    o.efield()
+    ctx.set_field(2, o.grid(2))
+    ctx.set_field(3, o.grid(3))
+    n0 = len(o.particles(0)[0])
+    for ts in range(3):
+        for s in range(2):
+            ctx.push(s)
+            o.push(s)
+        for s in range(2):
+            g = ctx.download(s)
+            c = o.particles(s)
+            assert len(g[0]) == len(c[0]), f"push {ts}: species {s} count {len(g[0])} vs {len(c[0])}"
+            for k in range(4):
+                assert np.array_equal(g[k], c[k]), f"push {ts}: species {s} component {k}"
+    assert len(o.particles(0)[0]) < n0
+
+
+def _fp32_run(st, env, steps=3):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        ctx = _lib.Context(st.domain, st.species)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    ctx.set_conductors(_lib.conductors(WIDE))
+    o = OracleSim(st.domain, st.species)
+    o.set_conductors(_lib.conductors(WIDE))
+    for s in range(2):
+        assert o.init_species(s) == 0
+        ctx.upload(s, *o.particles(s))
+    ctx.initial_field()
+    for ts in range(steps):
+        ctx.step(ts)
+    out = [ctx.download(s)[:4] for s in range(2)], ctx.get_field(0).copy()
+    ctx.close()
+    return out, o
+
+
+@pytest.mark.parametrize("bc", ["open", "pbc"])
+def test_fp32_conductor_kernels_identical(bc):
+    """FP32 windowed pushes with conductors: the one-launch kernel and the
+    per-species kernel give the same particles and rho bit for bit, and the
+    particles track the FP64 oracle."""
+    print(f"[conductors] fp32 kernels {bc}", flush=True)
+    st = small_setup(N=100000, Nx=128, Ny=128, bc=bc, precision=_lib.FP32, solver=_lib.SOLVER_GS, tol=1e-6,
+                     maxit=20000, vsx_beam=2.0 if bc != "pbc" else 0.0)
+    (a, ra), o = _fp32_run(st, {})
+    (b, rb), _ = _fp32_run(st, {"EPIC_PUSH_MULTI": "0"})
+    for s in range(2):
+        for k in range(4):
+            assert np.array_equal(a[s][k], b[s][k], equal_nan=True), (s, k)
+    assert np.array_equal(ra, rb)
+    o.initial_field()
+    for ts in range(3):
+        o.step()
+    for s in range(2):
+        c = o.particles(s)
+        assert abs(len(a[s][0]) - len(c[0])) <= 3
