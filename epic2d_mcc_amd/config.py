@@ -164,7 +164,8 @@ def derive(Nx, Ny, x0, species: List[SpeciesSpec], density, DT_coeff, norm_schem
     Lx = (nx - 1) * dx
     Ly = (ny - 1) * dy
     if poisson_op is None:
-        poisson_op = _lib.OP_DIRICHLET_WALL if solver == _lib.SOLVER_GS else _lib.OP_REF_GHOST
+        poisson_op = (_lib.OP_DIRICHLET_WALL if solver in (_lib.SOLVER_GS, _lib.SOLVER_GS_BOLTZMANN)
+                      else _lib.OP_REF_GHOST)
     dom = _lib.DomainDesc(x0=x0, y0=x0, dx=dx, dy=dy, nx=nx, ny=ny, L=L, W=W, LDe=LDe, wpe=wpe, vel_norm=vel_norm,
                           density=density, DT=DT, bc=_lib.BC[bc], shape=_lib.SHAPE[shape], solver=solver,
                           poisson_op=poisson_op, tolerance=tolerance, tol_relative=tol_relative,
@@ -203,7 +204,8 @@ def setup_from_ini(sections, precision=_lib.FP64, seed=1, capacity_factor=1.0):
     tol = stod(g["solver"]["tolerance"])
     maxit = stoi(g["solver"]["max_iteration"])
     solver = {"gs": _lib.SOLVER_GS, "cg": _lib.SOLVER_PCG, "iter": _lib.SOLVER_PCG, "iterative": _lib.SOLVER_PCG,
-              "pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL}[solvertype]
+              "pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL,
+              "gshybrid": _lib.SOLVER_GS_BOLTZMANN}[solvertype]
     specs = []
     for key in sorted(g.get("species", {}), key=int):
         tok = g["species"][key].split(",")

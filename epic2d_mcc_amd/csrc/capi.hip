@@ -167,7 +167,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (!dom || !sp || !out || nsp < 1 || nsp > 16) return EPIC_ERR_ARG;
   if (dom->nx < 3 || dom->ny < 3) return EPIC_ERR_ARG;
   if (dom->precision != EPIC_FP64 && dom->precision != EPIC_FP32) return EPIC_ERR_ARG;
-  if (dom->solver < EPIC_SOLVER_PCG || dom->solver > EPIC_SOLVER_SPECTRAL) return EPIC_ERR_ARG;
+  if (dom->solver < EPIC_SOLVER_PCG || dom->solver > EPIC_SOLVER_GS_BOLTZMANN) return EPIC_ERR_ARG;
   if (dom->bc < EPIC_BC_PBC || dom->bc > EPIC_BC_REFLECTIVE) return EPIC_ERR_ARG;
   if (dom->shape != EPIC_SHAPE_CIC && dom->shape != EPIC_SHAPE_NGP) return EPIC_ERR_ARG;
   if ((int64_t)dom->nx * dom->ny > (1ll << 30)) return EPIC_ERR_ARG;

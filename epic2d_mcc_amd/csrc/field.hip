@@ -267,7 +267,9 @@ __global__ __launch_bounds__(BS) void pcg_kernel(PcgArgs<T> a) {
 //   phi += omega (phi_new - phi)
 // with the RMS residual sqrt(sum R^2/(nx ny)) checked every 25 iterations.
 // Lexicographic and red-black SOR converge to the same discrete solution;
-// their iterates differ.
+// their iterates differ.  BOLTZ: gsfluidsolver's problem (field.cpp:7-68),
+// rho s replaced by (rho - exp(phi)) s in the update and the residual, and
+// the residual checked every 50 iterations.
 struct SorArgs {
   int nx, ny;
   const double* rho;
@@ -280,8 +282,9 @@ struct SorArgs {
   unsigned* bar;
 };
 
-template <bool COOP>
+template <bool COOP, bool BOLTZ>
 __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
+  constexpr int CHECK = BOLTZ ? 50 : 25;
   __shared__ double sh[64];
   GridBar grid{a.bar, 0u};
   const int n = a.nx * a.ny;
@@ -297,24 +300,26 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
         const int i = k / a.ny, j = k % a.ny;
         if (i < 1 || i > a.nx - 2 || j < 1 || j > a.ny - 2 || ((i + j) & 1) != color) continue;
         const double* p = a.phi;
-        const double phi_new = a.coeff_inv * (a.rho[k] * a.s + ((p[k + a.ny] + p[k - a.ny]) / a.dx2) +
+        const double src = BOLTZ ? (a.rho[k] - exp(p[k])) * a.s : a.rho[k] * a.s;
+        const double phi_new = a.coeff_inv * (src + ((p[k + a.ny] + p[k - a.ny]) / a.dx2) +
                                               ((p[k + 1] + p[k - 1]) / a.dy2));
         a.phi[k] = p[k] + a.omega * (phi_new - p[k]);
       }
       gsync<COOP>(grid);
     }
-    if (it % 25 == 0) {
+    if (it % CHECK == 0) {
       double sum = 0;
       for (int k = g0; k < n; k += TT) {
         const int i = k / a.ny, j = k % a.ny;
         if (i < 1 || i > a.nx - 2 || j < 1 || j > a.ny - 2) continue;
         const double* p = a.phi;
+        const double src = BOLTZ ? (a.rho[k] - exp(p[k])) * a.s : (a.rho[k]) * a.s;
         const double R = ((p[k + a.ny] - 2 * p[k] + p[k - a.ny]) / a.dx2) +
-                         ((p[k + 1] - 2 * p[k] + p[k - 1]) / a.dy2) + (a.rho[k]) * a.s;
+                         ((p[k + 1] - 2 * p[k] + p[k - 1]) / a.dy2) + src;
         sum += R * R;
       }
       double d[1] = {sum};
-      grid_sum<1, COOP>(d, a.part, (it / 25) & 1, sh, grid);
+      grid_sum<1, COOP>(d, a.part, (it / CHECK) & 1, sh, grid);
       L2 = sqrt(d[0] / (a.nx * a.ny));
       if (L2 < a.tol) {
         it++;
@@ -582,6 +587,18 @@ hipError_t launch_conductor_rho(Ctx* c) {
   return hipSuccess;
 }
 
+// gsfluidsolver's boundary rho (field.cpp:23-33): 0 on the bottom, left and
+// right edges, top_v on the top edge (j = ny-1) except its two corners, which
+// the second loop resets to 0.
+__global__ void boltz_boundary_rho_kernel(double* rho, int nx, int ny, double top_v) {
+  const int n = nx * ny;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int i = k / ny, j = k % ny;
+    if (i == 0 || i == nx - 1 || j == 0) rho[k] = 0.0;
+    else if (j == ny - 1) rho[k] = top_v;
+  }
+}
+
 hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
@@ -589,8 +606,17 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   const double s = (d.L * d.L) / (d.LDe * d.LDe);
   const int grid = std::min((n + 255) / 256, 4096);
   const long long cap = std::min<long long>((long long)n * n, (long long)d.max_iteration);
-  if (d.solver == EPIC_SOLVER_GS) {
-    if (c->cond.any()) EPIC_HIP_TRY(launch_conductor_rho(c));
+  if (d.solver == EPIC_SOLVER_GS || d.solver == EPIC_SOLVER_GS_BOLTZMANN) {
+    const bool boltz = d.solver == EPIC_SOLVER_GS_BOLTZMANN;
+    // gssolver runs the conductors' initialize(); gsfluidsolver does not
+    if (!boltz && c->cond.any()) EPIC_HIP_TRY(launch_conductor_rho(c));
+    if (boltz) {
+      // 10000 * (eV / (K_b * EV_to_K)) with the reference's constants (domain.h:204-215)
+      const double QE_ = 1.602176565e-19, K_b = 1.380648e-23, EV_to_K = QE_ / K_b, eV_ = 1.602176565e-19;
+      const double top_v = 10000 * (eV_ / (K_b * EV_to_K));
+      hipLaunchKernelGGL(boltz_boundary_rho_kernel, dim3(grid), dim3(256), 0, c->stream, c->rho, d.nx, d.ny, top_v);
+      EPIC_HIP_TRY(hipGetLastError());
+    }
     SorArgs a{};
     a.nx = d.nx; a.ny = d.ny; a.rho = c->rho; a.phi = c->phi;
     a.dx2 = d.dx * d.dx; a.dy2 = d.dy * d.dy;
@@ -602,12 +628,13 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
     const int G = solver_blocks(c, n, &npt);
     hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
     if (G == 1) {
-      hipLaunchKernelGGL(sor_kernel<false>, dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
+      auto k1 = boltz ? sor_kernel<false, true> : sor_kernel<false, false>;
+      hipLaunchKernelGGL(k1, dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
       EPIC_HIP_TRY(hipGetLastError());
     } else {
       void* args[] = {&a};
-      EPIC_HIP_TRY(hipLaunchCooperativeKernel((const void*)sor_kernel<true>, dim3(G), dim3(SOLVER_BLOCK), args, 0,
-                                              c->stream));
+      const void* k = boltz ? (const void*)sor_kernel<true, true> : (const void*)sor_kernel<true, false>;
+      EPIC_HIP_TRY(hipLaunchCooperativeKernel(k, dim3(G), dim3(SOLVER_BLOCK), args, 0, c->stream));
     }
     double sc[3];
     EPIC_HIP_TRY(read_scalars(c, iters, sc));

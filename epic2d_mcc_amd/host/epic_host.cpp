@@ -167,11 +167,15 @@ epic_ctx* Domain::ctx() {
   else if (shapefunction == "NGP") d.shape = EPIC_SHAPE_NGP;
   else throw EpicError(EPIC_ERR_ARG, "unknown shapefunction " + shapefunction);
   if (SolverType == "gs") d.solver = EPIC_SOLVER_GS;
+  else if (SolverType == "gshybrid") d.solver = EPIC_SOLVER_GS_BOLTZMANN;
   else if (SolverType == "cg" || SolverType == "iter" || SolverType == "iterative") d.solver = EPIC_SOLVER_PCG;
   else if (SolverType == "pcg_mixed") d.solver = EPIC_SOLVER_PCG_MIXED;
   else if (SolverType == "spectral") d.solver = EPIC_SOLVER_SPECTRAL;
   else throw EpicError(EPIC_ERR_ARG, "solver " + SolverType + " is not part of this build");
-  d.poisson_op = poisson_op >= 0 ? poisson_op : (d.solver == EPIC_SOLVER_GS ? EPIC_OP_DIRICHLET_WALL : EPIC_OP_REF_GHOST);
+  d.poisson_op = poisson_op >= 0 ? poisson_op
+                                 : (d.solver == EPIC_SOLVER_GS || d.solver == EPIC_SOLVER_GS_BOLTZMANN
+                                        ? EPIC_OP_DIRICHLET_WALL
+                                        : EPIC_OP_REF_GHOST);
   d.tolerance = tolerance;
   d.tol_relative = tol_relative;
   d.max_iteration = max_iteration;

@@ -52,6 +52,10 @@ extern "C" {
                                    operator in FP64 (the transforms are FP64
                                    matrix products on the matrix cores), plus
                                    FP64 residual refinement                    */
+#define EPIC_SOLVER_GS_BOLTZMANN 4 /* "gshybrid": SOR on the Boltzmann-electron
+                                   problem lap phi = -(rho - exp(phi)) L^2/LDe^2
+                                   (gsfluidsolver, field.cpp:7-68), red-black,
+                                   with its boundary rho values               */
 /* inner solve of EPIC_SOLVER_PCG_MIXED */
 #define EPIC_PRECOND_JACOBI 0   /* FP32 Jacobi-preconditioned CG               */
 #define EPIC_PRECOND_SPECTRAL 1 /* FP32 sine-transform solve (one application

@@ -61,7 +61,7 @@ const double E_ION_TH = 15.8;
 
 enum { BC_PBC = 0, BC_OPEN = 1, BC_REFLECTIVE = 2 };
 enum { SHAPE_CIC = 0, SHAPE_NGP = 1 };
-enum { SOLVER_PCG = 0, SOLVER_GS = 1 };
+enum { SOLVER_PCG = 0, SOLVER_GS = 1, SOLVER_GS_FLUID = 4 };
 enum { LOAD_RANDOM = 0, LOAD_UNIFORM = 1, LOAD_SIN = 2, LOAD_COS = 3, LOAD_GAUSSIAN = 4 };
 
 struct Particle {
@@ -677,11 +677,60 @@ static int gssolver(Sim* m) {
   return 0;
 }
 
+// FieldSolve::gsfluidsolver, field.cpp:7-68: SOR on the Boltzmann-electron
+// problem, with its boundary rho values and a residual check every 50
+// iterations; conductors are not initialised here.
+static void gsfluidsolver(Sim* m) {
+  double dx2 = m->dx * m->dx, dy2 = m->dy * m->dy;
+  int nx = m->nx, ny = m->ny;
+  std::fill(m->phi.begin(), m->phi.end(), 0.0);
+  double L2 = 0;
+  double coeff = (2 / dx2) + (2 / dy2);
+  auto phi = [&](int i, int j) -> double& { return m->phi[m->idx(i, j)]; };
+  auto rho = [&](int i, int j) -> double& { return m->rho[m->idx(i, j)]; };
+  for (int i = 0; i < nx; i++) {
+    rho(i, 0) = 0;
+    rho(i, ny - 1) = 10000 * (eV / (K_b * EV_to_K));
+  }
+  for (int i = 0; i < ny; i++) {
+    rho(0, i) = 0;
+    rho(nx - 1, i) = 0;
+  }
+  double s = (m->L * m->L) / (m->LDe * m->LDe);
+  int it_done = 0;
+  for (int iter = 0; iter < m->max_iteration; iter++) {
+    for (int i = 1; i < nx - 1; i++)
+      for (int j = 1; j < ny - 1; j++) {
+        double phi_new = (1 / coeff) * ((rho(i, j) - exp(phi(i, j))) * s +
+                                        ((phi(i + 1, j) + phi(i - 1, j)) / dx2) +
+                                        ((phi(i, j + 1) + phi(i, j - 1)) / dy2));
+        phi(i, j) = phi(i, j) + 1.4 * (phi_new - phi(i, j));
+      }
+    it_done = iter + 1;
+    if (iter % 50 == 0) {
+      double sum = 0;
+      for (int i = 1; i < nx - 1; i++)
+        for (int j = 1; j < ny - 1; j++) {
+          double R = ((phi(i + 1, j) - 2 * phi(i, j) + phi(i - 1, j)) / dx2) +
+                     ((phi(i, j + 1) - 2 * phi(i, j) + phi(i, j - 1)) / dy2) +
+                     (rho(i, j) - exp(phi(i, j))) * s;
+          sum += R * R;
+        }
+      L2 = sqrt(sum / (nx * ny));
+      m->error = L2;
+      if (L2 < m->tolerance) break;
+    }
+  }
+  m->last_iters = it_done;
+}
+
 // FieldSolve::PotentialSolver, field.cpp:266-272.
 int orc_solve(void* h, double* err) {
   Sim* m = (Sim*)h;
   if (m->solver == SOLVER_GS) {
     if (gssolver(m)) return -1;
+  } else if (m->solver == SOLVER_GS_FLUID) {
+    gsfluidsolver(m);
   } else {
     pcgsolver(m);
   }
