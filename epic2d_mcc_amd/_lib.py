@@ -20,7 +20,7 @@ ERR_NAMES = {-1: "EPIC_ERR_ARG", -2: "EPIC_ERR_HIP", -3: "EPIC_ERR_CAPACITY", -4
 
 BC = {"pbc": 0, "open": 1, "reflective": 2}
 SHAPE = {"CIC": 0, "NGP": 1}
-SOLVER_PCG, SOLVER_GS, SOLVER_PCG_MIXED, SOLVER_SPECTRAL, SOLVER_GS_BOLTZMANN = 0, 1, 2, 3, 4
+SOLVER_PCG, SOLVER_GS, SOLVER_PCG_MIXED, SOLVER_SPECTRAL, SOLVER_GS_BOLTZMANN, SOLVER_NEWTON_CG = 0, 1, 2, 3, 4, 5
 OP_REF_GHOST, OP_DIRICHLET_WALL = 0, 1
 PRECOND_JACOBI, PRECOND_SPECTRAL = 0, 1
 FP64, FP32 = 0, 1

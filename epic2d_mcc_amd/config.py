@@ -205,7 +205,7 @@ def setup_from_ini(sections, precision=_lib.FP64, seed=1, capacity_factor=1.0):
     maxit = stoi(g["solver"]["max_iteration"])
     solver = {"gs": _lib.SOLVER_GS, "cg": _lib.SOLVER_PCG, "iter": _lib.SOLVER_PCG, "iterative": _lib.SOLVER_PCG,
               "pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL,
-              "gshybrid": _lib.SOLVER_GS_BOLTZMANN}[solvertype]
+              "gshybrid": _lib.SOLVER_GS_BOLTZMANN, "hybrid": _lib.SOLVER_NEWTON_CG}[solvertype]
     specs = []
     for key in sorted(g.get("species", {}), key=int):
         tok = g["species"][key].split(",")

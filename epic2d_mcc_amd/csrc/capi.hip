@@ -103,7 +103,7 @@ static void free_all(Ctx* c) {
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
                 c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
                 c->mcc.event_count, c->mcc.cand, c->mcc.cand_n, c->sp_S, c->sp_T, c->sp_lam,
-                c->cond.table, c->cond.node, c->cond.value};
+                c->cond.table, c->cond.node, c->cond.value, c->nr_ws};
   for (void* p : gs)
     if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -167,7 +167,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (!dom || !sp || !out || nsp < 1 || nsp > 16) return EPIC_ERR_ARG;
   if (dom->nx < 3 || dom->ny < 3) return EPIC_ERR_ARG;
   if (dom->precision != EPIC_FP64 && dom->precision != EPIC_FP32) return EPIC_ERR_ARG;
-  if (dom->solver < EPIC_SOLVER_PCG || dom->solver > EPIC_SOLVER_GS_BOLTZMANN) return EPIC_ERR_ARG;
+  if (dom->solver < EPIC_SOLVER_PCG || dom->solver > EPIC_SOLVER_NEWTON_CG) return EPIC_ERR_ARG;
   if (dom->bc < EPIC_BC_PBC || dom->bc > EPIC_BC_REFLECTIVE) return EPIC_ERR_ARG;
   if (dom->shape != EPIC_SHAPE_CIC && dom->shape != EPIC_SHAPE_NGP) return EPIC_ERR_ARG;
   if ((int64_t)dom->nx * dom->ny > (1ll << 30)) return EPIC_ERR_ARG;

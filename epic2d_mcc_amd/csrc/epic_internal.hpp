@@ -147,6 +147,7 @@ struct Ctx {
   double* sp_S = nullptr;       // sine-transform matrices
   double* sp_T = nullptr;       // temporaries
   double* sp_lam = nullptr;
+  double* nr_ws = nullptr;      // Newton-CG solver: 6 * nnodes, allocated on first use
   int sp_m = 0, sp_n = 0;
   void* blas = nullptr;
 

@@ -587,6 +587,99 @@ hipError_t launch_conductor_rho(Ctx* c) {
   return hipSuccess;
 }
 
+// nrpcgsolver (field.cpp:141-210) in one workgroup: ten Newton steps, each
+// cg() (solvers.h:32-69) from zero on J = A - diag(exp(x)) with F = A x -
+// rhovec, ||r|| < 1e-4, at most min(n, 10000) iterations, then x -= y.  As
+// written in the reference, rhovec accumulates exp(x) L^2/LDe^2 over the
+// steps while J's diagonal takes exp(x) alone.  A is pcgsolver's matrix
+// (flat index k = row*nx + col, neighbours k +- 1, k +- nx, summed in column
+// order); dot products are workgroup sums.
+struct NrArgs {
+  int nx, ny;
+  const double* rho;
+  double* x;  // phi
+  double dx2, s;
+  double* ws;  // rhovec, ex, y, r, p, Ap: 6n
+  int* iters_out;
+  double* scalars;
+};
+
+__device__ __forceinline__ double nr_row(const double* v, const double* ex, int k, int nx, int ny, bool jac) {
+  const int row = k / nx, col = k % nx;
+  double sum = 0;
+  if (row > 0) sum += 1.0 * v[k - nx];
+  if (col > 0) sum += 1.0 * v[k - 1];
+  sum += (jac ? (-4.0 - ex[k]) : -4.0) * v[k];
+  if (col < nx - 1) sum += 1.0 * v[k + 1];
+  if (row < ny - 1) sum += 1.0 * v[k + nx];
+  return sum;
+}
+
+__global__ __launch_bounds__(SOLVER_BLOCK) void newton_cg_kernel(NrArgs a) {
+  __shared__ double sh[SOLVER_BLOCK / 64];
+  const int nx = a.nx, ny = a.ny, n = nx * ny, t = threadIdx.x, T = blockDim.x;
+  double* rv = a.ws;
+  double* ex = a.ws + n;
+  double* y = a.ws + 2 * (size_t)n;
+  double* r = a.ws + 3 * (size_t)n;
+  double* p = a.ws + 4 * (size_t)n;
+  double* Ap = a.ws + 5 * (size_t)n;
+  auto dot = [&](const double* u, const double* v) {
+    double d[1] = {0.0};
+    for (int k = t; k < n; k += T) d[0] += u[k] * v[k];
+    block_sum<1>(d, sh);
+    return d[0];
+  };
+  for (int k = t; k < n; k += T) {
+    rv[k] = rhs_at(a.rho[k], k / ny, k % ny, nx, ny, a.dx2, a.s);
+    a.x[k] = 0.0;
+  }
+  __syncthreads();
+  int total = 0;
+  const int cap = min(n, 10000);
+  for (int newton = 0; newton < 10; newton++) {
+    for (int k = t; k < n; k += T) {
+      const double e = exp(a.x[k]);
+      rv[k] = rv[k] + e * a.s;
+      ex[k] = e;
+    }
+    __syncthreads();
+    // F = A x - rhovec; r0 = F - J 0 = F; p0 = r0; y = 0
+    for (int k = t; k < n; k += T) {
+      const double F = nr_row(a.x, ex, k, nx, ny, false) - rv[k];
+      r[k] = F;
+      p[k] = F;
+      y[k] = 0.0;
+    }
+    __syncthreads();
+    double rr = dot(r, r);
+    for (int it = 0; it < cap; it++) {
+      if (sqrt(rr) < 1e-4) break;
+      for (int k = t; k < n; k += T) Ap[k] = nr_row(p, ex, k, nx, ny, true);
+      __syncthreads();
+      const double pAp = dot(p, Ap);
+      const double alpha = rr / pAp;
+      for (int k = t; k < n; k += T) {
+        y[k] = y[k] + p[k] * alpha;
+        r[k] = r[k] - Ap[k] * alpha;
+      }
+      __syncthreads();
+      const double r12 = dot(r, r);
+      const double beta = r12 / rr;
+      for (int k = t; k < n; k += T) p[k] = r[k] + p[k] * beta;
+      __syncthreads();
+      rr = r12;
+      total++;
+    }
+    for (int k = t; k < n; k += T) a.x[k] = a.x[k] - y[k];
+    __syncthreads();
+  }
+  if (t == 0) {
+    *a.iters_out = total;
+    a.scalars[1] = 0.0;  // nrpcgsolver sets error = 0
+  }
+}
+
 // gsfluidsolver's boundary rho (field.cpp:23-33): 0 on the bottom, left and
 // right edges, top_v on the top edge (j = ny-1) except its two corners, which
 // the second loop resets to 0.
@@ -606,6 +699,19 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   const double s = (d.L * d.L) / (d.LDe * d.LDe);
   const int grid = std::min((n + 255) / 256, 4096);
   const long long cap = std::min<long long>((long long)n * n, (long long)d.max_iteration);
+  if (d.solver == EPIC_SOLVER_NEWTON_CG) {
+    if (!c->nr_ws) EPIC_HIP_TRY(hipMalloc(&c->nr_ws, 6 * (size_t)n * sizeof(double)));
+    NrArgs a{};
+    a.nx = d.nx; a.ny = d.ny; a.rho = c->rho; a.x = c->phi;
+    a.dx2 = dx2; a.s = s;
+    a.ws = c->nr_ws; a.iters_out = c->ws_iter; a.scalars = c->ws_scalar;
+    hipLaunchKernelGGL(newton_cg_kernel, dim3(1), dim3(SOLVER_BLOCK), 0, c->stream, a);
+    EPIC_HIP_TRY(hipGetLastError());
+    double sc[3];
+    EPIC_HIP_TRY(read_scalars(c, iters, sc));
+    *err = sc[1];
+    return hipSuccess;
+  }
   if (d.solver == EPIC_SOLVER_GS || d.solver == EPIC_SOLVER_GS_BOLTZMANN) {
     const bool boltz = d.solver == EPIC_SOLVER_GS_BOLTZMANN;
     // gssolver runs the conductors' initialize(); gsfluidsolver does not

@@ -168,6 +168,7 @@ epic_ctx* Domain::ctx() {
   else throw EpicError(EPIC_ERR_ARG, "unknown shapefunction " + shapefunction);
   if (SolverType == "gs") d.solver = EPIC_SOLVER_GS;
   else if (SolverType == "gshybrid") d.solver = EPIC_SOLVER_GS_BOLTZMANN;
+  else if (SolverType == "hybrid") d.solver = EPIC_SOLVER_NEWTON_CG;
   else if (SolverType == "cg" || SolverType == "iter" || SolverType == "iterative") d.solver = EPIC_SOLVER_PCG;
   else if (SolverType == "pcg_mixed") d.solver = EPIC_SOLVER_PCG_MIXED;
   else if (SolverType == "spectral") d.solver = EPIC_SOLVER_SPECTRAL;

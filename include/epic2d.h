@@ -56,6 +56,11 @@ extern "C" {
                                    problem lap phi = -(rho - exp(phi)) L^2/LDe^2
                                    (gsfluidsolver, field.cpp:7-68), red-black,
                                    with its boundary rho values               */
+#define EPIC_SOLVER_NEWTON_CG 5 /* "hybrid": nrpcgsolver (field.cpp:141-210), ten
+                                   Newton steps each solved by plain CG on
+                                   A - diag(exp(phi)), as written (its
+                                   right-hand side accumulates over the steps);
+                                   one workgroup                              */
 /* inner solve of EPIC_SOLVER_PCG_MIXED */
 #define EPIC_PRECOND_JACOBI 0   /* FP32 Jacobi-preconditioned CG               */
 #define EPIC_PRECOND_SPECTRAL 1 /* FP32 sine-transform solve (one application

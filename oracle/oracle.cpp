@@ -61,7 +61,7 @@ const double E_ION_TH = 15.8;
 
 enum { BC_PBC = 0, BC_OPEN = 1, BC_REFLECTIVE = 2 };
 enum { SHAPE_CIC = 0, SHAPE_NGP = 1 };
-enum { SOLVER_PCG = 0, SOLVER_GS = 1, SOLVER_GS_FLUID = 4 };
+enum { SOLVER_PCG = 0, SOLVER_GS = 1, SOLVER_GS_FLUID = 4, SOLVER_NEWTON_CG = 5 };
 enum { LOAD_RANDOM = 0, LOAD_UNIFORM = 1, LOAD_SIN = 2, LOAD_COS = 3, LOAD_GAUSSIAN = 4 };
 
 struct Particle {
@@ -724,6 +724,69 @@ static void gsfluidsolver(Sim* m) {
   m->last_iters = it_done;
 }
 
+// FieldSolve::nrpcgsolver, field.cpp:141-210: ten Newton steps on the
+// Boltzmann-electron problem, each a cg() (solvers.h:32-69: plain CG from
+// x = 0, absolute ||r|| < 1e-4, at most min(n, 10000) iterations) on
+// J = A - diag(exp(x)).  Kept as written: the right-hand side accumulates
+// exp(x) L^2/LDe^2 over the Newton steps (rhovec = rhovec + rhovecx), while
+// the Jacobian's diagonal takes exp(x) without the factor.  A is the
+// reference's matrix (apply_A; J x adds (-exp(x_i)) x_i to the diagonal term
+// in column order).
+static void nrpcgsolver(Sim* m) {
+  double dx2 = m->dx * m->dx;
+  int nx = m->nx, ny = m->ny, n = nx * ny;
+  double s = (m->L * m->L) / (m->LDe * m->LDe);
+  std::vector<double> rhovec(n), rhovecx(n), x(n, 0.0), Ax(n), F(n), ex(n);
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) rhovec[m->idx(i, j)] = -m->rho[m->idx(i, j)] * dx2 * s;
+  for (int i = 0; i < nx; i++) { rhovec[m->idx(i, 0)] = 0; rhovec[m->idx(i, ny - 1)] = 0; }
+  for (int j = 0; j < ny; j++) { rhovec[m->idx(0, j)] = 0; rhovec[m->idx(nx - 1, j)] = 0; }
+  // J v: the row of A with its diagonal -4 replaced by (-4) - exp(x_i)
+  auto apply_J = [&](const std::vector<double>& v, std::vector<double>& u) {
+    for (int i = 0; i < n; i++) {
+      int row = i / nx, col = i % nx;
+      double sum = 0;
+      if (row > 0) sum += 1.0 * v[i - nx];
+      if (col > 0) sum += 1.0 * v[i - 1];
+      sum += (-4.0 - ex[i]) * v[i];
+      if (col < nx - 1) sum += 1.0 * v[i + 1];
+      if (row < ny - 1) sum += 1.0 * v[i + nx];
+      u[i] = sum;
+    }
+  };
+  int total = 0;
+  for (int iter = 0; iter < 10; iter++) {
+    for (int i = 0; i < n; i++) rhovecx[i] = exp(x[i]) * s;
+    for (int i = 0; i < n; i++) rhovec[i] = rhovec[i] + rhovecx[i];
+    apply_A(nx, ny, x, Ax);
+    for (int i = 0; i < n; i++) F[i] = Ax[i] - rhovec[i];
+    for (int i = 0; i < n; i++) ex[i] = exp(x[i]);
+    // cg(J, x_ini = 0, F, 10000, 1e-4)
+    std::vector<double> y(n, 0.0), r0(n), p0(n), Ap(n);
+    apply_J(y, Ap);
+    for (int i = 0; i < n; i++) r0[i] = F[i] - Ap[i];
+    p0 = r0;
+    int cap = std::min(n, 10000);
+    for (int it = 0; it < cap; it++) {
+      if (std::sqrt(dot(r0, r0)) < 1e-4) break;
+      apply_J(p0, Ap);
+      double r02 = dot(r0, r0);
+      double pAp = dot(p0, Ap);
+      double alpha = r02 / pAp;
+      for (int i = 0; i < n; i++) y[i] = y[i] + p0[i] * alpha;
+      for (int i = 0; i < n; i++) r0[i] = r0[i] - Ap[i] * alpha;
+      double r12 = dot(r0, r0);
+      double beta = r12 / r02;
+      for (int i = 0; i < n; i++) p0[i] = r0[i] + p0[i] * beta;
+      total++;
+    }
+    for (int i = 0; i < n; i++) x[i] = x[i] - y[i];
+  }
+  m->error = 0;
+  m->phi = x;
+  m->last_iters = total;
+}
+
 // FieldSolve::PotentialSolver, field.cpp:266-272.
 int orc_solve(void* h, double* err) {
   Sim* m = (Sim*)h;
@@ -731,6 +794,8 @@ int orc_solve(void* h, double* err) {
     if (gssolver(m)) return -1;
   } else if (m->solver == SOLVER_GS_FLUID) {
     gsfluidsolver(m);
+  } else if (m->solver == SOLVER_NEWTON_CG) {
+    nrpcgsolver(m);
   } else {
     pcgsolver(m);
   }

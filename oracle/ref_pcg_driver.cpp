@@ -40,4 +40,49 @@ int ref_pcg(int nx, int ny, const double* rhs, double* phi, int max_iteration, d
   return 0;
 }
 
+// FieldSolve::nrpcgsolver (field.cpp:141-210) on the reference's dense
+// Matrix, vec and cg(): ten Newton steps, each cg(J, 0, F, 10000, 1e-4) with
+// J = A - diag(exp(x)); rhs is rho itself (the driver forms rhom the way the
+// solver does).  s = L^2/LDe^2, dx2 = dx^2.
+int ref_nrpcg(int nx, int ny, const double* rho, double dx2, double s, double* phi) {
+  int n = nx * ny;
+  Matrix<double> A(n, n), rhom(nx, ny);
+  for (int i = 0; i < n; ++i) {
+    int row = i / nx;
+    int col = i % nx;
+    A(i, i) = -4;
+    if (col > 0) A(i, i - 1) = 1;
+    if (col < nx - 1) A(i, i + 1) = 1;
+    if (row > 0) A(i, i - nx) = 1;
+    if (row < ny - 1) A(i, i + nx) = 1;
+  }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) rhom(i, j) = -rho[i * ny + j] * dx2 * s;
+  for (int i = 0; i < nx; i++) {
+    rhom(i, 0) = 0;
+    rhom(i, ny - 1) = 0;
+  }
+  for (int i = 0; i < ny; i++) {
+    rhom(0, i) = 0;
+    rhom(nx - 1, i) = 0;
+  }
+  vec<double> rhovec = flatten(rhom);
+  vec<double> rhovecx(n);
+  vec<double> x(n);
+  for (int iter = 0; iter < 10; iter++) {
+    for (int i = 0; i < n; i++) rhovecx(i) = exp(x(i)) * s;
+    rhovec = rhovec + rhovecx;
+    Matrix<double> J = A;
+    vec<double> F = A * x - rhovec;
+    for (int i = 0; i < n; i++) J(i, i) = J(i, i) - exp(x(i));
+    vec<double> x_ini(n);
+    vec<double> y = cg(J, x_ini, F, 10000, 1e-4);
+    x = x - y;
+  }
+  Matrix<double> xarr = unflatten(x, nx, ny);
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) phi[i * ny + j] = xarr(i, j);
+  return 0;
+}
+
 }  // extern "C"

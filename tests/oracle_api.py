@@ -103,6 +103,7 @@ def ref_pcg_lib():
         return None
     r = ctypes.CDLL(REF_PCG)
     r.ref_pcg.argtypes = [ctypes.c_int, ctypes.c_int, D, D, ctypes.c_int, ctypes.c_double, D]
+    r.ref_nrpcg.argtypes = [ctypes.c_int, ctypes.c_int, D, ctypes.c_double, ctypes.c_double, D]
     return r
 
 
@@ -114,7 +115,7 @@ class OracleSim:
         self.o = o
         d = OrcDomain(x0=dom.x0, y0=dom.y0, dx=dom.dx, dy=dom.dy, nx=dom.nx, ny=dom.ny, L=dom.L, W=dom.W,
                       LDe=dom.LDe, wpe=dom.wpe, vel_norm=dom.vel_norm, density=dom.density, DT=dom.DT, bc=dom.bc,
-                      shape=dom.shape, solver=dom.solver if dom.solver in (1, 4) else 0, tolerance=dom.tolerance,
+                      shape=dom.shape, solver=dom.solver if dom.solver in (1, 4, 5) else 0, tolerance=dom.tolerance,
                       max_iteration=dom.max_iteration, elastic=0, excitation=0, ionization=0, gas_density=0.0)
         if gas:
             d.elastic, d.excitation, d.ionization, d.gas_density = gas

@@ -62,3 +62,45 @@ def test_boltzmann_sor_steps_track_oracle():
             c = o.particles(s)
             assert len(g[0]) == len(c[0])
             assert rel(g[0], c[0]) < 1e-9 and rel(g[2], c[2]) < 1e-7
+
+
+@pytest.mark.parametrize("Nx,Ny", [(16, 16), (20, 12)])
+def test_newton_cg_matches_oracle(Nx, Ny):
+    """nrpcgsolver on the device (one workgroup, workgroup dot products)
+    against the oracle's sequential restatement: the CG iteration counts
+    agree and phi agrees to rounding."""
+    print(f"[hybrid] newton-cg {Nx}x{Ny}", flush=True)
+    st = small_setup(N=10, Nx=Nx, Ny=Ny, solver=_lib.SOLVER_NEWTON_CG)
+    d = st.domain
+    rng = np.random.default_rng(6)
+    rho = rng.normal(scale=0.3, size=(d.nx, d.ny))
+    ctx = _lib.Context(d, st.species)
+    ctx.set_field(0, rho)
+    itg, err = ctx.solve()
+    o = OracleSim(d, st.species)
+    o.set_grid(0, rho)
+    ito, _ = o.solve()
+    assert abs(itg - ito) <= 2, (itg, ito)
+    assert err == 0.0
+    assert rel(ctx.get_field(1), o.grid(1)) < 1e-9
+    ctx.close()
+
+
+def test_newton_cg_steps_track_oracle():
+    print("[hybrid] newton-cg steps", flush=True)
+    st = small_setup(N=2000, Nx=16, Ny=16, bc="pbc", solver=_lib.SOLVER_NEWTON_CG)
+    o, parts = reference_load(st)
+    ctx = _lib.Context(st.domain, st.species)
+    for s, p in enumerate(parts):
+        ctx.upload(s, *p)
+    ctx.initial_field()
+    o.initial_field()
+    for ts in range(4):
+        ctx.step(ts)
+        o.step()
+        assert rel(ctx.get_field(1), o.grid(1)) < 1e-8, ts
+        for s in range(2):
+            g = ctx.download(s)
+            c = o.particles(s)
+            assert len(g[0]) == len(c[0])
+            assert rel(g[0], c[0]) < 1e-9 and rel(g[2], c[2]) < 1e-7

@@ -1,7 +1,10 @@
-"""Pins of the oracle's Boltzmann-electron SOR (FieldSolve::gsfluidsolver,
-field.cpp:7-68): a manufactured solution of the nonlinear problem and the
-closed form of the boundary rho it writes."""
+"""Pins of the oracle's Boltzmann-electron solvers: the SOR
+(FieldSolve::gsfluidsolver, field.cpp:7-68) by a manufactured solution of
+the nonlinear problem and the closed form of the boundary rho it writes; the
+Newton-CG (FieldSolve::nrpcgsolver, field.cpp:141-210) bit for bit against
+the same loop on the reference's own dense Matrix/vec/cg() templates."""
 import numpy as np
+import pytest
 
 from epic2d_mcc_amd import _lib, config
 from harness import small_setup
@@ -55,3 +58,26 @@ def test_gsfluid_boundary_rho():
     assert np.all(rho[1:-1, -1] == top) and abs(top - 10000) < 1e-9
     assert np.all(rho[:, 0] == 0) and np.all(rho[0, :] == 0) and np.all(rho[-1, :] == 0)
     assert np.all(rho[1:-1, 1:-1] == 0.25)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(11, 11), (13, 9)])
+def test_newton_cg_bitwise_equal_to_reference_templates(Nx, Ny):
+    # nrpcgsolver restated matrix-free == the same Newton loop on the
+    # reference's own dense Matrix/vec/cg() (oracle/_ref), bit for bit,
+    # including the nx-stride matrix on a non-square grid
+    from oracle_api import dp, ref_pcg_lib
+    r = ref_pcg_lib()
+    if r is None:
+        pytest.skip("reference tree absent: oracle/_ref not built")
+    st = small_setup(N=1, Nx=Nx, Ny=Ny, solver=_lib.SOLVER_NEWTON_CG)
+    d = st.domain
+    rng = np.random.default_rng(4)
+    rho = rng.normal(scale=0.3, size=(d.nx, d.ny))
+    o = OracleSim(d, st.species)
+    o.set_grid(0, rho)
+    o.solve()
+    want = np.empty(d.nx * d.ny)
+    s = (d.L * d.L) / (d.LDe * d.LDe)
+    r.ref_nrpcg(d.nx, d.ny, dp(np.ascontiguousarray(rho)), d.dx * d.dx, s, dp(want))
+    assert np.array_equal(o.grid(1).ravel(), want)
+    assert np.any(want != 0)
