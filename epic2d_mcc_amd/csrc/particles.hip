@@ -611,7 +611,8 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
 // workgroups walk the chunks round-robin (the whole device streams through
 // one region at a time).
 template <bool DEP, int PPT, int WAVES, bool COND = false>
-__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, int rr) {
+__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, int rr, const CondDev* cond,
+                                                                     int ncond) {
   __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
   constexpr int CHW = 64 * PPT;
   constexpr int CH = PART_BLOCK * PPT;
@@ -634,7 +635,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
   if (cb < ce_) load_chunk<PPT>(a, cb * CH + (int64_t)wid * CHW, n, lane, r);
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
     ChunkDep<PPT> dp;
-    push_chunk_regs<DEP, PPT, COND>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp);
+    push_chunk_regs<DEP, PPT, COND>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp, cond, ncond);
     const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
     const bool more = chunk + cs < ce_;
     const bool full = more && nb + 64 * PPT <= n;
@@ -653,6 +654,8 @@ constexpr int PUSH_MAXSP = 8;
 struct PushF32Set {
   PushF32 sp[PUSH_MAXSP];
   int nsp;
+  const CondDev* cond;  // absorbing conductors (the first ncond are active)
+  int ncond;
 };
 
 template <int PPT, bool COND>
@@ -698,7 +701,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S
   for (; g < tot; g += gridDim.x) {
     ChunkDep<PPT> dp;
     const bool active = wb < n;
-    if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp);
+    if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp, S.cond, S.ncond);
     const int64_t gn = g + gridDim.x;
     const bool more = gn < tot;
     int sn = s;
@@ -836,8 +839,6 @@ PushF32 make_push_f32(Ctx* c, int s, double kv, double kx) {
   a.nx = d.nx; a.ny = d.ny; a.bc = d.bc;
   a.Q = S.Q;
   a.errc = c->errc;
-  a.cond = c->cond.table;
-  a.ncond = c->cond.active();
   return a;
 }
 
@@ -862,14 +863,16 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   const bool cond = c->cond.any();
   if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
     const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2);
+    const CondDev* ct = c->cond.table;
+    const int nct = c->cond.active();
     if (cond) {
       // with conductors: the default shape with the conductor test
       if (deposit) {
         auto k = push_f32_kernel<true, 4, 1, true>;
-        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map);
+        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct);
       } else {
         auto k = push_f32_kernel<false, 4, 1, true>;
-        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map);
+        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct);
       }
       return hipGetLastError();
     }
@@ -880,10 +883,10 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   case ID:                                                                                         \
     if (deposit) {                                                                                 \
       auto k = push_f32_kernel<true, PPT, WV>;                                                     \
-      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map); \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct); \
     } else {                                                                                       \
       auto k = push_f32_kernel<false, PPT, WV>;                                                    \
-      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map); \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct); \
     }                                                                                              \
     break;
       EPIC_V(1, 8, 5)
@@ -942,6 +945,8 @@ hipError_t launch_push_all(Ctx* c) {
     push_consts(c, s, 1, false, &c1, &c2, &c3);
     S.sp[S.nsp++] = make_push_f32(c, s, c1 * c2, c3 * c2);
   }
+  S.cond = c->cond.table;
+  S.ncond = c->cond.active();
   auto k = c->cond.any() ? push_f32_multi_kernel<4, true> : push_f32_multi_kernel<4, false>;
   hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, S);
   return hipGetLastError();

@@ -145,8 +145,6 @@ struct PushF32 {
   int nx, ny, bc;
   unsigned long long* Q;
   unsigned long long* errc;
-  const CondDev* cond;  // absorbing conductors (the first ncond entries are active)
-  int ncond;
 };
 
 // Bilinear interpolation of node values e00, e10 (i+1), e01 (j+1), e11 over
@@ -207,11 +205,14 @@ __device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, in
 
 // Gather, push, walls and store of a loaded chunk; fills the deposit
 // information of the new positions (DEP).  COND: the walls are followed by
-// the conductor test (a separate instantiation, so the kernels without
-// conductors keep their code).
+// the conductor test against the first ncond shapes of cond (a separate
+// instantiation, so the kernels without conductors keep their code; the
+// table is an argument of its own, not part of PushF32, whose size the
+// batched kernel's descriptor loads depend on).
 template <bool DEP, int PPT, bool COND = false>
 __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, int64_t n, int lane,
-                                                ChunkRegs<PPT>& r, ChunkDep<PPT>& dp) {
+                                                ChunkRegs<PPT>& r, ChunkDep<PPT>& dp,
+                                                const CondDev* cond = nullptr, int ncond = 0) {
   constexpr int V = 4;
   const int nyc = a.ny - 1;
   const float fmax_i = (float)(a.nx - 1), fmax_j = (float)(a.ny - 1);
@@ -284,7 +285,7 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
       if (py < a.y0) { py = a.y0 + (a.y0 - py); r.vy[k] = -r.vy[k]; }
       else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); r.vy[k] = -r.vy[k]; }
     }
-    if (COND && alive && in_conductor(a.cond, a.ncond, (double)px, (double)py)) {
+    if (COND && alive && in_conductor(cond, ncond, (double)px, (double)py)) {
       alive = false;
       px = __builtin_nanf("");
     }
