@@ -1,0 +1,97 @@
+"""Writes the committed fixtures of tests/golden/ (numpy .npz, no pickles).
+
+    python tests/golden/make_fixtures.py
+
+Sources:
+  ref_pcg_9x7.npz, ref_pcg_12x12.npz
+      the reference's own dense pcg() (linearalgebra/solvers.h:72-100, compiled
+      from the reference tree into oracle/_ref by oracle/Makefile) on seeded
+      right-hand sides: the only part of the path the reference's code can
+      compute here (its physics sources include output.h -> H5Cpp.h)
+  ref_nrpcg_11x9.npz
+      the Newton-CG loop of nrpcgsolver (field.cpp:141-210) on the reference's
+      own dense Matrix / vec / cg() (oracle/_ref)
+  mt19937_uniform_1000.npz
+      the first 1000 draws of the reference's global stream (mt19937 seeded 0,
+      uniform_real_distribution<double>(0, 1), domain.h:218-231), from the
+      oracle; pinned by the C++ standard's 10000th-output value
+  cross_sections_64.npz
+      the e-Ar cross sections of cross_section.cpp:4-39 at 64 log-spaced
+      energies, from the oracle's restatement
+  loop_16x16_pbc_cg.npz
+      the reference loop (main.cpp:366-423) from the mt19937(0) load of a
+      16 x 16-cell periodic electron-ion plasma, 2 x 1000 particles, reference
+      PCG at 1e-13 absolute: rho, phi, Ex, Ey after each of 10 steps, the
+      particles after the last, and the PCG iteration counts, from the oracle
+      (the oracle's PCG equals oracle/_ref's bit for bit)
+The ref_* files need the reference tree at generation time only; the tests
+read the committed files.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from epic2d_mcc_amd import _lib  # noqa: E402
+from harness import small_setup  # noqa: E402
+from oracle_api import OracleSim, dp, oracle, ref_pcg_lib  # noqa: E402
+
+
+def rhs(nx, ny, seed):
+    """pcgsolver's right-hand side form: zero on the boundary nodes."""
+    b = np.random.default_rng(seed).normal(size=(nx, ny))
+    b[0, :] = b[-1, :] = b[:, 0] = b[:, -1] = 0.0
+    return np.ascontiguousarray(b)
+
+
+def main():
+    import ctypes
+    r = ref_pcg_lib()
+    if r is None:
+        sys.exit("the reference tree is needed to build oracle/_ref (make -C oracle)")
+    for nx, ny, seed in ((9, 7, 1), (12, 12, 2)):
+        b = rhs(nx, ny, seed)
+        phi = np.empty(nx * ny)
+        err = ctypes.c_double()
+        r.ref_pcg(nx, ny, dp(b), dp(phi), 10000, 1e-10, ctypes.byref(err))
+        np.savez(os.path.join(HERE, f"ref_pcg_{nx}x{ny}.npz"), nx=nx, ny=ny, b=b.ravel(), phi=phi,
+                 max_iteration=10000, tolerance=1e-10)
+    st = small_setup(N=1, Nx=10, Ny=8, solver=_lib.SOLVER_NEWTON_CG)
+    d = st.domain
+    rho = np.ascontiguousarray(np.random.default_rng(3).normal(scale=0.3, size=(d.nx, d.ny)))
+    phi = np.empty(d.nx * d.ny)
+    s = (d.L * d.L) / (d.LDe * d.LDe)
+    r.ref_nrpcg(d.nx, d.ny, dp(rho), d.dx * d.dx, s, dp(phi))
+    np.savez(os.path.join(HERE, f"ref_nrpcg_{d.nx}x{d.ny}.npz"), nx=d.nx, ny=d.ny, rho=rho.ravel(), phi=phi,
+             dx2=d.dx * d.dx, s=s)
+
+    u = np.empty(1000)
+    oracle().orc_rnd_draws(0, 1000, dp(u))
+    np.savez(os.path.join(HERE, "mt19937_uniform_1000.npz"), u=u)
+
+    e = np.logspace(-2, 3, 64)
+    o = oracle()
+    sig = np.array([[o.orc_sigma_elastic(x), o.orc_sigma_excitation(x), o.orc_sigma_ionization(x)] for x in e])
+    np.savez(os.path.join(HERE, "cross_sections_64.npz"), energy_ev=e, sigma=sig)
+
+    st = small_setup(N=1000, Nx=16, Ny=16, bc="pbc", tol=1e-13)
+    o = OracleSim(st.domain, st.species)
+    for s_ in range(2):
+        o.init_species(s_, 0)
+    o.initial_field()
+    fields, iters = [], []
+    for ts in range(10):
+        iters.append(o.step())
+        fields.append(np.stack([o.grid(w) for w in range(4)]))
+    parts = [np.stack(o.particles(s_)) for s_ in range(2)]
+    np.savez(os.path.join(HERE, "loop_16x16_pbc_cg.npz"), fields=np.stack(fields), iters=np.array(iters),
+             electrons=parts[0], ions=parts[1])
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
