@@ -208,3 +208,28 @@ def test_windowed_fp64_shapes_identical(env):
         for k in range(4):
             assert np.array_equal(outs[0][0][s][k], outs[1][0][s][k]), (s, k)
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("Nx,N", [(16, 2000), (128, 60000)])
+def test_sub_cycled_push_bitwise(Nx, N):
+    """Push_species_serial(sub_cycle, ...) with sub_cycle = 3 (the ion
+    sub-cycling main.cpp parses but never uses, main.cpp:237,420): the step is
+    DT*sub_cycle in both updates (species.cpp:44-51); from the oracle's own E
+    every particle is bit-identical, on the whole-grid and the windowed
+    kernel."""
+    print(f"[parity] sub-cycled push {Nx}", flush=True)
+    st = small_setup(N=N, Nx=Nx, Ny=Nx, bc="open", tol=1e-10, vsx_beam=2.0)
+    o, ctx = _pair(st)
+    for s in range(2):
+        o.scatter(s)
+    o.compute_rho()
+    o.solve()
+    o.efield()
+    ctx.set_field(2, o.grid(2))
+    ctx.set_field(3, o.grid(3))
+    for ts in range(2):
+        ctx.push(1, 3)
+        o.push(1, 3)
+        ctx.push(0, 1)
+        o.push(0, 1)
+        _exact_particles(o, ctx, 2, f"sub-cycled push {ts}")
