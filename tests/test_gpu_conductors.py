@@ -219,3 +219,18 @@ def test_fp32_conductor_kernels_identical(bc):
     for s in range(2):
         c = o.particles(s)
         assert abs(len(a[s][0]) - len(c[0])) <= 3
+
+
+def test_batch_member_rejects_conductors():
+    """The batched kernels have no conductor test, so a batch member refuses
+    a conductor list (an empty one is accepted)."""
+    print("[conductors] batch member", flush=True)
+    st = small_setup(N=1000, Nx=32, Ny=32, precision=_lib.FP32, solver=_lib.SOLVER_PCG_MIXED, tol=1e-10)
+    st.domain.tol_relative = 1
+    st.domain.precond = _lib.PRECOND_SPECTRAL
+    B = _lib.Batch()
+    mid, m = B.add(st.domain, st.species)
+    with pytest.raises(_lib.EpicError, match="EPIC_ERR_ARG"):
+        m.set_conductors(_lib.conductors([RECT]))
+    m.set_conductors(_lib.conductors([]))
+    B.close()
