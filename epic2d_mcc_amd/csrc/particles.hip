@@ -651,6 +651,13 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
 // software pipeline of push_f32_kernel, the next chunk possibly of the next
 // species.  Per-particle arithmetic is push_f32_kernel's.
 constexpr int PUSH_MAXSP = 8;
+
+// A wavefront-uniform 64-bit value moved into scalar registers.
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(uint64_t)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 struct PushF32Set {
   PushF32 sp[PUSH_MAXSP];
   int nsp;
@@ -669,17 +676,20 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S
   __shared__ int64_t ns[PUSH_MAXSP];
   if (threadIdx.x < PUSH_MAXSP) ns[threadIdx.x] = (int)threadIdx.x < S.nsp ? *S.sp[threadIdx.x].d_n : 0;
   __syncthreads();
-  int64_t cend[PUSH_MAXSP];
-  int64_t tot = 0;
+  // the chunk ranges are the same for every lane: read into scalar
+  // registers as 32-bit chunk counts (a species holds fewer than 2^31 * CH
+  // slots), so that locating a chunk's species is scalar work
+  int cend[PUSH_MAXSP];
+  int tot = 0;
 #pragma unroll
   for (int k = 0; k < PUSH_MAXSP; k++) {
-    tot += (ns[k] + CH - 1) / CH;
+    tot += (int)((uniform_i64(ns[k]) + CH - 1) / CH);
     cend[k] = tot;
   }
   // species and first slot of global chunk g (wavefront-uniform)
-  auto locate = [&](int64_t g, int& sp, int64_t& wb, int64_t& n) {
+  auto locate = [&](int g, int& sp, int64_t& wb, int64_t& n) {
     int s = 0;
-    int64_t beg = 0;
+    int beg = 0;
 #pragma unroll
     for (int k = 0; k + 1 < PUSH_MAXSP; k++)
       if (g >= cend[k]) {
@@ -687,10 +697,10 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S
         beg = cend[k];
       }
     sp = s;
-    n = ns[s];
-    wb = (g - beg) * CH + (int64_t)wid * CHW;
+    n = uniform_i64(ns[s]);
+    wb = (int64_t)(g - beg) * CH + (int64_t)wid * CHW;
   };
-  int64_t g = blockIdx.x;
+  int g = blockIdx.x;
   if (g >= tot) return;
   int s;
   int64_t wb, n;
@@ -702,7 +712,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S
     ChunkDep<PPT> dp;
     const bool active = wb < n;
     if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp, S.cond, S.ncond);
-    const int64_t gn = g + gridDim.x;
+    const int gn = g + (int)gridDim.x;
     const bool more = gn < tot;
     int sn = s;
     int64_t wbn = 0, nn = 0;
