@@ -541,7 +541,7 @@ def main():
         main_single(args, st, "periodic 256x256 cells, electron + Ar+ at 1e15 m^-3, 200 ppc each (26,214,400), "
                               "FP32, Ar neutrals 1e21 m^-3 at 300 K (null-collision MCC, e-Ar elastic/excitation/"
                               "ionisation, Ar+-Ar elastic/CX), mixed Poisson 1e-10, sort every 20 steps", "fp32", 32,
-                    "push_f32_multi_kernel<4,false> (gather + push + wrap + deposit, all species in one launch)", mcc=mcc, gas=(1, 1, 1, 1e21),
+                    "push_f32_multi_kernel<4,false,8> (gather + push + wrap + deposit, all species in one launch)", mcc=mcc, gas=(1, 1, 1, 1e21),
                     cpu_note=", electron collisions by the reference's direct method")
         return
     if args.precision == "fp64":
@@ -589,7 +589,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": PUSH_TRAFFIC_BYTES_PER_PARTICLE * parts_per_launch,
-                     "kernel": "push_f32_multi_kernel<4,false> (CIC gather + leapfrog push + walls + deposit, all species in one launch)",
+                     "kernel": "push_f32_multi_kernel<4,false,8> (CIC gather + leapfrog push + walls + deposit, all species in one launch)",
                      "bytes_per_particle_step": bytes_per_ps, "avg_launch_ms": avg_launch_s * 1e3,
                      "particles_per_launch": parts_per_launch},
         "device_errors": r["errors"],

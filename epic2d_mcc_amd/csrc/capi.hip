@@ -204,6 +204,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
   if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
+  if (const char* v = getenv("EPIC_PUSH_MULTI_WAVES")) c->push_multi_waves = atoi(v);
   if (const char* v = getenv("EPIC_SPEC_MFMA")) c->spec_mfma = atoi(v);
   if (const char* v = getenv("EPIC_PCG_BLOCK")) c->pcg_block = atoi(v);
   // many-workgroup PCG / SOR: one workgroup per two CUs.  Every iteration

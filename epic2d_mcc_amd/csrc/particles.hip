@@ -665,8 +665,8 @@ struct PushF32Set {
   int ncond;
 };
 
-template <int PPT, bool COND>
-__global__ __launch_bounds__(PART_BLOCK) void push_f32_multi_kernel(PushF32Set S) {
+template <int PPT, bool COND, int WAVES>
+__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_multi_kernel(PushF32Set S) {
   __shared__ unsigned long long lds[(PART_BLOCK / 64) * WAVE_WIN];
   constexpr int CHW = 64 * PPT;
   constexpr int CH = PART_BLOCK * PPT;
@@ -957,7 +957,9 @@ hipError_t launch_push_all(Ctx* c) {
   }
   S.cond = c->cond.table;
   S.ncond = c->cond.active();
-  auto k = c->cond.any() ? push_f32_multi_kernel<4, true> : push_f32_multi_kernel<4, false>;
+  // WAVES = 8 caps the registers at 64 (eight wavefronts per SIMD)
+  auto k = c->cond.any() ? (c->push_multi_waves == 8 ? push_f32_multi_kernel<4, true, 8> : push_f32_multi_kernel<4, true, 1>)
+                         : (c->push_multi_waves == 8 ? push_f32_multi_kernel<4, false, 8> : push_f32_multi_kernel<4, false, 1>);
   hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, S);
   return hipGetLastError();
 }

@@ -102,7 +102,8 @@ def _run_env(st, parts, env, steps=3):
 _SHAPES = [("open", {"EPIC_PUSH_MULTI": "0"}), ("open", {"EPIC_PUSH_VARIANT": "1"}),
            ("open", {"EPIC_PUSH_VARIANT": "2"}), ("open", {"EPIC_PUSH_VARIANT": "3"}),
            ("open", {"EPIC_PUSH_VARIANT": "4"}), ("open", {"EPIC_PUSH_MAP": "0"}),
-           ("open", {"EPIC_PUSH_GRID": "2"}), ("pbc", {"EPIC_PUSH_MULTI": "0"}),
+           ("open", {"EPIC_PUSH_GRID": "2"}), ("open", {"EPIC_PUSH_MULTI_WAVES": "1"}),
+           ("pbc", {"EPIC_PUSH_MULTI": "0"}),
            ("reflective", {"EPIC_PUSH_MULTI": "0"})]
 
 
