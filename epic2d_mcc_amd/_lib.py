@@ -27,6 +27,7 @@ FP64, FP32 = 0, 1
 NERR = 8
 NSTAGE = 8
 STAGES = ["deposit", "rho", "solve", "efield", "push", "collide", "sort", "inject"]
+TIMING_ONLY = 0x10000
 MCC_E_ELASTIC, MCC_E_EXCITATION, MCC_E_IONIZATION, MCC_I_ELASTIC, MCC_I_CX = range(5)
 MCC_NPROC = 5
 
@@ -372,8 +373,12 @@ class Context:
     def sync(self):
         self._c(self.L.epic_sync(self.h), "sync")
 
-    def timing(self, on=True):
-        self._c(self.L.epic_timing_enable(self.h, int(on)), "timing")
+    def timing(self, on=True, stages=None):
+        """Stage timers on (every stage, or only the named ones) or off."""
+        v = int(bool(on))
+        if on and stages is not None:
+            v = TIMING_ONLY | sum(1 << STAGES.index(n) for n in stages)
+        self._c(self.L.epic_timing_enable(self.h, v), "timing")
 
     def timing_read(self):
         ms = np.zeros(NSTAGE)

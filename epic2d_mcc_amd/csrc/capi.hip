@@ -75,8 +75,8 @@ static void resolve_events(Ctx* c) {
   c->ev_used = 0;
 }
 
-void stage_begin(Ctx* c) {
-  if (!c->timing) return;
+void stage_begin(Ctx* c, int stage) {
+  if (!c->timing || !((c->timing_mask >> stage) & 1u)) return;
   if (c->ev_used * 2 + 2 > (int)c->ev_pool.size()) resolve_events(c);
   if (c->ev_used * 2 + 2 > (int)c->ev_pool.size()) return;
   hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream);
@@ -106,6 +106,8 @@ static void free_all(Ctx* c) {
                 c->cond.table, c->cond.node, c->cond.value, c->nr_ws};
   for (void* p : gs)
     if (p) hipFree(p);
+  if (c->hscal) hipHostFree(c->hscal);
+  c->hscal = nullptr;
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (hipEvent_t ev : c->ev_pool) hipEventDestroy(ev);
@@ -457,7 +459,7 @@ int epic_scatter(epic_ctx* h, int s) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
   if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
-  stage_begin(c);
+  stage_begin(c, 0);
   CK(launch_zero_Q(c), "zero");
   CK(launch_deposit(c, s), "deposit");
   CK(launch_den_only(c, s), "den");
@@ -469,7 +471,7 @@ int epic_scatter(epic_ctx* h, int s) {
 int epic_compute_rho(epic_ctx* h) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
-  stage_begin(c);
+  stage_begin(c, 1);
   CK(launch_den_only(c, -1), "rho");
   stage_end(c, 1);
   return EPIC_OK;
@@ -480,7 +482,7 @@ int epic_solve_potential(epic_ctx* h, int* iters, double* err) {
   Ctx* c = &h->c;
   int it = 0;
   double e = 0;
-  stage_begin(c);
+  stage_begin(c, 2);
   CK(launch_solve(c, &it, &e), "solve");
   stage_end(c, 2);
   if (iters) *iters = it;
@@ -491,7 +493,7 @@ int epic_solve_potential(epic_ctx* h, int* iters, double* err) {
 int epic_efield(epic_ctx* h) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
-  stage_begin(c);
+  stage_begin(c, 3);
   CK(launch_efield(c), "efield");
   stage_end(c, 3);
   return EPIC_OK;
@@ -501,7 +503,7 @@ int epic_push(epic_ctx* h, int s, int sub_cycle) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
   if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
-  stage_begin(c);
+  stage_begin(c, 4);
   CK(launch_push(c, s, sub_cycle, false, 0.0), "push");
   stage_end(c, 4);
   c->deposit_valid = false;
@@ -631,7 +633,7 @@ int epic_set_conductors(epic_ctx* h, const epic_conductor_desc* cd, int n) {
 int epic_inject(epic_ctx* h, int step) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
-  stage_begin(c);
+  stage_begin(c, 7);
   CK(launch_inject(c, step), "inject");
   stage_end(c, 7);
   return EPIC_OK;
@@ -689,7 +691,7 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
 int epic_collide(epic_ctx* h, int step) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
-  stage_begin(c);
+  stage_begin(c, 5);
   CK(launch_mcc(c, step), "collide");
   stage_end(c, 5);
   return EPIC_OK;
@@ -713,7 +715,7 @@ int epic_sort(epic_ctx* h, int s) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
   if (s < 0 || s >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
-  stage_begin(c);
+  stage_begin(c, 6);
   CK(sync_counts(c), "counts");
   CK(launch_sort(c, s), "sort");
   stage_end(c, 6);
@@ -735,45 +737,45 @@ int epic_step(epic_ctx* h, int ts, int* iters) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
   if (!c->emitters.empty()) {
-    stage_begin(c);
+    stage_begin(c, 7);
     CK(launch_inject(c, ts), "inject");
     stage_end(c, 7);
   }
   if (!c->deposit_valid) {
-    stage_begin(c);
+    stage_begin(c, 0);
     CK(launch_zero_Q(c), "zero");
     for (int s = 0; s < c->nsp; s++) CK(launch_deposit(c, s), "deposit");
     stage_end(c, 0);
   }
-  stage_begin(c);
+  stage_begin(c, 1);
   CK(launch_rho_finalize(c), "rho");
   stage_end(c, 1);
   int it = 0;
   double err = 0;
-  stage_begin(c);
+  stage_begin(c, 2);
   CK(launch_solve(c, &it, &err), "solve");
   stage_end(c, 2);
   if (iters) *iters = it;
-  stage_begin(c);
+  stage_begin(c, 3);
   CK(launch_efield(c), "efield");
   stage_end(c, 3);
   // every pushed species in one launch where the FP32 hot kernel applies
   const bool all = push_all_ok(c);
   if (all) {
-    stage_begin(c);
+    stage_begin(c, 4);
     CK(launch_push_all(c), "push");
     stage_end(c, 4);
   }
   for (int s = 0; s < c->nsp; s++) {
     if (all && c->sp[s].desc.pushed) continue;
-    stage_begin(c);
+    stage_begin(c, 4);
     if (c->sp[s].desc.pushed) CK(launch_push(c, s, 1, true, 0.0), "push");
     else CK(launch_deposit(c, s), "deposit");
     stage_end(c, 4);
   }
   c->deposit_valid = true;
   if (c->mcc.on) {
-    stage_begin(c);
+    stage_begin(c, 5);
     CK(launch_mcc(c, ts), "collide");
     stage_end(c, 5);
   }
@@ -781,7 +783,7 @@ int epic_step(epic_ctx* h, int ts, int* iters) {
   bool any_due = false;
   for (int s = 0; s < c->nsp; s++) any_due = any_due || c->sort_due(s);
   if (any_due) {
-    stage_begin(c);
+    stage_begin(c, 6);
     CK(sync_counts(c), "counts");
     for (int s = 0; s < c->nsp; s++)
       if (c->sort_due(s)) CK(launch_sort(c, s), "sort");
@@ -913,6 +915,8 @@ int epic_timing_enable(epic_ctx* h, int on) {
   c->ev_used = 0;
   c->ev_open = -1;
   c->timing = on != 0;
+  // EPIC_TIMING_ONLY | stage bits: only those stages record events
+  c->timing_mask = (on & EPIC_TIMING_ONLY) ? (unsigned)on & ((1u << EPIC_NSTAGE) - 1) : ~0u;
   for (int k = 0; k < EPIC_NSTAGE; k++) {
     c->stage_ms[k] = 0;
     c->stage_calls[k] = 0;

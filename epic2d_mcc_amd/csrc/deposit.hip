@@ -49,9 +49,35 @@ __global__ void rho_finalize_kernel(FinArgs a) {
   }
 }
 
-__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-    p[k] = 0ull;
+// Clears the accumulators of every species in mask in one launch.  Not
+// folded into rho_finalize_kernel: under periodic walls a node's fold reads
+// its images, which other threads read too.
+struct ZeroQArgs {
+  unsigned long long* Q[MAXSP];
+  int nsp;
+  unsigned mask;
+  int64_t nn;
+};
+
+__global__ void zero_q_kernel(ZeroQArgs a) {
+  for (int s = 0; s < a.nsp; s++) {
+    if (!(a.mask & (1u << s))) continue;
+    unsigned long long* q = a.Q[s];
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < a.nn; k += (int64_t)gridDim.x * blockDim.x)
+      q[k] = 0ull;
+  }
+}
+
+static hipError_t zero_q(Ctx* c, unsigned mask) {
+  if (!mask) return hipSuccess;
+  ZeroQArgs a{};
+  a.nsp = c->nsp;
+  a.mask = mask;
+  a.nn = (int64_t)c->dom.nx * c->dom.ny;
+  for (int s = 0; s < c->nsp; s++) a.Q[s] = c->sp[s].Q;
+  const int grid = (int)std::min<int64_t>((a.nn + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_q_kernel, dim3(grid), dim3(256), 0, c->stream, a);
+  return hipGetLastError();
 }
 
 static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
@@ -76,12 +102,7 @@ static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
   int grid = (int)std::min<int64_t>((nn + 255) / 256, 4096);
   hipLaunchKernelGGL(rho_finalize_kernel, dim3(grid), dim3(256), 0, c->stream, a);
   EPIC_HIP_TRY(hipGetLastError());
-  for (int s = 0; s < c->nsp; s++) {
-    if (!(mask & (1u << s))) continue;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid), dim3(256), 0, c->stream, c->sp[s].Q, nn);
-    EPIC_HIP_TRY(hipGetLastError());
-  }
-  return hipSuccess;
+  return zero_q(c, mask);
 }
 
 // All species' Q -> den and rho (the fused-step path).
@@ -99,13 +120,9 @@ hipError_t launch_den_only(Ctx* c, int s) {
 }
 
 hipError_t launch_zero_Q(Ctx* c) {
-  const int64_t nn = (int64_t)c->dom.nx * c->dom.ny;
-  int grid = (int)std::min<int64_t>((nn + 255) / 256, 4096);
-  for (int s = 0; s < c->nsp; s++) {
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid), dim3(256), 0, c->stream, c->sp[s].Q, nn);
-    EPIC_HIP_TRY(hipGetLastError());
-  }
-  return hipSuccess;
+  unsigned mask = 0;
+  for (int s = 0; s < c->nsp; s++) mask |= 1u << s;
+  return zero_q(c, mask);
 }
 
 }  // namespace epic

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/epic2d.h"
@@ -114,6 +115,14 @@ struct Conductors {
   int active() const { return nrect + (elec_live ? nelec : 0); }
 };
 
+// Host-visible result of a device reduction: the values are stored first,
+// seq last with a system-scope release, so a host that reads seq (acquire)
+// sees the values of that launch.
+struct HostScal {
+  unsigned long long bb, rr;  // bit patterns of two doubles
+  unsigned long long seq;
+};
+
 struct Ctx {
   epic_domain_desc dom{};
   int nsp = 0;
@@ -141,8 +150,14 @@ struct Ctx {
   void* ws_x = nullptr;   // nnodes of T (inner solution)
   double* ws_part = nullptr;  // per-block partial sums (3 * max_blocks)
   double* ws_scalar = nullptr;  // solver scalars written back to host
+  // pinned host words of the mixed solver's convergence check (written by
+  // the device, polled by the host) and the last sequence number used
+  HostScal* hscal = nullptr;
+  unsigned long long hscal_seq = 0;
   int* ws_iter = nullptr;
   int max_solver_blocks = 0;
+  // co-resident workgroups per CU of each particle kernel (queried once)
+  std::unordered_map<const void*, int> occupancy;
   // spectral solver workspace
   double* sp_S = nullptr;       // sine-transform matrices
   double* sp_T = nullptr;       // temporaries
@@ -196,6 +211,7 @@ struct Ctx {
   // timing: event pairs recorded on the stream without synchronising,
   // resolved when read
   bool timing = false;
+  unsigned timing_mask = ~0u;  // stages that record events while timing is on
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_stage;  // stage of each recorded pair
   int ev_used = 0;
@@ -248,7 +264,7 @@ hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int 
 // helpers (capi.hip)
 int64_t live_count(Ctx* c, int s, hipError_t* e);
 hipError_t sync_counts(Ctx* c);
-void stage_begin(Ctx* c);
+void stage_begin(Ctx* c, int stage);
 void stage_end(Ctx* c, int stage);
 // context creation on an existing stream (ext non-null: not owned)
 int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, hipStream_t ext, epic_ctx** out);

@@ -106,16 +106,26 @@ struct Stencil {
   }
 };
 
-// r_k = b_k - (A phi)_k over the unknowns, 0 elsewhere.
-__device__ __forceinline__ double residual_at(int op, const double* b, const double* phi, int k, int nx, int ny) {
-  auto pv = [&](int m) -> double { return phi[m]; };
+// r_k = b_k - (A phi)_k over the unknowns, 0 elsewhere; pv(m) reads phi_m.
+template <class PV>
+__device__ __forceinline__ double residual_with(int op, const double* b, PV pv, int k, int nx, int ny) {
   if (op == EPIC_OP_REF_GHOST) {
     const Stencil<EPIC_OP_REF_GHOST> S{nx, ny};
-    return b[k] - S.apply(k, phi[k], pv);
+    return b[k] - S.apply(k, pv(k), pv);
   }
   const Stencil<EPIC_OP_DIRICHLET_WALL> S{nx, ny};
   if (!S.unknown(k)) return 0.0;
-  return b[k] - S.apply(k, phi[k], pv);
+  return b[k] - S.apply(k, pv(k), pv);
+}
+
+__device__ __forceinline__ double residual_at(int op, const double* b, const double* phi, int k, int nx, int ny) {
+  return residual_with(op, b, [&](int m) -> double { return phi[m]; }, k, nx, ny);
+}
+
+// The residual of phi = 0, in the same arithmetic (so the same bits as
+// residual_at over a zeroed phi).
+__device__ __forceinline__ double residual_at_zero(int op, const double* b, int k, int nx, int ny) {
+  return residual_with(op, b, [](int) -> double { return 0.0; }, k, nx, ny);
 }
 
 // FieldSolve::CalculateEfield (field.cpp:276-331) at node (i, j): central

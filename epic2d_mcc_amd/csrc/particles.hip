@@ -777,9 +777,13 @@ static void push_consts(const Ctx* c, int s, int sub_cycle, bool rewind, double*
 // 8 XCDs, so the XCD-contiguous chunk walk applies).
 template <typename K>
 static int particle_grid(Ctx* c, K kernel) {
+  const void* key = (const void*)kernel;
+  auto it = c->occupancy.find(key);
+  if (it != c->occupancy.end()) return c->num_cu * it->second;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, PART_BLOCK, 0) != hipSuccess || per_cu < 1)
     per_cu = 2;
+  c->occupancy[key] = per_cu;
   return c->num_cu * per_cu;
 }
 

@@ -18,6 +18,8 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -66,20 +68,23 @@ __global__ void hadamard_kernel(T* Y, const T* G, int n) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) Y[k] *= G[k];
 }
 
-__global__ void zero_kernel(double* p, int n) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0.0;
-}
-
 // residual r = b - A phi over the unknowns (FP64) and ||r||^2 partials;
 // X (when given): r packed in FP32 as the transform's operand
 template <int OP>
 __global__ void spec_residual_kernel(const double* b, const double* phi, double* r, int nx, int ny, double* part,
-                                     float* X, SpecLayout L) {
+                                     float* X, SpecLayout L, double* zero_phi) {
   __shared__ double sh[64];
   const int n = nx * ny;
   double acc = 0;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const double rk = residual_at(OP, b, phi, k, nx, ny);
+    // zero_phi: the solve's start, phi = 0 (written here, not read)
+    double rk;
+    if (zero_phi) {
+      rk = residual_at_zero(OP, b, k, nx, ny);
+      zero_phi[k] = 0.0;
+    } else {
+      rk = residual_at(OP, b, phi, k, nx, ny);
+    }
     r[k] = rk;
     acc += rk * rk;
     if (X) {
@@ -92,11 +97,23 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
-__global__ void sum2_kernel(const double* part, int n, double* out) {
+// hs (when given): *bb_dev and this sum, then seq, to pinned host memory,
+// for the host's polling wait.
+__global__ void sum2_kernel(const double* part, int n, double* out, HostScal* hs, const double* bb_dev,
+                            unsigned long long seq) {
   double s = 0;
   for (int b = threadIdx.x; b < n; b += 64) s += part[b];
   s = wave_sum(s);
-  if (threadIdx.x == 0) *out = s;
+  if (threadIdx.x == 0) {
+    *out = s;
+    if (hs) {
+      __hip_atomic_store(&hs->bb, (unsigned long long)__double_as_longlong(*bb_dev), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&hs->rr, (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&hs->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
 }
 
 struct SpecState {
@@ -333,21 +350,48 @@ static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwr
   return hipGetLastError();
 }
 
+// zero_phi: phi is set to 0 by the same kernel and the residual is that of
+// phi = 0 (phi itself is not read).  seq > 0: the sum and *bb_dev also go to
+// the pinned host words under that sequence number.
 static hipError_t residual_norm2(Ctx* c, const double* b, const double* phi, double* r, double* out_dev,
-                                 float* X = nullptr) {
+                                 float* X = nullptr, double* zero_phi = nullptr, const double* bb_dev = nullptr,
+                                 unsigned long long seq = 0) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
   const int g = std::min((n + 255) / 256, 1024);
   const SpecLayout L = c->blas ? ((SpecState*)c->blas)->L : SpecLayout{};
   if (d.poisson_op == EPIC_OP_REF_GHOST)
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_REF_GHOST>, dim3(g), dim3(256), 0, c->stream, b, phi, r, d.nx,
-                       d.ny, c->ws_part, X, L);
+                       d.ny, c->ws_part, X, L, zero_phi);
   else
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_DIRICHLET_WALL>, dim3(g), dim3(256), 0, c->stream, b, phi, r,
-                       d.nx, d.ny, c->ws_part, X, L);
+                       d.nx, d.ny, c->ws_part, X, L, zero_phi);
   EPIC_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(64), 0, c->stream, c->ws_part, g, out_dev);
+  hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(64), 0, c->stream, c->ws_part, g, out_dev,
+                     seq ? c->hscal : nullptr, bb_dev, seq);
   return hipGetLastError();
+}
+
+// Waits until the pinned host words carry seq and reads them.  A stream
+// synchronisation spins only briefly and then sleeps on an interrupt, whose
+// wake-up costs tens of microseconds once per solve; polling the words
+// costs one PCIe read per poll.  After 50 ms the wait falls back to a stream
+// synchronisation (which also reports a failed launch).
+static hipError_t wait_host_scalars(Ctx* c, unsigned long long seq, double* bb, double* rr) {
+  HostScal* h = c->hscal;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned spin = 1; __atomic_load_n(&h->seq, __ATOMIC_ACQUIRE) != seq; spin++) {
+    if ((spin & 255u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
+      if (__atomic_load_n(&h->seq, __ATOMIC_ACQUIRE) != seq) return hipErrorUnknown;
+      break;
+    }
+  }
+  const unsigned long long b = __atomic_load_n(&h->bb, __ATOMIC_RELAXED);
+  const unsigned long long r = __atomic_load_n(&h->rr, __ATOMIC_RELAXED);
+  std::memcpy(bb, &b, sizeof(double));
+  std::memcpy(rr, &r, sizeof(double));
+  return hipSuccess;
 }
 
 // Solve A phi = b to ||b - A phi|| <= tol ||b||.  fp32_inner: the corrections
@@ -357,14 +401,15 @@ static hipError_t residual_norm2(Ctx* c, const double* b, const double* phi, dou
 hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32_inner, double tol, int max_ref,
                                  int* solves, double* relres) {
   EPIC_HIP_TRY(spec_init(c));
+  if (!c->hscal) {
+    EPIC_HIP_TRY(hipHostMalloc((void**)&c->hscal, sizeof(HostScal), hipHostMallocCoherent));
+    std::memset(c->hscal, 0, sizeof(HostScal));
+  }
   SpecState* S = (SpecState*)c->blas;
-  const int n = c->dom.nx * c->dom.ny;
-  const int grid = std::min((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(256), 0, c->stream, phi, n);
-  EPIC_HIP_TRY(hipGetLastError());
   double* dscal = c->ws_scalar + 4;  // [4] ||b||^2, [5] ||r||^2
   float* Xpack = fp32_inner && spec_mfma_ok(c) ? S->X32 : nullptr;
-  EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal, Xpack));  // phi = 0: r = b
+  // phi := 0 and r = b in one kernel (the corrections write phi's interior)
+  EPIC_HIP_TRY(residual_norm2(c, b, nullptr, c->r64, dscal, Xpack, phi));
   int k = 0;
   double host[2] = {0, 0};
   for (; k < max_ref; k++) {
@@ -372,12 +417,13 @@ hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32
       EPIC_HIP_TRY(spec_apply<float>(c, c->r64, phi, k == 0, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32));
     else
       EPIC_HIP_TRY(spec_apply<double>(c, c->r64, phi, k == 0, S->Sp64, S->Sq64, S->G64, S->X64, S->Y64));
-    EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal + 1, Xpack));
     // the FP32 correction gains ~6 digits: two applications reach 1e-10 and
     // more; check the true residual after the second
-    if (k + 1 >= (fp32_inner ? 2 : 1)) {
-      EPIC_HIP_TRY(hipMemcpyAsync(host, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
+    const bool check = k + 1 >= (fp32_inner ? 2 : 1);
+    const unsigned long long seq = check ? ++c->hscal_seq : 0;
+    EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal + 1, Xpack, nullptr, dscal, seq));
+    if (check) {
+      EPIC_HIP_TRY(wait_host_scalars(c, seq, &host[0], &host[1]));
       if (sqrt(host[1]) <= tol * sqrt(host[0]) || host[1] == 0) {
         k++;
         break;

@@ -253,6 +253,10 @@ int epic_clear_error_counters(epic_ctx* ctx);
  * 0 deposit, 1 rho, 2 solve, 3 efield, 4 push(+deposit, one pair per species
  * launch), 5 collide, 6 sort, 7 inject. */
 #define EPIC_NSTAGE 8
+/* on: 0 off, 1 every stage, or EPIC_TIMING_ONLY | (1 << stage) | ... to record
+ * only those stages (each event pair costs the stream a few microseconds).
+ * Enabling clears the accumulated times. */
+#define EPIC_TIMING_ONLY 0x10000
 int epic_timing_enable(epic_ctx* ctx, int on);
 int epic_timing_read(epic_ctx* ctx, double ms[EPIC_NSTAGE], int64_t calls[EPIC_NSTAGE]);
 
