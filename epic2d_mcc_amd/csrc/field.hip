@@ -564,8 +564,8 @@ static hipError_t read_scalars(Ctx* c, int* iters, double* scal3) {
   return hipStreamSynchronize(c->stream);
 }
 
-hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32_inner, double tol, int max_ref,
-                                 int* solves, double* relres);
+hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner, double tol, int max_ref,
+                                 int* solves, double* relres, const double* rho, double dx2, double s);
 
 // rho := voltage on the conductors' nodes (the grids' initialize() at the
 // start of gssolver, field.cpp:100-103); the host resolved repeated nodes to
@@ -748,6 +748,13 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
     return hipSuccess;
   }
 
+  if (d.solver == EPIC_SOLVER_SPECTRAL || (d.solver == EPIC_SOLVER_PCG_MIXED && d.precond == EPIC_PRECOND_SPECTRAL)) {
+    // b is built from rho by the solve's first kernel
+    const bool fp32 = d.solver == EPIC_SOLVER_PCG_MIXED;
+    const double tol = d.tol_relative ? d.tolerance : 1e-12;
+    return launch_spectral_solve(c, c->b64, c->phi, fp32, tol, fp32 ? 6 : 3, iters, err, c->rho, dx2, s);
+  }
+
   hipLaunchKernelGGL(rhs_kernel, dim3(grid), dim3(256), 0, c->stream, c->rho, c->b64, d.nx, d.ny, dx2, s);
   EPIC_HIP_TRY(hipGetLastError());
 
@@ -757,11 +764,6 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
     EPIC_HIP_TRY(read_scalars(c, iters, sc));
     *err = sc[1];
     return hipSuccess;
-  }
-  if (d.solver == EPIC_SOLVER_SPECTRAL || (d.solver == EPIC_SOLVER_PCG_MIXED && d.precond == EPIC_PRECOND_SPECTRAL)) {
-    const bool fp32 = d.solver == EPIC_SOLVER_PCG_MIXED;
-    const double tol = d.tol_relative ? d.tolerance : 1e-12;
-    return launch_spectral_solve(c, c->b64, c->phi, fp32, tol, fp32 ? 6 : 3, iters, err);
   }
 
   // Mixed precision: FP64 true residual + refinement around FP32 Jacobi-PCG

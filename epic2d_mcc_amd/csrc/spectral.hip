@@ -70,9 +70,17 @@ __global__ void hadamard_kernel(T* Y, const T* G, int n) {
 
 // residual r = b - A phi over the unknowns (FP64) and ||r||^2 partials;
 // X (when given): r packed in FP32 as the transform's operand
+// The solve's first residual can also build b from rho: pcgsolver's
+// right-hand side (field.cpp:234-251), when rho is given.
+struct RhsSrc {
+  const double* rho;
+  double* b;
+  double dx2, s;
+};
+
 template <int OP>
 __global__ void spec_residual_kernel(const double* b, const double* phi, double* r, int nx, int ny, double* part,
-                                     float* X, SpecLayout L, double* zero_phi) {
+                                     float* X, SpecLayout L, double* zero_phi, RhsSrc src) {
   __shared__ double sh[64];
   const int n = nx * ny;
   double acc = 0;
@@ -80,6 +88,7 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
     // zero_phi: the solve's start, phi = 0 (written here, not read)
     double rk;
     if (zero_phi) {
+      if (src.rho) src.b[k] = rhs_at(src.rho[k], k / ny, k % ny, nx, ny, src.dx2, src.s);
       rk = residual_at_zero(OP, b, k, nx, ny);
       zero_phi[k] = 0.0;
     } else {
@@ -97,19 +106,25 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
-// hs (when given): *bb_dev and this sum, then seq, to pinned host memory,
-// for the host's polling wait.
-__global__ void sum2_kernel(const double* part, int n, double* out, HostScal* hs, const double* bb_dev,
-                            unsigned long long seq) {
-  double s = 0;
-  for (int b = threadIdx.x; b < n; b += 64) s += part[b];
-  s = wave_sum(s);
+// ||r||^2 and ||b||^2 from the residual kernels' partials (pr: the latest
+// residual's, pb: the first residual's, r = b) into out[1], out[0]; hs (when
+// given): both, then seq, to pinned host memory for the host's polling wait.
+__global__ void norms_kernel(const double* pr, const double* pb, int n, double* out, HostScal* hs,
+                             unsigned long long seq) {
+  double sr = 0, sb = 0;
+  for (int b = threadIdx.x; b < n; b += 64) {
+    sr += pr[b];
+    sb += pb[b];
+  }
+  sr = wave_sum(sr);
+  sb = wave_sum(sb);
   if (threadIdx.x == 0) {
-    *out = s;
+    out[0] = sb;
+    out[1] = sr;
     if (hs) {
-      __hip_atomic_store(&hs->bb, (unsigned long long)__double_as_longlong(*bb_dev), __ATOMIC_RELAXED,
+      __hip_atomic_store(&hs->bb, (unsigned long long)__double_as_longlong(sb), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&hs->rr, (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED,
+      __hip_atomic_store(&hs->rr, (unsigned long long)__double_as_longlong(sr), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_store(&hs->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
@@ -350,25 +365,37 @@ static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwr
   return hipGetLastError();
 }
 
-// zero_phi: phi is set to 0 by the same kernel and the residual is that of
-// phi = 0 (phi itself is not read).  seq > 0: the sum and *bb_dev also go to
-// the pinned host words under that sequence number.
-static hipError_t residual_norm2(Ctx* c, const double* b, const double* phi, double* r, double* out_dev,
-                                 float* X = nullptr, double* zero_phi = nullptr, const double* bb_dev = nullptr,
-                                 unsigned long long seq = 0) {
+// Partial sums of the residual kernels: the latest residual's at
+// ws_part[0, g), the first one's (r = b) at ws_part[SPEC_PART_B, +g).
+constexpr int SPEC_PART_B = 1024;
+
+static int residual_blocks(const Ctx* c) { return std::min((c->dom.nx * c->dom.ny + 255) / 256, SPEC_PART_B); }
+
+// r = b - A phi and its partial sums.  First residual of a solve (zero_phi
+// given): phi := 0 in the same kernel and r = b (phi itself is not read),
+// partials to the second region; b is built from rho there when src.rho is
+// given.
+static hipError_t residual(Ctx* c, const double* b, const double* phi, double* r, float* X, double* zero_phi = nullptr,
+                           RhsSrc src = RhsSrc{nullptr, nullptr, 0, 0}) {
   const epic_domain_desc& d = c->dom;
-  const int n = d.nx * d.ny;
-  const int g = std::min((n + 255) / 256, 1024);
+  const int g = residual_blocks(c);
   const SpecLayout L = c->blas ? ((SpecState*)c->blas)->L : SpecLayout{};
+  double* part = c->ws_part + (zero_phi ? SPEC_PART_B : 0);
   if (d.poisson_op == EPIC_OP_REF_GHOST)
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_REF_GHOST>, dim3(g), dim3(256), 0, c->stream, b, phi, r, d.nx,
-                       d.ny, c->ws_part, X, L, zero_phi);
+                       d.ny, part, X, L, zero_phi, src);
   else
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_DIRICHLET_WALL>, dim3(g), dim3(256), 0, c->stream, b, phi, r,
-                       d.nx, d.ny, c->ws_part, X, L, zero_phi);
-  EPIC_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(64), 0, c->stream, c->ws_part, g, out_dev,
-                     seq ? c->hscal : nullptr, bb_dev, seq);
+                       d.nx, d.ny, part, X, L, zero_phi, src);
+  return hipGetLastError();
+}
+
+// ||b||^2 and the latest residual's ||r||^2 into out (first_only: no
+// correction ran, r = b); seq > 0: also to the pinned host words.
+static hipError_t norms(Ctx* c, double* out, bool first_only, unsigned long long seq) {
+  const double* pb = c->ws_part + SPEC_PART_B;
+  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, c->stream, first_only ? pb : c->ws_part, pb,
+                     residual_blocks(c), out, seq ? c->hscal : nullptr, seq);
   return hipGetLastError();
 }
 
@@ -395,11 +422,12 @@ static hipError_t wait_host_scalars(Ctx* c, unsigned long long seq, double* bb, 
 }
 
 // Solve A phi = b to ||b - A phi|| <= tol ||b||.  fp32_inner: the corrections
-// are FP32 sine-transform solves (mixed precision); otherwise FP64.
+// are FP32 sine-transform solves (mixed precision); otherwise FP64.  rho
+// (when given): b is first built from it, b = -rho dx^2 s on the unknowns.
 // Returns the number of transform solves in *solves and the final relative
 // residual in *relres.
-hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32_inner, double tol, int max_ref,
-                                 int* solves, double* relres) {
+hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner, double tol, int max_ref,
+                                 int* solves, double* relres, const double* rho, double dx2, double s) {
   EPIC_HIP_TRY(spec_init(c));
   if (!c->hscal) {
     EPIC_HIP_TRY(hipHostMalloc((void**)&c->hscal, sizeof(HostScal), hipHostMallocCoherent));
@@ -408,29 +436,34 @@ hipError_t launch_spectral_solve(Ctx* c, const double* b, double* phi, bool fp32
   SpecState* S = (SpecState*)c->blas;
   double* dscal = c->ws_scalar + 4;  // [4] ||b||^2, [5] ||r||^2
   float* Xpack = fp32_inner && spec_mfma_ok(c) ? S->X32 : nullptr;
-  // phi := 0 and r = b in one kernel (the corrections write phi's interior)
-  EPIC_HIP_TRY(residual_norm2(c, b, nullptr, c->r64, dscal, Xpack, phi));
+  // b from rho, phi := 0 and r = b in one kernel (the corrections write
+  // phi's interior)
+  EPIC_HIP_TRY(residual(c, b, nullptr, c->r64, Xpack, phi, RhsSrc{rho, b, dx2, s}));
   int k = 0;
+  bool checked = false;
   double host[2] = {0, 0};
   for (; k < max_ref; k++) {
     if (fp32_inner)
       EPIC_HIP_TRY(spec_apply<float>(c, c->r64, phi, k == 0, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32));
     else
       EPIC_HIP_TRY(spec_apply<double>(c, c->r64, phi, k == 0, S->Sp64, S->Sq64, S->G64, S->X64, S->Y64));
+    EPIC_HIP_TRY(residual(c, b, phi, c->r64, Xpack));
     // the FP32 correction gains ~6 digits: two applications reach 1e-10 and
-    // more; check the true residual after the second
-    const bool check = k + 1 >= (fp32_inner ? 2 : 1);
-    const unsigned long long seq = check ? ++c->hscal_seq : 0;
-    EPIC_HIP_TRY(residual_norm2(c, b, phi, c->r64, dscal + 1, Xpack, nullptr, dscal, seq));
-    if (check) {
+    // more; check the true residual after the second (the norms are formed
+    // only for a check)
+    if (k + 1 >= (fp32_inner ? 2 : 1)) {
+      const unsigned long long seq = ++c->hscal_seq;
+      EPIC_HIP_TRY(norms(c, dscal, false, seq));
       EPIC_HIP_TRY(wait_host_scalars(c, seq, &host[0], &host[1]));
+      checked = true;
       if (sqrt(host[1]) <= tol * sqrt(host[0]) || host[1] == 0) {
         k++;
         break;
       }
     }
   }
-  if (host[0] == 0 && host[1] == 0) {
+  if (!checked) {
+    EPIC_HIP_TRY(norms(c, dscal, k == 0, 0));
     EPIC_HIP_TRY(hipMemcpyAsync(host, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
   }
