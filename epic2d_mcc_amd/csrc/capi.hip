@@ -103,7 +103,7 @@ static void free_all(Ctx* c) {
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
                 c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
                 c->mcc.event_count, c->mcc.cand, c->mcc.cand_n, c->sp_S, c->sp_T, c->sp_lam,
-                c->cond.table, c->cond.node, c->cond.value, c->nr_ws};
+                c->cond.table, c->cond.cmask, c->cond.node, c->cond.value, c->nr_ws};
   for (void* p : gs)
     if (p) hipFree(p);
   if (c->hscal) hipHostFree(c->hscal);
@@ -206,6 +206,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
   if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
+  if (const char* v = getenv("EPIC_COND_CELL_MASK")) c->cond_cell_mask = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI_WAVES")) c->push_multi_waves = atoi(v);
   if (const char* v = getenv("EPIC_SPEC_MFMA")) c->spec_mfma = atoi(v);
   if (const char* v = getenv("EPIC_PCG_BLOCK")) c->pcg_block = atoi(v);
@@ -606,9 +607,11 @@ int epic_set_conductors(epic_ctx* h, const epic_conductor_desc* cd, int n) {
   Conductors& q = c->cond;
   CK(hipStreamSynchronize(c->stream), "sync");
   if (q.table) hipFree(q.table);
+  if (q.cmask) hipFree(q.cmask);
   if (q.node) hipFree(q.node);
   if (q.value) hipFree(q.value);
   q.table = nullptr;
+  q.cmask = nullptr;
   q.node = nullptr;
   q.value = nullptr;
   q.desc.assign(cd, cd + n);
@@ -620,6 +623,35 @@ int epic_set_conductors(epic_ctx* h, const epic_conductor_desc* cd, int n) {
   if (!table.empty()) {
     CK(hipMalloc(&q.table, table.size() * sizeof(CondDev)), "conductors");
     CK(hipMemcpy(q.table, table.data(), table.size() * sizeof(CondDev), hipMemcpyHostToDevice), "conductors");
+    // Cells (i * (ny - 1) + j) that a shape's bounding box reaches, widened
+    // by one cell on each side: a particle in an unflagged cell is on no
+    // conductor (the kernels compute its cell in FP32, whose rounding moves
+    // it by at most one cell), so the FP32 pushes skip the exact test there.
+    const epic_domain_desc& d = c->dom;
+    const int ncx = d.nx - 1, ncy = d.ny - 1;
+    std::vector<uint32_t> cm(((size_t)ncx * ncy + 31) / 32, c->cond_cell_mask ? 0u : ~0u);
+    auto cell_of = [](double v, double v0, double h, int ncell) {
+      const double t = std::floor((v - v0) / h);
+      return (int)std::max(-2.0, std::min(t, (double)ncell + 1));
+    };
+    for (const CondDev& t : table) {
+      double xl, xh, yl, yh;
+      if (t.kind == COND_DISK) {
+        const double r = std::sqrt(std::max(t.c, 0.0));
+        xl = t.a - r; xh = t.a + r; yl = t.b - r; yh = t.b + r;
+      } else {
+        xl = t.a; xh = t.b; yl = t.c; yh = t.d;
+      }
+      const int i0 = std::max(0, cell_of(xl, d.x0, d.dx, ncx) - 1), i1 = std::min(ncx - 1, cell_of(xh, d.x0, d.dx, ncx) + 1);
+      const int j0 = std::max(0, cell_of(yl, d.y0, d.dy, ncy) - 1), j1 = std::min(ncy - 1, cell_of(yh, d.y0, d.dy, ncy) + 1);
+      for (int i = i0; i <= i1; i++)
+        for (int j = j0; j <= j1; j++) {
+          const size_t k = (size_t)i * ncy + j;
+          cm[k >> 5] |= 1u << (k & 31);
+        }
+    }
+    CK(hipMalloc(&q.cmask, cm.size() * sizeof(uint32_t)), "conductors");
+    CK(hipMemcpy(q.cmask, cm.data(), cm.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "conductors");
   }
   if (q.nnode > 0) {
     CK(hipMalloc(&q.node, node.size() * sizeof(int)), "conductors");

@@ -108,6 +108,8 @@ struct Conductors {
   CondDev* table = nullptr;  // device: rectangles, then electrodes
   int nrect = 0, nelec = 0;
   bool elec_live = false;    // the electrodes exist once an SOR solve has run
+  uint32_t* cmask = nullptr; // device: one bit per cell, set where a shape's
+                             // bounding box (widened by one cell) reaches
   int* node = nullptr;       // device: nodes an SOR solve sets to a voltage
   double* value = nullptr;
   int nnode = 0;
@@ -196,7 +198,8 @@ struct Ctx {
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
   int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
-  int push_multi_waves = 8;    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
+  int push_multi_waves = 8;
+  int cond_cell_mask = 1;  // 0: every cell flagged (the exact test everywhere)    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
   int spec_mfma = 1;           // FP32 sine transforms on the hand-written MFMA GEMM (EPIC_SPEC_MFMA=0: rocBLAS)
   int pcg_block = 512;         // single-workgroup PCG size above 4096 nodes: 1024, 512 or 256 threads (EPIC_PCG_BLOCK)
 

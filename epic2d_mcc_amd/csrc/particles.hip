@@ -612,7 +612,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
 // one region at a time).
 template <bool DEP, int PPT, int WAVES, bool COND = false>
 __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, int rr, const CondDev* cond,
-                                                                     int ncond) {
+                                                                     int ncond, const uint32_t* cmask) {
   __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
   constexpr int CHW = 64 * PPT;
   constexpr int CH = PART_BLOCK * PPT;
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
   if (cb < ce_) load_chunk<PPT>(a, cb * CH + (int64_t)wid * CHW, n, lane, r);
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
     ChunkDep<PPT> dp;
-    push_chunk_regs<DEP, PPT, COND>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp, cond, ncond);
+    push_chunk_regs<DEP, PPT, COND>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp, cond, ncond, cmask);
     const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
     const bool more = chunk + cs < ce_;
     const bool full = more && nb + 64 * PPT <= n;
@@ -663,6 +663,7 @@ struct PushF32Set {
   int nsp;
   const CondDev* cond;  // absorbing conductors (the first ncond are active)
   int ncond;
+  const uint32_t* cmask;  // cells where a conductor can be (one bit per cell)
 };
 
 template <int PPT, bool COND, int WAVES>
@@ -711,7 +712,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_multi_kernel(PushF
   for (; g < tot; g += gridDim.x) {
     ChunkDep<PPT> dp;
     const bool active = wb < n;
-    if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp, S.cond, S.ncond);
+    if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp, S.cond, S.ncond, S.cmask);
     const int gn = g + (int)gridDim.x;
     const bool more = gn < tot;
     int sn = s;
@@ -883,10 +884,10 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
       // with conductors: the default shape with the conductor test
       if (deposit) {
         auto k = push_f32_kernel<true, 4, 1, true>;
-        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct);
+        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, c->cond.cmask);
       } else {
         auto k = push_f32_kernel<false, 4, 1, true>;
-        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct);
+        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, c->cond.cmask);
       }
       return hipGetLastError();
     }
@@ -897,10 +898,10 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   case ID:                                                                                         \
     if (deposit) {                                                                                 \
       auto k = push_f32_kernel<true, PPT, WV>;                                                     \
-      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct); \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, c->cond.cmask); \
     } else {                                                                                       \
       auto k = push_f32_kernel<false, PPT, WV>;                                                    \
-      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct); \
+      hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, c->cond.cmask); \
     }                                                                                              \
     break;
       EPIC_V(1, 8, 5)
@@ -961,6 +962,7 @@ hipError_t launch_push_all(Ctx* c) {
   }
   S.cond = c->cond.table;
   S.ncond = c->cond.active();
+  S.cmask = c->cond.cmask;
   // WAVES = 8 caps the registers at 64 (eight wavefronts per SIMD)
   auto k = c->cond.any() ? (c->push_multi_waves == 8 ? push_f32_multi_kernel<4, true, 8> : push_f32_multi_kernel<4, true, 1>)
                          : (c->push_multi_waves == 8 ? push_f32_multi_kernel<4, false, 8> : push_f32_multi_kernel<4, false, 1>);

@@ -205,14 +205,16 @@ __device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, in
 
 // Gather, push, walls and store of a loaded chunk; fills the deposit
 // information of the new positions (DEP).  COND: the walls are followed by
-// the conductor test against the first ncond shapes of cond (a separate
+// the conductor test against the first ncond shapes of cond, in the cells
+// that cmask flags (a separate
 // instantiation, so the kernels without conductors keep their code; the
 // table is an argument of its own, not part of PushF32, whose size the
 // batched kernel's descriptor loads depend on).
 template <bool DEP, int PPT, bool COND = false>
 __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, int64_t n, int lane,
                                                 ChunkRegs<PPT>& r, ChunkDep<PPT>& dp,
-                                                const CondDev* cond = nullptr, int ncond = 0) {
+                                                const CondDev* cond = nullptr, int ncond = 0,
+                                                const uint32_t* cmask = nullptr) {
   constexpr int V = 4;
   const int nyc = a.ny - 1;
   const float fmax_i = (float)(a.nx - 1), fmax_j = (float)(a.ny - 1);
@@ -285,9 +287,16 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
       if (py < a.y0) { py = a.y0 + (a.y0 - py); r.vy[k] = -r.vy[k]; }
       else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); r.vy[k] = -r.vy[k]; }
     }
-    if (COND && alive && in_conductor(cond, ncond, (double)px, (double)py)) {
-      alive = false;
-      px = __builtin_nanf("");
+    if (COND && alive) {
+      // the cell's flag first (cmask: cells a shape's bounding box reaches,
+      // widened by one), the exact FP64 test only in flagged cells
+      const int ci = min(max((int)((px - a.x0) * a.idx), 0), a.nx - 2);
+      const int cj = min(max((int)((py - a.y0) * a.idy), 0), a.ny - 2);
+      const int cell = ci * nyc + cj;
+      if (((cmask[cell >> 5] >> (cell & 31)) & 1u) && in_conductor(cond, ncond, (double)px, (double)py)) {
+        alive = false;
+        px = __builtin_nanf("");
+      }
     }
     r.x[k] = px;
     r.y[k] = py;

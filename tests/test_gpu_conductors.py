@@ -202,17 +202,21 @@ def _fp32_run(st, env, steps=3):
 @pytest.mark.parametrize("bc", ["open", "pbc"])
 def test_fp32_conductor_kernels_identical(bc):
     """FP32 windowed pushes with conductors: the one-launch kernel and the
-    per-species kernel give the same particles and rho bit for bit, and the
-    particles track the FP64 oracle."""
+    per-species kernel give the same particles and rho bit for bit, so does
+    the exact test in every cell (no cell bitmap), and the particles track
+    the FP64 oracle."""
     print(f"[conductors] fp32 kernels {bc}", flush=True)
     st = small_setup(N=100000, Nx=128, Ny=128, bc=bc, precision=_lib.FP32, solver=_lib.SOLVER_GS, tol=1e-6,
                      maxit=20000, vsx_beam=2.0 if bc != "pbc" else 0.0)
     (a, ra), o = _fp32_run(st, {})
     (b, rb), _ = _fp32_run(st, {"EPIC_PUSH_MULTI": "0"})
+    (m, rm), _ = _fp32_run(st, {"EPIC_COND_CELL_MASK": "0"})
     for s in range(2):
         for k in range(4):
             assert np.array_equal(a[s][k], b[s][k], equal_nan=True), (s, k)
+            assert np.array_equal(a[s][k], m[s][k], equal_nan=True), ("no bitmap", s, k)
     assert np.array_equal(ra, rb)
+    assert np.array_equal(ra, rm)
     o.initial_field()
     for ts in range(3):
         o.step()
