@@ -785,12 +785,21 @@ int epic_step(epic_ctx* h, int ts, int* iters) {
   int it = 0;
   double err = 0;
   stage_begin(c, 2);
-  CK(launch_solve(c, &it, &err), "solve");
+  // the sine-transform solve launches the E-field itself after its last
+  // convergence check (timed with the solve then)
+  c->efield_in_solve = true;
+  c->efield_done = false;
+  const hipError_t solve_rc = launch_solve(c, &it, &err);
+  c->efield_in_solve = false;
+  CK(solve_rc, "solve");
   stage_end(c, 2);
   if (iters) *iters = it;
-  stage_begin(c, 3);
-  CK(launch_efield(c), "efield");
-  stage_end(c, 3);
+  if (!c->efield_done) {
+    stage_begin(c, 3);
+    CK(launch_efield(c), "efield");
+    stage_end(c, 3);
+  }
+  c->efield_done = false;
   // every pushed species in one launch where the FP32 hot kernel applies
   const bool all = push_all_ok(c);
   if (all) {

@@ -156,6 +156,11 @@ struct Ctx {
   // the device, polled by the host) and the last sequence number used
   HostScal* hscal = nullptr;
   unsigned long long hscal_seq = 0;
+  // fused step: the sine-transform solve launches the E-field right after
+  // its convergence check (so it runs while the host waits on the check);
+  // efield_done: the launched E-field is that of the final phi
+  bool efield_in_solve = false;
+  bool efield_done = false;
   int* ws_iter = nullptr;
   int max_solver_blocks = 0;
   // co-resident workgroups per CU of each particle kernel (queried once)

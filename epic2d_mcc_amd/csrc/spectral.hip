@@ -443,6 +443,7 @@ hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner
   bool checked = false;
   double host[2] = {0, 0};
   for (; k < max_ref; k++) {
+    c->efield_done = false;
     if (fp32_inner)
       EPIC_HIP_TRY(spec_apply<float>(c, c->r64, phi, k == 0, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32));
     else
@@ -454,6 +455,10 @@ hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner
     if (k + 1 >= (fp32_inner ? 2 : 1)) {
       const unsigned long long seq = ++c->hscal_seq;
       EPIC_HIP_TRY(norms(c, dscal, false, seq));
+      // E of this phi, in case it is the final one (a further correction
+      // launches it again)
+      if (c->efield_in_solve) EPIC_HIP_TRY(launch_efield(c));
+      c->efield_done = c->efield_in_solve;
       EPIC_HIP_TRY(wait_host_scalars(c, seq, &host[0], &host[1]));
       checked = true;
       if (sqrt(host[1]) <= tol * sqrt(host[0]) || host[1] == 0) {
