@@ -23,7 +23,7 @@ namespace epic {
 constexpr int SORT_TILE = 16;
 // Particles per thread per pass of the sort kernels: the loads of SORT_U
 // particles are in flight together; a workgroup chunk is 256 * SORT_U.
-constexpr int SORT_U = 4;
+constexpr int SORT_U = 8;
 constexpr int SORT_CHUNK = 256 * SORT_U;
 
 struct KeyMap {
