@@ -15,7 +15,7 @@ namespace epic {
 constexpr int MAXSP = 16;
 
 struct FinArgs {
-  const unsigned long long* Q[MAXSP];
+  unsigned long long* Q[MAXSP];
   double* den[MAXSP];
   double scale[MAXSP];   // 2^-B * spwt
   double factor[MAXSP];  // charge * (1/QE)
@@ -26,6 +26,7 @@ struct FinArgs {
   int nx, ny, bc;
   double* rho;
   unsigned long long* errc;
+  int clear_q;           // clear each read accumulator (walls without images)
 };
 
 __global__ void rho_finalize_kernel(FinArgs a) {
@@ -37,6 +38,8 @@ __global__ void rho_finalize_kernel(FinArgs a) {
       double den;
       if (a.mask & (1u << s)) {
         const long long qs = (long long)fold_q(a.Q[s], i, j, a.nx, a.ny, a.bc);
+        // without periodic images a node reads only its own accumulator
+        if (a.clear_q) a.Q[s][k] = 0ull;
         if (qs < 0) atomicAdd(&a.errc[EPIC_ERRC_FIXED_OVF], 1ull);
         den = den_from_q(qs, a.scale[s], a.norm, i, j, a.nx, a.ny, a.bc);
         a.den[s][k] = den;
@@ -49,9 +52,9 @@ __global__ void rho_finalize_kernel(FinArgs a) {
   }
 }
 
-// Clears the accumulators of every species in mask in one launch.  Not
-// folded into rho_finalize_kernel: under periodic walls a node's fold reads
-// its images, which other threads read too.
+// Clears the accumulators of every species in mask in one launch (under
+// periodic walls a node's fold reads its images, which other threads read
+// too, so rho_finalize_kernel cannot clear them as it reads).
 struct ZeroQArgs {
   unsigned long long* Q[MAXSP];
   int nsp;
@@ -98,11 +101,12 @@ static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
   a.nx = d.nx; a.ny = d.ny; a.bc = d.bc;
   a.rho = c->rho;
   a.errc = c->errc;
+  a.clear_q = d.bc != EPIC_BC_PBC;
   const int64_t nn = (int64_t)d.nx * d.ny;
   int grid = (int)std::min<int64_t>((nn + 255) / 256, 4096);
   hipLaunchKernelGGL(rho_finalize_kernel, dim3(grid), dim3(256), 0, c->stream, a);
   EPIC_HIP_TRY(hipGetLastError());
-  return zero_q(c, mask);
+  return a.clear_q ? hipSuccess : zero_q(c, mask);
 }
 
 // All species' Q -> den and rho (the fused-step path).
