@@ -378,10 +378,20 @@ int epic_upload_particles(epic_ctx* h, int s, const double* x, const double* y, 
       CK(hipMemcpy(dst[k], src[k], n * sizeof(double), hipMemcpyHostToDevice), "upload");
     }
   } else {
+    // Rounding to FP32 can lift a position just below the upper wall onto
+    // it (x0 + Lx), which the reference never holds for a live particle
+    // (its loader and push keep x < x0 + Lx): such positions are stored as
+    // the largest float below the wall.
+    const double hi[2] = {c->dom.x0 + (c->dom.nx - 1) * c->dom.dx, c->dom.y0 + (c->dom.ny - 1) * c->dom.dy};
     std::vector<float> tmp(n);
     for (int k = 0; k < 5; k++) {
       if (!src[k]) return fail(c, EPIC_ERR_ARG, "null array");
       for (int64_t i = 0; i < n; i++) tmp[i] = (float)src[k][i];
+      if (k < 2) {
+        const float hf = (float)hi[k], below = std::nextafter(hf, -INFINITY);
+        for (int64_t i = 0; i < n; i++)
+          if (src[k][i] < hi[k] && tmp[i] >= hf) tmp[i] = below;
+      }
       CK(hipMemcpy(dst[k], tmp.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
     }
   }
@@ -933,7 +943,7 @@ int epic_error_counters(epic_ctx* h, int64_t out[EPIC_NERR]) {
   unsigned long long v[EPIC_NERR];
   CK(hipMemcpyAsync(v, c->errc, sizeof(v), hipMemcpyDeviceToHost, c->stream), "errc");
   CK(hipStreamSynchronize(c->stream), "errc");
-  for (int k = 0; k < EPIC_NERR; k++) out[k] = (int64_t)v[k];
+  for (int k = 0; k < EPIC_NERR; k++) out[k] = (int64_t)v[k] + (int64_t)c->errc_host[k];
   return EPIC_OK;
 }
 
@@ -942,6 +952,7 @@ int epic_clear_error_counters(epic_ctx* h) {
   Ctx* c = &h->c;
   CK(hipMemsetAsync(c->errc, 0, EPIC_NERR * sizeof(unsigned long long), c->stream), "errc");
   CK(hipStreamSynchronize(c->stream), "errc");
+  for (int k = 0; k < EPIC_NERR; k++) c->errc_host[k] = 0;
   return EPIC_OK;
 }
 

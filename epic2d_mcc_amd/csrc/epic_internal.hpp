@@ -190,6 +190,9 @@ struct Ctx {
   int red_blocks = 0;
 
   unsigned long long* errc = nullptr;  // EPIC_NERR device counters
+  // counts the host observes (a solve whose convergence test runs on the
+  // host and fails), added to the device counters when they are read
+  long long errc_host[EPIC_NERR] = {};
 
   std::vector<Emitter> emitters;
   MccState mcc;
@@ -203,8 +206,8 @@ struct Ctx {
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
   int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
-  int push_multi_waves = 8;
-  int cond_cell_mask = 1;  // 0: every cell flagged (the exact test everywhere)    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
+  int push_multi_waves = 8;    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
+  int cond_cell_mask = 1;      // 0: every cell flagged (the exact test everywhere)
   int spec_mfma = 1;           // FP32 sine transforms on the hand-written MFMA GEMM (EPIC_SPEC_MFMA=0: rocBLAS)
   int pcg_block = 512;         // single-workgroup PCG size above 4096 nodes: 1024, 512 or 256 threads (EPIC_PCG_BLOCK)
 

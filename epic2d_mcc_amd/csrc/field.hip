@@ -46,6 +46,7 @@ struct PcgArgs {
   int relative;         // tol relative to ||b||
   int compute_error;    // ||A x - b|| / n (field.cpp:258-259)
   unsigned* bar;        // grid barrier counter
+  unsigned long long* noconv;  // +1 when the cap ends the loop above tol (nullptr: not counted)
 };
 
 template <typename T>
@@ -153,8 +154,12 @@ __global__ __launch_bounds__(BS) void pcg_kernel(PcgArgs<T> a) {
   double s = 0;
   int cur = 0;
   long long it = 0;
+  bool conv = false;
   for (; it < a.cap; it++) {
-    if (sqrt(rr) < tol) break;
+    if (sqrt(rr) < tol) {
+      conv = true;
+      break;
+    }
     // v = w + v_old * s ; u = A v
     T v[NPT], u[NPT];
     const T* vo = a.vb + (size_t)cur * n;
@@ -257,6 +262,9 @@ __global__ __launch_bounds__(BS) void pcg_kernel(PcgArgs<T> a) {
     a.scalars[0] = sqrt(rr);
     a.scalars[2] = alpha;
     *a.iters_out = (int)it;
+    // the reference stops silently at the cap (solvers.h:81-97); the final
+    // residual is still tested, as the next iteration would test it
+    if (!conv && !(sqrt(rr) < tol) && a.noconv) atomicAdd(a.noconv, 1ull);
   }
 }
 
@@ -280,6 +288,7 @@ struct SorArgs {
   double* scalars;
   int* iters_out;
   unsigned* bar;
+  unsigned long long* noconv;  // +1 when maxit ends the loop above tol
 };
 
 template <bool COOP, bool BOLTZ>
@@ -294,6 +303,7 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
   gsync<COOP>(grid);
   int it = 0;
   double L2 = 0;
+  bool conv = false;
   for (; it < a.maxit; it++) {
     for (int color = 0; color < 2; color++) {
       for (int k = g0; k < n; k += TT) {
@@ -322,6 +332,7 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
       grid_sum<1, COOP>(d, a.part, (it / CHECK) & 1, sh, grid);
       L2 = sqrt(d[0] / (a.nx * a.ny));
       if (L2 < a.tol) {
+        conv = true;
         it++;
         break;
       }
@@ -330,6 +341,8 @@ __global__ __launch_bounds__(SOLVER_BLOCK) void sor_kernel(SorArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.scalars[1] = L2;
     *a.iters_out = it;
+    // gssolver reports this on stderr (field.cpp:137)
+    if (!conv && a.noconv) atomicAdd(a.noconv, 1ull);
   }
 }
 
@@ -539,9 +552,12 @@ static hipError_t run_pcg_op(Ctx* c, int G, int npt, PcgArgs<T>& a) {
   }
 }
 
+// count_noconv: a cap reached above tolerance adds to the context's
+// EPIC_ERRC_NOCONV counter (the outer solves; not the mixed scheme's inner
+// FP32 solves, whose tolerance is only a stopping rule).
 template <typename T>
 static hipError_t run_pcg(Ctx* c, const double* b, T* x_out, long long cap, double tol, int relative,
-                          int compute_error) {
+                          int compute_error, bool count_noconv = false) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
   int npt = 1;
@@ -554,6 +570,7 @@ static hipError_t run_pcg(Ctx* c, const double* b, T* x_out, long long cap, doub
   a.part = c->ws_part; a.scalars = c->ws_scalar; a.iters_out = c->ws_iter;
   a.cap = cap; a.tol = tol; a.relative = relative; a.compute_error = compute_error;
   a.bar = (unsigned*)(c->ws_iter + 1);
+  a.noconv = count_noconv ? c->errc + EPIC_ERRC_NOCONV : nullptr;
   if (d.poisson_op == EPIC_OP_REF_GHOST) return run_pcg_op<T, EPIC_OP_REF_GHOST>(c, G, npt, a);
   return run_pcg_op<T, EPIC_OP_DIRICHLET_WALL>(c, G, npt, a);
 }
@@ -730,6 +747,7 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
     a.s = s; a.omega = d.sor_omega; a.tol = d.tolerance; a.maxit = d.max_iteration;
     a.part = c->ws_part; a.scalars = c->ws_scalar; a.iters_out = c->ws_iter;
     a.bar = (unsigned*)(c->ws_iter + 1);
+    a.noconv = c->errc + EPIC_ERRC_NOCONV;
     int npt;
     const int G = solver_blocks(c, n, &npt);
     hipLaunchKernelGGL(zero_bar_kernel, dim3(1), dim3(1), 0, c->stream, a.bar);
@@ -759,7 +777,7 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   EPIC_HIP_TRY(hipGetLastError());
 
   if (d.solver == EPIC_SOLVER_PCG) {
-    EPIC_HIP_TRY(run_pcg<double>(c, c->b64, c->phi, cap, d.tolerance, d.tol_relative, 1));
+    EPIC_HIP_TRY(run_pcg<double>(c, c->b64, c->phi, cap, d.tolerance, d.tol_relative, 1, true));
     double sc[3];
     EPIC_HIP_TRY(read_scalars(c, iters, sc));
     *err = sc[1];
@@ -799,6 +817,7 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err) {
   }
   *iters = total;
   *err = bnorm > 0 ? rnorm / bnorm : 0;
+  if (!(rnorm <= d.tolerance * bnorm || rnorm == 0)) c->errc_host[EPIC_ERRC_NOCONV]++;
   return hipSuccess;
 }
 

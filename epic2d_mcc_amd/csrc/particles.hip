@@ -374,6 +374,13 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
           else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); vx[k] = -vx[k]; }
           if (py < a.y0) { py = a.y0 + (a.y0 - py); vy[k] = -vy[k]; }
           else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); vy[k] = -vy[k]; }
+          if (sizeof(P) == 4) {
+            // FP32: a position exactly on the upper wall mirrors onto
+            // itself; keep live positions below it (the FP64 path keeps the
+            // reference's arithmetic)
+            if (px >= a.xhi) px = nextafter(a.xhi, a.x0);
+            if (py >= a.yhi) py = nextafter(a.yhi, a.y0);
+          }
         }
         // a conductor absorbs what the walls left (species.cpp:84-98,
         // 111-124, 152-165)

@@ -5,6 +5,7 @@
 
 #include "device_common.hpp"
 #include "epic_internal.hpp"
+#include <type_traits>
 
 namespace epic {
 
@@ -286,6 +287,10 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
       else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); r.vx[k] = -r.vx[k]; }
       if (py < a.y0) { py = a.y0 + (a.y0 - py); r.vy[k] = -r.vy[k]; }
       else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); r.vy[k] = -r.vy[k]; }
+      // a position exactly on the upper wall mirrors onto itself: keep live
+      // positions below it
+      if (px >= a.xhi) px = nextafterf(a.xhi, a.x0);
+      if (py >= a.yhi) py = nextafterf(a.yhi, a.y0);
     }
     if (COND && alive) {
       // the cell's flag first (cmask: cells a shape's bounding box reaches,
@@ -352,7 +357,14 @@ __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsign
   wave_fence();
 
   unsigned rk = NO_DEPOSIT;
-  unsigned acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+  // A particle adds at most 16384^2 = 2^28 units to a corner, so a run of
+  // PPT particles in one cell sums to at most PPT * 2^28: 32-bit sums hold
+  // up to 15 particles, longer chunks sum in 64 bits.
+  using acc_t = typename std::conditional<((unsigned long long)PPT << 28) < (1ull << 32), unsigned,
+                                          unsigned long long>::type;
+  static_assert((unsigned long long)PPT * (1ull << 28) <= (unsigned long long)(acc_t)~(acc_t)0,
+                "run accumulator too narrow for PPT particles");
+  acc_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
 #pragma unroll
   for (int k = 0; k < PPT; k++) {
     if (dp.cp[k] == NO_DEPOSIT) continue;
@@ -366,10 +378,10 @@ __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsign
     }
     const unsigned dI = dp.wp[k] >> 16, dJ = dp.wp[k] & 0xffffu;
     const unsigned oI = 16384u - dI, oJ = 16384u - dJ;
-    acc0 += oI * oJ;
-    acc1 += dI * oJ;
-    acc2 += oI * dJ;
-    acc3 += dI * dJ;
+    acc0 += (acc_t)(oI * oJ);
+    acc1 += (acc_t)(dI * oJ);
+    acc2 += (acc_t)(oI * dJ);
+    acc3 += (acc_t)(dI * dJ);
   }
   if (rk != NO_DEPOSIT) {
     const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};

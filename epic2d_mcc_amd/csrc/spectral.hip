@@ -474,6 +474,8 @@ hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner
   }
   *solves = k;
   *relres = host[0] > 0 ? sqrt(host[1] / host[0]) : 0.0;
+  // max_ref corrections left the residual above tolerance (or not finite)
+  if (!(*relres <= tol)) c->errc_host[EPIC_ERRC_NOCONV]++;
   return hipSuccess;
 }
 

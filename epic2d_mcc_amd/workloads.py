@@ -21,19 +21,20 @@ def periodic_64(precision=_lib.FP64, N=409600, solver=_lib.SOLVER_PCG, tol=1e-8,
 
 
 def bounded_512(ppc=100, precision=_lib.FP32, solver=_lib.SOLVER_PCG_MIXED, sort_interval=20, seed=2,
-                tol=1e-10, precond=_lib.PRECOND_SPECTRAL):
+                tol=1e-10, precond=_lib.PRECOND_SPECTRAL, cells=512):
     """Four-species bounded plasma: 512x512 cells, Dirichlet phi = 0 walls,
     absorbing walls; electrons, positive ions, negative ions (alpha = 0.1) and
     a negative beam (beta = 0.05, drift 10 v_th,e along x); 100 particles/
     cell/species (104,857,600 total); Te = 1 eV, Ti = Tn = 0.026 eV; FP32
     particles, mixed-precision PCG to 1e-10 relative; counting sort every 20
-    steps.  500 steps."""
-    N = ppc * 512 * 512
+    steps.  500 steps.  `cells` scales the square grid down for parity runs
+    (the species, walls and solver stay those of the 512x512 workload)."""
+    N = ppc * cells * cells
     specs = [config.SpeciesSpec("electron", ME, N, 1.0, -1, 1.0),
              config.SpeciesSpec("ion", M_ION, N, 0.026, 1, 0.0),
              config.SpeciesSpec("negion", M_ION, N, 0.026, -1, 0.1),
              config.SpeciesSpec("beam", ME, N, 1.0, -1, 0.05, 10.0, 0.0)]
-    st = config.derive(512, 512, 0.0, specs, 1e13, 0.05, bc="open", solver=solver,
+    st = config.derive(cells, cells, 0.0, specs, 1e13, 0.05, bc="open", solver=solver,
                        poisson_op=_lib.OP_DIRICHLET_WALL, tolerance=tol, tol_relative=1, max_iteration=20000,
                        precision=precision, sort_interval=sort_interval, seed=seed, precond=precond)
     st.info["steps"] = 500

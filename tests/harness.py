@@ -38,3 +38,69 @@ def rel(a, b):
     if den == 0:
         return float(np.max(np.abs(a)))
     return float(np.max(np.abs(a - b)) / den)
+
+
+def dirichlet_solver(dom):
+    """Exact solve of gssolver's discrete problem (field.cpp:89-116 at its
+    fixed point): (phi[i+1,j] - 2 phi[i,j] + phi[i-1,j])/dx^2 + (phi[i,j+1] -
+    2 phi[i,j] + phi[i,j-1])/dy^2 = -rho[i,j] L^2/LDe^2 on the interior nodes,
+    phi = 0 on the boundary.  One sparse LU factorisation, reused per call.
+    The oracle's own SOR converges to the same solution (pinned against a
+    sparse direct solve in test_oracle_conductors); LU reaches it to round-off
+    at once, where SOR at omega = 1.4 needs ~2e4 sweeps at 129^2 nodes."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import splu
+    mx, my = dom.nx - 2, dom.ny - 2
+
+    def lap1(m, h):
+        return sp.diags([np.ones(m - 1), -2.0 * np.ones(m), np.ones(m - 1)], [-1, 0, 1]) / (h * h)
+
+    A = (sp.kron(lap1(mx, dom.dx), sp.identity(my)) + sp.kron(sp.identity(mx), lap1(my, dom.dy))).tocsc()
+    lu = splu(A)
+    s = (dom.L * dom.L) / (dom.LDe * dom.LDe)
+
+    def solve(rho):
+        rho = np.asarray(rho, dtype=np.float64).reshape(dom.nx, dom.ny)
+        phi = np.zeros((dom.nx, dom.ny))
+        phi[1:-1, 1:-1] = lu.solve(-rho[1:-1, 1:-1].ravel() * s).reshape(mx, my)
+        return phi
+
+    return solve
+
+
+class OracleDirichletLoop:
+    """The oracle's loop (main.cpp:366-423: scatter, ComputeRho, Poisson, E,
+    push) with the Poisson step on gssolver's Dirichlet-wall problem solved
+    exactly (dirichlet_solver), for comparison with the mixed-precision
+    sine-transform solve of the same operator."""
+
+    def __init__(self, o, dom):
+        self.o = o
+        self.solve_phi = dirichlet_solver(dom)
+
+    def _fields(self):
+        o = self.o
+        for s in range(o.nsp):
+            o.scatter(s)
+        o.compute_rho()
+        o.set_grid(1, self.solve_phi(o.grid(0)))
+        o.efield()
+
+    def initial_field(self):
+        self._fields()
+        for s in range(self.o.nsp):
+            self.o.rewind(s)
+
+    def step(self):
+        self._fields()
+        for s in range(self.o.nsp):
+            self.o.push(s)
+
+
+def total_energy(src, nsp, norm_sp=0, components=3):
+    """KE of every species plus PE, in the reference's normalisation
+    (Compute_KE / Compute_PE, species.cpp:248-271, domain.cpp:266-286);
+    `src` is a Context or an OracleSim.  components=2 sums KE_x + KE_y only:
+    without collisions the reference's vz stays 0 (init.cpp:86-92), so KE_z
+    is 0, and tests that carry a particle id in vz leave it out."""
+    return float(sum(np.sum(src.ke(s, norm_sp)[:components]) for s in range(nsp)) + src.pe(norm_sp))

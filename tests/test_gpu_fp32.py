@@ -129,3 +129,96 @@ def test_fp32_kernel_shapes_identical(bc, env):
         for k in range(4):
             assert np.array_equal(ref_parts[s][k], got_parts[s][k], equal_nan=True), (s, k)
     assert np.array_equal(ref_rho, got_rho)
+
+
+def test_long_runs_on_one_node_do_not_wrap():
+    """16 particles per thread (EPIC_PUSH_VARIANT=4) all sitting exactly on
+    one node: each adds 2^28 fixed-point units to that node, so a thread's
+    run sums to 2^32, past a 32-bit accumulator.  Electrons and ions share
+    the node, so rho = 0, E = 0 and nothing moves; the fused deposit must
+    still give each species' full charge at that node and nothing elsewhere,
+    equal to the default four-per-thread kernel's."""
+    print("[fp32] 16-particle runs on one node", flush=True)
+    n = 64 * 16 * 8
+    st = small_setup(N=n, Nx=128, Ny=128, precision=_lib.FP32, tol=1e-8)
+    x = np.full(n, 40.0)
+    y = np.full(n, 50.0)
+    z = np.zeros(n)
+    parts = [(x, y, z, z, z), (x, y, z, z, z)]
+    d = st.domain
+    expect = n * st.species[0].spwt / (d.dx * d.dy * d.L * d.L * d.density)
+    dens = []
+    for env in ({"EPIC_PUSH_VARIANT": "4"}, {}):
+        import os
+        old = {k: os.environ.get(k) for k in env}
+        os.environ.update(env)
+        try:
+            ctx = _lib.Context(st.domain, st.species)
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        for s, p in enumerate(parts):
+            ctx.upload(s, *p)
+        ctx.initial_field()
+        ctx.step(0)   # push + fused deposit of the (unmoved) positions
+        ctx.step(1)   # its den
+        den = [ctx.get_field(10 + s) for s in range(2)]
+        assert not ctx.errors().any()
+        ctx.close()
+        for s in range(2):
+            assert den[s][40, 50] == pytest.approx(expect, rel=1e-12), (env, s, den[s][40, 50], expect)
+            rest = den[s].copy()
+            rest[40, 50] = 0.0
+            assert not rest.any(), env
+        dens.append(den)
+    for s in range(2):
+        assert np.array_equal(dens[0][s], dens[1][s])
+
+
+@pytest.mark.parametrize("bc", ["pbc", "open", "reflective"])
+def test_fp32_positions_stay_below_upper_wall(bc):
+    """FP64 positions just below the upper walls (x0 + Lx - 1e-9) round to
+    the wall itself in FP32; the reference never holds a live particle there
+    (its loader and push keep x < x0 + Lx).  Stored in FP32 they must stay
+    below it: the stand-alone deposit and rewind see them inside the grid
+    (no out-of-grid count) and every particle's charge is deposited."""
+    print(f"[fp32] upper-wall rounding bc={bc}", flush=True)
+    st = small_setup(N=4096, Nx=128, Ny=128, precision=_lib.FP32, bc=bc, tol=1e-8)
+    d = st.domain
+    Lx, Ly = (d.nx - 1) * d.dx, (d.ny - 1) * d.dy
+    rng = np.random.default_rng(11)
+    n = 4096
+    x = rng.random(n) * Lx
+    y = rng.random(n) * Ly
+    x[:64] = Lx - 1e-9
+    y[64:128] = Ly - 1e-9
+    x[128:192] = Lx - 1e-9
+    y[128:192] = Ly - 1e-9
+    z = np.zeros(n)
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, x, y, z, z, z)
+    ctx.upload(1, y, x, z, z, z)
+    for s in range(2):
+        g = ctx.download(s)
+        assert len(g[0]) == n
+        assert np.all(g[0] < Lx) and np.all(g[1] < Ly)
+    norm = d.dx * d.dy * d.L * d.L * d.density
+    w = np.ones((d.nx, d.ny))
+    if bc != "pbc":
+        w[0, :] = w[-1, :] = w[:, 0] = w[:, -1] = 2.0
+        w[0, 0] = w[0, -1] = w[-1, 0] = w[-1, -1] = 4.0
+    for s in range(2):
+        ctx.scatter(s)
+        den = ctx.get_field(10 + s)
+        if bc == "pbc":
+            tot = float(den[:-1, :-1].sum())  # the folded copies counted once
+        else:
+            tot = float(np.sum(den * w))
+        assert tot == pytest.approx(n * st.species[s].spwt / norm, rel=1e-12), s
+    ctx.initial_field()
+    ctx.step(0)
+    assert not ctx.errors().any(), ctx.errors()
+    ctx.close()

@@ -121,3 +121,42 @@ def test_mixed_and_spectral_match_fp64_oracle(Nx, Ny):
     it, err = ctx.solve()
     assert err < 1e-12
     assert rel(ctx.get_field(1), ref) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["pcg", "pcg_multi", "sor", "mixed_jacobi", "spectral_mixed", "spectral_fp64"])
+def test_nonconvergence_is_counted(case):
+    """A solve that stops at its cap above tolerance adds one to
+    EPIC_ERRC_NOCONV (the reference's PCG stops silently at max_iteration,
+    solvers.h:81-97; its SOR prints to stderr, field.cpp:137); a converged
+    solve adds nothing."""
+    print(f"[solver] noconv {case}", flush=True)
+    Nx = 128 if case == "pcg_multi" else 32
+    st = small_setup(N=1, Nx=Nx, Ny=Nx, tol=1e-9)
+    solver, op, prec = {
+        "pcg": (_lib.SOLVER_PCG, _lib.OP_REF_GHOST, 0), "pcg_multi": (_lib.SOLVER_PCG, _lib.OP_REF_GHOST, 0),
+        "sor": (_lib.SOLVER_GS, _lib.OP_DIRICHLET_WALL, 0),
+        "mixed_jacobi": (_lib.SOLVER_PCG_MIXED, _lib.OP_DIRICHLET_WALL, _lib.PRECOND_JACOBI),
+        "spectral_mixed": (_lib.SOLVER_PCG_MIXED, _lib.OP_DIRICHLET_WALL, _lib.PRECOND_SPECTRAL),
+        "spectral_fp64": (_lib.SOLVER_SPECTRAL, _lib.OP_REF_GHOST, 0)}[case]
+    rel_tol = solver in (_lib.SOLVER_PCG_MIXED, _lib.SOLVER_SPECTRAL)
+    rho = _rho(st.domain.nx, st.domain.ny, 7)
+    for capped in (False, True):
+        d = _lib.DomainDesc.from_buffer_copy(st.domain)
+        d.solver, d.poisson_op, d.precond = solver, op, prec
+        d.tol_relative = 1 if rel_tol else 0
+        d.tolerance = 1e-10 if rel_tol else (1e-9 if solver == _lib.SOLVER_PCG else 1e-6)
+        d.max_iteration = 20000
+        if capped:
+            if solver in (_lib.SOLVER_PCG, _lib.SOLVER_GS):
+                d.max_iteration = 1
+            else:
+                d.tolerance = 1e-30  # below FP64 round-off of the residual
+        ctx = _lib.Context(d, st.species)
+        ctx.set_field(0, rho)
+        ctx.solve()
+        ctx.solve()
+        nc = int(ctx.errors()[_lib.ERRC_NOCONV])
+        ctx.clear_errors()
+        assert int(ctx.errors()[_lib.ERRC_NOCONV]) == 0
+        ctx.close()
+        assert nc == (2 if capped else 0), (case, capped, nc)
