@@ -25,7 +25,8 @@ OP_REF_GHOST, OP_DIRICHLET_WALL = 0, 1
 PRECOND_JACOBI, PRECOND_SPECTRAL = 0, 1
 FP64, FP32 = 0, 1
 NERR = 8
-(ERRC_GATHER_OOB, ERRC_DEPOSIT_OOB, ERRC_FIXED_OVF, ERRC_CAPACITY, ERRC_NOCONV, ERRC_NAN) = range(6)
+(ERRC_GATHER_OOB, ERRC_DEPOSIT_OOB, ERRC_FIXED_OVF, ERRC_CAPACITY, ERRC_NOCONV, ERRC_NAN,
+ ERRC_MCC_BOUND) = range(7)
 NSTAGE = 8
 STAGES = ["deposit", "rho", "solve", "efield", "push", "collide", "sort", "inject"]
 TIMING_ONLY = 0x10000

@@ -354,10 +354,13 @@ __global__ void inject_kernel(P* x, P* y, P* vx, P* vy, P* vz, uint32_t* id, int
     const u32x4 rp = philox4x32_10({(uint32_t)t, step, RNG_INJECT, emitter}, k0, k1);
     const double px = e.x0 + (e.x1 - e.x0) * u01_53(rp.a, rp.b);
     const double py = e.y0 + (e.y1 - e.y0) * u01_53(rp.c, rp.d);
+    // Init::SampleVel's three uniforms per component (init.cpp:110-114),
+    // six independent 53-bit draws from three Philox blocks
     const u32x4 r1 = philox4x32_10({(uint32_t)t, step, RNG_INJECT2 + 2 * which, emitter}, k0, k1);
     const u32x4 r2 = philox4x32_10({(uint32_t)t, step, RNG_INJECT2 + 2 * which + 1, emitter}, k0, k1);
+    const u32x4 r3 = philox4x32_10({(uint32_t)t, step, RNG_INJECT3 + which, emitter}, k0, k1);
     const double sx = u01_53(r1.a, r1.b) + u01_53(r1.c, r1.d) + u01_53(r2.a, r2.b) - 1.5;
-    const double sy = u01_53(r2.c, r2.d) + u01_24(r1.a ^ r2.d) + u01_24(r1.c ^ r2.b) - 1.5;
+    const double sy = u01_53(r2.c, r2.d) + u01_53(r3.a, r3.b) + u01_53(r3.c, r3.d) - 1.5;
     const double vxp = (vth_sqrt2 * sx + e.vdx * vel_norm) / vel_norm;
     const double vyp = (vth_sqrt2 * sy + e.vdy * vel_norm) / vel_norm;
     x[slot] = (P)px; y[slot] = (P)py; vx[slot] = (P)vxp; vy[slot] = (P)vyp; vz[slot] = (P)0;

@@ -56,6 +56,7 @@ enum : uint32_t {
   RNG_MCC_NEWION = 13,
   RNG_MCC_NEWION2 = 14,
   RNG_INJECT2 = 16,  // 16 + 2*target, 17 + 2*target
+  RNG_INJECT3 = 20,  // 20 + target
 };
 
 // ---- reductions ------------------------------------------------------------

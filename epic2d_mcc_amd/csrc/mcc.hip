@@ -71,6 +71,7 @@ struct MccArgs {
   int slot;  // batched runs: index of the simulation's step counter
   uint16_t* cand;  // candidate offsets, MCC_CHUNK per selection chunk
   int* cand_n;     // candidates per selection chunk
+  unsigned long long* errc;  // EPIC_ERRC_MCC_BOUND: collision probability above p_null
 };
 
 __device__ __forceinline__ void rotate(double g, double chi, double eta, double st, double ct, double sp, double cp,
@@ -112,7 +113,11 @@ __device__ __forceinline__ void mcc_candidate(const MccArgs<P>& a, uint32_t step
       double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, energy) : 0.0;
       const double t0 = s_el, t1 = t0 + s_ex, t2 = t1 + s_io;
       const double nu = a.gas_density * t2 * velocity;
-      if (R1 * a.p_null >= -expm1(-nu * a.dt)) continue;  // null collision
+      const double pc = -expm1(-nu * a.dt);
+      // a speed beyond the one nu_max covers: the particle collides with
+      // p_null instead of pc (counted, as an undercount)
+      if (pc > a.p_null) atomicAdd(&a.errc[EPIC_ERRC_MCC_BOUND], 1ull);
+      if (R1 * a.p_null >= pc) continue;  // null collision
       int type;
       if (a.ref_selection) {
         // two independent draws (collision.cpp:117,125)
@@ -189,6 +194,7 @@ __device__ __forceinline__ void mcc_candidate(const MccArgs<P>& a, uint32_t step
       const double nu_cx = a.gas_density * s_cx * g * a.vel_norm;
       const double nu = nu_el + nu_cx;
       const double pc = -expm1(-nu * a.dt);
+      if (pc > a.p_null) atomicAdd(&a.errc[EPIC_ERRC_MCC_BOUND], 1ull);
       if (R1 * a.p_null >= pc) continue;  // null collision
       // real collision: process by frequency share
       const double R = (R1 * a.p_null / pc) * nu;
@@ -479,6 +485,7 @@ static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   a.event_cap = M.event_cap;
   a.cand = kind == 0 ? M.cand : M.cand + M.cand_len_e;
   a.cand_n = kind == 0 ? M.cand_n : M.cand_n + M.cand_len_e / MCC_CHUNK;
+  a.errc = c->errc;
   return a;
 }
 

@@ -80,8 +80,14 @@ extern "C" {
 #define EPIC_ERRC_DEPOSIT_OOB 1 /* scatter outside the node grid (reference throws) */
 #define EPIC_ERRC_FIXED_OVF 2   /* fixed-point deposit overflow                     */
 #define EPIC_ERRC_CAPACITY 3    /* births/injections dropped for lack of capacity   */
-#define EPIC_ERRC_NOCONV 4      /* Poisson solve reached max_iteration              */
+#define EPIC_ERRC_NOCONV 4      /* a Poisson solve stopped above its tolerance (PCG
+                                   or SOR at max_iteration, the mixed and sine-
+                                   transform solves at their correction limit)     */
 #define EPIC_ERRC_NAN 5         /* non-finite velocity met in a push                */
+#define EPIC_ERRC_MCC_BOUND 6   /* a collision candidate's probability exceeded the
+                                   null-collision bound 1 - exp(-nu_max dt): a speed
+                                   beyond what nu_max covers (4x the table's top
+                                   speed); it collided with the bound instead      */
 
 typedef struct epic_ctx epic_ctx;
 

@@ -12,11 +12,13 @@
 // operation order, so that this file reproduces the reference bit for bit
 // wherever the reference is deterministic.
 //
-// Pinning (what checks each part; see tests/test_oracle_*.py):
+// Pinning (what checks each part, tests/test_oracle_*.py):
 //   setup/normalisation   closed-form plasma parameters (LDe, wpe, spwt) at
 //                         the 1e13 m^-3 / 1 eV workload, independent of code
 //   mt19937 + uniform     the C++ standard's 10000th-output value 4123659995
-//   Init (random)         moments of the Irwin-Hall velocity sum, bounds
+//   SampleVel / Init      Irwin-Hall moments of the three-uniform sum (mean,
+//                         variance kT/m, excess kurtosis -0.4, the 3 sigma
+//                         bound and its tail mass), uniform positions (KS)
 //   Scatter / ComputeRho  brute-force CIC/NGP weights on hand-placed
 //                         particles, charge conservation, edge/fold rules
 //   pcg()                 the reference's own linearalgebra/ templates,
@@ -27,11 +29,18 @@
 //   Push / Rewind         uniform-field closed form, harmonic-oscillator
 //                         frequency 2 asin(w dt/2)/dt, wrap/absorb rules
 //   KE / PE / momentum    closed forms on hand-built states
-//   cross sections        thresholds, zero below threshold, flag gating
-//   direct MCC            collision probability 1-exp(-nu dt) vs counts,
-//                         energy-loss bookkeeping (11.5 / 15.8 eV)
-//   null-collision MCC    same collision rate as the direct method (3 sigma),
-//                         elastic |g| conservation, isotropy
+//   cross sections        thresholds and continuity there, the elastic
+//                         fit's E -> 0 limit, the measured argon ionisation
+//                         maximum; the other fit coefficients are unpinned
+//   direct MCC            per-process counts against 1 - exp(-nu dt) and the
+//                         two-draw selection rule (5 sigma); per-process
+//                         kinematics: |g| kept (elastic), 11.5 eV removed
+//                         (excitation), e_sc + e_ej = E - 15.8 eV with the
+//                         ejected-energy law and the coplanar angles
+//                         (ionisation), isotropic cos(chi) (KS)
+//   Emitter::inject       positions uniform on the emitter box, velocity
+//                         moments per target species (positions drawn from a
+//                         seeded stream in place of std::random_device)
 //   Philox4x32-10         the Random123 known-answer vectors
 
 #include <algorithm>
@@ -1114,6 +1123,65 @@ void orc_set_conductors(void* h, const orc_conductor* c, int n) {
 }
 
 int orc_is_conductor(void* h, double x, double y) { return on_any_grid((Sim*)h, x, y) ? 1 : 0; }
+
+// n draws of Init::SampleVel(species, temp) (init.cpp:110-114) from the
+// global stream, in m/s.
+void orc_sample_vel(void* h, double mass, double temp, int n, double* out) {
+  Sim* m = (Sim*)h;
+  for (int k = 0; k < n; k++) out[k] = sample_vel(m, mass, temp);
+}
+
+// One call of collision_electron (collision.cpp:60-188) on the electron
+// velocity v3 (normalised), at table index eindex, position (x, y).  v3 is
+// updated in place; births are appended to species se and st as the
+// reference appends them.  Returns the process (0 elastic, 1 excitation,
+// 2 ionisation).
+int orc_collide_one(void* h, int se, int st, int eindex, double x, double y, double* v3) {
+  Sim* m = (Sim*)h;
+  long counts[3] = {0, 0, 0};
+  collision_electron(m, x, y, v3[0], v3[1], v3[2], eindex, se, st, counts);
+  return counts[0] ? 0 : (counts[1] ? 1 : 2);
+}
+
+// The emitter descriptor (emitter.h:8-20), laid out as the device
+// library's epic_emitter_desc.
+struct orc_emitter {
+  double x0, y0, x1, y1, temp;
+  int numparticle;
+  double vdx, vdy;
+  int species_idx1, species_idx2;
+};
+
+// Emitter::inject, emitter.cpp:5-36.  Positions: Domain::unirand
+// (domain.cpp:288-299) draws from a generator seeded by std::random_device
+// on every call, which no run can reproduce; here they come from one
+// mt19937(pos_seed) through the same uniform_real_distribution form
+// (x in [x0, x1), then y in [y0, y1)).  Velocities: SampleVel on the global
+// stream (vx, then vy, per target species), plus the drift times vel_norm,
+// normalised by vel_norm; vz = 0.
+void orc_inject(void* h, const orc_emitter* e, unsigned pos_seed) {
+  Sim* m = (Sim*)h;
+  std::mt19937 pg(pos_seed);
+  for (int p = 0; p < e->numparticle; p++) {
+    double x = std::uniform_real_distribution<double>(e->x0, e->x1)(pg);
+    double y = std::uniform_real_distribution<double>(e->y0, e->y1)(pg);
+    int targets[2] = {e->species_idx1, e->species_idx2};
+    int nt = e->species_idx1 != e->species_idx2 ? 2 : 1;
+    for (int w = 0; w < nt; w++) {
+      Sim::Sp& sp = m->sp[targets[w]];
+      double vx = sample_vel(m, sp.mass, e->temp) + e->vdx * m->vel_norm;
+      double vy = sample_vel(m, sp.mass, e->temp) + e->vdy * m->vel_norm;
+      double vz = 0.0;
+      vx /= m->vel_norm;
+      vy /= m->vel_norm;
+      vz /= m->vel_norm;
+      Particle q;
+      q.pos[0] = x; q.pos[1] = y;
+      q.vel[0] = vx; q.vel[1] = vy; q.vel[2] = vz;
+      sp.part_list.push_back(q);
+    }
+  }
+}
 
 int orc_last_iters(void* h) { return ((Sim*)h)->last_iters; }
 

@@ -94,6 +94,11 @@ def oracle():
         o.orc_init_species.restype = ctypes.c_int
         o.orc_solve.restype = ctypes.c_int
         o.orc_last_iters.argtypes = [ctypes.c_void_p]
+        o.orc_sample_vel.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, D]
+        o.orc_collide_one.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, D]
+        o.orc_collide_one.restype = ctypes.c_int
+        o.orc_inject.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
         o.orc_set_tolerance.argtypes = [ctypes.c_void_p, ctypes.c_double]
     return _o
 
@@ -137,6 +142,31 @@ class OracleSim:
 
     def init_species(self, s, gauss_seed=0):
         return self.o.orc_init_species(self.h, s, gauss_seed)
+
+    def sample_vel(self, mass, temp, n):
+        out = np.empty(n)
+        self.o.orc_sample_vel(self.h, mass, temp, n, dp(out))
+        return out
+
+    def build_mcc_tables(self):
+        self.o.orc_mcc_build_tables(self.h)
+
+    def collide_direct(self, se=0, st=1):
+        """handle_collisions once; returns (counts[3], energy removed in eV)."""
+        counts = (ctypes.c_long * 3)()
+        eloss = ctypes.c_double()
+        self.o.orc_collide_direct(self.h, se, st, counts, ctypes.byref(eloss))
+        return list(counts), eloss.value
+
+    def collide_one(self, v3, eindex, x=0.0, y=0.0, se=0, st=1):
+        v = np.ascontiguousarray(v3, dtype=np.float64).copy()
+        proc = self.o.orc_collide_one(self.h, se, st, eindex, x, y, dp(v))
+        return proc, v
+
+    def inject(self, e, pos_seed):
+        """e: the library's EmitterDesc (the oracle's orc_emitter has the same
+        layout)."""
+        self.o.orc_inject(self.h, ctypes.byref(e), pos_seed)
 
     def set_conductors(self, conds):
         """conds: a ctypes array of the library's epic_conductor_desc (the

@@ -90,3 +90,62 @@ def test_fused_step_with_emitter_equals_discrete():
         assert len(ga[0]) == len(gb[0])
         for k in range(6):
             assert np.array_equal(ga[k], gb[k]), (s, k)
+
+
+def _ks2(a, b):
+    """Two-sample Kolmogorov-Smirnov statistic and its asymptotic p-value."""
+    a, b = np.sort(a), np.sort(b)
+    allv = np.concatenate([a, b])
+    fa = np.searchsorted(a, allv, side="right") / len(a)
+    fb = np.searchsorted(b, allv, side="right") / len(b)
+    d = float(np.max(np.abs(fa - fb)))
+    ne = len(a) * len(b) / (len(a) + len(b))
+    lam = (np.sqrt(ne) + 0.12 + 0.11 / np.sqrt(ne)) * d
+    p = 2 * sum((-1) ** (j - 1) * np.exp(-2 * j * j * lam * lam) for j in range(1, 101))
+    return d, float(min(max(p, 0.0), 1.0))
+
+
+def test_emitter_distribution_matches_oracle():
+    """The device injection against the oracle's restatement of
+    Emitter::inject (emitter.cpp:5-36: positions uniform on the emitter box,
+    v = SampleVel(temp) + drift, normalised, vz = 0).  The two draw different
+    numbers (Philox keyed by (emitter, step, k) here; the reference's
+    positions come from std::random_device, the oracle's from a seeded
+    stream), so they are compared in distribution: two-sample KS on x, y,
+    vx, vy for each target species; the Irwin-Hall bound |v - drift| <=
+    3 sqrt(kT/m) on every device draw; vx and vy uncorrelated."""
+    print("[emitter] distribution vs oracle", flush=True)
+    n, steps = 20000, 5
+    st = small_setup(N=16, Nx=32, Ny=32, capacity_factor=(16 + n * steps) / 16 + 1)
+    d = st.domain
+    e = _lib.EmitterDesc(x0=2.0, y0=4.0, x1=6.0, y1=28.0, temp=3.0, numparticle=n, vdx=1.5, vdy=-0.5,
+                         species_idx1=0, species_idx2=1)
+    ctx = _lib.Context(d, st.species)
+    for s in range(2):
+        ctx.upload(s, [], [], [], [], [])
+    ctx.add_emitter(e)
+    for ts in range(steps):
+        ctx.inject(ts)
+    assert not ctx.errors().any()
+    from oracle_api import OracleSim
+    o = OracleSim(d, st.species)
+    for s in range(2):
+        o.set_particles(s, [], [], [], [], [])
+    for ts in range(steps):
+        o.inject(e, 1000 + ts)
+    for s in range(2):
+        g = ctx.download(s)
+        c = o.particles(s)
+        assert len(g[0]) == len(c[0]) == n * steps
+        sig = np.sqrt(3.0 * config.QE / st.species[s].mass) / d.vel_norm
+        for k, name in enumerate(("x", "y", "vx", "vy")):
+            dks, p = _ks2(g[k], c[k])
+            print(f"  species {s} {name}: KS {dks:.4f} p {p:.3f}", flush=True)
+            assert p > 1e-3, (s, name, dks, p)
+        assert np.all(np.abs(g[2] - 1.5) <= 3 * sig * (1 + 1e-6))
+        assert np.all(np.abs(g[3] + 0.5) <= 3 * sig * (1 + 1e-6))
+        assert np.all(g[4] == 0.0)
+        r = np.corrcoef(g[2], g[3])[0, 1]
+        assert abs(r) < 5 / np.sqrt(len(g[2])), r
+        assert np.all((g[0] >= 2.0) & (g[0] < 6.0) & (g[1] >= 4.0) & (g[1] < 28.0))
+    ctx.close()

@@ -203,3 +203,132 @@ def test_candidate_lists_independent_of_layout():
         assert np.array_equal(outs[0][k], outs[1][k])
     assert np.array_equal(outs[0][3], outs[1][3])
     assert outs[0][3][0] > 0
+
+
+def test_coupled_loop_matches_direct_method_1000_steps():
+    """The whole loop with collisions on both sides, 1000 steps: periodic
+    16x16 cells, 25,600 electrons loaded at 8 eV and 25,600 static Ar+
+    (ionfixed, main.cpp:419), argon at 2e21 m^-3 with the reference's e-Ar
+    cross sections, FP64, PCG.  Each step scatters, solves, pushes the
+    electrons and collides them (main.cpp:395-423); the electrons cool by
+    excitation and ionisation while ionisation adds electrons and ions.  The
+    GPU (null collision, Philox per (id, step)) and the oracle (the
+    reference's direct method on its global mt19937) draw different numbers,
+    so the runs agree in distribution: per-process counts and the energy
+    removed by thresholds within 3 sigma (sigma^2 = the sum of both runs'
+    Poisson variances), and the births equal the ionisations on each side."""
+    print("[mcc] coupled loop, 1000 steps", flush=True)
+    N, K = 25600, 1000
+    specs = [config.SpeciesSpec("electron", ME, N, 1.0, -1, 1.0),
+             config.SpeciesSpec("ion", xsec.M_AR, N, 0.026, 1, 0.0)]
+    st = config.derive(16, 16, 0.0, specs, 1e15, 0.05, bc="pbc", tolerance=1e-8, tol_relative=0, ionfixed=1,
+                       capacity_factor=1.5)
+    d = st.domain
+    rng = np.random.default_rng(21)
+    sig = np.sqrt(8.0 * config.QE / ME) / d.vel_norm
+    e = (rng.random(N) * 16, rng.random(N) * 16, rng.normal(0, sig, N), rng.normal(0, sig, N),
+         rng.normal(0, sig, N))
+    i = (rng.random(N) * 16, rng.random(N) * 16, np.zeros(N), np.zeros(N), np.zeros(N))
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, *e)
+    ctx.upload(1, *i)
+    ctx.mcc_setup(xsec.mcc_desc(xsec.reference_e_ar(), gas_density=GAS, gas_temp_K=0.0, enable=(1, 1, 1, 0, 0)))
+    ctx.initial_field()
+    for ts in range(K):
+        ctx.step(ts)
+    gs = ctx.mcc_stats()
+    g = np.array(gs["count"][:3], dtype=np.float64)
+    g_loss = gs["eloss_ev"]
+    n_e, n_i = ctx.count(0), ctx.count(1)
+    assert not ctx.errors().any(), ctx.errors()
+    ctx.close()
+    o = OracleSim(d, st.species, gas=(1, 1, 1, GAS))
+    o.build_mcc_tables()
+    o.set_particles(0, *e)
+    o.set_particles(1, *i)
+    o.initial_field()
+    c = np.zeros(3)
+    c_loss = 0.0
+    for ts in range(K):
+        o.step(pushed=[1, 0])
+        cnt, loss = o.collide_direct()
+        c += cnt
+        c_loss += loss
+    print(f"  gpu {g.astype(int).tolist()} loss {g_loss:.0f} eV; oracle {c.astype(int).tolist()} loss {c_loss:.0f} eV",
+          flush=True)
+    for p in range(3):
+        assert abs(g[p] - c[p]) <= 3 * np.sqrt(g[p] + c[p]), (p, g[p], c[p])
+    var_loss = 11.5 ** 2 * (g[1] + c[1]) + 15.8 ** 2 * (g[2] + c[2])
+    assert abs(g_loss - c_loss) <= 3 * np.sqrt(var_loss)
+    assert g_loss == pytest.approx(11.5 * g[1] + 15.8 * g[2], rel=1e-9)
+    assert n_e == N + g[2] and n_i == N + g[2]
+    assert o.o.orc_count(o.h, 0) == N + c[2] and o.o.orc_count(o.h, 1) == N + c[2]
+    assert c[1] > 1000 and c[2] > 500
+
+
+def _ion_only(N, gas_temp_K, enable, v3):
+    specs = [config.SpeciesSpec("electron", ME, 16, 1.0, -1, 1.0),
+             config.SpeciesSpec("ion", xsec.M_AR, N, 0.026, 1, 0.0)]
+    st = config.derive(16, 16, 0.0, specs, 1e15, 0.05, bc="pbc")
+    d = st.domain
+    rng = np.random.default_rng(13)
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, np.full(16, 1.0), np.full(16, 1.0), np.zeros(16), np.zeros(16), np.zeros(16))
+    v = np.tile(np.asarray(v3, dtype=np.float64) / d.vel_norm, (N, 1))
+    ctx.upload(1, rng.random(N) * 16, rng.random(N) * 16, v[:, 0], v[:, 1], v[:, 2])
+    ctx.mcc_setup(xsec.mcc_desc(xsec.synthetic_ar(), electron_sp=-1, ion_sp=1, gas_density=1e24,
+                                gas_temp_K=gas_temp_K, enable=enable))
+    return ctx, d, v
+
+
+def test_ion_charge_exchange_takes_the_neutral_maxwellian():
+    """Resonant charge exchange hands the ion the neutral's velocity: ions
+    loaded as a cold beam (every ion at 20 thermal speeds along x) that
+    exchange charge with 300 K argon leave with velocities that are the
+    300 K Maxwellian, N(0, kT/M) per component (KS), independent of the
+    beam; the rest keep the beam velocity exactly."""
+    print("[mcc] Ar+ charge exchange", flush=True)
+    N = 200000
+    s_n = np.sqrt(config.K_b * 300.0 / xsec.M_AR)
+    ctx, d, v = _ion_only(N, 300.0, (0, 0, 0, 0, 1), (20 * s_n, 0.0, 0.0))
+    ctx.collide(0)
+    x, y, vx, vy, vz, ids = ctx.download(1)
+    vv = np.stack([vx, vy, vz], axis=1) * d.vel_norm
+    changed = np.any(vv != v[ids] * d.vel_norm, axis=1)
+    ncx = int(np.count_nonzero(changed))
+    assert ncx == ctx.mcc_stats()["count"][4] and ncx > 2000
+    from math import erf, sqrt
+    for k in range(3):
+        z = np.sort(vv[changed, k] / s_n)
+        cdf = 0.5 * (1 + np.array([erf(t / sqrt(2)) for t in z]))
+        n = len(z)
+        dks = max(np.max(np.arange(1, n + 1) / n - cdf), np.max(cdf - np.arange(n) / n))
+        assert dks * np.sqrt(n) < 1.95, (k, dks)   # KS at the 0.1 % level
+    ctx.close()
+
+
+def test_ion_elastic_conserves_relative_speed():
+    """Ar+ - Ar elastic scattering off neutrals at rest (gas_temp_K = 0):
+    equal masses put the centre of mass at v/2 and the relative speed |v|
+    is kept, so each scattered ion ends at v/2 + (|v|/2) n with n a unit
+    vector, isotropic in the centre-of-mass frame (cos(v, n) uniform)."""
+    print("[mcc] Ar+ elastic", flush=True)
+    N = 200000
+    s_n = np.sqrt(config.K_b * 300.0 / xsec.M_AR)
+    ctx, d, v = _ion_only(N, 0.0, (0, 0, 0, 1, 0), (3 * s_n, 4 * s_n, 0.0))
+    ctx.collide(0)
+    x, y, vx, vy, vz, ids = ctx.download(1)
+    v1 = np.stack([vx, vy, vz], axis=1)
+    v0 = v[ids]
+    changed = np.any(v1 != v0, axis=1)
+    nel = int(np.count_nonzero(changed))
+    assert nel == ctx.mcc_stats()["count"][3] and nel > 2000
+    half = np.linalg.norm(v0[changed], axis=1) / 2
+    nvec = v1[changed] - v0[changed] / 2
+    assert np.allclose(np.linalg.norm(nvec, axis=1), half, rtol=1e-12, atol=0)
+    c = np.sum(nvec * v0[changed], axis=1) / (half * 2 * half)
+    u = np.sort((c + 1) / 2)
+    n = len(u)
+    dks = max(np.max(np.arange(1, n + 1) / n - u), np.max(u - np.arange(n) / n))
+    assert dks * np.sqrt(n) < 1.95
+    ctx.close()
