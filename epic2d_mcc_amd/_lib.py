@@ -152,6 +152,7 @@ def lib():
         "epic_scatter": [P, ctypes.c_int],
         "epic_compute_rho": [P],
         "epic_solve_potential": [P, ctypes.POINTER(ctypes.c_int), D],
+        "epic_solve_potential_noalloc": [P, ctypes.POINTER(ctypes.c_int), D],
         "epic_efield": [P],
         "epic_push": [P, ctypes.c_int, ctypes.c_int],
         "epic_rewind": [P, ctypes.c_int],

@@ -501,6 +501,23 @@ int epic_solve_potential(epic_ctx* h, int* iters, double* err) {
   return EPIC_OK;
 }
 
+int epic_solve_potential_noalloc(epic_ctx* h, int* iters, double* err) {
+  if (!h) return EPIC_ERR_ARG;
+  Ctx* c = &h->c;
+  const epic_domain_desc& d = c->dom;
+  if (!(d.solver == EPIC_SOLVER_PCG_MIXED && d.precond == EPIC_PRECOND_SPECTRAL))
+    return fail(c, EPIC_ERR_ARG, "epic_solve_potential_noalloc: the context's solver must be the FP32 sine-transform "
+                                 "mixed solve (EPIC_SOLVER_PCG_MIXED with EPIC_PRECOND_SPECTRAL)");
+  int it = 0, status = 0;
+  double e = 0;
+  const char* why = "";
+  CK(launch_spectral_solve_noalloc(c, &it, &e, &status, &why), "solve (no allocation)");
+  if (status) return fail(c, status, why);
+  if (iters) *iters = it;
+  if (err) *err = e;
+  return EPIC_OK;
+}
+
 int epic_efield(epic_ctx* h) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;

@@ -251,6 +251,10 @@ hipError_t launch_inject(Ctx* c, int step);
 int64_t sort_key_space(int nx, int ny);  // size of the tile-major cell key space
 // field.hip
 hipError_t launch_solve(Ctx* c, int* iters, double* err);
+// The FP32 sine-transform solve with its workspace in a static device array
+// (no allocation); *status = an EPIC_ERR_* code and *why its text when the
+// context cannot use it.
+hipError_t launch_spectral_solve_noalloc(Ctx* c, int* solves, double* relres, int* status, const char** why);
 hipError_t launch_conductor_rho(Ctx* c);
 hipError_t launch_efield(Ctx* c);
 hipError_t launch_pack_E2(Ctx* c);

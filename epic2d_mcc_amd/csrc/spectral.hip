@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "device_common.hpp"
@@ -305,8 +306,7 @@ static bool tile32_ok(int m, int n, int k) { return m % 32 == 0 && n % 32 == 0 &
 
 // The FP32 transform runs on sgemm_tile32 (the residual kernel then writes
 // its packed FP32 operand).
-static bool spec_mfma_ok(const Ctx* c) {
-  const SpecState* S = (const SpecState*)c->blas;
+static bool spec_mfma_ok(const Ctx* c, const SpecState* S) {
   const int P = S->L.pp, Q = S->L.qq;
   return c->spec_mfma && tile32_ok(P, Q, P) && tile32_ok(P, Q, Q);
 }
@@ -326,15 +326,14 @@ static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const float* A
 
 // phi (+)= A^-1 src over the unknowns, in precision T.
 template <typename T>
-static hipError_t spec_apply(Ctx* c, const double* src, double* phi, bool overwrite, const T* Sp, const T* Sq,
-                             const T* G, T* X, T* Y) {
-  SpecState* S = (SpecState*)c->blas;
+static hipError_t spec_apply(Ctx* c, SpecState* S, const double* src, double* phi, bool overwrite, const T* Sp,
+                             const T* Sq, const T* G, T* X, T* Y) {
   const SpecLayout L = S->L;
   const int n = L.p * L.q;
   const int P = L.pp, Q = L.qq;
   const int grid = std::min((n + 255) / 256, 4096);
   const int gridP = std::min((P * Q + 255) / 256, 4096);
-  if (sizeof(T) == 4 && spec_mfma_ok(c)) {
+  if (sizeof(T) == 4 && spec_mfma_ok(c, S)) {
     // FP32 on the hand-written MFMA kernel: the Hadamard scaling by G in
     // the second product's epilogue, the unpacking into phi in the fourth's
     const float* Spf = (const float*)Sp;
@@ -375,11 +374,10 @@ static int residual_blocks(const Ctx* c) { return std::min((c->dom.nx * c->dom.n
 // given): phi := 0 in the same kernel and r = b (phi itself is not read),
 // partials to the second region; b is built from rho there when src.rho is
 // given.
-static hipError_t residual(Ctx* c, const double* b, const double* phi, double* r, float* X, double* zero_phi = nullptr,
-                           RhsSrc src = RhsSrc{nullptr, nullptr, 0, 0}) {
+static hipError_t residual(Ctx* c, const SpecLayout& L, const double* b, const double* phi, double* r, float* X,
+                           double* zero_phi = nullptr, RhsSrc src = RhsSrc{nullptr, nullptr, 0, 0}) {
   const epic_domain_desc& d = c->dom;
   const int g = residual_blocks(c);
-  const SpecLayout L = c->blas ? ((SpecState*)c->blas)->L : SpecLayout{};
   double* part = c->ws_part + (zero_phi ? SPEC_PART_B : 0);
   if (d.poisson_op == EPIC_OP_REF_GHOST)
     hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_REF_GHOST>, dim3(g), dim3(256), 0, c->stream, b, phi, r, d.nx,
@@ -421,45 +419,49 @@ static hipError_t wait_host_scalars(Ctx* c, unsigned long long seq, double* bb, 
   return hipSuccess;
 }
 
-// Solve A phi = b to ||b - A phi|| <= tol ||b||.  fp32_inner: the corrections
-// are FP32 sine-transform solves (mixed precision); otherwise FP64.  rho
-// (when given): b is first built from it, b = -rho dx^2 s on the unknowns.
+// Solve A phi = b to ||b - A phi|| <= tol ||b|| on the workspace S.
+// fp32_inner: the corrections are FP32 sine-transform solves (mixed
+// precision); otherwise FP64.  rho (when given): b is first built from it,
+// b = -rho dx^2 s on the unknowns.  poll: the convergence check's norms come
+// through the pinned host words (c->hscal); otherwise by a plain copy.
 // Returns the number of transform solves in *solves and the final relative
 // residual in *relres.
-hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner, double tol, int max_ref,
-                                 int* solves, double* relres, const double* rho, double dx2, double s) {
-  EPIC_HIP_TRY(spec_init(c));
-  if (!c->hscal) {
-    EPIC_HIP_TRY(hipHostMalloc((void**)&c->hscal, sizeof(HostScal), hipHostMallocCoherent));
-    std::memset(c->hscal, 0, sizeof(HostScal));
-  }
-  SpecState* S = (SpecState*)c->blas;
+static hipError_t spectral_solve_on(Ctx* c, SpecState* S, double* b, double* phi, bool fp32_inner, double tol,
+                                    int max_ref, int* solves, double* relres, const double* rho, double dx2,
+                                    double s, bool poll) {
   double* dscal = c->ws_scalar + 4;  // [4] ||b||^2, [5] ||r||^2
-  float* Xpack = fp32_inner && spec_mfma_ok(c) ? S->X32 : nullptr;
+  float* Xpack = fp32_inner && spec_mfma_ok(c, S) ? S->X32 : nullptr;
+  const SpecLayout& L = S->L;
   // b from rho, phi := 0 and r = b in one kernel (the corrections write
   // phi's interior)
-  EPIC_HIP_TRY(residual(c, b, nullptr, c->r64, Xpack, phi, RhsSrc{rho, b, dx2, s}));
+  EPIC_HIP_TRY(residual(c, L, b, nullptr, c->r64, Xpack, phi, RhsSrc{rho, b, dx2, s}));
   int k = 0;
   bool checked = false;
   double host[2] = {0, 0};
   for (; k < max_ref; k++) {
     c->efield_done = false;
     if (fp32_inner)
-      EPIC_HIP_TRY(spec_apply<float>(c, c->r64, phi, k == 0, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32));
+      EPIC_HIP_TRY(spec_apply<float>(c, S, c->r64, phi, k == 0, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32));
     else
-      EPIC_HIP_TRY(spec_apply<double>(c, c->r64, phi, k == 0, S->Sp64, S->Sq64, S->G64, S->X64, S->Y64));
-    EPIC_HIP_TRY(residual(c, b, phi, c->r64, Xpack));
+      EPIC_HIP_TRY(spec_apply<double>(c, S, c->r64, phi, k == 0, S->Sp64, S->Sq64, S->G64, S->X64, S->Y64));
+    EPIC_HIP_TRY(residual(c, L, b, phi, c->r64, Xpack));
     // the FP32 correction gains ~6 digits: two applications reach 1e-10 and
     // more; check the true residual after the second (the norms are formed
     // only for a check)
     if (k + 1 >= (fp32_inner ? 2 : 1)) {
-      const unsigned long long seq = ++c->hscal_seq;
-      EPIC_HIP_TRY(norms(c, dscal, false, seq));
-      // E of this phi, in case it is the final one (a further correction
-      // launches it again)
-      if (c->efield_in_solve) EPIC_HIP_TRY(launch_efield(c));
-      c->efield_done = c->efield_in_solve;
-      EPIC_HIP_TRY(wait_host_scalars(c, seq, &host[0], &host[1]));
+      if (poll) {
+        const unsigned long long seq = ++c->hscal_seq;
+        EPIC_HIP_TRY(norms(c, dscal, false, seq));
+        // E of this phi, in case it is the final one (a further correction
+        // launches it again)
+        if (c->efield_in_solve) EPIC_HIP_TRY(launch_efield(c));
+        c->efield_done = c->efield_in_solve;
+        EPIC_HIP_TRY(wait_host_scalars(c, seq, &host[0], &host[1]));
+      } else {
+        EPIC_HIP_TRY(norms(c, dscal, false, 0));
+        EPIC_HIP_TRY(hipMemcpyAsync(host, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
+      }
       checked = true;
       if (sqrt(host[1]) <= tol * sqrt(host[0]) || host[1] == 0) {
         k++;
@@ -477,6 +479,141 @@ hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner
   // max_ref corrections left the residual above tolerance (or not finite)
   if (!(*relres <= tol)) c->errc_host[EPIC_ERRC_NOCONV]++;
   return hipSuccess;
+}
+
+hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner, double tol, int max_ref,
+                                 int* solves, double* relres, const double* rho, double dx2, double s) {
+  EPIC_HIP_TRY(spec_init(c));
+  if (!c->hscal) {
+    EPIC_HIP_TRY(hipHostMalloc((void**)&c->hscal, sizeof(HostScal), hipHostMallocCoherent));
+    std::memset(c->hscal, 0, sizeof(HostScal));
+  }
+  return spectral_solve_on(c, (SpecState*)c->blas, b, phi, fp32_inner, tol, max_ref, solves, relres, rho, dx2, s,
+                           true);
+}
+
+// ---- the same FP32 solve without allocation ---------------------------------
+// The transform matrices and work arrays of launch_spectral_solve come from a
+// file-scope device array instead of hipMalloc on the context's first solve,
+// and the convergence check reads the norms by a copy instead of pinned host
+// words.  Dimensions are padded to multiples of 64 (at least 64) so that
+// every product runs on sgemm_tile32, which takes K <= 512: transform
+// dimensions up to 512, i.e. node grids up to 514 x 514 (Dirichlet walls) or
+// 512 x 512 (pcgsolver's operator).
+constexpr int NOALLOC_MAX_DIM = 512;
+constexpr size_t NOALLOC_FLOATS = (size_t)5 * NOALLOC_MAX_DIM * NOALLOC_MAX_DIM;  // Sp, Sq, G, X, Y: 5 MiB
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static float* noalloc_ptr[NOALLOC_MAX_DEVICES];       // scratch's device address, per device
+static SpecLayout noalloc_built[NOALLOC_MAX_DEVICES];  // layout of the matrices scratch holds
+static bool noalloc_valid[NOALLOC_MAX_DEVICES];
+
+static int pad64(int p) { return std::max(64, (p + 63) / 64 * 64); }
+
+// S_p, S_q, G of the layout into scratch, X and Y zeroed (their padding
+// must be zero), in the values spec_init computes (FP64 arithmetic rounded
+// to FP32).
+__global__ void noalloc_build_kernel(float* ws, SpecLayout L) {
+  const int P = L.pp, Q = L.qq, p = L.p, q = L.q;
+  float* Sp = ws;
+  float* Sq = Sp + (size_t)P * P;
+  float* G = Sq + (size_t)Q * Q;
+  float* X = G + (size_t)P * Q;
+  float* Y = X + (size_t)P * Q;
+  const double cc = (2.0 / (p + 1)) * (2.0 / (q + 1));
+  const size_t tot = (size_t)P * P + (size_t)Q * Q + (size_t)3 * P * Q;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    if (t < (size_t)P * P) {
+      const int a = (int)(t % P), k = (int)(t / P);
+      Sp[t] = (a < p && k < p) ? (float)sin(M_PI * (double)(a + 1) * (k + 1) / (p + 1)) : 0.f;
+    } else if (t < (size_t)P * P + (size_t)Q * Q) {
+      const size_t u = t - (size_t)P * P;
+      const int a = (int)(u % Q), k = (int)(u / Q);
+      Sq[u] = (a < q && k < q) ? (float)sin(M_PI * (double)(a + 1) * (k + 1) / (q + 1)) : 0.f;
+    } else {
+      const size_t u = t - (size_t)P * P - (size_t)Q * Q;
+      const size_t m = u % ((size_t)P * Q);
+      const int which = (int)(u / ((size_t)P * Q));
+      if (which == 0) {
+        const int k = (int)(m % P), l = (int)(m / P);
+        float v = 0.f;
+        if (k < p && l < q) {
+          const double lam = 2 * cos(M_PI * (k + 1) / (p + 1)) - 2 + 2 * cos(M_PI * (l + 1) / (q + 1)) - 2;
+          v = (float)(cc / lam);
+        }
+        G[m] = v;
+      } else {
+        (which == 1 ? X : Y)[m] = 0.f;
+      }
+    }
+  }
+}
+
+hipError_t launch_spectral_solve_noalloc(Ctx* c, int* solves, double* relres, int* status, const char** why) {
+  const epic_domain_desc& d = c->dom;
+  SpecLayout L = spec_layout(c);
+  L.pp = pad64(L.p);
+  L.qq = pad64(L.q);
+  *status = 0;
+  if (L.pp > NOALLOC_MAX_DIM || L.qq > NOALLOC_MAX_DIM) {
+    *status = EPIC_ERR_CAPACITY;
+    *why = "grid beyond the no-allocation solve's maximum (transform dimensions up to 512: nodes up to 514 x 514 "
+           "with Dirichlet walls, 512 x 512 with pcgsolver's operator)";
+    return hipSuccess;
+  }
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES) {
+    *status = EPIC_ERR_ARG;
+    *why = "device index beyond the no-allocation solve's table";
+    return hipSuccess;
+  }
+  float* ws = nullptr;
+  bool build = false;
+  {
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[c->device]) {
+      // the array's device address on this device, looked up once with the
+      // device current (the host-side symbol is not a device address)
+      int cur = 0;
+      EPIC_HIP_TRY(hipGetDevice(&cur));
+      if (cur != c->device) EPIC_HIP_TRY(hipSetDevice(c->device));
+      void* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (cur != c->device) EPIC_HIP_TRY(hipSetDevice(cur));
+      EPIC_HIP_TRY(e);
+      noalloc_ptr[c->device] = (float*)ptr;
+    }
+    ws = noalloc_ptr[c->device];
+    const SpecLayout& B = noalloc_built[c->device];
+    build = !noalloc_valid[c->device] || B.p != L.p || B.q != L.q || B.pp != L.pp || B.qq != L.qq;
+    if (build) {
+      noalloc_built[c->device] = L;
+      noalloc_valid[c->device] = true;
+    }
+  }
+  if (build) {
+    hipLaunchKernelGGL(noalloc_build_kernel, dim3(1024), dim3(256), 0, c->stream, ws, L);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
+  SpecState S{};
+  S.L = L;
+  S.Sp32 = ws;
+  S.Sq32 = S.Sp32 + (size_t)L.pp * L.pp;
+  S.G32 = S.Sq32 + (size_t)L.qq * L.qq;
+  S.X32 = S.G32 + (size_t)L.pp * L.qq;
+  S.Y32 = S.X32 + (size_t)L.pp * L.qq;
+  const double tol = d.tol_relative ? d.tolerance : 1e-12;
+  const double s = (d.L * d.L) / (d.LDe * d.LDe);
+  const bool mfma = c->spec_mfma;
+  c->spec_mfma = 1;  // the products run on sgemm_tile32 whatever EPIC_SPEC_MFMA says
+  const bool eis = c->efield_in_solve;
+  c->efield_in_solve = false;
+  const hipError_t e =
+      spectral_solve_on(c, &S, c->b64, c->phi, true, tol, 6, solves, relres, c->rho, d.dx * d.dx, s, false);
+  c->spec_mfma = mfma;
+  c->efield_in_solve = eis;
+  return e;
 }
 
 }  // namespace epic

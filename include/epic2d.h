@@ -210,6 +210,19 @@ int epic_load_species(epic_ctx* ctx, int sp, int64_t n, int vz_thermal);
 int epic_scatter(epic_ctx* ctx, int sp);               /* Species::ScatterSpecies_serial (species.h:71) */
 int epic_compute_rho(epic_ctx* ctx);                   /* Domain::ComputeRho (domain.h:169) */
 int epic_solve_potential(epic_ctx* ctx, int* iters, double* err); /* FieldSolve::PotentialSolver (field.h:26) */
+/* No counterpart in the reference: epic_solve_potential without any device
+ * or pinned-host allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The context's solver must be EPIC_SOLVER_PCG_MIXED with
+ * EPIC_PRECOND_SPECTRAL; the result is that solve's, within its tolerance.
+ * epic_solve_potential allocates the sine-transform matrices and work
+ * arrays on a context's first solve; this variant keeps them in a static
+ * 5 MiB device array per GPU (held until the process ends), which limits the
+ * transform dimensions to 512: node grids up to 514 x 514 with Dirichlet
+ * walls, 512 x 512 with pcgsolver's operator (EPIC_ERR_CAPACITY beyond).
+ * Every call on a GPU shares that array: calls that overlap on one GPU (on
+ * different streams or threads) race. */
+int epic_solve_potential_noalloc(epic_ctx* ctx, int* iters, double* err);
 int epic_efield(epic_ctx* ctx);                        /* FieldSolve::CalculateEfield (field.h:25) */
 int epic_push(epic_ctx* ctx, int sp, int sub_cycle);   /* Species::Push_species_serial (species.h:70) */
 int epic_rewind(epic_ctx* ctx, int sp);                /* Species::Rewind_species (species.h:72) */
