@@ -524,7 +524,8 @@ class ServeConfig(ctypes.Structure):
                 ("gas_density", ctypes.c_double), ("dt_coeff", ctypes.c_double), ("sort_interval", ctypes.c_int32),
                 ("budget_bytes", ctypes.c_double), ("max_members", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("tune_steps", ctypes.c_int32), ("refine", ctypes.c_int32),
-                ("mcc_template", ctypes.POINTER(MccDesc)), ("progress_s", ctypes.c_double)]
+                ("mcc_template", ctypes.POINTER(MccDesc)), ("progress_s", ctypes.c_double),
+                ("health_interval", ctypes.c_int32), ("fail_relres", ctypes.c_double), ("fault_job", ctypes.c_int32)]
 
 
 class ServeJob(ctypes.Structure):
@@ -533,7 +534,8 @@ class ServeJob(ctypes.Structure):
                 ("steps", ctypes.c_int32), ("collisional", ctypes.c_int32), ("two_stream", ctypes.c_int32),
                 ("te", ctypes.c_double), ("ti", ctypes.c_double), ("vd", ctypes.c_double),
                 ("particles", ctypes.c_int64), ("bytes", ctypes.c_int64), ("ke", ctypes.c_double * 3),
-                ("phi_l2", ctypes.c_double), ("collisions", ctypes.c_int64)]
+                ("phi_l2", ctypes.c_double), ("collisions", ctypes.c_int64), ("failed", ctypes.c_int32),
+                ("relres", ctypes.c_double), ("errors", ctypes.c_int64 * NERR)]
 
 
 class ServeReport(ctypes.Structure):
@@ -543,7 +545,8 @@ class ServeReport(ctypes.Structure):
                 ("wait_p50_s", ctypes.c_double), ("wait_p99_s", ctypes.c_double),
                 ("particle_steps", ctypes.c_double), ("throughput", ctypes.c_double),
                 ("max_members", ctypes.c_int32), ("peak_bytes", ctypes.c_double), ("batch_steps", ctypes.c_int64),
-                ("graph_captures", ctypes.c_int64), ("push_variant", ctypes.c_int32), ("node_block", ctypes.c_int32)]
+                ("graph_captures", ctypes.c_int64), ("push_variant", ctypes.c_int32), ("node_block", ctypes.c_int32),
+                ("jobs_failed", ctypes.c_int32)]
 
 
 _host = None

@@ -154,7 +154,7 @@ __global__ void bnorm_kernel(const BSim* sims, const double* part_b, const doubl
   if (threadIdx.x == 0) {
     const double rel = b > 0 ? sqrt(r / b) : 0.0;
     relres[blockIdx.x] = rel;
-    if (rel > S.tol) atomicAdd(&S.errc[EPIC_ERRC_NOCONV], 1ull);
+    if (!(rel <= S.tol)) atomicAdd(&S.errc[EPIC_ERRC_NOCONV], 1ull);  // NaN counts too
   }
 }
 

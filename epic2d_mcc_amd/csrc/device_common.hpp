@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "epic2d.h"
+
 namespace epic {
 
 // ---- Philox4x32-10 ---------------------------------------------------------
@@ -194,6 +196,15 @@ __device__ __forceinline__ void xcd_chunk_range(int64_t nchunks, int64_t* begin,
   *begin = lo + local;
   *end = hi;
   *step = per;
+}
+
+// Push error tally of one thread: gathers outside the grid in the low half,
+// particles whose new position is not finite (a non-finite velocity or
+// field) in the high half; such a particle leaves the list.
+constexpr int NAN_UNIT = 1 << 16;
+__device__ __forceinline__ void flush_push_errors(unsigned long long* errc, int bad) {
+  if (bad & (NAN_UNIT - 1)) atomicAdd(&errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)(bad & (NAN_UNIT - 1)));
+  if (bad >= NAN_UNIT) atomicAdd(&errc[EPIC_ERRC_NAN], (unsigned long long)(bad / NAN_UNIT));
 }
 
 }  // namespace epic

@@ -353,7 +353,11 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
         vy[k] = vy[k] + (a.c1 * ey) * a.c2;
         P px = x[k] + (a.c3 * vx[k]) * a.c2;
         P py = y[k] + (a.c3 * vy[k]) * a.c2;
-        if (a.bc == EPIC_BC_PBC) {
+        if (isnan(px) || isnan(py)) {
+          bad += NAN_UNIT;
+          px = p_nan<P>();
+          live[k] = false;
+        } else if (a.bc == EPIC_BC_PBC) {
           if (px < a.x0) px = px + a.Lx;
           else if (px >= a.xhi) px = px - a.Lx;
           if (py < a.y0) py = py + a.Ly;
@@ -391,7 +395,7 @@ __global__ __launch_bounds__(PART_BLOCK) void particle_kernel(PartArgs<P> a) {
         x[k] = px;
         y[k] = py;
       }
-      if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
+      if (bad) flush_push_errors(a.errc, bad);
     }
 
     // deposit cells and offsets of the (new) positions, computed once
@@ -573,7 +577,11 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
       vy[k] = vy[k] + (a.c1 * ey) * a.c2;
       P px = x[k] + (a.c3 * vx[k]) * a.c2;
       P py = y[k] + (a.c3 * vy[k]) * a.c2;
-      if (a.bc == EPIC_BC_PBC) {
+      if (isnan(px) || isnan(py)) {
+        bad += NAN_UNIT;
+        px = p_nan<P>();
+        live[k] = false;
+      } else if (a.bc == EPIC_BC_PBC) {
         if (px < a.x0) px = px + a.Lx;
         else if (px >= a.xhi) px = px - a.Lx;
         if (py < a.y0) py = py + a.Ly;
@@ -596,7 +604,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
       x[k] = px;
       y[k] = py;
     }
-    if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
+    if (bad) flush_push_errors(a.errc, bad);
     DepCells<P, PPT> dc;
     if (DEP) dep_cells<P, PPT>(a, live, x, y, dc);
     st_run<P, PPT>(a.vx, wb, lane, n, vx);

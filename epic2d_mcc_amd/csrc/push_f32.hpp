@@ -273,7 +273,11 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
     float px = fmaf(a.kx, r.vx[k], r.x[k]);
     float py = fmaf(a.kx, r.vy[k], r.y[k]);
     bool alive = true;
-    if (a.bc == EPIC_BC_PBC) {
+    if (isnan(px) || isnan(py)) {
+      bad += NAN_UNIT;
+      alive = false;
+      px = __builtin_nanf("");
+    } else if (a.bc == EPIC_BC_PBC) {
       px = px < a.x0 ? px + a.Lx : (px >= a.xhi ? px - a.Lx : px);
       py = py < a.y0 ? py + a.Ly : (py >= a.yhi ? py - a.Ly : py);
       // x0 - tiny + Lx can round up to xhi itself: the periodic image of x0
@@ -332,7 +336,7 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
   st_run<float, PPT>(a.vy, wb, lane, n, r.vy);
   st_run<float, PPT>(a.x, wb, lane, n, r.x);
   st_run<float, PPT>(a.y, wb, lane, n, r.y);
-  if (bad) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)bad);
+  if (bad) flush_push_errors(a.errc, bad);
 }
 
 // Deposit of a pushed chunk through the wavefront's LDS window wl: each

@@ -82,6 +82,7 @@ int epic_serve_make_jobs(const epic_serve_config* c, epic_serve_job* out, int ca
     const double u_ts = u01(r), u_vd = u01(r);
     j.two_stream = (!j.collisional && u_ts < c->two_stream_fraction) ? 1 : 0;
     j.vd = j.two_stream ? u_vd * c->vd_max : 0.0;
+    if (c->fault_job == k + 1) j.ti = std::nan("");
     j.particles = 2ll * j.ppc * j.grid * j.grid;
     if (out && k < cap) out[k] = j;
     k++;
@@ -254,6 +255,25 @@ void digest(epic_ctx* ctx, int nsp, int nn, epic_serve_job& j) {
 
 using Clock = std::chrono::steady_clock;
 
+// Health of an admitted job from its error counters and last residual: 0,
+// or 1 + the EPIC_ERRC_* index that fails it.  A member's NOCONV count
+// alone (a step that ended just above the solver tolerance) is recorded
+// but does not fail it; its residual has to exceed fail_relres.
+int job_health(epic_batch* B, const Active& a, const epic_serve_config* c, epic_serve_job& j) {
+  int64_t e[EPIC_NERR] = {};
+  if (epic_error_counters(a.ctx, e) != EPIC_OK) return 1 + EPIC_ERRC_NAN;
+  for (int k = 0; k < EPIC_NERR; k++) j.errors[k] = e[k];
+  double rr = 0;
+  if (epic_batch_relres(B, a.id, &rr) == EPIC_OK) j.relres = rr;
+  static const int fatal[] = {EPIC_ERRC_NAN, EPIC_ERRC_CAPACITY, EPIC_ERRC_FIXED_OVF, EPIC_ERRC_GATHER_OOB,
+                              EPIC_ERRC_DEPOSIT_OOB};
+  for (int k : fatal)
+    if (e[k]) return 1 + k;
+  const double lim = c->fail_relres > 0 ? c->fail_relres : 1e-6;
+  if (!(j.relres <= lim)) return 1 + EPIC_ERRC_NOCONV;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep, epic_serve_job* jobs_out,
@@ -311,12 +331,13 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
 
   std::deque<int> queue;
   std::vector<Active> active;
-  int next = 0, done = 0, max_members = 0;
+  int next = 0, done = 0, failed = 0, max_members = 0, since_check = 0;
+  const int interval = c->health_interval > 0 ? c->health_interval : 10;
   int64_t used = 0;
   double peak = 0, busy = 0, last_print = 0;
   const auto t0 = Clock::now();
   auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
-  while (done < n) {
+  while (done + failed < n) {
     double t = now();
     while (next < n && jobs[next].arrival_s <= t) queue.push_back(next++);
     while (!queue.empty()) {
@@ -342,24 +363,38 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
     max_members = std::max(max_members, (int)active.size());
     const double ts = now();
     if ((rc = epic_batch_step(B, 1)) != EPIC_OK) return done_batch(fail(rc, epic_batch_last_error(B)));
-    bool any_done = false;
-    for (size_t m = 0; m < active.size();) {
+    // which jobs finish, and (every interval steps, and for the finishing
+    // ones) which have failed; read before any member leaves the batch
+    const bool check = ++since_check >= interval;
+    if (check) since_check = 0;
+    std::vector<int> verdict(active.size(), 0);  // 0 stays, 1 completes, 2 evicted
+    for (size_t m = 0; m < active.size(); m++) {
       int64_t st = 0;
       epic_batch_member_steps(B, active[m].id, &st);
       epic_serve_job& j = jobs[active[m].job];
-      if (st >= j.steps) {
-        digest(active[m].ctx, nsp[active[m].job], nn[active[m].job], j);  // synchronises
-        j.done_s = now();
-        epic_batch_remove(B, active[m].id);
-        used -= active[m].bytes;
-        active.erase(active.begin() + m);
-        done++;
-        any_done = true;
-      } else {
-        m++;
+      const bool fin = st >= j.steps;
+      if (check || fin) {
+        j.failed = job_health(B, active[m], c, j);
+        if (j.failed) verdict[m] = 2;
+        else if (fin) verdict[m] = 1;
       }
     }
-    (void)any_done;
+    for (size_t m = active.size(); m-- > 0;) {
+      if (!verdict[m]) continue;
+      epic_serve_job& j = jobs[active[m].job];
+      if (verdict[m] == 1) {
+        digest(active[m].ctx, nsp[active[m].job], nn[active[m].job], j);  // synchronises
+        done++;
+      } else {
+        failed++;
+        std::fprintf(stderr, "[serve] job %d evicted (error counter %d) after %.1f s\n", j.index, j.failed - 1,
+                     now() - j.arrival_s);
+      }
+      j.done_s = now();
+      epic_batch_remove(B, active[m].id);
+      used -= active[m].bytes;
+      active.erase(active.begin() + m);
+    }
     busy += now() - ts;
     if (c->progress_s > 0 && now() - last_print >= c->progress_s) {
       last_print = now();
@@ -375,10 +410,11 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
   std::vector<double> lat, wait;
   double ps = 0, first = n ? jobs[0].arrival_s : 0, last = 0;
   for (const epic_serve_job& j : jobs) {
+    last = std::max(last, j.done_s);
+    if (j.failed) continue;
     lat.push_back(j.done_s - j.arrival_s);
     wait.push_back(j.admit_s - j.arrival_s);
     ps += (double)j.particles * j.steps;
-    last = std::max(last, j.done_s);
   }
   epic_serve_report r{};
   r.jobs = n;
@@ -392,7 +428,7 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
     sum += v;
     mx = std::max(mx, v);
   }
-  r.latency_mean_s = n ? sum / n : 0;
+  r.latency_mean_s = lat.empty() ? 0 : sum / (double)lat.size();
   r.latency_max_s = mx;
   r.wait_p50_s = pct(wait, 0.5);
   r.wait_p99_s = pct(wait, 0.99);
@@ -404,6 +440,7 @@ extern "C" int epic_serve_run(const epic_serve_config* c, epic_serve_report* rep
   r.graph_captures = bs.graph_captures;
   r.push_variant = bs.push_variant;
   r.node_block = bs.node_block;
+  r.jobs_failed = failed;
   *rep = r;
   if (jobs_out)
     for (int k = 0; k < std::min(n, cap); k++) jobs_out[k] = jobs[k];

@@ -83,7 +83,9 @@ extern "C" {
 #define EPIC_ERRC_NOCONV 4      /* a Poisson solve stopped above its tolerance (PCG
                                    or SOR at max_iteration, the mixed and sine-
                                    transform solves at their correction limit)     */
-#define EPIC_ERRC_NAN 5         /* non-finite velocity met in a push                */
+#define EPIC_ERRC_NAN 5         /* a push met a non-finite new position (a non-finite
+                                   velocity or field); the particle is dropped (the
+                                   reference's next bounds-checked gather throws)  */
 #define EPIC_ERRC_MCC_BOUND 6   /* a collision candidate's probability exceeded the
                                    null-collision bound 1 - exp(-nu_max dt): a speed
                                    beyond what nu_max covers (4x the table's top

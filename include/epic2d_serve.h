@@ -56,6 +56,16 @@ typedef struct {
   int32_t refine;           /* Poisson corrections per step (0: 2) */
   const epic_mcc_desc* mcc_template; /* tables for collisional jobs (species indices are set per job) */
   double progress_s;        /* > 0: a progress line on stderr at this interval */
+  /* failure handling: every health_interval steps (0: 10) and at completion
+   * each admitted job's device error counters and last Poisson residual are
+   * read; a job with non-finite particles (EPIC_ERRC_NAN), dropped births
+   * (CAPACITY), a fixed-point overflow, particles outside the grid, or a
+   * relative residual that is not finite or above fail_relres (0: 1e-6) is
+   * evicted and reported failed; the others go on undisturbed */
+  int32_t health_interval;
+  double fail_relres;
+  int32_t fault_job;        /* fault injection for tests of that path: 1 + the index of a job whose
+                               ions are loaded with a non-finite temperature (NaN velocities); 0: none */
 } epic_serve_config;
 
 typedef struct {
@@ -68,6 +78,10 @@ typedef struct {
   double ke[3];                       /* digest at completion: kinetic energy of species 0..2 (x+y+z) */
   double phi_l2;                      /* digest: ||phi||_2 */
   int64_t collisions;                 /* digest: all MCC events */
+  int32_t failed;                     /* 0: completed; else 1 + the EPIC_ERRC_* index that evicted it
+                                         (EPIC_ERRC_NOCONV for a bad residual); done_s is then the eviction time */
+  double relres;                      /* last Poisson ||b - A phi|| / ||b|| read for it */
+  int64_t errors[EPIC_NERR];          /* its device error counters when last read */
 } epic_serve_job;
 
 typedef struct {
@@ -83,6 +97,7 @@ typedef struct {
   double peak_bytes;          /* largest admitted total */
   int64_t batch_steps, graph_captures;
   int32_t push_variant, node_block;
+  int32_t jobs_failed;        /* evicted; latencies and particle_steps count completed jobs only */
 } epic_serve_report;
 
 /* The job stream of a configuration (no device work). */

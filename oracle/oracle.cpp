@@ -51,7 +51,17 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
+
+// Host threads of the all-cores timing variant (orc_set_threads); 1 runs
+// the sequential restatement.  With more, the push and the deposit split the
+// particle list into contiguous per-thread ranges (the push keeps the
+// survivors' order; the deposit sums per-thread grids, so its FP64 sums are
+// not the sequential ones) and the PCG's operator product splits its rows.
+// Timing only: every parity check runs on one thread.
+int g_threads = 1;
 
 // Physical constants, include/domain.h:204-215.
 const double EPS_0 = 8.85418782e-12;
@@ -540,10 +550,26 @@ int orc_scatter(void* h, int s) {
   Sim::Sp& sp = m->sp[s];
   std::fill(sp.den.begin(), sp.den.end(), 0.0);
   int bad = 0;
-  for (const Particle& p : sp.part_list) {
-    double lx = XtoL(m, p.pos[0]);
-    double ly = YtoL(m, p.pos[1]);
-    if (scatter(m, lx, ly, sp.spwt, sp.den)) bad++;
+  if (g_threads <= 1) {
+    for (const Particle& p : sp.part_list) {
+      double lx = XtoL(m, p.pos[0]);
+      double ly = YtoL(m, p.pos[1]);
+      if (scatter(m, lx, ly, sp.spwt, sp.den)) bad++;
+    }
+  } else {
+    const size_t np = sp.part_list.size();
+    std::vector<std::vector<double>> part(g_threads, std::vector<double>(sp.den.size(), 0.0));
+#pragma omp parallel num_threads(g_threads) reduction(+ : bad)
+    {
+      const int t = omp_get_thread_num();
+      const size_t b = np * t / g_threads, e = np * (t + 1) / g_threads;
+      for (size_t k = b; k < e; k++) {
+        const Particle& p = sp.part_list[k];
+        if (scatter(m, XtoL(m, p.pos[0]), YtoL(m, p.pos[1]), sp.spwt, part[t])) bad++;
+      }
+    }
+    for (int t = 0; t < g_threads; t++)
+      for (size_t k = 0; k < sp.den.size(); k++) sp.den[k] += part[t][k];
   }
   double cell_area = m->dx * m->dy;
   double L2 = m->L * m->L;
@@ -595,6 +621,22 @@ static double dot(const std::vector<double>& a, const std::vector<double>& b) {
   return r;
 }
 
+static void apply_A_par(int nx, int ny, const std::vector<double>& v, std::vector<double>& u) {
+  if (g_threads <= 1) return apply_A(nx, ny, v, u);
+  const int n = nx * ny;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 0; i < n; i++) {
+    int row = i / nx, col = i % nx;
+    double sum = 0;
+    if (row > 0) sum += 1.0 * v[i - nx];
+    if (col > 0) sum += 1.0 * v[i - 1];
+    sum += -4.0 * v[i];
+    if (col < nx - 1) sum += 1.0 * v[i + 1];
+    if (row < ny - 1) sum += 1.0 * v[i + nx];
+    u[i] = sum;
+  }
+}
+
 // pcg(), solvers.h:72-100, with M_inv = diagonalPreconditioner(A) = -1/4 I
 // (function.h:96-106).  Iteration cap min(n*n, max_iteration) taken in 64 bits.
 int orc_pcg(int nx, int ny, const double* b_in, double* x_out, int max_iteration, double tolerance) {
@@ -609,7 +651,7 @@ int orc_pcg(int nx, int ny, const double* b_in, double* x_out, int max_iteration
   int iters = 0;
   for (long long it = 0; it < cap; it++) {
     if (std::sqrt(dot(r, r)) < tolerance) break;
-    apply_A(nx, ny, v, u);
+    apply_A_par(nx, ny, v, u);
     double t = alpha / dot(v, u);
     for (int i = 0; i < n; i++) x[i] = x[i] + v[i] * t;
     for (int i = 0; i < n; i++) r[i] = r[i] - u[i] * t;
@@ -853,10 +895,7 @@ int orc_push(void* h, int s, int sub_cycle) {
   Sim::Sp& sp = m->sp[s];
   int bad = 0;
   auto& pl = sp.part_list;
-  std::vector<Particle> kept;
-  kept.reserve(pl.size());
-  for (size_t k = 0; k < pl.size(); k++) {
-    Particle part = pl[k];
+  auto one = [&](Particle part, std::vector<Particle>& kept, int& bad) {
     double qm = sp.charge / sp.mass;
     double lx = XtoL(m, part.pos[0]);
     double ly = YtoL(m, part.pos[1]);
@@ -876,7 +915,7 @@ int orc_push(void* h, int s, int sub_cycle) {
     } else if (m->bc == BC_OPEN) {
       if (part.pos[0] < m->x0 || part.pos[0] >= m->x0 + m->Lx || part.pos[1] < m->x0 ||
           part.pos[1] >= m->x0 + m->Ly)
-        continue;  // erase (species.cpp:103-107); order of the rest is kept
+        return;  // erase (species.cpp:103-107); order of the rest is kept
     } else {
       if (part.pos[0] < m->x0) { part.pos[0] = m->x0 + (m->x0 - part.pos[0]); part.vel[0] = -part.vel[0]; }
       else if (part.pos[0] >= m->x0 + m->Lx) {
@@ -887,8 +926,27 @@ int orc_push(void* h, int s, int sub_cycle) {
         part.pos[1] = m->x0 + m->Ly - (part.pos[1] - (m->x0 + m->Ly)); part.vel[1] = -part.vel[1];
       }
     }
-    if (on_any_grid(m, part.pos[0], part.pos[1])) continue;
+    if (on_any_grid(m, part.pos[0], part.pos[1])) return;
     kept.push_back(part);
+  };
+  std::vector<Particle> kept;
+  if (g_threads <= 1) {
+    kept.reserve(pl.size());
+    for (size_t k = 0; k < pl.size(); k++) one(pl[k], kept, bad);
+  } else {
+    const size_t np = pl.size();
+    std::vector<std::vector<Particle>> part(g_threads);
+#pragma omp parallel num_threads(g_threads) reduction(+ : bad)
+    {
+      const int t = omp_get_thread_num();
+      const size_t b = np * t / g_threads, e = np * (t + 1) / g_threads;
+      part[t].reserve(e - b);
+      for (size_t k = b; k < e; k++) one(pl[k], part[t], bad);
+    }
+    size_t tot = 0;
+    for (auto& v : part) tot += v.size();
+    kept.reserve(tot);
+    for (auto& v : part) kept.insert(kept.end(), v.begin(), v.end());
   }
   pl.swap(kept);
   return bad;
@@ -1184,6 +1242,9 @@ void orc_inject(void* h, const orc_emitter* e, unsigned pos_seed) {
 }
 
 int orc_last_iters(void* h) { return ((Sim*)h)->last_iters; }
+
+// Host threads of the all-cores timing variant (1: the sequential oracle).
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
 void orc_set_tolerance(void* h, double tol) { ((Sim*)h)->tolerance = tol; }
 

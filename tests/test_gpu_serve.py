@@ -97,3 +97,35 @@ def test_budget_forces_queueing():
     assert rep["peak_bytes"] <= cfg.budget_bytes
     assert rep["wait_p99_s"] > 0
     assert max(j.admit_s - j.arrival_s for j in jobs) > 0
+
+
+def test_failed_job_is_evicted_and_the_rest_are_undisturbed():
+    """One job of the stream is admitted with a non-finite ion temperature
+    (fault_job), so its ions carry NaN velocities: the reference would throw
+    at their next bounds-checked gather (linalg.h:473-480); here the push
+    drops them and counts EPIC_ERRC_NAN, and the loop's health check evicts
+    the job within health_interval steps and reports it failed.  Every other
+    job completes and equals its run alone, as in a mix without the failure."""
+    tmpl = _mcc_template()
+    probe = _cfg(tmpl)
+    cand = [j for j in _lib.serve_make_jobs(probe) if not j.collisional]
+    victim = cand[1].index
+    cfg = _cfg(tmpl, fault_job=victim + 1, health_interval=4)
+    rep, jobs = _lib.serve_run(cfg)
+    print(f"[serve] fault job {victim}: {rep}", flush=True)
+    assert rep["jobs_failed"] == 1 and rep["jobs_done"] == rep["jobs"] - 1
+    bad = jobs[victim]
+    assert bad.failed == 1 + _lib.ERRC_NAN, bad.failed
+    assert bad.errors[_lib.ERRC_NAN] > 0
+    # evicted at the first health check after its first push, long before its steps ran out
+    assert bad.done_s > bad.admit_s
+    assert rep["particle_steps"] == pytest.approx(sum(j.particles * j.steps for j in jobs if not j.failed))
+    for j in jobs:
+        if j.index == victim:
+            continue
+        assert j.failed == 0 and not any(j.errors[k] for k in (_lib.ERRC_NAN, _lib.ERRC_CAPACITY))
+        ke, phi, coll = _alone(j, cfg, tmpl)
+        assert abs(j.phi_l2 - phi) <= 1e-9 * phi, (j.index, j.phi_l2, phi)
+        for a, b in zip(j.ke, ke):
+            assert abs(a - b) <= 1e-9 * abs(b)
+        assert abs(j.collisions - coll) <= 2
