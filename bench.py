@@ -80,23 +80,53 @@ def gpu_run(st, steps, warmup, solver_name, mcc=None):
                 errors=errs.tolist(), collisions=coll)
 
 
-def cpu_baseline(st, sample_frac=1.0, steps=2, gas=None):
-    """The oracle (FP64 CPU restatement of the reference loop, one thread) on a
-    bounded sample of the same workload: the full 512x512 grid and one step,
-    with `sample_frac` of the particles.  Particle stages (scatter, push) are
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, the ones this process
+    may use, the OpenMP thread setting and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(usable, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else usable
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "omp_num_threads": omp, "model": model,
+            "threads": threads}
+
+
+def cpu_baseline(st, sample_frac=1.0, steps=2, gas=None, threads=1, dense_ref=False):
+    """The oracle (FP64 CPU restatement of the reference loop) on a bounded
+    sample of the same workload: the full grid, `steps` steps, with
+    `sample_frac` of the particles.  Particle stages (scatter, push) are
     timed on the sample and scaled to the full count; the field stages
-    (rho, Poisson, E) are per step and counted once.  Poisson: the
-    matrix-free restatement of the reference's pcg() recurrence (dense PCG at
-    263,169 unknowns needs 1.1 TB), absolute tolerance = the workload's
-    relative one times ||b||."""
+    (rho, Poisson, E) are per step and counted once.
+    threads: 1 is the reference's serial loop; more runs the oracle's
+    all-cores variant (per-thread particle ranges, threaded operator
+    products).
+    Poisson: dense_ref times the reference's own dense pcg() with its
+    threaded Matrix x vec, compiled from the reference's headers into
+    oracle/_ref (pcgsolver builds the dense A, and pcg its dense
+    preconditioner, on every solve: field.cpp:221-232, solvers.h:75-76);
+    otherwise the matrix-free restatement of the same pcg() recurrence (dense
+    PCG at 263,169 unknowns needs 1.1 TB).  Absolute tolerance = the
+    workload's relative one times ||b||."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_api import OracleSim
+    from oracle_api import OracleSim, oracle, ref_pcg_lib
+    oracle().orc_set_threads(threads)
+    ref = ref_pcg_lib() if dense_ref else None
     d = st.domain
     dom = _lib.DomainDesc.from_buffer_copy(d)
     dom.solver = _lib.SOLVER_PCG
     o = OracleSim(dom, st.species, gas=gas)
     if gas is not None:
-        from oracle_api import oracle
         oracle().orc_mcc_build_tables(o.h)
     rng = np.random.default_rng(7)
     Lx = (d.nx - 1) * d.dx
@@ -114,8 +144,24 @@ def cpu_baseline(st, sample_frac=1.0, steps=2, gas=None):
         o.scatter(s)
     o.compute_rho()
     rho = o.grid(0)
-    b = -rho[1:-1, 1:-1] * d.dx * d.dx * ((d.L * d.L) / (d.LDe * d.LDe))
-    o.set_tolerance(d.tolerance * float(np.sqrt((b * b).sum())))
+    s2 = (d.L * d.L) / (d.LDe * d.LDe)
+    b = -rho[1:-1, 1:-1] * d.dx * d.dx * s2
+    tol_abs = d.tolerance * float(np.sqrt((b * b).sum()))
+    o.set_tolerance(tol_abs)
+
+    def dense_solve():
+        import ctypes
+        rhom = -o.grid(0) * d.dx * d.dx * s2
+        rhom[0, :] = rhom[-1, :] = rhom[:, 0] = rhom[:, -1] = 0.0
+        rhom = np.ascontiguousarray(rhom.ravel())
+        phi = np.empty_like(rhom)
+        err = ctypes.c_double()
+        D = ctypes.POINTER(ctypes.c_double)
+        ref.ref_pcg(d.nx, d.ny, rhom.ctypes.data_as(D), phi.ctypes.data_as(D), d.max_iteration, tol_abs,
+                    ctypes.byref(err))
+        o.set_grid(1, phi.reshape(d.nx, d.ny))
+        return -1
+
     t = time.perf_counter
     tp = tf = 0.0
     its = []
@@ -126,28 +172,52 @@ def cpu_baseline(st, sample_frac=1.0, steps=2, gas=None):
             o.scatter(s)
         t1 = t()
         o.compute_rho()
-        it, _ = o.solve()
+        if ref is not None:
+            it = dense_solve()
+        else:
+            it, _ = o.solve()
         o.efield()
         t2 = t()
         for s in range(nsp):
             o.push(s)
         if gas is not None:
-            import ctypes
-            from oracle_api import oracle
-            counts = (ctypes.c_long * 3)()
-            eloss = ctypes.c_double()
-            oracle().orc_collide_direct(o.h, 0, 1, counts, ctypes.byref(eloss))
+            o.collide_direct()
         t3 = t()
         tp += (t1 - t0) + (t3 - t2)
         tf += t2 - t1
         its.append(it)
     wall = t() - start
+    oracle().orc_set_threads(1)
     scale = n_full / n_samp
     t_particles = tp * scale / steps
     t_fields = tf / steps
     t_step = t_particles + t_fields
     return dict(value=n_full / t_step, t=wall, iters=its, npart=n_full, nsamp=n_samp, steps=steps,
-                t_particles=t_particles, t_fields=t_fields)
+                t_particles=t_particles, t_fields=t_fields, threads=threads, dense_ref=ref is not None)
+
+
+def cpu_baseline_entry(st, args, gas=None, note="", dense_ref=False):
+    """cpu_baseline on one thread (the reference's serial loops: the
+    reported baseline) and on every usable core (the all-cores variant)."""
+    hi = host_info()
+    cb = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps, gas=gas, dense_ref=dense_ref)
+    solver = ("the reference's own dense pcg() (oracle/_ref, its Matrix x vec on %d hardware threads)"
+              % (os.cpu_count() or 1) if cb["dense_ref"] else "the matrix-free pcg() restatement")
+    out = {"value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+           "sample": "%d steps of the same workload by the FP64 oracle (restatement of the reference loop) on one "
+                     "thread: %d of the %d particles (particle stages scaled x%.2f to the full count), %.2f s "
+                     "particle + %.2f s field stages per step, Poisson by %s (%s iterations)%s; %.1f s of CPU "
+                     "work" % (cb["steps"], cb["nsamp"], cb["npart"], cb["npart"] / cb["nsamp"], cb["t_particles"],
+                               cb["t_fields"], solver, cb["iters"], note, cb["t"]),
+           "host": {k: hi[k] for k in ("nproc", "usable_cpus", "omp_num_threads", "model")}}
+    if hi["threads"] > 1 and not args.no_all_cores:
+        ca = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps, gas=gas, threads=hi["threads"],
+                          dense_ref=dense_ref)
+        out["all_cores"] = {"value": ca["value"], "cores": hi["threads"], "kind": "port",
+                            "t_particles": ca["t_particles"], "t_fields": ca["t_fields"],
+                            "sample": "the same sample with the oracle's OpenMP variant on %d threads "
+                                      "(per-thread particle ranges, threaded operator products)" % hi["threads"]}
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -356,6 +426,7 @@ def main_serve(args):
                    "steps_scale": sc},
         "latency_p50_s": rep["latency_p50_s"],
         "latency_p99_s": rep["latency_p99_s"],
+        "jobs_failed": rep["jobs_failed"],
         "report": rep,
         "run_s": t_run,
     }
@@ -365,7 +436,7 @@ def main_serve(args):
     print(json.dumps(out))
 
 
-def main_single(args, st, label, dtype, bytes_ps, kernel, mcc=None, gas=None, cpu_note=""):
+def main_single(args, st, label, dtype, bytes_ps, kernel, mcc=None, gas=None, cpu_note="", dense_ref=False):
     """One simulation of the given set-up: whole-step particle-steps/s, the
     fused push's roofline, and the oracle on the host beside it."""
     r = gpu_run(st, args.steps, args.warmup, label, mcc=mcc)
@@ -401,15 +472,56 @@ def main_single(args, st, label, dtype, bytes_ps, kernel, mcc=None, gas=None, cp
         "device_errors": r["errors"],
     }
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps, gas=gas)
-        out["cpu_baseline"] = {
-            "value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the same workload by the FP64 oracle on one thread, %d of the %d particles "
-                      "(particle stages scaled x%.2f), %.2f s particle + %.2f s field stages per step, Poisson by "
-                      "the matrix-free pcg() restatement (%s iterations)%s; %.1f s of CPU work" % (
-                          cb["steps"], cb["nsamp"], cb["npart"], cb["npart"] / cb["nsamp"], cb["t_particles"],
-                          cb["t_fields"], cb["iters"], cpu_note, cb["t"])}
+        out["cpu_baseline"] = cpu_baseline_entry(st, args, gas=gas, note=cpu_note, dense_ref=dense_ref)
     print(json.dumps(out))
+
+
+def fp64_plain_leg(args):
+    """The plain FP64 path on the headline workload (FP64 particles, the
+    reference's Jacobi PCG on the Dirichlet-wall operator to 1e-10): a few
+    steps, for the deposit+push fraction of HBM peak (64 B per
+    particle-step, event-timed push launches) and its particle-steps/s."""
+    st = workloads.bounded_512(precision=_lib.FP64, solver=_lib.SOLVER_PCG)
+    r = gpu_run(st, args.fp64_steps, 1, "pcg")
+    push_ms, push_calls = r["timing"]["push"]
+    avg_launch_s = push_ms / max(push_calls, 1) / 1e3
+    parts_per_launch = 0.5 * (sum(r["per_sp0"]) + sum(r["per_sp1"])) * args.fp64_steps / max(push_calls, 1)
+    achieved = 64 * parts_per_launch / avg_launch_s / 1e9
+    return {"workload": "the same 512x512 four-species workload in FP64 (plain path: FP64 particles, the "
+                        "reference PCG, Jacobi, FP64, to 1e-10 relative)",
+            "steps": args.fp64_steps, "value": 0.5 * (r["n0"] + r["n1"]) * args.fp64_steps / r["t"],
+            "ms_per_step": 1e3 * r["t"] / args.fp64_steps,
+            "pcg_iterations_mean": float(np.mean(r["iters"])) if r["iters"] else 0,
+            "push_kernel": "push_f64_kernel<2,true,false> (gather + push + walls + deposit, one launch per species)",
+            "push_avg_launch_ms": avg_launch_s * 1e3, "push_achieved_GBs": achieved,
+            "push_frac": achieved / HBM_PEAK_GBS, "device_errors": r["errors"]}
+
+
+def serve_leg(args):
+    """A short request stream through the serving loop (continuous batching,
+    FP32 + FP64-refined) and the plain FP64 path one job at a time on a
+    sample of the same jobs: p50/p99 time to completion, aggregate
+    particle-steps/s, and the loop's speed-up over the FP64 path."""
+    import ctypes
+    from epic2d_mcc_amd import xsec
+    tmpl = xsec.mcc_desc(xsec.synthetic_ar(), gas_density=1e21, gas_temp_K=300.0, enable=(1, 1, 1, 1, 1))
+    sc = args.serve_steps_scale_leg
+    cfg = _lib.serve_config(duration_s=args.serve_duration_leg, steps_min=max(1, round(100 * sc)),
+                            steps_max=max(1, round(2000 * sc)), mcc_template=ctypes.pointer(tmpl))
+    t0 = time.perf_counter()
+    rep, jobs = _lib.serve_run(cfg)
+    t_run = time.perf_counter() - t0
+    k = max(1, args.serve_baseline_jobs_leg)
+    sample = jobs[::max(1, len(jobs) // k)][:k]
+    base = fp64_sequential_jobs(cfg, sample, tmpl, args.baseline_steps)
+    return {"workload": "Poisson arrivals 20/s for %.0f s (%d jobs), grids 64^2-512^2, 50-400 ppc, %d-%d steps "
+                        "(the 100-2000 range x%.2f), half collisional, half of the rest two-stream, 200 GB budget"
+                        % (cfg.duration_s, rep["jobs"], cfg.steps_min, cfg.steps_max, sc),
+            "value": rep["throughput"], "unit": "particle-steps/s", "latency_p50_s": rep["latency_p50_s"],
+            "latency_p99_s": rep["latency_p99_s"], "jobs_done": rep["jobs_done"], "jobs_failed": rep["jobs_failed"],
+            "max_members": rep["max_members"], "run_s": t_run,
+            "fp64_plain_path": {"value": base, "jobs_sampled": len(sample), "steps_per_job": args.baseline_steps},
+            "speedup_over_fp64_plain_path": rep["throughput"] / base}
 
 
 def main_ops(args):
@@ -520,6 +632,13 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=30.0)
     ap.add_argument("--serve-baseline-jobs", type=int, default=12)
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--no-all-cores", action="store_true", help="skip the all-cores CPU variant")
+    ap.add_argument("--no-fp64-leg", action="store_true", help="headline: skip the plain FP64 path leg")
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-serve-leg", action="store_true", help="headline: skip the short serving leg")
+    ap.add_argument("--serve-duration-leg", type=float, default=5.0)
+    ap.add_argument("--serve-steps-scale-leg", type=float, default=0.1)
+    ap.add_argument("--serve-baseline-jobs-leg", type=int, default=4)
     ap.add_argument("--baseline-steps", type=int, default=3)
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -542,7 +661,8 @@ def main():
         st = workloads.periodic_64()
         main_single(args, st, "periodic 64x64 cells, electron + ion (409,600 each), FP64 particles, reference PCG "
                               "(Jacobi, FP64) to 1e-8 relative", "fp64", 64,
-                    "particle_kernel<double,2,PUSH,whole-grid LDS,deposit> (gather + push + wrap + deposit)")
+                    "particle_kernel<double,2,PUSH,whole-grid LDS,deposit> (gather + push + wrap + deposit)",
+                    dense_ref=True)
         return
     if args.workload == "mcc256":
         from epic2d_mcc_amd import xsec
@@ -591,9 +711,14 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
         "config": {"workload": "bounded 512x512, 4 species x %d ppc (%d particles), open walls, "
-                               "Dirichlet phi=0, mixed PCG 1e-10, sort every 20 steps" % (
+                               "Dirichlet phi=0, Poisson by FP32 sine-transform corrections with an FP64 true "
+                               "residual to 1e-10 relative, sort every 20 steps" % (
                                    args.ppc, sum(int(s.numparticle) for s in st.species)),
-                   "solver": args.solver, "particles_start": r["n0"], "particles_end": r["n1"],
+                   "solver": {"pcg_mixed": "FP32 sine-transform corrections, FP64 true residual and refinement "
+                                            "(EPIC_SOLVER_PCG_MIXED + EPIC_PRECOND_SPECTRAL)",
+                              "spectral": "FP64 sine-transform solve + FP64 refinement",
+                              "pcg": "reference Jacobi PCG, FP64", "gs": "red-black SOR, FP64"}[args.solver],
+                   "particles_start": r["n0"], "particles_end": r["n1"],
                    "poisson_iterations_mean": float(np.mean(r["iters"])) if r["iters"] else 0,
                    "stage_ms_per_step": stage},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -606,16 +731,15 @@ def main():
     }
     if copy_cal:
         out["copy_calibration"] = copy_cal
+    if not args.no_fp64_leg:
+        print("[bench] plain FP64 path leg", file=sys.stderr, flush=True)
+        out["fp64_plain_path"] = fp64_plain_leg(args)
+    if not args.no_serve_leg:
+        print("[bench] serving leg", file=sys.stderr, flush=True)
+        out["serving"] = serve_leg(args)
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps)
-        out["cpu_baseline"] = {
-            "value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the same workload by the FP64 oracle (restatement of the reference loop) on one "
-                      "thread: 512x512 grid, %d of the %d particles (particle stages scaled x%.2f to the full "
-                      "count), %.2f s particle + %.2f s field stages per step, Poisson by the matrix-free pcg() "
-                      "restatement (%s iterations); %.1f s of CPU work" % (
-                          cb["steps"], cb["nsamp"], cb["npart"], cb["npart"] / cb["nsamp"], cb["t_particles"],
-                          cb["t_fields"], cb["iters"], cb["t"])}
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_entry(st, args)
     print(json.dumps(out))
 
 

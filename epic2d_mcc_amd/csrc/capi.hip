@@ -319,8 +319,14 @@ int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp
 // What epic_mcc_setup allocates: table, counters, event list, candidate
 // lists (16-bit offsets, capacity rounded up to MCC_CHUNK) and their
 // per-chunk counts.
-static size_t mcc_bytes_of(int ntab, int event_cap, int64_t len) {
-  const int cap = event_cap > 0 ? event_cap : 16384;
+// Default ionisation event list: 16384 events, or one per 64 electron slots
+// for large species (a hot 1e8-electron plasma ionises ~1e5 per step).
+static int mcc_event_cap(int event_cap, int64_t len_e) {
+  return event_cap > 0 ? event_cap : (int)std::max<int64_t>(16384, (len_e + 63) / 64);
+}
+
+static size_t mcc_bytes_of(int ntab, int event_cap, int64_t len, int64_t len_e) {
+  const int cap = mcc_event_cap(event_cap, len_e);
   return (size_t)EPIC_MCC_NPROC * std::max(ntab, 1) * sizeof(double) + (EPIC_MCC_NPROC + 2) * sizeof(uint64_t) +
          (size_t)cap * 8 * sizeof(double) + sizeof(int) + (size_t)std::max<int64_t>(len, 1) * sizeof(uint16_t) +
          (size_t)std::max<int64_t>(len / MCC_CHUNK, 1) * sizeof(int);
@@ -334,7 +340,8 @@ static int64_t mcc_list_len(const epic_species_desc* sp, int s) {
 
 int epic_mcc_estimate_bytes(const epic_mcc_desc* d, const epic_species_desc* sp, int nsp, int64_t* bytes) {
   if (!d || !sp || !bytes || nsp < 1 || d->electron_sp >= nsp || d->ion_sp >= nsp) return EPIC_ERR_ARG;
-  *bytes = (int64_t)mcc_bytes_of(d->ntab, d->event_cap, mcc_list_len(sp, d->electron_sp) + mcc_list_len(sp, d->ion_sp));
+  const int64_t le = mcc_list_len(sp, d->electron_sp);
+  *bytes = (int64_t)mcc_bytes_of(d->ntab, d->event_cap, le + mcc_list_len(sp, d->ion_sp), le);
   return EPIC_OK;
 }
 
@@ -714,7 +721,9 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   CK(dalloc_c(c, &M.sigma, sig.size()), "mcc table");
   CK(hipMemcpy(M.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice), "mcc table");
   M.d.sigma = nullptr;
-  M.event_cap = d->event_cap > 0 ? d->event_cap : 16384;
+  M.event_cap = mcc_event_cap(d->event_cap, d->electron_sp >= 0 ? (c->sp[d->electron_sp].cap + MCC_CHUNK - 1) /
+                                                                      MCC_CHUNK * MCC_CHUNK
+                                                                : 0);
   if (!M.counts) CK(dalloc_c(c, &M.counts, EPIC_MCC_NPROC + 2), "mcc counts");
   if (M.events) hipFree(M.events);
   M.events = nullptr;

@@ -90,10 +90,12 @@ __device__ __forceinline__ double gauss(uint32_t a, uint32_t b, uint32_t c, uint
 }
 
 // One candidate of the null-collision method (R0 < P_null already drawn):
-// the real/null decision, the process and the kinematics.
+// the real/null decision, the process and the kinematics.  Returns the
+// index of the ionisation event it stored, or -1.
 template <typename P>
-__device__ __forceinline__ void mcc_candidate(const MccArgs<P>& a, uint32_t step, int64_t p,
-                                              unsigned long long (&cnt)[EPIC_MCC_NPROC]) {
+__device__ __forceinline__ int mcc_candidate(const MccArgs<P>& a, uint32_t step, int64_t p,
+                                             unsigned long long (&cnt)[EPIC_MCC_NPROC]) {
+  int stored = -1;
   do {
     const uint32_t pid = a.id[p];
     const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
@@ -169,7 +171,7 @@ __device__ __forceinline__ void mcc_candidate(const MccArgs<P>& a, uint32_t step
           ev[4] = wy + F2 * hy;
           ev[5] = wz + F2 * hz;
           ev[6] = (double)pid;
-          ev[7] = 0;
+          stored = e;  // ev[7]: the event's rank within its chunk, set by the caller
         }
       }
       double ngx, ngy, ngz;
@@ -218,6 +220,7 @@ __device__ __forceinline__ void mcc_candidate(const MccArgs<P>& a, uint32_t step
       }
     }
   } while (false);
+  return stored;
 }
 
 // A selection chunk (MCC_CHUNK consecutive slots) is one wavefront's work
@@ -265,7 +268,11 @@ __device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t st
 // Pass 2: the candidates, one per lane, so the collision arithmetic runs on
 // full wavefronts instead of on the few lanes of each wavefront that drew a
 // candidate.  Each particle's draws are keyed by (id, step): the order of
-// processing changes nothing.
+// processing changes nothing.  Electrons: a chunk's candidates are taken in
+// slot order, so each stored ionisation event gets its rank among the
+// chunk's events (ev[7]), and the chunk's event count replaces its
+// candidate count in cand_n[c] (read at the start, not needed after); the
+// birth append orders the births by (chunk, rank), i.e. by parent slot.
 template <typename P>
 __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
   const int64_t n = a.d_n[0];
@@ -276,7 +283,20 @@ __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int
   for (int64_t c = (int64_t)bid * wpb + (threadIdx.x >> 6); c < nch; c += (int64_t)nblk * wpb) {
     const int64_t base = c * MCC_CHUNK;
     const int nc = a.cand_n[c];
-    for (int t = lane; t < nc; t += 64) mcc_candidate(a, step, base + a.cand[base + t], cnt);
+    int nev = 0;
+    for (int t0 = 0; t0 < nc; t0 += 64) {
+      const int t = t0 + lane;
+      const int e = t < nc ? mcc_candidate(a, step, base + a.cand[base + t], cnt) : -1;
+      if (a.kind == 0) {
+        const unsigned long long m = __ballot(e >= 0);
+        if (e >= 0) {
+          const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+          a.events[(size_t)e * EVW + 7] = (double)(nev + r);
+        }
+        nev += (int)__popcll(m);
+      }
+    }
+    if (a.kind == 0 && lane == 0) a.cand_n[c] = nev;
   }
 #pragma unroll
   for (int q = 0; q < EPIC_MCC_NPROC; q++) {
@@ -312,11 +332,13 @@ __global__ __launch_bounds__(MCC_BLOCK, 2) void mcc_batch_kernel(const MccArgs<P
   mcc_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x);
 }
 
-// Append ionisation births in parent-slot order: one workgroup ranks the
-// events (slots are unique) and writes the new electron at n_e + rank and the
-// new ion at n_i + rank, ids next_id + rank, and deposits both.
+// Append ionisation births in parent-slot order: one workgroup scans the
+// per-chunk event counts the collision pass left; an event's rank is its
+// chunk's offset plus its rank within the chunk (O(chunks + events), no
+// bound on the events per step but the event list).  The new electron goes
+// to n_e + rank and the new ion to n_i + rank, ids next_id + rank, and both
+// are deposited.
 constexpr int APPEND_BLOCK = 1024;
-constexpr int APPEND_LDS = 16384;
 
 template <typename P>
 struct AppendArgs {
@@ -341,11 +363,13 @@ struct AppendArgs {
   unsigned long long* counts;
   unsigned long long* errc;
   int slot;  // batched runs: index of the simulation's step counter
+  int* chunk_n;  // per selection chunk: its ionisation events (scanned in place)
 };
 
 template <typename P>
-__device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t step, double* slots) {
+__device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t step, int* scan) {
   const double* events = A.events;
+  int* chunk_n = A.chunk_n;
   int* event_count = A.event_count;
   const int cap_events = A.cap_events;
   P *ex = A.ex, *ey = A.ey, *evx = A.evx, *evy = A.evy, *evz = A.evz;
@@ -365,16 +389,35 @@ __device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t
   unsigned long long* counts = A.counts;
   unsigned long long* errc = A.errc;
   const int E = min(*event_count, cap_events);
-  for (int k = threadIdx.x; k < E; k += blockDim.x) slots[k] = events[(size_t)k * EVW];
-  __syncthreads();
   const int64_t ne = e_n[0], ni = i_n[0];
   const uint32_t ide = (uint32_t)e_n[1], idi = (uint32_t)i_n[1];
+  // exclusive scan of the per-chunk event counts the collision pass left in
+  // chunk_n (chunk order = slot order): each thread a contiguous run of
+  // chunks, then a scan of the run totals
+  const int nch = (int)((ne + MCC_CHUNK - 1) / MCC_CHUNK);
+  const int per = (nch + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int c0 = min((int)threadIdx.x * per, nch), c1 = min(c0 + per, nch);
+  int run = 0;
+  for (int c = c0; c < c1; c++) run += chunk_n[c];
+  scan[threadIdx.x] = run;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const int v = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int acc = scan[threadIdx.x] - run;
+  for (int c = c0; c < c1; c++) {
+    const int v = chunk_n[c];
+    chunk_n[c] = acc;
+    acc += v;
+  }
+  __syncthreads();
   int dropped = 0;
   for (int k = threadIdx.x; k < E; k += blockDim.x) {
-    const double me = slots[k];
-    int rank = 0;
-    for (int t = 0; t < E; t++) rank += slots[t] < me ? 1 : 0;
     const double* ev = events + (size_t)k * EVW;
+    const int rank = chunk_n[(int64_t)ev[0] / MCC_CHUNK] + (int)ev[7];
     const int64_t se = ne + rank, si = ni + rank;
     if (se >= e_cap || si >= i_cap) {
       dropped++;
@@ -434,16 +477,16 @@ __device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t
 
 template <typename P>
 __global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_kernel(AppendArgs<P> A) {
-  __shared__ double slots[APPEND_LDS];
-  mcc_append_body(A, A.step, slots);
+  __shared__ int scan[APPEND_BLOCK];
+  mcc_append_body(A, A.step, scan);
 }
 
 // Batched: one workgroup per simulation with ionisation.
 template <typename P>
 __global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_batch_kernel(const AppendArgs<P>* args, const int* steps) {
-  __shared__ double slots[APPEND_LDS];
+  __shared__ int scan[APPEND_BLOCK];
   const AppendArgs<P>& A = args[blockIdx.x];
-  mcc_append_body(A, (uint32_t)steps[A.slot], slots);
+  mcc_append_body(A, (uint32_t)steps[A.slot], scan);
 }
 
 // Arguments of one species' collision pass (kind 0 electron, 1 ion).
@@ -499,7 +542,8 @@ static AppendArgs<P> append_args(Ctx* c, int step) {
   AppendArgs<P> A{};
   A.events = M.events;
   A.event_count = M.event_count;
-  A.cap_events = std::min(M.event_cap, APPEND_LDS);
+  A.cap_events = M.event_cap;
+  A.chunk_n = M.cand_n;  // the electrons' per-chunk event counts (see mcc_body)
   A.ex = (P*)S.x; A.ey = (P*)S.y; A.evx = (P*)S.vx; A.evy = (P*)S.vy; A.evz = (P*)S.vz;
   A.eid = S.id; A.e_n = S.d_n; A.e_cap = S.cap;
   A.ix = (P*)I.x; A.iy = (P*)I.y; A.ivx = (P*)I.vx; A.ivy = (P*)I.vy; A.ivz = (P*)I.vz;

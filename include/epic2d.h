@@ -175,7 +175,8 @@ typedef struct epic_mcc_desc {
   int32_t ref_selection;         /* 1: the reference's two-draw type choice
                                     (collision.cpp:117,125); 0: one draw against
                                     cumulative frequencies */
-  int32_t event_cap;             /* ionisation births per step (0 = 16384) */
+  int32_t event_cap;             /* ionisation births per step (0: 16384, or one per
+                                    64 electron slots when that is more) */
 } epic_mcc_desc;
 
 typedef struct epic_mcc_stats {

@@ -100,6 +100,7 @@ def oracle():
         o.orc_collide_one.restype = ctypes.c_int
         o.orc_inject.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
         o.orc_set_tolerance.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        o.orc_set_threads.argtypes = [ctypes.c_int]
     return _o
 
 

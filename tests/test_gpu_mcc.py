@@ -332,3 +332,44 @@ def test_ion_elastic_conserves_relative_speed():
     dks = max(np.max(np.arange(1, n + 1) / n - u), np.max(u - np.arange(n) / n))
     assert dks * np.sqrt(n) < 1.95
     ctx.close()
+
+
+def test_many_ionisations_per_step_appended_in_parent_order():
+    """More ionisation events in one step than the former 16384-event
+    limit: 2,000,000 electrons at 40 eV in 1e22 m^-3 argon ionise ~5e4
+    times in one collision call.  Every event gives one electron and one ion
+    (no CAPACITY count), appended in parent-slot order: the k-th new
+    electron sits at the position of the k-th ionising parent in slot order,
+    and ids continue in that order."""
+    print("[mcc] many ionisations per step", flush=True)
+    N = 2000000
+    st = _setup(N, 40.0)
+    d = st.domain
+    rng = np.random.default_rng(31)
+    speed = np.sqrt(2 * 40.0 * config.QE / ME) / d.vel_norm
+    c = rng.uniform(-1, 1, N)
+    ph = rng.uniform(0, 2 * np.pi, N)
+    sn = np.sqrt(1 - c * c)
+    e = (rng.random(N) * 16, rng.random(N) * 16, speed * sn * np.cos(ph), speed * sn * np.sin(ph), speed * c)
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, *e)
+    ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    ctx.mcc_setup(xsec.mcc_desc(xsec.reference_e_ar(), gas_density=1e22, gas_temp_K=0.0, enable=(1, 1, 1, 0, 0)))
+    ctx.collide(0)
+    st_ = ctx.mcc_stats()
+    nion = st_["count"][2]
+    print(f"  ionisations {nion}, births {st_['births']}", flush=True)
+    assert nion > 20000
+    assert st_["births"] == nion
+    assert not ctx.errors().any(), ctx.errors()
+    x, y, vx, vy, vz, ids = ctx.download(0)
+    assert len(x) == N + nion
+    # parents: the loaded electrons whose position the births share
+    pos = {(a, b): k for k, (a, b) in enumerate(zip(e[0].tolist(), e[1].tolist()))}
+    parents = np.array([pos[(a, b)] for a, b in zip(x[N:].tolist(), y[N:].tolist())])
+    assert np.all(np.diff(parents) > 0)
+    assert np.array_equal(ids[N:], np.arange(N, N + nion, dtype=np.uint32))
+    xi, yi, _, _, _, idi = ctx.download(1)
+    assert len(xi) == 1 + nion
+    assert np.array_equal(xi[1:], x[N:]) and np.array_equal(yi[1:], y[N:])
+    ctx.close()
