@@ -49,11 +49,8 @@ def build(verbose=True, jobs=8):
             objs.append(obj)
             if verbose:
                 print("compiled", os.path.basename(obj), flush=True)
-    extra = []
-    if os.path.exists(os.path.join(CSRC, "spectral.hip")):
-        extra = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, *extra])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     if verbose:
         print("linked", LIB, flush=True)

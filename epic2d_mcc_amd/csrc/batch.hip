@@ -30,7 +30,6 @@
 // and occupancy target of the push, NB of the node stages) do not change any
 // result: the deposit is an integer sum and each node value is computed by
 // one thread.  They are chosen by measurement (epic_batch_tune).
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -39,6 +38,7 @@
 #include <string>
 #include <vector>
 
+#include "gemm_tile.hpp"
 #include "node_ops.hpp"
 #include "push_f32.hpp"
 #include "sort_ops.hpp"
@@ -53,8 +53,8 @@ struct BSim {
   double* b;
   float2* E2;
   float4* Ec;      // FP32 gather table (cell_coeffs)
-  float* X;        // this member's p*q block of the group's transform workspace
-  const float* G;  // group's spectral scaling c / (lam_k + mu_l)
+  float* X;        // this member's P*Q block of the group's transform workspace
+  const float* G;  // group's spectral scaling c / (lam_k + mu_l), P*Q
   SpecLayout L;
   int nx, ny, bc, op;
   int sp0, nsp;      // species records [sp0, sp0 + nsp)
@@ -75,6 +75,10 @@ struct BTile {
 };
 
 // ---- node stages -----------------------------------------------------------
+__global__ void fill_f32_kernel(float* p, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0.f;
+}
+
 __global__ void bfinalize_kernel(const BSim* sims, const BSpecies* sps, const BTile* tiles, double* part_b) {
   __shared__ double sh[16];
   const BTile t = tiles[blockIdx.x];
@@ -103,15 +107,6 @@ __global__ void bfinalize_kernel(const BSim* sims, const BSpecies* sps, const BT
   double v[1] = {bb};
   block_sum<1>(v, sh);
   if (threadIdx.x == 0) part_b[blockIdx.x] = v[0];
-}
-
-__global__ void bhadamard_kernel(const BSim* sims, const BTile* tiles) {
-  const BTile t = tiles[blockIdx.x];
-  const BSim& S = sims[t.sim];
-  const int k = t.k0 + threadIdx.x;
-  if (k >= S.nx * S.ny) return;
-  const int x = spec_index(S.L, k);
-  if (x >= 0) S.X[x] *= S.G[x];
 }
 
 __global__ void bunpack_kernel(const BSim* sims, const BTile* tiles, int first) {
@@ -401,7 +396,8 @@ void spec_destroy(Ctx* c);
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t bytes) {
+  hipError_t ensure(size_t bytes, bool* grown = nullptr) {
+    if (grown) *grown = false;
     if (bytes <= cap) return hipSuccess;
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -409,6 +405,7 @@ struct DevBuf {
     const size_t want = std::max<size_t>(bytes + bytes / 2, 256);
     hipError_t e = hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
+    if (grown) *grown = e == hipSuccess;
     return e;
   }
   void release() {
@@ -418,10 +415,12 @@ struct DevBuf {
   }
 };
 
-// Sine-transform matrices of one distinct (p, q) and the workspace of the
-// members that share it.
+// Sine-transform matrices of one distinct (p, q), zero-padded to P x Q
+// (multiples of 64, the tile products' shapes), and the workspace of the
+// members that share it (P*Q per member, its padding kept zero).
 struct Group {
   int p = 0, q = 0;
+  int P = 0, Q = 0;
   int nb = 0;  // members in the current layout
   float* Sp = nullptr;
   float* Sq = nullptr;
@@ -439,8 +438,6 @@ struct Member {
 struct Batch {
   hipStream_t stream = nullptr;
   int device = 0, num_cu = 0;
-  rocblas_handle blas = nullptr;
-  DevBuf blas_ws;
   epic_batch_opts opts{};
   int push_variant = 0;
   int node_block = 256;
@@ -484,25 +481,27 @@ struct Batch {
 
 static Ctx* member_ctx(const Member& m) { return ctx_of(m.h); }
 
+// The single-simulation solver's layout (spectral.hip), padded the same way.
 static SpecLayout layout_of(const epic_domain_desc& d) {
-  if (d.poisson_op == EPIC_OP_DIRICHLET_WALL) return SpecLayout{d.ny - 2, d.nx - 2, d.ny, d.ny + 1, d.ny - 2, d.nx - 2};
-  return SpecLayout{d.nx, d.ny, d.nx, 0, d.nx, d.ny};
+  if (d.poisson_op == EPIC_OP_DIRICHLET_WALL)
+    return SpecLayout{d.ny - 2, d.nx - 2, d.ny, d.ny + 1, spec_pad32(d.ny - 2), spec_pad32(d.nx - 2)};
+  return SpecLayout{d.nx, d.ny, d.nx, 0, spec_pad32(d.nx), spec_pad32(d.ny)};
 }
 
 // FP32 sine matrices and scaling, computed in FP64 and rounded (the FP32
-// copies of the single-simulation solver).
+// copies of the single-simulation solver), zero outside p x p, q x q, p x q.
 static hipError_t group_init(Group& g) {
-  const int p = g.p, q = g.q;
-  std::vector<float> sp((size_t)p * p), sq((size_t)q * q), gg((size_t)p * q);
+  const int p = g.p, q = g.q, P = g.P, Q = g.Q;
+  std::vector<float> sp((size_t)P * P, 0.f), sq((size_t)Q * Q, 0.f), gg((size_t)P * Q, 0.f);
   for (int a = 0; a < p; a++)
-    for (int k = 0; k < p; k++) sp[a + (size_t)k * p] = (float)sin(M_PI * (double)(a + 1) * (k + 1) / (p + 1));
+    for (int k = 0; k < p; k++) sp[a + (size_t)k * P] = (float)sin(M_PI * (double)(a + 1) * (k + 1) / (p + 1));
   for (int a = 0; a < q; a++)
-    for (int k = 0; k < q; k++) sq[a + (size_t)k * q] = (float)sin(M_PI * (double)(a + 1) * (k + 1) / (q + 1));
+    for (int k = 0; k < q; k++) sq[a + (size_t)k * Q] = (float)sin(M_PI * (double)(a + 1) * (k + 1) / (q + 1));
   const double cc = (2.0 / (p + 1)) * (2.0 / (q + 1));
   for (int k = 0; k < p; k++)
     for (int l = 0; l < q; l++) {
       const double lam = 2 * cos(M_PI * (k + 1) / (p + 1)) - 2 + 2 * cos(M_PI * (l + 1) / (q + 1)) - 2;
-      gg[k + (size_t)l * p] = (float)(cc / lam);
+      gg[k + (size_t)l * P] = (float)(cc / lam);
     }
   EPIC_HIP_TRY(hipMalloc(&g.Sp, sp.size() * 4));
   EPIC_HIP_TRY(hipMalloc(&g.Sq, sq.size() * 4));
@@ -536,6 +535,8 @@ static hipError_t build_groups(Batch* B, std::vector<int>& grp, std::vector<int>
       Group g;
       g.p = L.p;
       g.q = L.q;
+      g.P = L.pp;
+      g.Q = L.qq;
       EPIC_HIP_TRY(group_init(g));
       B->groups.push_back(g);
       gi = (int)B->groups.size() - 1;
@@ -545,11 +546,22 @@ static hipError_t build_groups(Batch* B, std::vector<int>& grp, std::vector<int>
   }
   for (Group& g : B->groups) {
     if (g.nb == 0) continue;
-    const size_t bytes = (size_t)g.p * g.q * g.nb * sizeof(float);
+    const size_t bytes = (size_t)g.P * g.Q * g.nb * sizeof(float);
     // growing frees the old workspace: the previous step must be done with it
     if (bytes > g.X.cap || bytes > g.Y.cap) EPIC_HIP_TRY(hipStreamSynchronize(B->stream));
-    EPIC_HIP_TRY(g.X.ensure(bytes));
-    EPIC_HIP_TRY(g.Y.ensure(bytes));
+    bool gx = false, gy = false;
+    EPIC_HIP_TRY(g.X.ensure(bytes, &gx));
+    EPIC_HIP_TRY(g.Y.ensure(bytes, &gy));
+    // a new workspace starts zero (its padding stays zero through the
+    // products); a fill kernel, not a memset, in case a step is captured
+    if (gx) {
+      hipLaunchKernelGGL(fill_f32_kernel, dim3(1024), dim3(256), 0, B->stream, (float*)g.X.p, g.X.cap / 4);
+      EPIC_HIP_TRY(hipGetLastError());
+    }
+    if (gy) {
+      hipLaunchKernelGGL(fill_f32_kernel, dim3(1024), dim3(256), 0, B->stream, (float*)g.Y.p, g.Y.cap / 4);
+      EPIC_HIP_TRY(hipGetLastError());
+    }
   }
   return hipSuccess;
 }
@@ -579,7 +591,7 @@ static hipError_t build_descriptors(Batch* B, bool full) {
     S.rho = c->rho; S.phi = c->phi; S.efx = c->efx; S.efy = c->efy; S.b = c->b64;
     S.E2 = (float2*)c->E2;
     S.Ec = (float4*)c->Ec;
-    S.X = (float*)g.X.p + (size_t)sig[m] * g.p * g.q;
+    S.X = (float*)g.X.p + (size_t)sig[m] * g.P * g.Q;
     S.G = g.G;
     S.L = layout_of(d);
     S.nx = d.nx; S.ny = d.ny; S.bc = d.bc; S.op = d.poisson_op;
@@ -699,21 +711,21 @@ static hipError_t refresh(Batch* B) {
   return hipSuccess;
 }
 
-static hipError_t rb(rocblas_status s) { return s == rocblas_status_success ? hipSuccess : hipErrorUnknown; }
-
-// X <- S_p X S_q for every member of group g (in place through Y).
-static hipError_t group_transform(Batch* B, Group& g) {
-  const float one = 1.0f, zero = 0.0f;
-  const int p = g.p, q = g.q;
+// X <- S_p X S_q (times G elementwise when scale) for every member of group
+// g (in place through Y), on the tile products: the same arithmetic for a
+// member whatever the group's size, and the same as the single-simulation
+// solver's.
+static hipError_t group_transform(Batch* B, Group& g, bool scale) {
+  const int P = g.P, Q = g.Q;
   float* X = (float*)g.X.p;
   float* Y = (float*)g.Y.p;
+  const SpecLayout none{};
   // Y = S_p [X_1 ... X_nb]: one product over the members' concatenated columns
-  EPIC_HIP_TRY(rb(rocblas_sgemm(B->blas, rocblas_operation_none, rocblas_operation_none, p, q * g.nb, p, &one, g.Sp,
-                                p, X, p, &zero, Y, p)));
-  // X_m = Y_m S_q
-  EPIC_HIP_TRY(rb(rocblas_sgemm_strided_batched(B->blas, rocblas_operation_none, rocblas_operation_none, p, q, q,
-                                                &one, Y, p, (rocblas_stride)p * q, g.Sq, q, 0, &zero, X, p,
-                                                (rocblas_stride)p * q, g.nb)));
+  const TileGemm t1{g.Sp, P, X, P, Y, P, nullptr, nullptr, none, 0, 0, 0, 0};
+  EPIC_HIP_TRY(gemm_tile32(B->stream, P, Q * g.nb, P, t1));
+  // X_m = Y_m S_q (o G), one launch for the group's members (blockIdx.z)
+  const TileGemm t2{Y, P, g.Sq, Q, X, P, scale ? g.G : nullptr, nullptr, none, 0, (size_t)P * Q, 0, (size_t)P * Q};
+  EPIC_HIP_TRY(gemm_tile32(B->stream, P, Q, Q, t2, g.nb));
   return hipSuccess;
 }
 
@@ -739,11 +751,9 @@ static hipError_t enqueue_step(Batch* B, bool events) {
     const int K = B->opts.refine;
     for (int k = 0; k < K; k++) {
       for (Group& g : B->groups)
-        if (g.nb) EPIC_HIP_TRY(group_transform(B, g));
-      hipLaunchKernelGGL(bhadamard_kernel, dim3(nt), dim3(NB), 0, B->stream, sims, tiles);
-      EPIC_HIP_TRY(hipGetLastError());
+        if (g.nb) EPIC_HIP_TRY(group_transform(B, g, true));
       for (Group& g : B->groups)
-        if (g.nb) EPIC_HIP_TRY(group_transform(B, g));
+        if (g.nb) EPIC_HIP_TRY(group_transform(B, g, false));
       hipLaunchKernelGGL(bunpack_kernel, dim3(nt), dim3(NB), 0, B->stream, sims, tiles, k == 0 ? 1 : 0);
       EPIC_HIP_TRY(hipGetLastError());
       hipLaunchKernelGGL(bresidual_kernel, dim3(nt), dim3(NB), 0, B->stream, sims, tiles, pr);
@@ -802,7 +812,7 @@ static hipError_t pre_step(Batch* B) {
   for (Member& m : B->members) {
     Ctx* c = member_ctx(m);
     if (c->blas) {
-      // the initial-field solver's matrices and handle are not needed again
+      // the initial-field solver's matrices are not needed again
       EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
       spec_destroy(c);
     }
@@ -1023,20 +1033,6 @@ int epic_batch_create(const epic_batch_opts* opts, epic_batch** out) {
   }
   B->num_cu = prop.multiProcessorCount;
   for (auto& e : B->ev) (void)hipEventCreate(&e);
-  if (rocblas_create_handle(&B->blas) != rocblas_status_success ||
-      rocblas_set_stream(B->blas, B->stream) != rocblas_status_success) {
-    (void)hipStreamDestroy(B->stream);
-    delete h;
-    return EPIC_ERR_HIP;
-  }
-  // a fixed rocBLAS workspace: no allocation inside a captured step
-  if (B->blas_ws.ensure(32u << 20) != hipSuccess ||
-      rocblas_set_workspace(B->blas, B->blas_ws.p, B->blas_ws.cap) != rocblas_status_success) {
-    rocblas_destroy_handle(B->blas);
-    (void)hipStreamDestroy(B->stream);
-    delete h;
-    return EPIC_ERR_HIP;
-  }
   *out = h;
   return EPIC_OK;
 }
@@ -1053,8 +1049,6 @@ int epic_batch_destroy(epic_batch* h) {
                     &B->d_app, &B->d_steps, &B->d_part_b, &B->d_part_r, &B->d_relres};
   for (DevBuf* d : bufs) d->release();
   for (DevBuf& d : B->d_sortsegs) d.release();
-  if (B->blas) rocblas_destroy_handle(B->blas);
-  B->blas_ws.release();
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(B->stream);
@@ -1224,7 +1218,8 @@ int epic_batch_get_stats(epic_batch* h, epic_batch_stats* out) {
   for (int k = 0; k < 4; k++) s.stage_ms[k] = B->stage_ms[k];
   int64_t bytes = 0;
   for (const Member& m : B->members) bytes += (int64_t)member_ctx(m)->bytes;
-  for (const Group& g : B->groups) bytes += (int64_t)(g.X.cap + g.Y.cap) + 4ll * ((int64_t)g.p * g.p + (int64_t)g.q * g.q + (int64_t)g.p * g.q);
+  for (const Group& g : B->groups)
+    bytes += (int64_t)(g.X.cap + g.Y.cap) + 4ll * ((int64_t)g.P * g.P + (int64_t)g.Q * g.Q + (int64_t)g.P * g.Q);
   s.device_bytes = bytes;
   s.relres_max = 0;
   if (!B->members.empty() && B->steps > 0 && !B->layout_dirty) {

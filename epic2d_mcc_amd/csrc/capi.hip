@@ -208,7 +208,6 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
   if (const char* v = getenv("EPIC_COND_CELL_MASK")) c->cond_cell_mask = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI_WAVES")) c->push_multi_waves = atoi(v);
-  if (const char* v = getenv("EPIC_SPEC_MFMA")) c->spec_mfma = atoi(v);
   if (const char* v = getenv("EPIC_PCG_BLOCK")) c->pcg_block = atoi(v);
   // many-workgroup PCG / SOR: one workgroup per two CUs.  Every iteration
   // ends in grid barriers whose cost grows with the number of workgroups

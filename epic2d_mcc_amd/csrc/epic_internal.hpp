@@ -208,7 +208,6 @@ struct Ctx {
   int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
   int push_multi_waves = 8;    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
   int cond_cell_mask = 1;      // 0: every cell flagged (the exact test everywhere)
-  int spec_mfma = 1;           // FP32 sine transforms on the hand-written MFMA GEMM (EPIC_SPEC_MFMA=0: rocBLAS)
   int pcg_block = 512;         // single-workgroup PCG size above 4096 nodes: 1024, 512 or 256 threads (EPIC_PCG_BLOCK)
 
   // species s is due for its counting sort after this step

@@ -10,13 +10,12 @@
 //   U = S_p ( (S_p X S_q) o G ) S_q,   G[k][l] = c / (lam_k + mu_l),
 //   c = (2/(p+1)) (2/(q+1))
 //
-// is four dense matrix products (rocBLAS GEMMs: FP64 or FP32 MFMA) and one
-// elementwise scaling.  In FP32 it is the inner solve of the mixed-precision
+// is four dense matrix products on the matrix cores (gemm_tile.hpp: FP32 or
+// FP64 MFMA, hand-written) with the scaling by G in the second product's
+// epilogue and the write into phi in the fourth's.  In FP32 it is the inner solve of the mixed-precision
 // scheme: FP64 true residual r = b - A phi, c = A^-1 r in FP32, phi += c,
 // repeated until ||r|| <= tol ||b|| (FP64 dot products); in FP64 it is a
 // direct solve followed by the same refinement.
-#include <rocblas/rocblas.h>
-
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -26,15 +25,14 @@
 
 #include "device_common.hpp"
 #include "epic_internal.hpp"
+#include "gemm_tile.hpp"
 #include "node_ops.hpp"
 
 namespace epic {
 
-// Transform dimensions padded to a multiple of 64 with zero rows and
-// columns (S, G and the work matrices are zero there, so the padding stays
-// zero through every product): rocBLAS runs 512^3 in 7.3 us and 511^3 in
-// 11.8 us on this device.
-static int spec_pad(int p) { return p >= 64 ? (p + 63) / 64 * 64 : p; }
+// Transform dimensions padded to a multiple of 32 with zero rows and
+// columns, the shapes the tile products take.
+static int spec_pad(int p) { return spec_pad32(p); }
 
 static SpecLayout spec_layout(const Ctx* c) {
   const epic_domain_desc& d = c->dom;
@@ -51,22 +49,6 @@ __global__ void pack_kernel(const double* src, T* X, SpecLayout L) {
     const int a = k % L.p, b = k / L.p;
     X[a + (size_t)b * L.pp] = (T)src[L.off + a + (size_t)b * L.ld];
   }
-}
-
-template <typename T>
-__global__ void unpack_add_kernel(const T* U, double* phi, SpecLayout L, int overwrite) {
-  const int n = L.p * L.q;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int a = k % L.p, b = k / L.p;
-    double* d = phi + L.off + a + (size_t)b * L.ld;
-    const double u = (double)U[a + (size_t)b * L.pp];
-    *d = overwrite ? u : *d + u;
-  }
-}
-
-template <typename T>
-__global__ void hadamard_kernel(T* Y, const T* G, int n) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) Y[k] *= G[k];
 }
 
 // residual r = b - A phi over the unknowns (FP64) and ||r||^2 partials;
@@ -133,7 +115,6 @@ __global__ void norms_kernel(const double* pr, const double* pb, int n, double* 
 }
 
 struct SpecState {
-  rocblas_handle h = nullptr;
   SpecLayout L{};
   // FP64 and FP32 copies of S_p, S_q, G
   double *Sp64 = nullptr, *Sq64 = nullptr, *G64 = nullptr, *X64 = nullptr, *Y64 = nullptr;
@@ -178,8 +159,6 @@ static hipError_t spec_init(Ctx* c) {
   EPIC_HIP_TRY(hipMemset(S->Y64, 0, g.size() * 8));
   EPIC_HIP_TRY(hipMemset(S->X32, 0, g.size() * 4));
   EPIC_HIP_TRY(hipMemset(S->Y32, 0, g.size() * 4));
-  if (rocblas_create_handle(&S->h) != rocblas_status_success) return hipErrorUnknown;
-  if (rocblas_set_stream(S->h, c->stream) != rocblas_status_success) return hipErrorUnknown;
   c->blas = S;
   return hipSuccess;
 }
@@ -190,178 +169,34 @@ void spec_destroy(Ctx* c) {
   void* ps[] = {S->Sp64, S->Sq64, S->G64, S->X64, S->Y64, S->Sp32, S->Sq32, S->G32, S->X32, S->Y32};
   for (void* p : ps)
     if (p) hipFree(p);
-  if (S->h) rocblas_destroy_handle(S->h);
   delete S;
   c->blas = nullptr;
 }
 
-// C = A B (times G elementwise when G is given), column-major FP32, on the
-// matrix cores: M, N multiples of 32, K a multiple of 64 up to 512.  One workgroup per
-// 32 x 32 tile of C; its four wavefronts take a quarter of K each, with a
-// 2 x 2 block of v_mfma_f32_16x16x4f32 accumulators; the four partial tiles
-// are summed in LDS in wavefront order, so the result does not depend on
-// scheduling.  Operand layout of the 16x16x4 MFMA: lane l holds A[l%16][k]
-// and B[k][l%16] for its k = l/16, and D[4(l/16)+r][l%16] in register r.
-// The contraction index is free to permute, so lane group g takes the four
-// consecutive k = kb + 4g + e over the four MFMAs e of a 16-deep step: the B
-// operand (contiguous along k) is then one 16-byte load per lane.
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// Operands of one tile GEMM.  dst: the product goes to dst through the
-// layout L (overwritten or added) instead of to C.  G: the product is scaled
-// by G elementwise.
-struct TileGemm {
-  const float* A;
-  int lda;
-  const float* B;
-  int ldb;
-  float* C;
-  int ldc;
-  const float* G;
-  double* dst;
-  SpecLayout L;
-  int overwrite;
-};
-
-// (one or two wavefronts per SIMD: the register budget lets the scheduler
-// keep every operand load of the K slice in flight at once)
-template <int NSTEP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void sgemm_tile32_kernel(TileGemm t) {
-  __shared__ float part[4][32 * 32];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int tm = blockIdx.x * 32, tn = blockIdx.y * 32;
-  const int r16 = lane & 15, g = lane >> 4;
-  const int k0 = w * NSTEP * 16;  // this wavefront's quarter of K = 64 NSTEP
-  v4f b[NSTEP][2];
-  float a[NSTEP][2][4];
-#pragma unroll
-  for (int st = 0; st < NSTEP; st++) {
-    const int kb = k0 + st * 16;
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-      const int col = tn + cb * 16 + r16;
-      b[st][cb] = *reinterpret_cast<const v4f*>(t.B + (size_t)col * t.ldb + kb + 4 * g);
-    }
-#pragma unroll
-    for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-      for (int e = 0; e < 4; e++)
-        a[st][rb][e] = t.A[(size_t)(tm + rb * 16 + r16) + (size_t)(kb + 4 * g + e) * t.lda];
-  }
-  // the scheduler may not move anything across: every load above is in
-  // flight before the first product waits for its own operands
-  __builtin_amdgcn_sched_barrier(0);
-  v4f acc[2][2];
-#pragma unroll
-  for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int st = 0; st < NSTEP; st++)
-#pragma unroll
-    for (int e = 0; e < 4; e++)
-#pragma unroll
-      for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-        for (int cb = 0; cb < 2; cb++)
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st][rb][e], b[st][cb][e], acc[rb][cb], 0, 0, 0);
-#pragma unroll
-  for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) part[w][(rb * 16 + 4 * g + r) + (cb * 16 + r16) * 32] = acc[rb][cb][r];
-  __syncthreads();
-  for (int q = threadIdx.x; q < 32 * 32; q += 256) {
-    float v = part[0][q] + part[1][q];
-    v += part[2][q];
-    v += part[3][q];
-    const int i = tm + (q & 31), j = tn + (q >> 5);
-    const size_t o = (size_t)i + (size_t)j * t.ldc;
-    if (t.G) v *= t.G[o];
-    if (t.dst) {
-      if (i < t.L.p && j < t.L.q) {
-        double* d = t.dst + t.L.off + i + (size_t)j * t.L.ld;
-        *d = t.overwrite ? (double)v : *d + (double)v;
-      }
-    } else {
-      t.C[o] = v;
-    }
-  }
-}
-
-// C (m x n) = A (m x k) B (k x n) (times G), k = 64 NSTEP
-static void sgemm_tile32(hipStream_t st, int m, int n, int k, const TileGemm& t) {
-  const dim3 grid(m / 32, n / 32);
-  switch (k / 64) {
-#define EPIC_G(N) \
-  case N: hipLaunchKernelGGL(sgemm_tile32_kernel<N>, grid, dim3(256), 0, st, t); break;
-    EPIC_G(1) EPIC_G(2) EPIC_G(3) EPIC_G(4) EPIC_G(5) EPIC_G(6) EPIC_G(7) EPIC_G(8)
-#undef EPIC_G
-    default: break;
-  }
-}
-
-static bool tile32_ok(int m, int n, int k) { return m % 32 == 0 && n % 32 == 0 && k % 64 == 0 && k <= 512; }
-
-// The FP32 transform runs on sgemm_tile32 (the residual kernel then writes
-// its packed FP32 operand).
-static bool spec_mfma_ok(const Ctx* c, const SpecState* S) {
-  const int P = S->L.pp, Q = S->L.qq;
-  return c->spec_mfma && tile32_ok(P, Q, P) && tile32_ok(P, Q, Q);
-}
-
-static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                           double* C, int ldc) {
-  const double one = 1.0, zero = 0.0;
-  return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C,
-                       ldc);
-}
-static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
-                           float* C, int ldc) {
-  const float one = 1.0f, zero = 0.0f;
-  return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C,
-                       ldc);
-}
-
-// phi (+)= A^-1 src over the unknowns, in precision T.
+// phi (+)= A^-1 src over the unknowns, in precision T: four tile products,
+// the scaling by G in the second's epilogue and the write into phi in the
+// fourth's.  FP32: X holds the packed residual (written by the residual
+// kernel); FP64: src is packed here.
 template <typename T>
 static hipError_t spec_apply(Ctx* c, SpecState* S, const double* src, double* phi, bool overwrite, const T* Sp,
                              const T* Sq, const T* G, T* X, T* Y) {
   const SpecLayout L = S->L;
-  const int n = L.p * L.q;
   const int P = L.pp, Q = L.qq;
-  const int grid = std::min((n + 255) / 256, 4096);
-  const int gridP = std::min((P * Q + 255) / 256, 4096);
-  if (sizeof(T) == 4 && spec_mfma_ok(c, S)) {
-    // FP32 on the hand-written MFMA kernel: the Hadamard scaling by G in
-    // the second product's epilogue, the unpacking into phi in the fourth's
-    const float* Spf = (const float*)Sp;
-    const float* Sqf = (const float*)Sq;
-    float* Xf = (float*)X;
-    float* Yf = (float*)Y;
-    // X holds the packed FP32 residual, written by the residual kernel
-    TileGemm t1{Spf, P, Xf, P, Yf, P, nullptr, nullptr, L, 0};
-    TileGemm t2{Yf, P, Sqf, Q, Xf, P, (const float*)G, nullptr, L, 0};
-    TileGemm t3{Spf, P, Xf, P, Yf, P, nullptr, nullptr, L, 0};
-    TileGemm t4{Yf, P, Sqf, Q, Xf, P, nullptr, phi, L, overwrite ? 1 : 0};
-    sgemm_tile32(c->stream, P, Q, P, t1);
-    sgemm_tile32(c->stream, P, Q, Q, t2);
-    sgemm_tile32(c->stream, P, Q, P, t3);
-    sgemm_tile32(c->stream, P, Q, Q, t4);
-    return hipGetLastError();
-  } else {
+  if (sizeof(T) == 8) {
+    const int n = L.p * L.q;
+    const int grid = std::min((n + 255) / 256, 4096);
     hipLaunchKernelGGL(pack_kernel<T>, dim3(grid), dim3(256), 0, c->stream, src, X, L);
     EPIC_HIP_TRY(hipGetLastError());
-    if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
-    if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
-    hipLaunchKernelGGL(hadamard_kernel<T>, dim3(gridP), dim3(256), 0, c->stream, X, G, P * Q);
-    EPIC_HIP_TRY(hipGetLastError());
-    if (gemm(S->h, P, Q, P, Sp, P, X, P, Y, P) != rocblas_status_success) return hipErrorUnknown;
-    if (gemm(S->h, P, Q, Q, Y, P, Sq, Q, X, P) != rocblas_status_success) return hipErrorUnknown;
   }
-  hipLaunchKernelGGL(unpack_add_kernel<T>, dim3(grid), dim3(256), 0, c->stream, X, phi, L, overwrite ? 1 : 0);
-  return hipGetLastError();
+  const TileGemmT<T> t1{Sp, P, X, P, Y, P, nullptr, nullptr, L, 0, 0, 0, 0};
+  const TileGemmT<T> t2{Y, P, Sq, Q, X, P, G, nullptr, L, 0, 0, 0, 0};
+  const TileGemmT<T> t3{Sp, P, X, P, Y, P, nullptr, nullptr, L, 0, 0, 0, 0};
+  const TileGemmT<T> t4{Y, P, Sq, Q, X, P, nullptr, phi, L, overwrite ? 1 : 0, 0, 0, 0};
+  EPIC_HIP_TRY(gemm_tile32(c->stream, P, Q, P, t1));
+  EPIC_HIP_TRY(gemm_tile32(c->stream, P, Q, Q, t2));
+  EPIC_HIP_TRY(gemm_tile32(c->stream, P, Q, P, t3));
+  EPIC_HIP_TRY(gemm_tile32(c->stream, P, Q, Q, t4));
+  return hipSuccess;
 }
 
 // Partial sums of the residual kernels: the latest residual's at
@@ -430,7 +265,7 @@ static hipError_t spectral_solve_on(Ctx* c, SpecState* S, double* b, double* phi
                                     int max_ref, int* solves, double* relres, const double* rho, double dx2,
                                     double s, bool poll) {
   double* dscal = c->ws_scalar + 4;  // [4] ||b||^2, [5] ||r||^2
-  float* Xpack = fp32_inner && spec_mfma_ok(c, S) ? S->X32 : nullptr;
+  float* Xpack = fp32_inner ? S->X32 : nullptr;
   const SpecLayout& L = S->L;
   // b from rho, phi := 0 and r = b in one kernel (the corrections write
   // phi's interior)
@@ -496,10 +331,9 @@ hipError_t launch_spectral_solve(Ctx* c, double* b, double* phi, bool fp32_inner
 // The transform matrices and work arrays of launch_spectral_solve come from a
 // file-scope device array instead of hipMalloc on the context's first solve,
 // and the convergence check reads the norms by a copy instead of pinned host
-// words.  Dimensions are padded to multiples of 64 (at least 64) so that
-// every product runs on sgemm_tile32, which takes K <= 512: transform
-// dimensions up to 512, i.e. node grids up to 514 x 514 (Dirichlet walls) or
-// 512 x 512 (pcgsolver's operator).
+// words.  Dimensions are padded as the allocating solve pads them; the
+// array holds transform dimensions up to 512, i.e. node grids up to
+// 514 x 514 (Dirichlet walls) or 512 x 512 (pcgsolver's operator).
 constexpr int NOALLOC_MAX_DIM = 512;
 constexpr size_t NOALLOC_FLOATS = (size_t)5 * NOALLOC_MAX_DIM * NOALLOC_MAX_DIM;  // Sp, Sq, G, X, Y: 5 MiB
 static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
@@ -510,7 +344,6 @@ static float* noalloc_ptr[NOALLOC_MAX_DEVICES];       // scratch's device addres
 static SpecLayout noalloc_built[NOALLOC_MAX_DEVICES];  // layout of the matrices scratch holds
 static bool noalloc_valid[NOALLOC_MAX_DEVICES];
 
-static int pad64(int p) { return std::max(64, (p + 63) / 64 * 64); }
 
 // S_p, S_q, G of the layout into scratch, X and Y zeroed (their padding
 // must be zero), in the values spec_init computes (FP64 arithmetic rounded
@@ -554,8 +387,8 @@ __global__ void noalloc_build_kernel(float* ws, SpecLayout L) {
 hipError_t launch_spectral_solve_noalloc(Ctx* c, int* solves, double* relres, int* status, const char** why) {
   const epic_domain_desc& d = c->dom;
   SpecLayout L = spec_layout(c);
-  L.pp = pad64(L.p);
-  L.qq = pad64(L.q);
+  L.pp = spec_pad32(L.p);
+  L.qq = spec_pad32(L.q);
   *status = 0;
   if (L.pp > NOALLOC_MAX_DIM || L.qq > NOALLOC_MAX_DIM) {
     *status = EPIC_ERR_CAPACITY;
@@ -605,13 +438,10 @@ hipError_t launch_spectral_solve_noalloc(Ctx* c, int* solves, double* relres, in
   S.Y32 = S.X32 + (size_t)L.pp * L.qq;
   const double tol = d.tol_relative ? d.tolerance : 1e-12;
   const double s = (d.L * d.L) / (d.LDe * d.LDe);
-  const bool mfma = c->spec_mfma;
-  c->spec_mfma = 1;  // the products run on sgemm_tile32 whatever EPIC_SPEC_MFMA says
   const bool eis = c->efield_in_solve;
   c->efield_in_solve = false;
   const hipError_t e =
       spectral_solve_on(c, &S, c->b64, c->phi, true, tol, 6, solves, relres, c->rho, d.dx * d.dx, s, false);
-  c->spec_mfma = mfma;
   c->efield_in_solve = eis;
   return e;
 }

@@ -188,7 +188,8 @@ double pct(std::vector<double> v, double p) {
 }
 
 // Device bytes a job holds while admitted: its context, MCC tables and its
-// share of the batch's FP32 transform workspace (X and Y, p*q each).
+// share of the batch's FP32 transform workspace (X and Y, P*Q each: the
+// transform dimensions padded to multiples of 32).
 int64_t job_bytes(const epic_domain_desc& d, const epic_species_desc* sp, int nsp, const epic_serve_job& j,
                   const epic_serve_config* c) {
   int64_t b = 0;
@@ -201,7 +202,8 @@ int64_t job_bytes(const epic_domain_desc& d, const epic_species_desc* sp, int ns
     if (epic_mcc_estimate_bytes(&m, sp, nsp, &mb) != EPIC_OK) return -1;
     b += mb;
   }
-  b += 2ll * d.nx * d.ny * 4 * 3 / 2;  // X, Y (the workspace grows by 1.5x)
+  auto pad32 = [](int64_t p) { return (p + 31) / 32 * 32; };
+  b += 2ll * pad32(d.nx) * pad32(d.ny) * 4 * 3 / 2;  // X, Y (the workspace grows by 1.5x)
   return b;
 }
 

@@ -617,7 +617,13 @@ static void apply_A(int nx, int ny, const std::vector<double>& v, std::vector<do
 
 static double dot(const std::vector<double>& a, const std::vector<double>& b) {
   double r = 0;  // vec::operator* (linalg.h:632-646)
-  for (size_t i = 0; i < a.size(); i++) r += a[i] * b[i];
+  if (g_threads <= 1) {
+    for (size_t i = 0; i < a.size(); i++) r += a[i] * b[i];
+    return r;
+  }
+  const long n = (long)a.size();
+#pragma omp parallel for num_threads(g_threads) reduction(+ : r) schedule(static)
+  for (long i = 0; i < n; i++) r += a[i] * b[i];
   return r;
 }
 
@@ -653,11 +659,15 @@ int orc_pcg(int nx, int ny, const double* b_in, double* x_out, int max_iteration
     if (std::sqrt(dot(r, r)) < tolerance) break;
     apply_A_par(nx, ny, v, u);
     double t = alpha / dot(v, u);
-    for (int i = 0; i < n; i++) x[i] = x[i] + v[i] * t;
-    for (int i = 0; i < n; i++) r[i] = r[i] - u[i] * t;
-    for (int i = 0; i < n; i++) w[i] = (1.0 / -4.0) * r[i];
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+    for (int i = 0; i < n; i++) {
+      x[i] = x[i] + v[i] * t;
+      r[i] = r[i] - u[i] * t;
+      w[i] = (1.0 / -4.0) * r[i];
+    }
     double beta = dot(w, r);
     double s = beta / alpha;
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int i = 0; i < n; i++) v[i] = w[i] + v[i] * s;
     alpha = beta;
     iters++;
@@ -943,10 +953,15 @@ int orc_push(void* h, int s, int sub_cycle) {
       part[t].reserve(e - b);
       for (size_t k = b; k < e; k++) one(pl[k], part[t], bad);
     }
-    size_t tot = 0;
-    for (auto& v : part) tot += v.size();
-    kept.reserve(tot);
-    for (auto& v : part) kept.insert(kept.end(), v.begin(), v.end());
+    std::vector<size_t> off(g_threads + 1, 0);
+    for (int t = 0; t < g_threads; t++) off[t + 1] = off[t] + part[t].size();
+    kept.resize(off[g_threads]);
+#pragma omp parallel num_threads(g_threads)
+    {
+      const int t = omp_get_thread_num();
+      std::copy(part[t].begin(), part[t].end(), kept.begin() + off[t]);
+      std::vector<Particle>().swap(part[t]);
+    }
   }
   pl.swap(kept);
   return bad;
