@@ -242,7 +242,7 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": moved / ms / 1e6}
 
 
-def ensemble_run(sims, steps, warmup, tune_steps=2, eager_steps=5):
+def ensemble_run(sims, steps, warmup, tune_steps=2, eager_steps=10):
     """The batched ensemble: every member in one batch, kernel shapes tuned on
     real steps, then `steps` timed graph replays."""
     B = _lib.Batch()
@@ -253,11 +253,15 @@ def ensemble_run(sims, steps, warmup, tune_steps=2, eager_steps=5):
         for s, sp in enumerate(st.species):
             ctx.load(s, sp.numparticle)
             ctx.sort(s)
-        ctx.initial_field()
+        B.init(mid)  # the initial field and half-step rewind inside the first batched step
         ids.append(mid)
     B.sync()
     t_setup = time.perf_counter() - t0
-    tuning = B.tune(tune_steps)
+    print(f"[bench] ensemble: {len(sims)} members set up in {t_setup:.1f} s", file=sys.stderr, flush=True)
+    tuning = None
+    if tune_steps > 0:
+        tuning = B.tune(tune_steps)
+        print("[bench] ensemble: kernel shapes tuned", file=sys.stderr, flush=True)
     B.step(warmup)
     B.sync()
 
@@ -273,7 +277,10 @@ def ensemble_run(sims, steps, warmup, tune_steps=2, eager_steps=5):
     stats = B.stats()
     n1 = count()
     # stage breakdown from a few eager steps (a graph replay is one event pair)
+    print("[bench] ensemble: timed steps done", file=sys.stderr, flush=True)
     B.configure(use_graph=False)
+    B.step(1)  # the first eager step after the graph replays is not timed
+    B.sync()
     B.timing(True)
     B.step(eager_steps)
     B.sync()
@@ -317,7 +324,7 @@ def sequential_run(sims, steps, precision, solver):
 
 def main_ensemble(args):
     sims = workloads.ensemble_64(n=args.members)
-    r = ensemble_run(sims, args.steps, args.warmup)
+    r = ensemble_run(sims, args.steps, args.warmup, tune_steps=args.tune_steps)
     npart = 0.5 * (r["n0"] + r["n1"])
     value = npart * args.steps / r["t"]
     tu = r["tuning"]
@@ -618,7 +625,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--solver", default="pcg_mixed", choices=["pcg_mixed", "spectral", "pcg", "gs"])
+    ap.add_argument("--solver", default=None, choices=["pcg_mixed", "spectral", "pcg", "gs"],
+                    help="bounded512 Poisson solver (default: pcg_mixed in fp32, the reference PCG in fp64)")
     ap.add_argument("--ppc", type=int, default=100)
     ap.add_argument("--cpu-sample", type=float, default=1.0)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -632,6 +640,9 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=30.0)
     ap.add_argument("--serve-baseline-jobs", type=int, default=12)
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--tune-steps", type=int, default=2,
+                    help="ensemble: timed steps per kernel-shape candidate (0: no tuning, the default shapes)")
+    ap.add_argument("--sort-interval", type=int, default=20, help="bounded512: steps between counting sorts")
     ap.add_argument("--no-all-cores", action="store_true", help="skip the all-cores CPU variant")
     ap.add_argument("--no-fp64-leg", action="store_true", help="headline: skip the plain FP64 path leg")
     ap.add_argument("--fp64-steps", type=int, default=3)
@@ -674,16 +685,21 @@ def main():
                     "push_f32_multi_kernel<4,false,8> (gather + push + wrap + deposit, all species in one launch)", mcc=mcc, gas=(1, 1, 1, 1e21),
                     cpu_note=", electron collisions by the reference's direct method")
         return
+    solvers = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
+               "gs": _lib.SOLVER_GS}
+    solver_text = {"pcg_mixed": "FP32 sine-transform refinement, FP64 residual", "spectral": "FP64 sine transform",
+                   "pcg": "reference PCG (Jacobi, FP64)", "gs": "red-black SOR (FP64)"}
     if args.precision == "fp64":
-        st = workloads.bounded_512(ppc=args.ppc, precision=_lib.FP64, solver=_lib.SOLVER_PCG)
-        main_single(args, st, "bounded 512x512, 4 species x %d ppc, FP64 particles (the plain path), reference PCG "
-                              "(Jacobi, FP64) on the Dirichlet-wall operator to 1e-10 relative" % args.ppc, "fp64", 64,
+        name = args.solver or "pcg"
+        st = workloads.bounded_512(ppc=args.ppc, precision=_lib.FP64, solver=solvers[name])
+        main_single(args, st, "bounded 512x512, 4 species x %d ppc, FP64 particles (the plain path), %s on the "
+                              "Dirichlet-wall operator to 1e-10 relative" % (args.ppc, solver_text[name]), "fp64", 64,
                     "push_f64_kernel<2,true,false> (gather + push + walls + deposit)")
         return
-    solver = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
-              "gs": _lib.SOLVER_GS}[args.solver]
+    args.solver = args.solver or "pcg_mixed"
+    solver = solvers[args.solver]
     copy_cal = copy_calibration() if args.copy_calibration else None
-    st = workloads.bounded_512(ppc=args.ppc, solver=solver)
+    st = workloads.bounded_512(ppc=args.ppc, solver=solver, sort_interval=args.sort_interval)
     r = gpu_run(st, args.steps, args.warmup, args.solver)
     nsp = len(st.species)
     part_steps = 0.5 * (r["n0"] + r["n1"]) * args.steps
@@ -712,8 +728,8 @@ def main():
         "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
         "config": {"workload": "bounded 512x512, 4 species x %d ppc (%d particles), open walls, "
                                "Dirichlet phi=0, Poisson by FP32 sine-transform corrections with an FP64 true "
-                               "residual to 1e-10 relative, sort every 20 steps" % (
-                                   args.ppc, sum(int(s.numparticle) for s in st.species)),
+                               "residual to 1e-10 relative, sort every %d steps" % (
+                                   args.ppc, sum(int(s.numparticle) for s in st.species), args.sort_interval),
                    "solver": {"pcg_mixed": "FP32 sine-transform corrections, FP64 true residual and refinement "
                                             "(EPIC_SOLVER_PCG_MIXED + EPIC_PRECOND_SPECTRAL)",
                               "spectral": "FP64 sine-transform solve + FP64 refinement",

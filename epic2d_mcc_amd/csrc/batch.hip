@@ -186,10 +186,14 @@ __global__ void bstep_kernel(int* steps, int n) {
 // Work item = one workgroup chunk (PART_BLOCK * PPT particles) of one
 // segment (a species of a member); wavefront w of the workgroup takes the
 // w-th 64*PPT run of it.  Chunks past a segment's live slot count are
-// skipped, so the list can cover capacity (births, injection).
+// skipped, so the list can cover capacity (births, injection).  The
+// descriptors are read-only and unaliased (__restrict__): that lets the
+// compiler fetch the work item and its segment with scalar loads after the
+// kernel's own particle stores; without it they arrive in vector registers
+// (92 VGPRs for the 4-particle shape, 5 waves per SIMD, instead of 64 and 8).
 template <int PPT, int WAVES>
-__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const PushF32* segs, const int2* chunks,
-                                                                        int nchunks) {
+__global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const PushF32* __restrict__ segs,
+                                                                        const int2* __restrict__ chunks, int nchunks) {
   __shared__ unsigned long long lds[(PART_BLOCK / 64) * WAVE_WIN];
   constexpr int CHW = 64 * PPT;
   constexpr int CH = PART_BLOCK * PPT;
@@ -362,7 +366,7 @@ __global__ void bsort_setn_kernel(const SortSeg* segs, int nseg) {
 struct PushShape {
   int ppt, waves;
 };
-static const PushShape kPushShapes[EPIC_BATCH_NPUSH] = {{4, 1}, {8, 1}, {16, 1}, {8, 2}, {4, 2}};
+static const PushShape kPushShapes[EPIC_BATCH_NPUSH] = {{4, 8}, {8, 1}, {16, 1}, {4, 1}, {8, 5}};
 static const int kNodeBlocks[EPIC_BATCH_NNODE] = {64, 128, 256, 512};
 
 template <int PPT, int W>
@@ -384,9 +388,9 @@ static hipError_t launch_push_batch(int variant, hipStream_t st, int num_cu, con
   switch (variant) {
     case 1: return launch_push_batch_t<8, 1>(st, num_cu, segs, chunks, nchunks);
     case 2: return launch_push_batch_t<16, 1>(st, num_cu, segs, chunks, nchunks);
-    case 3: return launch_push_batch_t<8, 2>(st, num_cu, segs, chunks, nchunks);
-    case 4: return launch_push_batch_t<4, 2>(st, num_cu, segs, chunks, nchunks);
-    default: return launch_push_batch_t<4, 1>(st, num_cu, segs, chunks, nchunks);
+    case 3: return launch_push_batch_t<4, 1>(st, num_cu, segs, chunks, nchunks);
+    case 4: return launch_push_batch_t<8, 5>(st, num_cu, segs, chunks, nchunks);
+    default: return launch_push_batch_t<4, 8>(st, num_cu, segs, chunks, nchunks);
   }
 }
 

@@ -306,7 +306,7 @@ int epic_mcc_estimate_bytes(const epic_mcc_desc* d, const epic_species_desc* sp,
  * random-stream key) and its own sort schedule. */
 typedef struct epic_batch epic_batch;
 
-#define EPIC_BATCH_NPUSH 5 /* push shapes: 0 (4 particles/thread, 1 wave/SIMD target), 1 (8,1), 2 (16,1), 3 (8,2), 4 (4,2) */
+#define EPIC_BATCH_NPUSH 5 /* push shapes (particles/thread, minimum waves/SIMD): 0 (4,8), 1 (8,1), 2 (16,1), 3 (4,1), 4 (8,5) */
 #define EPIC_BATCH_NNODE 4 /* node-stage workgroup sizes: 64, 128, 256, 512 */
 
 typedef struct {
