@@ -571,8 +571,68 @@ def host_lib():
         f.restype = ctypes.c_int
     H.epic_serve_last_error.argtypes = []
     H.epic_serve_last_error.restype = ctypes.c_char_p
+    P = ctypes.c_void_p
+    H.epic_h5_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
+    H.epic_h5_group.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+    H.epic_h5_dataset_f64.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    H.epic_h5_attr_i32.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int32]
+    H.epic_h5_attr_f64.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_double]
+    H.epic_h5_attr_str.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32]
+    H.epic_h5_close.argtypes = [P]
+    for f in (H.epic_h5_create, H.epic_h5_group, H.epic_h5_dataset_f64, H.epic_h5_attr_i32, H.epic_h5_attr_f64,
+              H.epic_h5_attr_str, H.epic_h5_close):
+        f.restype = ctypes.c_int
+    H.epic_h5_lookup3.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint32]
+    H.epic_h5_lookup3.restype = ctypes.c_uint32
     _host = H
     return H
+
+
+class H5File:
+    """The result.h5 writer (include/epic2d_h5.h).  Objects are numbers; 0 is
+    the root group.  close() writes the metadata."""
+    ROOT = 0
+
+    def __init__(self, path):
+        self.H = host_lib()
+        self.h = ctypes.c_void_p()
+        self._c(self.H.epic_h5_create(os.fsencode(path), ctypes.byref(self.h)), "create")
+
+    def _c(self, rc, what):
+        if rc != EPIC_OK:
+            raise EpicError(f"epic_h5_{what} failed: {ERR_NAMES.get(rc, rc)}")
+
+    def group(self, parent, name):
+        out = ctypes.c_int()
+        self._c(self.H.epic_h5_group(self.h, parent, name.encode(), ctypes.byref(out)), "group")
+        return out.value
+
+    def dataset(self, parent, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        dims = (ctypes.c_uint64 * a.ndim)(*a.shape)
+        out = ctypes.c_int()
+        self._c(self.H.epic_h5_dataset_f64(self.h, parent, name.encode(), a.ndim, dims, a.ctypes.data,
+                                           ctypes.byref(out)), "dataset_f64")
+        return out.value
+
+    def attr_i32(self, obj, name, v):
+        self._c(self.H.epic_h5_attr_i32(self.h, obj, name.encode(), int(v)), "attr_i32")
+
+    def attr_f64(self, obj, name, v):
+        self._c(self.H.epic_h5_attr_f64(self.h, obj, name.encode(), float(v)), "attr_f64")
+
+    def attr_str(self, obj, name, s, size):
+        self._c(self.H.epic_h5_attr_str(self.h, obj, name.encode(), s.encode(), size), "attr_str")
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, ctypes.c_void_p()
+            self._c(self.H.epic_h5_close(h), "close")
+
+
+def h5_lookup3(data: bytes, initval=0):
+    return host_lib().epic_h5_lookup3(data, len(data), initval)
 
 
 def serve_config(**kw):

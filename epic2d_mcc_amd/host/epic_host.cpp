@@ -254,6 +254,26 @@ int64_t Species::count() {
 }
 std::vector<double> Species::den() { return domain_->field(10 + index); }
 
+void Species::particles(std::vector<double>& pos, std::vector<double>& vel) {
+  epic_ctx* c = domain_->ctx();
+  int64_t n = 0;
+  domain_->check(epic_count(c, index, &n), "epic_count");
+  std::vector<double> x(n), y(n), vx(n), vy(n), vz(n);
+  std::vector<uint32_t> id(n);
+  int64_t got = 0;
+  domain_->check(epic_download_particles(c, index, x.data(), y.data(), vx.data(), vy.data(), vz.data(), id.data(), n,
+                                         &got),
+                 "epic_download_particles");
+  pos.resize(2 * got);
+  vel.resize(2 * got);
+  for (int64_t k = 0; k < got; k++) {
+    pos[2 * k] = x[k];
+    pos[2 * k + 1] = y[k];
+    vel[2 * k] = vx[k];
+    vel[2 * k + 1] = vy[k];
+  }
+}
+
 // ---- Init (init.cpp:3-114) -------------------------------------------------------
 double Init::SampleVel(Species& species) { return SampleVel(species, species.temp); }
 double Init::SampleVel(Species& species, double temp) {
@@ -509,6 +529,151 @@ void Emitter::inject(std::vector<Species>& species_list) {
   epic_ctx* c = domain.ctx();
   if (!first_) return;
   domain.check(epic_inject(c, step++), "epic_inject");
+}
+
+// ---- Output (output.cpp:4-315) ---------------------------------------------------
+Output::Output(const std::filesystem::path& outputfolder, Domain& domain) : outputfolder(outputfolder), domain(domain) {
+  // output.cpp:7-8: the folder is cleared and recreated
+  std::filesystem::remove_all(outputfolder);
+  std::filesystem::create_directories(outputfolder);
+  file = std::make_unique<H5Lite>((outputfolder / "result.h5").string());
+  if (!file->ok()) throw std::runtime_error("Error opening HDF5 file");
+  field_data_group = file->group(H5Lite::ROOT, "fielddata");
+  time_group = file->group(H5Lite::ROOT, "time_var");
+  metadata_group = file->group(H5Lite::ROOT, "metadata");
+  metadata_species = file->group(H5Lite::ROOT, "metadata_species");
+  sp_no = domain.species_no;
+  t_step = int(domain.NUM_TS / domain.write_interval) + 1;
+  store_ke.assign((size_t)t_step * (3 * sp_no + 2), 0.0);
+}
+
+Output::~Output() {
+  try {
+    if (file) close();
+  } catch (const std::exception& e) {
+    std::cerr << "Output: " << e.what() << std::endl;
+  }
+}
+
+void Output::close() {
+  if (!file) return;
+  const bool good = file->close();
+  file.reset();
+  if (!good) throw std::runtime_error("Error writing " + (outputfolder / "result.h5").string());
+}
+
+static void need(int id, const std::string& what) {
+  if (id < 0) throw std::runtime_error("result.h5: cannot create " + what + " (name taken?)");
+}
+
+void Output::write_particle_data(int ts, Species& species) {
+  const int g = file->group(H5Lite::ROOT, "particle_" + species.name);
+  need(g, "group particle_" + species.name);
+  std::vector<double> pos, vel;
+  species.particles(pos, vel);
+  const uint64_t n = pos.size() / 2;
+  need(file->dataset_f64(g, "pos" + std::to_string(ts), {n, 2}, pos.data()), "pos" + std::to_string(ts));
+  need(file->dataset_f64(g, "vel" + std::to_string(ts), {n, 2}, vel.data()), "vel" + std::to_string(ts));
+}
+
+int Output::field_group(const std::string& name) {
+  const int g = file->group(field_data_group, name);
+  need(g, "group fielddata/" + name);
+  return g;
+}
+
+// Node (i, j) of an nx x ny field (device layout i*ny + j) to element
+// j*nx + i of the dataset, as output.cpp:131-139 and 186-193 fill it.
+static std::vector<double> transposed(const std::vector<double>& f, int nx, int ny) {
+  std::vector<double> t((size_t)nx * ny);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) t[(size_t)j * nx + i] = f[(size_t)i * ny + j];
+  return t;
+}
+
+void Output::write_field_data(int ts) {
+  const int gp = field_group("pot"), gx = field_group("efieldx"), gy = field_group("efieldy");
+  const int nx = domain.nx, ny = domain.ny;
+  const std::vector<uint64_t> dims{(uint64_t)nx, (uint64_t)ny};
+  const std::string name = std::to_string(ts);
+  need(file->dataset_f64(gp, name, dims, transposed(domain.phi(), nx, ny).data()), "pot/" + name);
+  need(file->dataset_f64(gx, name, dims, transposed(domain.efx(), nx, ny).data()), "efieldx/" + name);
+  need(file->dataset_f64(gy, name, dims, transposed(domain.efy(), nx, ny).data()), "efieldy/" + name);
+}
+
+void Output::write_den_data(int ts, Species& species) {
+  const int g = field_group("den_" + species.name);
+  const int nx = domain.nx, ny = domain.ny;
+  const std::string name = std::to_string(ts);
+  need(file->dataset_f64(g, name, {(uint64_t)nx, (uint64_t)ny}, transposed(species.den(), nx, ny).data()),
+       "den_" + species.name + "/" + name);
+}
+
+void Output::write_metadata(int NC, int NUM_TS, int write_int, int write_int_phase, double DT, double density,
+                            int save_fig, int normscheme, int subcycleint, double LDe, double LDi, double wpe,
+                            double wpi, int spno, double GAS_DENSITY) {
+  const int g = metadata_group;
+  bool ok = file->attr_i32(g, "NC", NC);
+  ok = ok && file->attr_i32(g, "NUM_TS", NUM_TS);
+  ok = ok && file->attr_i32(g, "write_int", write_int);
+  ok = ok && file->attr_i32(g, "write_int_phase", write_int_phase);
+  ok = ok && file->attr_f64(g, "DT_coeff", DT);
+  ok = ok && file->attr_f64(g, "density", density);
+  ok = ok && file->attr_i32(g, "save_fig", save_fig);
+  ok = ok && file->attr_i32(g, "norm_scheme", normscheme);
+  ok = ok && file->attr_i32(g, "sub_cycle_interval", subcycleint);
+  ok = ok && file->attr_f64(g, "LDe", LDe);
+  ok = ok && file->attr_f64(g, "LDi", LDi);
+  ok = ok && file->attr_f64(g, "wpe", wpe);
+  ok = ok && file->attr_f64(g, "wpi", wpi);
+  ok = ok && file->attr_i32(g, "spno", spno);
+  ok = ok && file->attr_f64(g, "GAS_DENSITY", GAS_DENSITY);
+  if (!ok) throw std::runtime_error("result.h5: metadata written twice");
+}
+
+void Output::write_species_metadata(std::vector<Species>& species_list) {
+  for (Species& sp : species_list) {
+    const int g = file->group(metadata_species, sp.name);
+    need(g, "group metadata_species/" + sp.name);
+    // output.cpp:239: the name is a PredType::C_S1 attribute, a one-byte
+    // string, so it holds the name's first character
+    bool ok = file->attr_str(g, "name", sp.name, 1);
+    ok = ok && file->attr_f64(g, "mass", sp.mass);
+    ok = ok && file->attr_f64(g, "charge", sp.charge);
+    ok = ok && file->attr_f64(g, "spwt", sp.spwt);
+    ok = ok && file->attr_f64(g, "temperature", sp.temp);
+    ok = ok && file->attr_f64(g, "density", sp.fract_den);
+    ok = ok && file->attr_i32(g, "num_particles", sp.numparticle);
+    ok = ok && file->attr_f64(g, "streaming_velocity", sp.vsx);
+    if (!ok) throw std::runtime_error("result.h5: species metadata written twice");
+  }
+}
+
+void Output::write_ke() {
+  const uint64_t cols = (uint64_t)(3 * sp_no + 2);
+  need(file->dataset_f64(time_group, "kinetic_energy", {(uint64_t)t_step, cols}, store_ke.data()),
+       "time_var/kinetic_energy");
+}
+
+void Output::storeKE_to_matrix(int ts, std::vector<Species>& species_list) {
+  // output.cpp:294: the normalising species
+  Species& norm_species = (domain.SolverType == "hybrid" || domain.SolverType == "gshybrid")
+                              ? species_list[0]
+                              : ((domain.normscheme == 2 || domain.normscheme == 4) ? species_list[1] : species_list[0]);
+  const int k = ts / domain.write_interval;
+  if (k < 0 || k >= t_step) throw std::runtime_error("storeKE_to_matrix: step outside the history");
+  const size_t cols = (size_t)(3 * sp_no + 2);
+  double* row = store_ke.data() + (size_t)k * cols;
+  row[0] = ts * domain.DT;
+  size_t j = 1;
+  for (Species& sp : species_list) {
+    const std::array<double, 3> ke = sp.Compute_KE(norm_species);
+    row[j] = ke[0];
+    row[j + 1] = ke[1];
+    row[j + 2] = ke[2];
+    j += 3;
+  }
+  row[j] = domain.Compute_PE(norm_species);
 }
 
 }  // namespace epic2d

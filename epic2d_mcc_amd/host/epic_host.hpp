@@ -6,13 +6,16 @@
 #pragma once
 
 #include <array>
+#include <filesystem>
 #include <map>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "epic2d.h"
+#include "h5lite.hpp"
 
 namespace epic2d {
 
@@ -157,6 +160,9 @@ class Species {
   std::array<double, 3> Compute_Momentum(Species& species);                  // species.cpp:273-297
   int64_t count();
   std::vector<double> den();
+  // The live particles in storage order as interleaved (x, y) and (vx, vy)
+  // pairs: Particle::pos[0..1], vel[0..1] of part_list (species.h:13-20).
+  void particles(std::vector<double>& pos, std::vector<double>& vel);
   Domain& domain() { return *domain_; }
 
  private:
@@ -286,6 +292,48 @@ class Emitter {
   EmitterParams params;
   Domain& domain;
   bool first_ = false;
+};
+
+// Output (output.h:37-101): result.h5 in the reference's layout, written by
+// the self-contained HDF5 writer of h5lite.hpp.
+//   /metadata                       scalar attributes (output.cpp:202-225)
+//   /metadata_species/<name>        scalar attributes (output.cpp:228-249)
+//   /particle_<name>/pos<ts>, vel<ts>   N x 2 doubles (output.cpp:33-78)
+//   /fielddata/{pot,efieldx,efieldy}/<ts>, /fielddata/den_<name>/<ts>
+//       nx x ny doubles holding the field transposed: element j*nx + i of
+//       the row-major data is node (i, j) (output.cpp:81-199)
+//   /time_var/kinetic_energy        (NUM_TS/write_interval + 1) x (3 nsp + 2):
+//       t, KE (x, y, z) per species, PE (output.cpp:253-315)
+// The file is complete when the Output is destroyed (or close() returns).
+class Output {
+ public:
+  Output(const std::filesystem::path& outputfolder, Domain& domain);  // output.cpp:4-31
+  ~Output();
+  Output(const Output&) = delete;
+  Output& operator=(const Output&) = delete;
+
+  void write_particle_data(int ts, Species& species);  // output.cpp:33-78
+  void write_field_data(int ts);                       // output.cpp:81-147
+  void write_den_data(int ts, Species& species);       // output.cpp:150-199
+  void write_metadata(int NC, int NUM_TS, int write_int, int write_int_phase, double DT, double density, int save_fig,
+                      int normscheme, int subcycleint, double LDe, double LDi, double wpe, double wpi, int spno,
+                      double GAS_DENSITY);             // output.cpp:202-225
+  void write_species_metadata(std::vector<Species>& species_list);     // output.cpp:228-249
+  void write_ke();                                                     // output.cpp:253-288
+  void storeKE_to_matrix(int ts, std::vector<Species>& species_list);  // output.cpp:291-315
+  // Writes the object headers and the superblock; throws on an I/O error.
+  void close();
+
+  int precision = 6;
+  std::vector<double> store_ke;  // t_step x (3 sp_no + 2), row-major
+  int sp_no = 0, t_step = 0;
+
+ private:
+  int field_group(const std::string& name);
+  std::filesystem::path outputfolder;
+  Domain& domain;
+  std::unique_ptr<H5Lite> file;
+  int field_data_group = -1, time_group = -1, metadata_group = -1, metadata_species = -1;
 };
 
 // Analytic e-Ar cross sections of the reference (cross_section.cpp:4-39).

@@ -3,10 +3,11 @@
 // builds Domain / Species / Emitters, loads particles with the reference's
 // seeded loader, and runs NUM_TS+1 steps of the reference's loop
 // (main.cpp:395-455).  Diagnostics go to stdout every write_diagnostics steps
-// (output.cpp:318-392 without plots) and the kinetic/potential energy
-// history of storeKE_to_matrix (output.cpp:291-315) to
-// <output>/kinetic_energy.csv.  HDF5 field and particle dumps and live
-// plotting are not part of this build.
+// (output.cpp:318-392 without plots); <output>/result.h5 gets the
+// reference's metadata, particle, field and density dumps and the energy
+// history (main.cpp:268-277, 430-457; output.cpp:4-315), and the energy
+// history also goes to <output>/kinetic_energy.csv.  Live plotting is not
+// part of this build.
 //
 // An optional [device] section selects device options:
 //   precision = fp64 | fp32, sort_interval = k, tol_relative = 0|1,
@@ -57,6 +58,8 @@ int main(int argc, char* argv[]) {
     const int write_interval = INIParser::getInt(ini["diagnostics"], "write_interval");
     const int write_interval_phase = INIParser::getInt(ini["diagnostics"], "write_interval_phase");
     const int write_diagnostics = INIParser::getInt(ini["diagnostics"], "write_diagnostics");
+    const int write_flag = INIParser::getInt(ini["diagnostics"], "write_flag");
+    const int save_fig = INIParser::getInt(ini["diagnostics"], "save_fig");
     const int sub_cycle_interval = INIParser::getInt(ini["diagnostics"], "sub_cycle_interval");
     const int norm_scheme = INIParser::getInt(ini["normalization"], "norm_scheme");
     const double L_scale = INIParser::getDouble(ini["normalization"], "lenght_scale");
@@ -201,6 +204,13 @@ int main(int argc, char* argv[]) {
       emitters.emplace_back(ep, domain);
     }
 
+    // main.cpp:268-277 (the output folder is cleared and recreated)
+    Output output(outputfolder, domain);
+    output.precision = precision;
+    output.write_metadata(nx, NUM_TS, write_interval, write_interval_phase, DT_coeff, den, save_fig,
+                          domain.normscheme, domain.sub_cycle_interval, LDe, LDi, wpe, wpi, species_no, 0);
+    output.write_species_metadata(species_list);
+
     const auto start_time = std::chrono::high_resolution_clock::now();
     for (Species& sp : species_list) Init init(sp, domain, grids);
     FieldSolve fieldsolver(domain, grids);
@@ -213,9 +223,10 @@ int main(int argc, char* argv[]) {
     testcoll.set_electron_cross_sections();
     testcoll.calc_total_cross_sections();
 
-    // output.cpp:291-315: norm species sp[1] for norm schemes 2 and 4
-    Species& norm_species = (norm_scheme == 2 || norm_scheme == 4) ? species_list[1] : species_list[0];
-    std::filesystem::create_directories(outputfolder);
+    // output.cpp:323-324: the normalising species
+    Species& norm_species = (SolverType == "hybrid" || SolverType == "gshybrid")
+                                ? species_list[0]
+                                : ((norm_scheme == 2 || norm_scheme == 4) ? species_list[1] : species_list[0]);
     std::ofstream ke_out(std::filesystem::path(outputfolder) / "kinetic_energy.csv");
     ke_out << std::setprecision(17);
     for (int ts = 0; ts < NUM_TS + 1; ts++) {
@@ -252,15 +263,24 @@ int main(int argc, char* argv[]) {
         if (bc == "pbc") std::cout << " p_x: " << p[0] << " p_y: " << p[1] << " p_z: " << p[2];
         std::cout << std::defaultfloat << std::endl;
       }
+      // main.cpp:432-453
+      if (ts % write_interval_phase == 0 && ts % write_interval == 0)
+        for (Species& sp : species_list) output.write_particle_data(ts, sp);
       if (ts % write_interval == 0) {
-        ke_out << ts * domain.DT;
-        for (Species& sp : species_list) {
-          auto ke = sp.Compute_KE(norm_species);
-          ke_out << "," << ke[0] << "," << ke[1] << "," << ke[2];
+        output.storeKE_to_matrix(ts, species_list);
+        const size_t cols = 3 * species_list.size() + 2;
+        const double* row = output.store_ke.data() + (size_t)(ts / write_interval) * cols;
+        for (size_t c = 0; c < cols; c++) ke_out << (c ? "," : "") << row[c];
+        ke_out << "\n";
+        if (write_flag == 1 || write_flag == 2) {
+          output.write_field_data(ts);
+          output.write_den_data(ts, species_list[0]);
+          output.write_den_data(ts, species_list[1]);
         }
-        ke_out << "," << domain.Compute_PE(norm_species) << "\n";
       }
     }
+    output.write_ke();  // main.cpp:457
+    output.close();
     const auto end_time = std::chrono::high_resolution_clock::now();
     const double el = std::chrono::duration<double>(end_time - start_time).count();
     std::cout << "Elapsed time: " << el << " seconds" << std::endl;

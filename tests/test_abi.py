@@ -112,3 +112,12 @@ def test_host_library_exports_serving_entry_points():
     assert len(names) == 4
     H = _lib.host_lib()
     assert [n for n in names if not hasattr(H, n)] == []
+
+
+def test_host_library_exports_h5_writer_entry_points():
+    import re
+    text = open(os.path.join(ROOT, "include", "epic2d_h5.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|uint32_t)\s+(epic_\w+)\s*\(", text, re.M)))
+    assert len(names) == 8
+    H = _lib.host_lib()
+    assert [n for n in names if not hasattr(H, n)] == []

@@ -1,7 +1,9 @@
 """The ePIC++ driver (epic2d_run: reference main loop over the C++ host
 classes) on an input.ini in the reference's format: its kinetic/potential
 energy history matches the oracle's replay of the same loop from the same
-mt19937(0) load within 1e-12 relative."""
+mt19937(0) load within 1e-12 relative, and its result.h5 (read back by the
+HDF5 library's h5dump) holds the reference's layout with the oracle's
+fields, densities and particles."""
 import os
 import subprocess
 
@@ -146,3 +148,77 @@ def test_driver_with_conductors_matches_oracle(tmp_path):
             row += list(o.ke(s, 0))
         row.append(o.pe(0))
         assert np.allclose(hist[ts], np.array(row), rtol=1e-12, atol=1e-300), (ts, hist[ts], row)
+
+
+def test_driver_result_h5_matches_oracle(tmp_path):
+    """write_flag = 1, dumps every 5 steps: result.h5 in the layout of
+    output.cpp:4-315 (main.cpp:268-277, 430-457), every value against the
+    oracle's replay: phi, E and the densities of species 0 and 1 stored
+    transposed (element j*nx + i is node (i, j)), the particles as N x 2
+    (x, y) and (vx, vy) in storage order, the energy history, the
+    metadata."""
+    import h5dump
+    if not h5dump.available():
+        pytest.skip("no h5dump in this image")
+    out = tmp_path / "data"
+    ini = tmp_path / "input.ini"
+    text = INI.format(out=str(out)).replace("write_interval = 1\n", "write_interval = 5\n")
+    text = text.replace("write_interval_phase = 100", "write_interval_phase = 5").replace("write_flag = 0", "write_flag = 1")
+    ini.write_text(text)
+    print("[driver] result.h5, 16x16 periodic, dumps at 0, 5, 10", flush=True)
+    r = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    h5 = out / "result.h5"
+    tree = h5dump.contents(h5)
+    want = {"/fielddata/%s/%d" % (f, t) for f in ("pot", "efieldx", "efieldy", "den_electron", "den_ion")
+            for t in (0, 5, 10)}
+    want |= {"/particle_%s/%s%d" % (n, k, t) for n in ("electron", "ion") for k in ("pos", "vel") for t in (0, 5, 10)}
+    want.add("/time_var/kinetic_energy")
+    assert {k for k, v in tree.items() if v == "dataset"} == want
+    st = small_setup(N=2000, Nx=16, Ny=16, tol=1e-10, ion_mass=1.6724823e-27)
+    d = st.domain
+    o = OracleSim(d, st.species)
+    for s in range(2):
+        o.init_species(s, 0)
+    o.initial_field()
+    rows = []
+    for ts in range(11):
+        o.step()
+        if ts % 5:
+            continue
+        row = [ts * d.DT]
+        for s in range(2):
+            row += list(o.ke(s, 0))
+        row.append(o.pe(0))
+        rows.append(row)
+        for name, which in (("pot", 1), ("efieldx", 2), ("efieldy", 3), ("den_electron", 10), ("den_ion", 11)):
+            got, _ = h5dump.dataset(h5, f"/fielddata/{name}/{ts}")
+            assert got.shape == (d.nx, d.ny)
+            field = got.ravel().reshape(d.ny, d.nx).T  # element j*nx + i is node (i, j)
+            ref = o.grid(which)
+            assert np.max(np.abs(field - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1e-300), (name, ts)
+        for s, name in enumerate(("electron", "ion")):
+            x, y, vx, vy, _ = o.particles(s)
+            pos, _ = h5dump.dataset(h5, f"/particle_{name}/pos{ts}")
+            vel, _ = h5dump.dataset(h5, f"/particle_{name}/vel{ts}")
+            assert pos.shape == (len(x), 2) and vel.shape == (len(x), 2)
+            assert np.allclose(pos, np.stack([x, y], 1), rtol=1e-12, atol=1e-12), (name, ts)
+            assert np.allclose(vel, np.stack([vx, vy], 1), rtol=1e-12, atol=1e-12 * np.max(np.abs(vx))), (name, ts)
+    ke, _ = h5dump.dataset(h5, "/time_var/kinetic_energy")
+    assert ke.shape == (3, 8)
+    assert np.allclose(ke, np.array(rows), rtol=1e-12, atol=1e-300)
+    csv = np.loadtxt(out / "kinetic_energy.csv", delimiter=",")
+    assert np.array_equal(ke, csv)
+    meta = {"NC": 17, "NUM_TS": 10, "write_int": 5, "write_int_phase": 5, "DT_coeff": 0.05, "density": 1e13,
+            "save_fig": 0, "norm_scheme": 1, "sub_cycle_interval": 1, "spno": 2, "GAS_DENSITY": 0.0}
+    for k, v in meta.items():
+        assert h5dump.attribute(h5, f"/metadata/{k}")[0] == v, k
+    assert abs(h5dump.attribute(h5, "/metadata/LDe")[0] - d.LDe) <= 1e-15 * d.LDe
+    for s, name in enumerate(("electron", "ion")):
+        sp = st.species[s]
+        assert h5dump.attribute(h5, f"/metadata_species/{name}/name")[0] == name[0]
+        assert h5dump.attribute(h5, f"/metadata_species/{name}/num_particles")[0] == 2000
+        assert h5dump.attribute(h5, f"/metadata_species/{name}/mass")[0] == sp.mass
+        spwt = h5dump.attribute(h5, f"/metadata_species/{name}/spwt")[0]
+        assert abs(spwt - sp.spwt) <= 1e-14 * sp.spwt

@@ -136,6 +136,43 @@ def test_loader_positions_are_philox():
         assert abs(np.std(comp) - 1.0) < 0.05 and abs(np.mean(comp)) < 0.05
 
 
+def test_sort_after_motion_equals_cpu_stable_sort():
+    """The sort's working state: a sorted four-species plasma (128x128 cells,
+    20 per cell per species, open walls) after 20 fused steps without a
+    sort.  About one particle in ten has changed cell, so each wavefront of
+    the slot reservation holds several runs of the same cell; the result
+    must still be the stable sort by key, element for element, with the
+    absorbed particles dropped."""
+    from epic2d_mcc_amd import workloads
+    print("[storage] sort after 20 steps of motion, 128^2 x 4 species", flush=True)
+    st = workloads.bounded_512(ppc=20, cells=128, sort_interval=0)
+    d = st.domain
+    ctx = _lib.Context(d, st.species)
+    for s, sp in enumerate(st.species):
+        ctx.load(s, sp.numparticle)
+        ctx.sort(s)
+    ctx.initial_field()
+    for ts in range(20):
+        ctx.step(ts)
+    moved = []
+    for s in range(len(st.species)):
+        x, y, vx, vy, vz, ids = ctx.download(s)
+        live = ~np.isnan(x)
+        key = _cell_keys(x[live], y[live], d)  # dx = dy = 1, x0 = y0 = 0: exact in FP32 too
+        order = np.nonzero(live)[0][np.argsort(key, kind="stable")]
+        moved.append(float(np.mean(np.diff(key) < 0)))
+        ctx.sort(s)
+        g = ctx.download(s)
+        assert len(g[0]) == len(order), (s, len(g[0]), len(order))
+        assert np.array_equal(g[5], ids[order]), s
+        assert np.array_equal(g[0], x[order]) and np.array_equal(g[3], vy[order]), s
+    print("  fraction of descents in the key sequence before the sort: "
+          + ", ".join(f"{m:.3f}" for m in moved), flush=True)
+    assert max(moved) > 0.01  # the electrons did move
+    assert not ctx.errors().any(), ctx.errors()
+    ctx.close()
+
+
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.FP32])
 def test_sort_keeps_particles_on_the_upper_boundary(precision):
     """Live positions within an ulp of the upper walls (whose logical
