@@ -203,9 +203,13 @@ def cpu_baseline_entry(st, args, gas=None, note="", dense_ref=False):
     cb = cpu_baseline(st, sample_frac=args.cpu_sample, steps=args.cpu_steps, gas=gas, dense_ref=dense_ref)
     solver = ("the reference's own dense pcg() (oracle/_ref, its Matrix x vec on %d hardware threads)"
               % (os.cpu_count() or 1) if cb["dense_ref"] else "the matrix-free pcg() restatement")
-    out = {"value": cb["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+    # the reference's dense Matrix x vec runs on hardware_concurrency() threads
+    # (linalg.h:397-424); everything else here is one thread
+    cores = (os.cpu_count() or 1) if cb["dense_ref"] else 1
+    out = {"value": cb["value"], "unit": "particle-steps/s", "cores": cores, "kind": "port",
            "sample": "%d steps of the same workload by the FP64 oracle (restatement of the reference loop) on one "
-                     "thread: %d of the %d particles (particle stages scaled x%.2f to the full count), %.2f s "
+                     "thread (the Poisson matrix products excepted when it is the reference's dense pcg()): "
+                     "%d of the %d particles (particle stages scaled x%.2f to the full count), %.2f s "
                      "particle + %.2f s field stages per step, Poisson by %s (%s iterations)%s; %.1f s of CPU "
                      "work" % (cb["steps"], cb["nsamp"], cb["npart"], cb["npart"] / cb["nsamp"], cb["t_particles"],
                                cb["t_fields"], solver, cb["iters"], note, cb["t"]),

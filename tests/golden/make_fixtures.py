@@ -2,7 +2,7 @@
 
     python tests/golden/make_fixtures.py
 
-Sources:
+Two kinds of file.  Golden vectors computed by the reference's own code:
   ref_pcg_9x7.npz, ref_pcg_12x12.npz
       the reference's own dense pcg() (linearalgebra/solvers.h:72-100, compiled
       from the reference tree into oracle/_ref by oracle/Makefile) on seeded
@@ -11,15 +11,21 @@ Sources:
   ref_nrpcg_11x9.npz
       the Newton-CG loop of nrpcgsolver (field.cpp:141-210) on the reference's
       own dense Matrix / vec / cg() (oracle/_ref)
+Oracle regression vectors, written by the oracle (they pin nothing about
+the reference on their own; they freeze the oracle's output, which its pin
+tests check independently, so that a later change to it shows):
   mt19937_uniform_1000.npz
       the first 1000 draws of the reference's global stream (mt19937 seeded 0,
       uniform_real_distribution<double>(0, 1), domain.h:218-231), from the
       oracle; pinned by the C++ standard's 10000th-output value
   cross_sections_64.npz
       the e-Ar cross sections of cross_section.cpp:4-39 at 64 log-spaced
-      energies, from the oracle's restatement
+      energies, from the oracle's restatement (the analytic fits
+      themselves are parity-unpinned: no reference fixture holds their
+      values and cross_section.cpp cannot be compiled here)
   loop_16x16_pbc_cg.npz
-      the reference loop (main.cpp:366-423) from the mt19937(0) load of a
+      the oracle's restatement of the reference loop (main.cpp:366-423) from
+      the mt19937(0) load of a
       16 x 16-cell periodic electron-ion plasma, 2 x 1000 particles, reference
       PCG at 1e-13 absolute: rho, phi, Ex, Ey after each of 10 steps, the
       particles after the last, and the PCG iteration counts, from the oracle

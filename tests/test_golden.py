@@ -1,7 +1,9 @@
 """The oracle against the committed fixtures of tests/golden/ (made by
-tests/golden/make_fixtures.py): the reference's own pcg() and Newton-CG loop
-on its dense templates, the reference's random stream, the cross sections and
-a 10-step run of the reference loop.  These hold without the reference tree."""
+tests/golden/make_fixtures.py): golden vectors of the reference's own pcg()
+and Newton-CG loop on its dense templates, and oracle regression vectors (the
+random stream, the cross sections, a 10-step run of the oracle's restatement
+of the reference loop) that freeze the oracle's output.  These hold without
+the reference tree."""
 import os
 
 import numpy as np
