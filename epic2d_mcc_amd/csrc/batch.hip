@@ -191,7 +191,7 @@ __global__ void bstep_kernel(int* steps, int n) {
 // compiler fetch the work item and its segment with scalar loads after the
 // kernel's own particle stores; without it they arrive in vector registers
 // (92 VGPRs for the 4-particle shape, 5 waves per SIMD, instead of 64 and 8).
-template <int PPT, int WAVES>
+template <int PPT, int WAVES, int BC>
 __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const PushF32* __restrict__ segs,
                                                                         const int2* __restrict__ chunks, int nchunks) {
   __shared__ unsigned long long lds[(PART_BLOCK / 64) * WAVE_WIN];
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const Pus
   for (; c < nchunks; c += gridDim.x) {
     ChunkDep<PPT> dp;
     const bool active = wb < n;
-    if (active) push_chunk_regs<true, PPT>(a, wb, n, lane, r, dp);
+    if (active) push_chunk_regs<true, PPT, false, BC>(a, wb, n, lane, r, dp);
     const int cn = c + (int)gridDim.x;
     PushF32 an = a;
     int64_t nn = 0, wbn = 0;
@@ -369,10 +369,10 @@ struct PushShape {
 static const PushShape kPushShapes[EPIC_BATCH_NPUSH] = {{4, 8}, {8, 1}, {16, 1}, {4, 1}, {8, 5}};
 static const int kNodeBlocks[EPIC_BATCH_NNODE] = {64, 128, 256, 512};
 
-template <int PPT, int W>
+template <int PPT, int W, int BC>
 static hipError_t launch_push_batch_t(hipStream_t st, int num_cu, const PushF32* segs, const int2* chunks,
                                       int nchunks) {
-  auto k = push_batch_kernel<PPT, W>;
+  auto k = push_batch_kernel<PPT, W, BC>;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PART_BLOCK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
   int grid = num_cu * per_cu;
@@ -382,15 +382,29 @@ static hipError_t launch_push_batch_t(hipStream_t st, int num_cu, const PushF32*
   return hipGetLastError();
 }
 
-static hipError_t launch_push_batch(int variant, hipStream_t st, int num_cu, const PushF32* segs, const int2* chunks,
-                                    int nchunks) {
-  if (nchunks <= 0) return hipSuccess;
+template <int BC>
+static hipError_t launch_push_batch_bc(int variant, hipStream_t st, int num_cu, const PushF32* segs,
+                                       const int2* chunks, int nchunks) {
   switch (variant) {
-    case 1: return launch_push_batch_t<8, 1>(st, num_cu, segs, chunks, nchunks);
-    case 2: return launch_push_batch_t<16, 1>(st, num_cu, segs, chunks, nchunks);
-    case 3: return launch_push_batch_t<4, 1>(st, num_cu, segs, chunks, nchunks);
-    case 4: return launch_push_batch_t<8, 5>(st, num_cu, segs, chunks, nchunks);
-    default: return launch_push_batch_t<4, 8>(st, num_cu, segs, chunks, nchunks);
+    case 1: return launch_push_batch_t<8, 1, BC>(st, num_cu, segs, chunks, nchunks);
+    case 2: return launch_push_batch_t<16, 1, BC>(st, num_cu, segs, chunks, nchunks);
+    case 3: return launch_push_batch_t<4, 1, BC>(st, num_cu, segs, chunks, nchunks);
+    case 4: return launch_push_batch_t<8, 5, BC>(st, num_cu, segs, chunks, nchunks);
+    default: return launch_push_batch_t<4, 8, BC>(st, num_cu, segs, chunks, nchunks);
+  }
+}
+
+// bc: the wall kind every member shares, or BC_ANY when they differ (the
+// kernel then reads it per chunk)
+static hipError_t launch_push_batch(int variant, int bc, hipStream_t st, int num_cu, const PushF32* segs,
+                                    const int2* chunks, int nchunks) {
+  if (nchunks <= 0) return hipSuccess;
+  switch (bc) {
+    case EPIC_BC_PBC: return launch_push_batch_bc<EPIC_BC_PBC>(variant, st, num_cu, segs, chunks, nchunks);
+    case EPIC_BC_OPEN: return launch_push_batch_bc<EPIC_BC_OPEN>(variant, st, num_cu, segs, chunks, nchunks);
+    case EPIC_BC_REFLECTIVE:
+      return launch_push_batch_bc<EPIC_BC_REFLECTIVE>(variant, st, num_cu, segs, chunks, nchunks);
+    default: return launch_push_batch_bc<BC_ANY>(variant, st, num_cu, segs, chunks, nchunks);
   }
 }
 
@@ -444,6 +458,7 @@ struct Batch {
   int device = 0, num_cu = 0;
   epic_batch_opts opts{};
   int push_variant = 0;
+  int push_bc = BC_ANY;  // the members' common wall kind (BC_ANY: mixed)
   int node_block = 256;
   std::vector<Member> members;  // slot order
   int next_id = 0;
@@ -587,10 +602,12 @@ static hipError_t build_descriptors(Batch* B, bool full) {
   }
   B->h_mcc.clear();
   B->h_app.clear();
+  B->push_bc = B->members.empty() ? BC_ANY : member_ctx(B->members[0])->dom.bc;
   for (size_t m = 0; m < B->members.size(); m++) {
     Ctx* c = member_ctx(B->members[m]);
     const epic_domain_desc& d = c->dom;
     const Group& g = B->groups[grp[m]];
+    if (d.bc != B->push_bc) B->push_bc = BC_ANY;
     BSim S{};
     S.rho = c->rho; S.phi = c->phi; S.efx = c->efx; S.efy = c->efy; S.b = c->b64;
     S.E2 = (float2*)c->E2;
@@ -773,7 +790,7 @@ static hipError_t enqueue_step(Batch* B, bool events) {
     EPIC_HIP_TRY(hipGetLastError());
   }
   if (ev) rec(B, 2);
-  EPIC_HIP_TRY(launch_push_batch(B->push_variant, B->stream, B->num_cu, (const PushF32*)B->d_segs.p,
+  EPIC_HIP_TRY(launch_push_batch(B->push_variant, B->push_bc, B->stream, B->num_cu, (const PushF32*)B->d_segs.p,
                                  (const int2*)B->d_chunks.p, (int)B->h_chunks.size()));
   if (ev) rec(B, 3);
   EPIC_HIP_TRY(launch_mcc_batch(B->stream, B->num_cu, B->d_mcc.p, B->n_mcc, B->d_app.p, B->n_app,

@@ -625,7 +625,7 @@ __global__ __launch_bounds__(PART_BLOCK) void push_f64_kernel(PartArgs<double> a
 // rr = 0: each XCD walks its own contiguous range of chunks; rr = 1: all
 // workgroups walk the chunks round-robin (the whole device streams through
 // one region at a time).
-template <bool DEP, int PPT, int WAVES, bool COND = false>
+template <bool DEP, int PPT, int WAVES, bool COND = false, int BC = BC_ANY>
 __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, int rr, const CondDev* cond,
                                                                      int ncond, const uint32_t* cmask) {
   __shared__ unsigned long long lds[DEP ? (PART_BLOCK / 64) * WAVE_WIN : 1];
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
   if (cb < ce_) load_chunk<PPT>(a, cb * CH + (int64_t)wid * CHW, n, lane, r);
   for (int64_t chunk = cb; chunk < ce_; chunk += cs) {
     ChunkDep<PPT> dp;
-    push_chunk_regs<DEP, PPT, COND>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp, cond, ncond, cmask);
+    push_chunk_regs<DEP, PPT, COND, BC>(a, chunk * CH + (int64_t)wid * CHW, n, lane, r, dp, cond, ncond, cmask);
     const int64_t nb = (chunk + cs) * CH + (int64_t)wid * CHW;
     const bool more = chunk + cs < ce_;
     const bool full = more && nb + 64 * PPT <= n;
@@ -681,7 +681,7 @@ struct PushF32Set {
   const uint32_t* cmask;  // cells where a conductor can be (one bit per cell)
 };
 
-template <int PPT, bool COND, int WAVES>
+template <int PPT, bool COND, int WAVES, int BC = BC_ANY>
 __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_multi_kernel(PushF32Set S) {
   __shared__ unsigned long long lds[(PART_BLOCK / 64) * WAVE_WIN];
   constexpr int CHW = 64 * PPT;
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_multi_kernel(PushF
   for (; g < tot; g += gridDim.x) {
     ChunkDep<PPT> dp;
     const bool active = wb < n;
-    if (active) push_chunk_regs<true, PPT, COND>(a, wb, n, lane, r, dp, S.cond, S.ncond, S.cmask);
+    if (active) push_chunk_regs<true, PPT, COND, BC>(a, wb, n, lane, r, dp, S.cond, S.ncond, S.cmask);
     const int gn = g + (int)gridDim.x;
     const bool more = gn < tot;
     int sn = s;
@@ -895,17 +895,29 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
     const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2);
     const CondDev* ct = c->cond.table;
     const int nct = c->cond.active();
-    if (cond) {
-      // with conductors: the default shape with the conductor test
-      if (deposit) {
-        auto k = push_f32_kernel<true, 4, 1, true>;
-        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, c->cond.cmask);
-      } else {
-        auto k = push_f32_kernel<false, 4, 1, true>;
-        hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, c->cond.cmask);
-      }
+    // the default shape, with the wall kind fixed at compile time
+#define EPIC_D(DEP, COND, BC)                                                                     \
+  do {                                                                                            \
+    auto k = push_f32_kernel<DEP, 4, 1, COND, BC>;                                                \
+    hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, a, c->push_map, ct, nct, \
+                       c->cond.cmask);                                                            \
+  } while (0)
+#define EPIC_DB(DEP, COND)                                                                        \
+  do {                                                                                            \
+    if (d.bc == EPIC_BC_OPEN) EPIC_D(DEP, COND, EPIC_BC_OPEN);                                    \
+    else if (d.bc == EPIC_BC_PBC) EPIC_D(DEP, COND, EPIC_BC_PBC);                                 \
+    else EPIC_D(DEP, COND, EPIC_BC_REFLECTIVE);                                                   \
+  } while (0)
+    if (cond || c->push_variant < 1 || c->push_variant > 4) {
+      // the default shape (with conductors always: its conductor test)
+      if (cond && deposit) EPIC_DB(true, true);
+      else if (cond) EPIC_DB(false, true);
+      else if (deposit) EPIC_DB(true, false);
+      else EPIC_DB(false, false);
       return hipGetLastError();
     }
+#undef EPIC_DB
+#undef EPIC_D
     // kernel shape: particles per thread x occupancy target (waves per SIMD);
     // EPIC_PUSH_VARIANT at context creation
     switch (c->push_variant) {
@@ -923,8 +935,6 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
       EPIC_V(2, 8, 1)
       EPIC_V(3, 4, 8)
       EPIC_V(4, 16, 1)
-      default:
-      EPIC_V(0, 4, 1)
 #undef EPIC_V
     }
     return hipGetLastError();
@@ -978,9 +988,21 @@ hipError_t launch_push_all(Ctx* c) {
   S.cond = c->cond.table;
   S.ncond = c->cond.active();
   S.cmask = c->cond.cmask;
-  // WAVES = 8 caps the registers at 64 (eight wavefronts per SIMD)
-  auto k = c->cond.any() ? (c->push_multi_waves == 8 ? push_f32_multi_kernel<4, true, 8> : push_f32_multi_kernel<4, true, 1>)
-                         : (c->push_multi_waves == 8 ? push_f32_multi_kernel<4, false, 8> : push_f32_multi_kernel<4, false, 1>);
+  // WAVES = 8 caps the registers at 64 (eight wavefronts per SIMD); the
+  // wall kind is fixed at compile time (the compiler's choice of registers,
+  // WAVES = 1, keeps the per-chunk wall dispatch)
+  const bool cd = c->cond.any();
+  const int bc = c->dom.bc;
+  decltype(&push_f32_multi_kernel<4, false, 8>) k;
+  if (c->push_multi_waves != 8)
+    k = cd ? push_f32_multi_kernel<4, true, 1> : push_f32_multi_kernel<4, false, 1>;
+  else if (bc == EPIC_BC_OPEN)
+    k = cd ? push_f32_multi_kernel<4, true, 8, EPIC_BC_OPEN> : push_f32_multi_kernel<4, false, 8, EPIC_BC_OPEN>;
+  else if (bc == EPIC_BC_PBC)
+    k = cd ? push_f32_multi_kernel<4, true, 8, EPIC_BC_PBC> : push_f32_multi_kernel<4, false, 8, EPIC_BC_PBC>;
+  else
+    k = cd ? push_f32_multi_kernel<4, true, 8, EPIC_BC_REFLECTIVE>
+           : push_f32_multi_kernel<4, false, 8, EPIC_BC_REFLECTIVE>;
   hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, S);
   return hipGetLastError();
 }

@@ -180,12 +180,19 @@ struct ChunkDep {
   int imin, imax, jmin, jmax;
 };
 
+// Slots past the end of the array (a partial last chunk) load x = NaN, the
+// mark of an absent particle, so the push needs no separate range test.
 template <int PPT>
 __device__ __forceinline__ void load_chunk(const PushF32& a, int64_t wb, int64_t n, int lane, ChunkRegs<PPT>& r) {
   ld_run<float, PPT>(a.x, wb, lane, n, r.x);
   ld_run<float, PPT>(a.y, wb, lane, n, r.y);
   ld_run<float, PPT>(a.vx, wb, lane, n, r.vx);
   ld_run<float, PPT>(a.vy, wb, lane, n, r.vy);
+  if (wb + 64 * PPT > n) {
+#pragma unroll
+    for (int k = 0; k < PPT; k++)
+      if (part_index<float, PPT>(wb, lane, k) >= n) r.x[k] = __builtin_nanf("");
+  }
 }
 
 // A chunk known to be full (wb + 64*PPT <= n): the vector loads only.
@@ -204,128 +211,158 @@ __device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, in
     }
 }
 
-// Gather, push, walls and store of a loaded chunk; fills the deposit
-// information of the new positions (DEP).  COND: the walls are followed by
-// the conductor test against the first ncond shapes of cond, in the cells
-// that cmask flags (a separate
-// instantiation, so the kernels without conductors keep their code; the
-// table is an argument of its own, not part of PushF32, whose size the
-// batched kernel's descriptor loads depend on).
-template <bool DEP, int PPT, bool COND = false>
-__device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, int64_t n, int lane,
-                                                ChunkRegs<PPT>& r, ChunkDep<PPT>& dp,
-                                                const CondDev* cond = nullptr, int ncond = 0,
-                                                const uint32_t* cmask = nullptr) {
+// Lanes of the wavefront where c holds (a ballot and a scalar popcount).
+__device__ __forceinline__ unsigned wave_count(bool c) {
+  return (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(c));
+}
+
+// Gather, push, walls and store of a loaded chunk, for wall kind BC; fills
+// the deposit information of the new positions (DEP).  The per-particle
+// code is straight-line: the wall kind is resolved at compile time, the rare
+// cases (absent or absorbed particles, positions outside the grid) are
+// selects, and a particle's index values (cells, offsets) are used only
+// where its range test holds.  COND: the walls are followed by the conductor
+// test against the first ncond shapes of cond, in the cells that cmask flags
+// (a separate instantiation, so the kernels without conductors keep their
+// code; the table is an argument of its own, not part of PushF32, whose size
+// the batched kernel's descriptor loads depend on).
+template <int BC, bool DEP, int PPT, bool COND>
+__device__ __forceinline__ void push_chunk_bc(const PushF32& a, int64_t wb, int64_t n, int lane,
+                                              ChunkRegs<PPT>& r, ChunkDep<PPT>& dp, const CondDev* cond,
+                                              int ncond, const uint32_t* cmask) {
   constexpr int V = 4;
+  constexpr int G = PPT / V;
   const int nyc = a.ny - 1;
   const float fmax_i = (float)(a.nx - 1), fmax_j = (float)(a.ny - 1);
-  const int rem = (int)(n - wb < (int64_t)(64 * PPT) ? n - wb : (int64_t)(64 * PPT));
-  dp.imin = 1 << 30;
-  dp.imax = -1;
-  dp.jmin = 1 << 30;
-  dp.jmax = -1;
+  const int imx = a.nx - 2, jmx = a.ny - 2;
+  // reflecting walls: the highest FP32 position below each upper wall
+  const float xtop = BC == EPIC_BC_REFLECTIVE ? nextafterf(a.xhi, a.x0) : 0.f;
+  const float ytop = BC == EPIC_BC_REFLECTIVE ? nextafterf(a.yhi, a.y0) : 0.f;
+  const float qnan = __builtin_nanf("");
+  int imin = 1 << 30, imax = -1, jmin = 1 << 30, jmax = -1;
+  // error counts of the chunk, wavefront-uniform (ballots in scalar
+  // registers: a per-lane count would hold a vector register all along)
+  unsigned n_oob = 0, n_nan = 0;
   // the cell of each run's first particle: its coefficients are fetched for
   // all runs at once, so the fetches overlap
-  constexpr int G = PPT / V;
   int gkey[G];
   float4 gx[G], gy[G];
 #pragma unroll
   for (int g = 0; g < G; g++) {
     gkey[g] = -1;
     gx[g] = gy[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int off = g * (64 * V) + lane * V;
     const float lx = (r.x[g * V] - a.x0) * a.idx;
     const float ly = (r.y[g * V] - a.y0) * a.idy;
-    if (off < rem && lx >= 0.f && ly >= 0.f && lx <= fmax_i && ly <= fmax_j) {
-      gkey[g] = min((int)lx, a.nx - 2) * nyc + min((int)ly, a.ny - 2);
+    if (lx >= 0.f && ly >= 0.f && lx <= fmax_i && ly <= fmax_j) {
+      gkey[g] = min((int)lx, imx) * nyc + min((int)ly, jmx);
       gx[g] = a.Ec[2 * (size_t)gkey[g]];
       gy[g] = a.Ec[2 * (size_t)gkey[g] + 1];
     }
   }
-  int bad = 0;
 #pragma unroll
   for (int k = 0; k < PPT; k++) {
-    dp.cp[k] = NO_DEPOSIT;
-    dp.wp[k] = 0;
-    const int off = (k / V) * (64 * V) + lane * V + (k % V);
-    if (off >= rem || isnan(r.x[k])) continue;
-    const float lx = (r.x[k] - a.x0) * a.idx;
-    const float ly = (r.y[k] - a.y0) * a.idy;
-    float ex = 0.f, ey = 0.f;
-    if (lx >= 0.f && ly >= 0.f && lx <= fmax_i && ly <= fmax_j) {
-      const int i = min((int)lx, a.nx - 2), j = min((int)ly, a.ny - 2);
-      const int key = i * nyc + j;
-      float4& cx = gx[k / V];
-      float4& cy = gy[k / V];
-      if (key != gkey[k / V]) {
-        cx = a.Ec[2 * (size_t)key];
-        cy = a.Ec[2 * (size_t)key + 1];
-        gkey[k / V] = key;
-      }
-      const float di = lx - (float)i, dj = ly - (float)j;
-      ex = fmaf(dj, fmaf(di, cx.w, cx.z), fmaf(di, cx.y, cx.x));
-      ey = fmaf(dj, fmaf(di, cy.w, cy.z), fmaf(di, cy.y, cy.x));
-    } else {
-      bad++;
+    const float x = r.x[k], y = r.y[k], vx = r.vx[k], vy = r.vy[k];
+    // NaN x: an absorbed particle or a slot past the end.  It stays NaN (its
+    // other coordinates are not used again: every consumer skips NaN x)
+    const bool live = !isnan(x);
+    const float lx = (x - a.x0) * a.idx;
+    const float ly = (y - a.y0) * a.idy;
+    const bool inr = lx >= 0.f && ly >= 0.f && lx <= fmax_i && ly <= fmax_j;
+    n_oob += wave_count(live && !inr);
+    const int i = min((int)lx, imx), j = min((int)ly, jmx);
+    const int key = i * nyc + j;
+    float4& cx = gx[k / V];
+    float4& cy = gy[k / V];
+    if (inr && key != gkey[k / V]) {
+      cx = a.Ec[2 * (size_t)key];
+      cy = a.Ec[2 * (size_t)key + 1];
+      gkey[k / V] = key;
     }
-    r.vx[k] = fmaf(a.kv, ex, r.vx[k]);
-    r.vy[k] = fmaf(a.kv, ey, r.vy[k]);
-    float px = fmaf(a.kx, r.vx[k], r.x[k]);
-    float py = fmaf(a.kx, r.vy[k], r.y[k]);
-    bool alive = true;
-    if (isnan(px) || isnan(py)) {
-      bad += NAN_UNIT;
-      alive = false;
-      px = __builtin_nanf("");
-    } else if (a.bc == EPIC_BC_PBC) {
-      px = px < a.x0 ? px + a.Lx : (px >= a.xhi ? px - a.Lx : px);
-      py = py < a.y0 ? py + a.Ly : (py >= a.yhi ? py - a.Ly : py);
+    const float di = lx - (float)i, dj = ly - (float)j;
+    float ex = fmaf(dj, fmaf(di, cx.w, cx.z), fmaf(di, cx.y, cx.x));
+    float ey = fmaf(dj, fmaf(di, cy.w, cy.z), fmaf(di, cy.y, cy.x));
+    ex = inr ? ex : 0.f;
+    ey = inr ? ey : 0.f;
+    float nvx = fmaf(a.kv, ex, vx);
+    float nvy = fmaf(a.kv, ey, vy);
+    const float px = fmaf(a.kx, nvx, x);
+    const float py = fmaf(a.kx, nvy, y);
+    // a non-finite new position: counted, the particle is dropped
+    const bool nanp = live & (isnan(px) | isnan(py));
+    n_nan += wave_count(nanp);
+    float xo, yo;
+    bool alive;
+    if (BC == EPIC_BC_PBC) {
+      float wx = px < a.x0 ? px + a.Lx : (px >= a.xhi ? px - a.Lx : px);
+      float wy = py < a.y0 ? py + a.Ly : (py >= a.yhi ? py - a.Ly : py);
       // x0 - tiny + Lx can round up to xhi itself: the periodic image of x0
-      px = px >= a.xhi ? px - a.Lx : px;
-      py = py >= a.yhi ? py - a.Ly : py;
-    } else if (a.bc == EPIC_BC_OPEN) {
-      alive = !(px < a.x0 || px >= a.xhi || py < a.y0 || py >= a.yhi);
-      if (!alive) px = __builtin_nanf("");
+      wx = wx >= a.xhi ? wx - a.Lx : wx;
+      wy = wy >= a.yhi ? wy - a.Ly : wy;
+      alive = live && !nanp;
+      xo = nanp ? qnan : wx;
+      yo = nanp ? py : wy;
+    } else if (BC == EPIC_BC_OPEN) {
+      // false for a NaN coordinate
+      alive = live && px >= a.x0 && px < a.xhi && py >= a.y0 && py < a.yhi;
+      xo = alive ? px : qnan;
+      yo = py;
     } else {
-      if (px < a.x0) { px = a.x0 + (a.x0 - px); r.vx[k] = -r.vx[k]; }
-      else if (px >= a.xhi) { px = a.xhi - (px - a.xhi); r.vx[k] = -r.vx[k]; }
-      if (py < a.y0) { py = a.y0 + (a.y0 - py); r.vy[k] = -r.vy[k]; }
-      else if (py >= a.yhi) { py = a.yhi - (py - a.yhi); r.vy[k] = -r.vy[k]; }
+      const bool lox = px < a.x0, hix = px >= a.xhi;
+      const bool loy = py < a.y0, hiy = py >= a.yhi;
+      float rx = lox ? a.x0 + (a.x0 - px) : (hix ? a.xhi - (px - a.xhi) : px);
+      float ry = loy ? a.y0 + (a.y0 - py) : (hiy ? a.yhi - (py - a.yhi) : py);
       // a position exactly on the upper wall mirrors onto itself: keep live
       // positions below it
-      if (px >= a.xhi) px = nextafterf(a.xhi, a.x0);
-      if (py >= a.yhi) py = nextafterf(a.yhi, a.y0);
+      rx = rx >= a.xhi ? xtop : rx;
+      ry = ry >= a.yhi ? ytop : ry;
+      const bool flipx = !nanp && (lox || hix), flipy = !nanp && (loy || hiy);
+      nvx = flipx ? -nvx : nvx;
+      nvy = flipy ? -nvy : nvy;
+      alive = live && !nanp;
+      xo = nanp ? qnan : rx;
+      yo = nanp ? py : ry;
     }
     if (COND && alive) {
       // the cell's flag first (cmask: cells a shape's bounding box reaches,
       // widened by one), the exact FP64 test only in flagged cells
-      const int ci = min(max((int)((px - a.x0) * a.idx), 0), a.nx - 2);
-      const int cj = min(max((int)((py - a.y0) * a.idy), 0), a.ny - 2);
+      const int ci = min(max((int)((xo - a.x0) * a.idx), 0), a.nx - 2);
+      const int cj = min(max((int)((yo - a.y0) * a.idy), 0), a.ny - 2);
       const int cell = ci * nyc + cj;
-      if (((cmask[cell >> 5] >> (cell & 31)) & 1u) && in_conductor(cond, ncond, (double)px, (double)py)) {
+      if (((cmask[cell >> 5] >> (cell & 31)) & 1u) && in_conductor(cond, ncond, (double)xo, (double)yo)) {
         alive = false;
-        px = __builtin_nanf("");
+        xo = qnan;
       }
     }
-    r.x[k] = px;
-    r.y[k] = py;
-    if (DEP && alive) {
-      const float nlx = (px - a.x0) * a.idx, nly = (py - a.y0) * a.idy;
+    r.x[k] = xo;
+    r.y[k] = yo;
+    r.vx[k] = nvx;
+    r.vy[k] = nvy;
+    if (DEP) {
+      const float nlx = (xo - a.x0) * a.idx, nly = (yo - a.y0) * a.idy;
       // a wrapped position can round onto the upper boundary node: it
       // deposits into the last cell
-      if (nlx >= 0.f && nly >= 0.f && nlx <= fmax_i && nly <= fmax_j) {
-        const int ni = min((int)nlx, a.nx - 2), nj = min((int)nly, a.ny - 2);
-        const int dI = min((int)rintf((nlx - (float)ni) * 16384.0f), 16384);
-        const int dJ = min((int)rintf((nly - (float)nj) * 16384.0f), 16384);
-        dp.cp[k] = ((unsigned)ni << 16) | (unsigned)nj;
-        dp.wp[k] = ((unsigned)dI << 16) | (unsigned)dJ;
-        dp.imin = min(dp.imin, ni); dp.imax = max(dp.imax, ni);
-        dp.jmin = min(dp.jmin, nj); dp.jmax = max(dp.jmax, nj);
-      } else {
-        bad++;
-      }
+      const bool inr2 = nlx >= 0.f && nly >= 0.f && nlx <= fmax_i && nly <= fmax_j;
+      n_oob += wave_count(alive && !inr2);
+      const bool dep = alive && inr2;
+      const int ni = min((int)nlx, imx), nj = min((int)nly, jmx);
+      const int dI = min((int)rintf((nlx - (float)ni) * 16384.0f), 16384);
+      const int dJ = min((int)rintf((nly - (float)nj) * 16384.0f), 16384);
+      dp.cp[k] = dep ? (((unsigned)ni << 16) | (unsigned)nj) : NO_DEPOSIT;
+      dp.wp[k] = ((unsigned)dI << 16) | (unsigned)dJ;
+      imin = min(imin, dep ? ni : 1 << 30);
+      imax = max(imax, dep ? ni : -1);
+      jmin = min(jmin, dep ? nj : 1 << 30);
+      jmax = max(jmax, dep ? nj : -1);
     }
+    // one particle at a time: interleaving the particles' straight-line code
+    // (eight waves per SIMD hide the latency anyway) needs more registers
+    // than the 64 of that occupancy
+    __builtin_amdgcn_sched_barrier(0);
   }
+  dp.imin = imin;
+  dp.imax = imax;
+  dp.jmin = jmin;
+  dp.jmax = jmax;
   // Every gather fetch has been consumed here, so this wait costs nothing;
   // it tells the compiler that no fetch is outstanding, which otherwise
   // (its analysis is path-insensitive) drains all vector memory -- the next
@@ -336,8 +373,37 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
   st_run<float, PPT>(a.vy, wb, lane, n, r.vy);
   st_run<float, PPT>(a.x, wb, lane, n, r.x);
   st_run<float, PPT>(a.y, wb, lane, n, r.y);
-  if (bad) flush_push_errors(a.errc, bad);
+  // one lane adds the wavefront's counts
+  if ((n_oob | n_nan) && lane == __builtin_amdgcn_readfirstlane(lane)) {
+    if (n_oob) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)n_oob);
+    if (n_nan) atomicAdd(&a.errc[EPIC_ERRC_NAN], (unsigned long long)n_nan);
+  }
 }
+
+// Gather, push, walls and store of a loaded chunk.  BC >= 0: the wall kind
+// of every chunk the kernel sees (a.bc == BC), fixed at compile time; BC < 0:
+// read from a.bc, one branch per chunk (three copies of the particle code,
+// which need more registers: for the measurement shapes and batches whose
+// members mix wall kinds).
+template <bool DEP, int PPT, bool COND, int BC>
+__device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, int64_t n, int lane,
+                                                ChunkRegs<PPT>& r, ChunkDep<PPT>& dp,
+                                                const CondDev* cond = nullptr, int ncond = 0,
+                                                const uint32_t* cmask = nullptr) {
+  if constexpr (BC >= 0) {
+    push_chunk_bc<BC, DEP, PPT, COND>(a, wb, n, lane, r, dp, cond, ncond, cmask);
+  } else {
+    if (a.bc == EPIC_BC_OPEN)
+      push_chunk_bc<EPIC_BC_OPEN, DEP, PPT, COND>(a, wb, n, lane, r, dp, cond, ncond, cmask);
+    else if (a.bc == EPIC_BC_PBC)
+      push_chunk_bc<EPIC_BC_PBC, DEP, PPT, COND>(a, wb, n, lane, r, dp, cond, ncond, cmask);
+    else
+      push_chunk_bc<EPIC_BC_REFLECTIVE, DEP, PPT, COND>(a, wb, n, lane, r, dp, cond, ncond, cmask);
+  }
+}
+
+// Wall kind of a kernel instantiation: -1 reads it per chunk.
+constexpr int BC_ANY = -1;
 
 // Deposit of a pushed chunk through the wavefront's LDS window wl: each
 // lane sums runs of equal cells in registers, runs go to the window (or
@@ -393,28 +459,18 @@ __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsign
   }
   wave_fence();
   const int W = win.wi * win.wj;
+  // row of window slot k as floor((k + 1/2) / wj) in FP32: exact, since
+  // k < 512 and wj <= 512 keep (k + 1/2) / wj at least 1/1024 from an integer
+  const float rwj = 1.0f / (float)win.wj;
   for (int k = lane; k < W; k += 64) {
     const unsigned long long v = wl[k];
     if (v) {
-      const int gi = win.i0 + k / win.wj, gj = win.j0 + k % win.wj;
+      const int ri = (int)(((float)k + 0.5f) * rwj);
+      const int gi = win.i0 + ri, gj = win.j0 + (k - ri * win.wj);
       if (gi >= 0 && gj >= 0 && gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
     }
   }
   wave_fence();
-}
-
-// One wavefront's chunk of 64*PPT particles starting at wb (n = particles in
-// the array): push, walls, store, and (DEP) deposit of the new positions.
-// Shared by the single-simulation kernel and the batched one, so both do
-// identical arithmetic.
-template <bool DEP, int PPT>
-__device__ __forceinline__ void push_f32_chunk(const PushF32& a, int64_t wb, int64_t n, int lane,
-                                               unsigned long long* wl) {
-  ChunkRegs<PPT> r;
-  ChunkDep<PPT> dp;
-  load_chunk<PPT>(a, wb, n, lane, r);
-  push_chunk_regs<DEP, PPT>(a, wb, n, lane, r, dp);
-  if (DEP) deposit_chunk<PPT>(a, lane, wl, dp);
 }
 
 }  // namespace epic
