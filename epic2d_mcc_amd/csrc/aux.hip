@@ -34,10 +34,11 @@ __global__ void load_kernel(P* x, P* y, P* vx, P* vy, P* vz, uint32_t* id, int64
     const double m1 = sqrt(-2.0 * log(u1)), m2 = sqrt(-2.0 * log(u3));
     const double g1 = m1 * cos(2.0 * M_PI * u2), g2 = m1 * sin(2.0 * M_PI * u2);
     const double g3 = m2 * cos(2.0 * M_PI * u4);
-    x[p] = px;
-    y[p] = py;
-    vx[p] = (P)(sig * g1 + ux);
-    vy[p] = (P)(sig * g2 + uy);
+    const int64_t q = pslot<P>(p);
+    x[q] = px;
+    y[q] = py;
+    vx[q] = (P)(sig * g1 + ux);
+    vy[q] = (P)(sig * g2 + uy);
     vz[p] = (P)(vz_thermal ? sig * g3 : 0.0);
     id[p] = (uint32_t)p;
   }
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(AUX_BLOCK) void live_count_kernel(const P* x, const
   int c = 0;
   for (int k = 0; k < CMP_PPT; k++) {
     const int64_t i = base + k;
-    if (i < n && !isnan(x[i])) c++;
+    if (i < n && !isnan(x[pslot<P>(i)])) c++;
   }
   int tot;
   block_excl_scan(c, sh, &tot);
@@ -189,14 +190,16 @@ __global__ __launch_bounds__(AUX_BLOCK) void compact_scatter_kernel(const P* x, 
   int c = 0;
   for (int k = 0; k < CMP_PPT; k++) {
     const int64_t i = base + k;
-    if (i < n && !isnan(x[i])) c++;
+    if (i < n && !isnan(x[pslot<P>(i)])) c++;
   }
   int tot;
   int off = blk_off[blockIdx.x] + block_excl_scan(c, sh, &tot);
   for (int k = 0; k < CMP_PPT; k++) {
     const int64_t i = base + k;
-    if (i < n && !isnan(x[i])) {
-      x2[off] = x[i]; y2[off] = y[i]; vx2[off] = vx[i]; vy2[off] = vy[i]; vz2[off] = vz[i]; id2[off] = id[i];
+    const int64_t si = pslot<P>(i);
+    if (i < n && !isnan(x[si])) {
+      const int64_t so = pslot<P>(off);
+      x2[so] = x[si]; y2[so] = y[si]; vx2[so] = vx[si]; vy2[so] = vy[si]; vz2[off] = vz[i]; id2[off] = id[i];
       off++;
     }
   }
@@ -363,7 +366,8 @@ __global__ void inject_kernel(P* x, P* y, P* vx, P* vy, P* vz, uint32_t* id, int
     const double sy = u01_53(r2.c, r2.d) + u01_53(r3.a, r3.b) + u01_53(r3.c, r3.d) - 1.5;
     const double vxp = (vth_sqrt2 * sx + e.vdx * vel_norm) / vel_norm;
     const double vyp = (vth_sqrt2 * sy + e.vdy * vel_norm) / vel_norm;
-    x[slot] = (P)px; y[slot] = (P)py; vx[slot] = (P)vxp; vy[slot] = (P)vyp; vz[slot] = (P)0;
+    const int64_t q = pslot<P>(slot);
+    x[q] = (P)px; y[q] = (P)py; vx[q] = (P)vxp; vy[q] = (P)vyp; vz[slot] = (P)0;
     id[slot] = id0 + (uint32_t)t;
     if (Q) {
       const double lx = (px - x0) / dx, ly = (py - y0) / dy;

@@ -94,8 +94,12 @@ void spec_destroy(Ctx* c);
 
 static void free_all(Ctx* c) {
   spec_destroy(c);
+  const bool tiled = c->dom.precision == EPIC_FP32;
   for (auto& S : c->sp) {
-    void* ps[] = {S.x, S.y, S.vx, S.vy, S.vz, S.id, S.x2, S.y2, S.vx2, S.vy2, S.vz2, S.id2, S.Q, S.den, S.d_n};
+    // tiled storage: y, vx, vy point into x's allocation
+    void* ps[] = {S.x, tiled ? nullptr : S.y, tiled ? nullptr : S.vx, tiled ? nullptr : S.vy, S.vz, S.id,
+                  S.x2, tiled ? nullptr : S.y2, tiled ? nullptr : S.vx2, tiled ? nullptr : S.vy2, S.vz2, S.id2,
+                  S.Q, S.den, S.d_n};
     for (void* p : ps)
       if (p) hipFree(p);
   }
@@ -129,7 +133,7 @@ size_t ctx_bytes_estimate(const epic_domain_desc* dom, const epic_species_desc* 
   for (int s = 0; s < nsp; s++) {
     const int64_t cap = sp[s].capacity > 0 ? sp[s].capacity : std::max<int64_t>(sp[s].numparticle, 1);
     maxcap = std::max(maxcap, cap);
-    const size_t cap_pad = (size_t)((cap + 7) / 8 * 8);
+    const size_t cap_pad = (size_t)((cap + PTILE - 1) / PTILE * PTILE);
     tot += 10 * b(cap_pad * ps, 1) + 2 * b(cap_pad, 4) + 2 * b(nn, 8) + b(2, 8);
   }
   tot += 4 * b(nn, 8) + b(2 * nn * ps, 1);
@@ -253,11 +257,23 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
       return EPIC_ERR_CAPACITY;
     }
     maxcap = std::max(maxcap, S.cap);
-    // pad to whole 32-byte runs so vector loads of the last run stay inside
-    const size_t cap_pad = (size_t)((S.cap + 7) / 8 * 8);
-    char** arrs[] = {(char**)&S.x, (char**)&S.y, (char**)&S.vx, (char**)&S.vy, (char**)&S.vz,
-                     (char**)&S.x2, (char**)&S.y2, (char**)&S.vx2, (char**)&S.vy2, (char**)&S.vz2};
-    for (char** a : arrs) CKC(dalloc_c(c, a, cap_pad * ps), "particle arrays");
+    // pad to whole tiles (FP32) or 32-byte runs (FP64) so vector loads of
+    // the last run stay inside
+    const size_t cap_pad = (size_t)((S.cap + PTILE - 1) / PTILE * PTILE);
+    if (dom->precision == EPIC_FP32) {
+      // x, y, vx, vy in tiles of PTILE slots: one allocation per buffer
+      CKC(dalloc_c(c, (float**)&S.x, 4 * cap_pad), "particle arrays");
+      CKC(dalloc_c(c, (float**)&S.x2, 4 * cap_pad), "particle arrays");
+      float* t[2] = {(float*)S.x, (float*)S.x2};
+      S.y = t[0] + PTILE; S.vx = t[0] + 2 * PTILE; S.vy = t[0] + 3 * PTILE;
+      S.y2 = t[1] + PTILE; S.vx2 = t[1] + 2 * PTILE; S.vy2 = t[1] + 3 * PTILE;
+      CKC(dalloc_c(c, (char**)&S.vz, cap_pad * ps), "particle arrays");
+      CKC(dalloc_c(c, (char**)&S.vz2, cap_pad * ps), "particle arrays");
+    } else {
+      char** arrs[] = {(char**)&S.x, (char**)&S.y, (char**)&S.vx, (char**)&S.vy, (char**)&S.vz,
+                       (char**)&S.x2, (char**)&S.y2, (char**)&S.vx2, (char**)&S.vy2, (char**)&S.vz2};
+      for (char** a : arrs) CKC(dalloc_c(c, a, cap_pad * ps), "particle arrays");
+    }
     CKC(dalloc_c(c, &S.id, cap_pad), "ids");
     CKC(dalloc_c(c, &S.id2, cap_pad), "ids");
     CKC(dalloc_c(c, &S.Q, nn), "Q");
@@ -389,7 +405,9 @@ int epic_upload_particles(epic_ctx* h, int s, const double* x, const double* y, 
     // (its loader and push keep x < x0 + Lx): such positions are stored as
     // the largest float below the wall.
     const double hi[2] = {c->dom.x0 + (c->dom.nx - 1) * c->dom.dx, c->dom.y0 + (c->dom.ny - 1) * c->dom.dy};
-    std::vector<float> tmp(n);
+    // x, y, vx, vy into the tiled layout (whole tiles), vz as it is
+    const int64_t ntile = (n + PTILE - 1) / PTILE;
+    std::vector<float> tiles((size_t)ntile * 4 * PTILE, 0.f), tmp(n);
     for (int k = 0; k < 5; k++) {
       if (!src[k]) return fail(c, EPIC_ERR_ARG, "null array");
       for (int64_t i = 0; i < n; i++) tmp[i] = (float)src[k][i];
@@ -398,8 +416,13 @@ int epic_upload_particles(epic_ctx* h, int s, const double* x, const double* y, 
         for (int64_t i = 0; i < n; i++)
           if (src[k][i] < hi[k] && tmp[i] >= hf) tmp[i] = below;
       }
-      CK(hipMemcpy(dst[k], tmp.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
+      if (k < 4) {
+        for (int64_t i = 0; i < n; i++) tiles[ptile(i) + k * PTILE] = tmp[i];
+      } else {
+        CK(hipMemcpy(dst[k], tmp.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
+      }
     }
+    CK(hipMemcpy(S.x, tiles.data(), tiles.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
   }
   std::vector<uint32_t> ids(n);
   for (int64_t i = 0; i < n; i++) ids[i] = (uint32_t)i;
@@ -421,10 +444,17 @@ int epic_download_particles(epic_ctx* h, int s, double* x, double* y, double* vx
   const int64_t n = S.n_host;
   std::vector<double> a[5];
   void* srcs[5] = {S.x, S.y, S.vx, S.vy, S.vz};
+  std::vector<float> tiles;
+  if (c->dom.precision == EPIC_FP32) {
+    tiles.resize((size_t)((n + PTILE - 1) / PTILE) * 4 * PTILE);
+    CK(hipMemcpy(tiles.data(), S.x, tiles.size() * sizeof(float), hipMemcpyDeviceToHost), "download");
+  }
   for (int k = 0; k < 5; k++) {
     a[k].resize(n);
     if (c->dom.precision == EPIC_FP64) {
       CK(hipMemcpy(a[k].data(), srcs[k], n * sizeof(double), hipMemcpyDeviceToHost), "download");
+    } else if (k < 4) {
+      for (int64_t i = 0; i < n; i++) a[k][i] = tiles[ptile(i) + k * PTILE];
     } else {
       std::vector<float> t(n);
       CK(hipMemcpy(t.data(), srcs[k], n * sizeof(float), hipMemcpyDeviceToHost), "download");

@@ -18,8 +18,9 @@ __global__ __launch_bounds__(DIAG_BLOCK) void moments_kernel(const P* x, const P
   const int64_t n = *d_n;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-    if (isnan(x[k])) continue;
-    const double a = vx[k], b = vy[k], c = vz[k];
+    const int64_t q = pslot<P>(k);
+    if (isnan(x[q])) continue;
+    const double a = vx[q], b = vy[q], c = vz[k];
     acc[0] += a * a; acc[1] += b * b; acc[2] += c * c;
     acc[3] += a; acc[4] += b; acc[5] += c;
     acc[6] += 1.0;

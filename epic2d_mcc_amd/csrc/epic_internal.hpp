@@ -33,6 +33,19 @@ constexpr int DEP_WINDOW = 8192;
 constexpr int MCC_CHUNK = 2048;  // particle slots per MCC selection chunk
 constexpr int PART_BLOCK = 256;  // threads per workgroup in particle kernels
 
+// FP32 particle storage is tiled: the x, y, vx, vy of 256 consecutive slots
+// form one 4 KiB tile (x[256] y[256] vx[256] vy[256]), so that a wavefront's
+// chunk of the push is one contiguous block of memory instead of four
+// streams.  A species' y, vx, vy pointers are its x pointer + 256, + 512,
+// + 768, and slot i of any of the four lives at ptile(i); vz and id are plain
+// arrays.  FP64 storage is one array per coordinate (pslot<double> = i).
+constexpr int64_t PTILE = 256;
+__host__ __device__ __forceinline__ int64_t ptile(int64_t i) { return ((i >> 8) << 10) + (i & 255); }
+template <typename P>
+__host__ __device__ __forceinline__ int64_t pslot(int64_t i) {
+  return sizeof(P) == 4 ? ptile(i) : i;
+}
+
 struct PushF32;
 
 struct SpeciesDev {

@@ -99,9 +99,11 @@ __device__ __forceinline__ int mcc_candidate(const MccArgs<P>& a, uint32_t step,
   do {
     const uint32_t pid = a.id[p];
     const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
-    const P xp = a.x[p];
+    // x, y, vx, vy at slot q (tiled FP32 storage), vz and id at p
+    const int64_t q = pslot<P>(p);
+    const P xp = a.x[q];
     if (isnan(xp)) continue;
-    double vxe = a.vx[p], vye = a.vy[p], vze = a.vz[p];
+    double vxe = a.vx[q], vye = a.vy[q], vze = a.vz[p];
     const double R1 = u01_53(r.c, r.d);
     const u32x4 r2 = philox4x32_10({pid, step, RNG_MCC2, a.sp}, a.k0, a.k1);
     const u32x4 r3 = philox4x32_10({pid, step, RNG_MCC3, a.sp}, a.k0, a.k1);
@@ -166,7 +168,7 @@ __device__ __forceinline__ int mcc_candidate(const MccArgs<P>& a, uint32_t step,
           double* ev = a.events + (size_t)e * EVW;
           ev[0] = (double)p;
           ev[1] = (double)xp;
-          ev[2] = (double)a.y[p];
+          ev[2] = (double)a.y[q];
           ev[3] = wx + F2 * hx;
           ev[4] = wy + F2 * hy;
           ev[5] = wz + F2 * hz;
@@ -176,8 +178,8 @@ __device__ __forceinline__ int mcc_candidate(const MccArgs<P>& a, uint32_t step,
       }
       double ngx, ngy, ngz;
       rotate(g, chi, eta, st, ct, sp, cp, ngx, ngy, ngz);
-      a.vx[p] = (P)(wx + F2 * ngx);
-      a.vy[p] = (P)(wy + F2 * ngy);
+      a.vx[q] = (P)(wx + F2 * ngx);
+      a.vy[q] = (P)(wy + F2 * ngy);
       a.vz[p] = (P)(wz + F2 * ngz);
       cnt[type]++;
     } else {
@@ -208,13 +210,13 @@ __device__ __forceinline__ int mcc_candidate(const MccArgs<P>& a, uint32_t step,
         const double ct = 1.0 - 2.0 * u01_53(r4.a, r4.b), stt = sqrt(fmax(0.0, 1.0 - ct * ct));
         const double ph = 2.0 * M_PI * u01_53(r4.c, r4.d);
         const double f = M / (m + M);
-        a.vx[p] = (P)(cx + f * g * stt * cos(ph));
-        a.vy[p] = (P)(cy + f * g * stt * sin(ph));
+        a.vx[q] = (P)(cx + f * g * stt * cos(ph));
+        a.vy[q] = (P)(cy + f * g * stt * sin(ph));
         a.vz[p] = (P)(cz + f * g * ct);
         cnt[3]++;
       } else {
-        a.vx[p] = (P)vnx;
-        a.vy[p] = (P)vny;
+        a.vx[q] = (P)vnx;
+        a.vy[q] = (P)vny;
         a.vz[p] = (P)vnz;
         cnt[4]++;
       }
@@ -424,8 +426,9 @@ __device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t
       continue;
     }
     const double px = ev[1], py = ev[2];
-    ex[se] = (P)px; ey[se] = (P)py;
-    evx[se] = (P)ev[3]; evy[se] = (P)ev[4]; evz[se] = (P)ev[5];
+    const int64_t qe = pslot<P>(se), qi = pslot<P>(si);
+    ex[qe] = (P)px; ey[qe] = (P)py;
+    evx[qe] = (P)ev[3]; evy[qe] = (P)ev[4]; evz[se] = (P)ev[5];
     eid[se] = ide + (uint32_t)rank;
     // new ion: Maxwellian at the ion species temperature (normalised)
     const uint32_t pid = (uint32_t)ev[6];
@@ -434,8 +437,8 @@ __device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t
     double s1, s2;
     const double g0 = gauss(r.a, r.b, r.c, r.d, &s1);
     const double g2 = gauss(r2.a, r2.b, r2.c, r2.d, &s2);
-    ix[si] = (P)px; iy[si] = (P)py;
-    ivx[si] = (P)(ion_vth * g0); ivy[si] = (P)(ion_vth * s1); ivz[si] = (P)(ion_vth * g2);
+    ix[qi] = (P)px; iy[qi] = (P)py;
+    ivx[qi] = (P)(ion_vth * g0); ivy[qi] = (P)(ion_vth * s1); ivz[si] = (P)(ion_vth * g2);
     iid[si] = idi + (uint32_t)rank;
     if (Qe) {
       const double lx = (px - x0) / dx, ly = (py - y0) / dy;

@@ -156,7 +156,7 @@ __device__ __forceinline__ void ld_run_full(const P* __restrict__ a, int64_t wba
 #pragma unroll
   for (int g = 0; g < PPT / V; g++) {
     const v4u32 t = __builtin_nontemporal_load(
-        reinterpret_cast<const v4u32*>(a + wbase + (int64_t)g * 64 * V + (int64_t)lane * V));
+        reinterpret_cast<const v4u32*>(a + pslot<P>(wbase + (int64_t)g * 64 * V) + (int64_t)lane * V));
     __builtin_memcpy(&o[g * V], &t, 16);
   }
 }

@@ -21,8 +21,12 @@ struct alignas(2 * sizeof(P)) Vec2 {
 // at offset g*64*V.  Load g of a wavefront is then one contiguous 1 KiB
 // block (fully coalesced), and each lane owns runs of V consecutive
 // particles (what the gather cell reuse and the run-length deposit rely on).
+// This is the slot number; its address in an FP32 array is pslot<float>.
+// Chunks start at multiples of 64*PPT slots, so for FP32 (PPT a multiple of
+// 4) every group of 64*V = 256 slots is one tile's run of one coordinate.
 template <typename P, int PPT>
 __device__ __forceinline__ int64_t part_index(int64_t wbase, int lane, int k) {
+  static_assert(sizeof(P) == 8 || PPT % 4 == 0, "FP32 chunks are whole tiles");
   constexpr int V = 16 / (int)sizeof(P);
   return wbase + (int64_t)(k / V) * (64 * V) + (int64_t)lane * V + (k % V);
 }
@@ -34,7 +38,8 @@ __device__ __forceinline__ void ld_run(const P* __restrict__ a, int64_t wbase, i
   if (wbase + 64 * PPT <= n) {
 #pragma unroll
     for (int g = 0; g < NG; g++) {
-      const v4u32* p = reinterpret_cast<const v4u32*>(a + wbase + (int64_t)g * 64 * V + (int64_t)lane * V);
+      const v4u32* p =
+          reinterpret_cast<const v4u32*>(a + pslot<P>(wbase + (int64_t)g * 64 * V) + (int64_t)lane * V);
       const v4u32 t = __builtin_nontemporal_load(p);
       __builtin_memcpy(&o[g * V], &t, 16);
     }
@@ -42,7 +47,7 @@ __device__ __forceinline__ void ld_run(const P* __restrict__ a, int64_t wbase, i
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       const int64_t i = part_index<P, PPT>(wbase, lane, k);
-      o[k] = i < n ? a[i] : P(0);
+      o[k] = i < n ? a[pslot<P>(i)] : P(0);
     }
   }
 }
@@ -56,13 +61,14 @@ __device__ __forceinline__ void st_run(P* __restrict__ a, int64_t wbase, int lan
     for (int g = 0; g < NG; g++) {
       v4u32 t;
       __builtin_memcpy(&t, &o[g * V], 16);
-      __builtin_nontemporal_store(t, reinterpret_cast<v4u32*>(a + wbase + (int64_t)g * 64 * V + (int64_t)lane * V));
+      __builtin_nontemporal_store(
+          t, reinterpret_cast<v4u32*>(a + pslot<P>(wbase + (int64_t)g * 64 * V) + (int64_t)lane * V));
     }
   } else {
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       const int64_t i = part_index<P, PPT>(wbase, lane, k);
-      if (i < n) a[i] = o[k];
+      if (i < n) a[pslot<P>(i)] = o[k];
     }
   }
 }
@@ -206,7 +212,7 @@ __device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, in
 #pragma unroll
     for (int g = 0; g < PPT / V; g++) {
       const v4u32 t = __builtin_nontemporal_load(
-          reinterpret_cast<const v4u32*>(arr[q] + wb + (int64_t)g * 64 * V + (int64_t)lane * V));
+          reinterpret_cast<const v4u32*>(arr[q] + ptile(wb + (int64_t)g * 64 * V) + (int64_t)lane * V));
       __builtin_memcpy(&dst[q][g * V], &t, 16);
     }
 }

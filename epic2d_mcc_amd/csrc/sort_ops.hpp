@@ -120,7 +120,7 @@ __device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, in
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-    key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
+    key[u] = p < n ? cell_key(x[pslot<P>(p)], y[pslot<P>(p)], x0, y0, dx, dy, m) : -1;
   }
   if (threadIdx.x == 0) *wbase_s = key[0] >= 0 && key[0] < m.K ? max(0, key[0] - HWIN / 2) : 0;
   for (int t = threadIdx.x; t < HWIN; t += blockDim.x) win[t] = 0;
@@ -157,7 +157,7 @@ __device__ __forceinline__ void fill_chunk(const P* x, const P* y, int64_t n, in
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-    key[u] = p < n ? cell_key(x[p], y[p], x0, y0, dx, dy, m) : -1;
+    key[u] = p < n ? cell_key(x[pslot<P>(p)], y[pslot<P>(p)], x0, y0, dx, dy, m) : -1;
   }
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
@@ -290,14 +290,16 @@ __device__ __forceinline__ void gather_chunk(const uint32_t* perm, int64_t n, in
   for (int u = 0; u < SORT_U; u++) {
     const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
     if (i < n) {
-      a[u] = x[p[u]]; b[u] = y[p[u]]; c[u] = vx[p[u]]; d[u] = vy[p[u]]; e[u] = vz[p[u]]; f[u] = id[p[u]];
+      const int64_t q = pslot<P>(p[u]);
+      a[u] = x[q]; b[u] = y[q]; c[u] = vx[q]; d[u] = vy[q]; e[u] = vz[p[u]]; f[u] = id[p[u]];
     }
   }
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
     if (i < n) {
-      x2[i] = a[u]; y2[i] = b[u]; vx2[i] = c[u]; vy2[i] = d[u]; vz2[i] = e[u]; id2[i] = f[u];
+      const int64_t q = pslot<P>(i);
+      x2[q] = a[u]; y2[q] = b[u]; vx2[q] = c[u]; vy2[q] = d[u]; vz2[i] = e[u]; id2[i] = f[u];
     }
   }
 }

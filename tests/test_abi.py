@@ -94,7 +94,7 @@ def test_estimate_bytes_counts_every_creation_allocation():
         hip_ok = False
     want = 0
     for cap in (1000, 64):
-        pad = (cap + 7) // 8 * 8
+        pad = (cap + 255) // 256 * 256  # whole 256-slot tiles
         want += 10 * pad * 4 + 2 * pad * 4 + 2 * nn * 8 + 16
     want += 4 * nn * 8 + 2 * nn * 4 + ncell * 8 * 4 + 2 * nn * 8 + 2 * nn * 8 + 2 * nn * 8
     want += 16 * max(ncu, 1024) * 8 + 64 + 16
