@@ -646,7 +646,8 @@ static hipError_t build_descriptors(Batch* B, bool full) {
         push_f32_consts(c, s, &kv, &kx);
       }
       const int seg = (int)B->h_segs.size();
-      B->h_segs.push_back(make_push_f32(c, s, kv, kx));
+      // in place: the descriptors (and the captured step) keep their pointers
+      B->h_segs.push_back(make_push_f32(c, s, kv, kx, false));
       if (full) {
         const int64_t nch = (P.cap + CH - 1) / CH;
         for (int64_t k = 0; k < nch; k++) B->h_chunks.push_back(make_int2(seg, (int)k));

@@ -278,7 +278,10 @@ hipError_t launch_mcc(Ctx* c, int step);
 double mcc_nu_max(const epic_mcc_desc& d, const std::vector<double>& sig, double m, bool electron);
 
 // batch support
-PushF32 make_push_f32(Ctx* c, int s, double kv, double kx);
+// out_of_place: the push writes species s's second buffer (x2), which the
+// caller then swaps in with swap_xyv; otherwise it updates x in place.
+PushF32 make_push_f32(Ctx* c, int s, double kv, double kx, bool out_of_place);
+void swap_xyv(SpeciesDev& S);
 void push_f32_consts(Ctx* c, int s, double* kv, double* kx);
 size_t mcc_args_bytes();
 size_t append_args_bytes();

@@ -852,12 +852,13 @@ static hipError_t launch_particles(Ctx* c, int s, int mode, bool dep, double c1,
 
 // Arguments of the FP32 push of species s: kv = c1 c2 multiplies E into a
 // velocity increment, kx = c3 c2 a velocity into a displacement.
-PushF32 make_push_f32(Ctx* c, int s, double kv, double kx) {
+PushF32 make_push_f32(Ctx* c, int s, double kv, double kx, bool out_of_place) {
   const epic_domain_desc& d = c->dom;
   SpeciesDev& S = c->sp[s];
   const double Lx = (d.nx - 1) * d.dx, Ly = (d.ny - 1) * d.dy;
   PushF32 a{};
-  a.x = (float*)S.x; a.y = (float*)S.y; a.vx = (float*)S.vx; a.vy = (float*)S.vy;
+  a.x = (const float*)S.x;
+  a.xo = (float*)(out_of_place ? S.x2 : S.x);
   a.d_n = S.d_n;
   a.Ec = (const float4*)c->Ec;
   a.x0 = (float)d.x0; a.y0 = (float)d.y0;
@@ -892,7 +893,7 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
   const bool whole = (size_t)d.nx * d.ny <= (size_t)DEP_WINDOW;
   const bool cond = c->cond.any();
   if (d.precision == EPIC_FP32 && d.shape == EPIC_SHAPE_CIC && !whole && d.nx <= 65535 && d.ny <= 65535) {
-    const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2);
+    const PushF32 a = make_push_f32(c, s, c1 * c2, c3 * c2, true);
     const CondDev* ct = c->cond.table;
     const int nct = c->cond.active();
     // the default shape, with the wall kind fixed at compile time
@@ -914,6 +915,7 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
       else if (cond) EPIC_DB(false, true);
       else if (deposit) EPIC_DB(true, false);
       else EPIC_DB(false, false);
+      swap_xyv(c->sp[s]);
       return hipGetLastError();
     }
 #undef EPIC_DB
@@ -937,6 +939,7 @@ hipError_t launch_push(Ctx* c, int s, int sub_cycle, bool deposit, double rewind
       EPIC_V(4, 16, 1)
 #undef EPIC_V
     }
+    swap_xyv(c->sp[s]);
     return hipGetLastError();
   }
   if (d.precision == EPIC_FP64 && d.shape == EPIC_SHAPE_CIC && !whole && c->push64) {
@@ -983,7 +986,7 @@ hipError_t launch_push_all(Ctx* c) {
     if (!c->sp[s].desc.pushed) continue;
     double c1, c2, c3;
     push_consts(c, s, 1, false, &c1, &c2, &c3);
-    S.sp[S.nsp++] = make_push_f32(c, s, c1 * c2, c3 * c2);
+    S.sp[S.nsp++] = make_push_f32(c, s, c1 * c2, c3 * c2, true);
   }
   S.cond = c->cond.table;
   S.ncond = c->cond.active();
@@ -1004,7 +1007,17 @@ hipError_t launch_push_all(Ctx* c) {
     k = cd ? push_f32_multi_kernel<4, true, 8, EPIC_BC_REFLECTIVE>
            : push_f32_multi_kernel<4, false, 8, EPIC_BC_REFLECTIVE>;
   hipLaunchKernelGGL(k, dim3(push_grid(c, k)), dim3(PART_BLOCK), 0, c->stream, S);
+  for (int s = 0; s < c->nsp; s++)
+    if (c->sp[s].desc.pushed) swap_xyv(c->sp[s]);
   return hipGetLastError();
+}
+
+// The pushed coordinates now live in the second buffer (vz and id stay).
+void swap_xyv(SpeciesDev& S) {
+  std::swap(S.x, S.x2);
+  std::swap(S.y, S.y2);
+  std::swap(S.vx, S.vx2);
+  std::swap(S.vy, S.vy2);
 }
 
 }  // namespace epic

@@ -140,11 +140,13 @@ constexpr int WAVE_WIN = 512;
 //   which sum to exactly 2^28 per particle (FIX_BITS_FP32).
 // The deposit stays an integer sum, so it is deterministic; the 2^-14 cell
 // resolution matches the FP32 position resolution at 512 cells.
+// x, xo: a species' tiled particle storage (x, y, vx, vy of each 256-slot
+// tile at +0, +256, +512, +768) read and written; xo is the other buffer of
+// the pair (the push streams from one into the other, which the memory
+// system serves faster than an in-place update) or x itself.
 struct PushF32 {
-  float* x;
-  float* y;
-  float* vx;
-  float* vy;
+  const float* x;
+  float* xo;
   const int64_t* d_n;
   const float4* Ec;  // per cell c = i*(ny-1) + j: bilinear coefficients of ex, then of ey
   float x0, y0, idx, idy, xhi, yhi, Lx, Ly;
@@ -191,9 +193,9 @@ struct ChunkDep {
 template <int PPT>
 __device__ __forceinline__ void load_chunk(const PushF32& a, int64_t wb, int64_t n, int lane, ChunkRegs<PPT>& r) {
   ld_run<float, PPT>(a.x, wb, lane, n, r.x);
-  ld_run<float, PPT>(a.y, wb, lane, n, r.y);
-  ld_run<float, PPT>(a.vx, wb, lane, n, r.vx);
-  ld_run<float, PPT>(a.vy, wb, lane, n, r.vy);
+  ld_run<float, PPT>(a.x + PTILE, wb, lane, n, r.y);
+  ld_run<float, PPT>(a.x + 2 * PTILE, wb, lane, n, r.vx);
+  ld_run<float, PPT>(a.x + 3 * PTILE, wb, lane, n, r.vy);
   if (wb + 64 * PPT > n) {
 #pragma unroll
     for (int k = 0; k < PPT; k++)
@@ -205,14 +207,13 @@ __device__ __forceinline__ void load_chunk(const PushF32& a, int64_t wb, int64_t
 template <int PPT>
 __device__ __forceinline__ void load_chunk_full(const PushF32& a, int64_t wb, int lane, ChunkRegs<PPT>& r) {
   constexpr int V = 4;
-  float* const arr[4] = {a.x, a.y, a.vx, a.vy};
   float* const dst[4] = {r.x, r.y, r.vx, r.vy};
 #pragma unroll
   for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int g = 0; g < PPT / V; g++) {
       const v4u32 t = __builtin_nontemporal_load(
-          reinterpret_cast<const v4u32*>(arr[q] + ptile(wb + (int64_t)g * 64 * V) + (int64_t)lane * V));
+          reinterpret_cast<const v4u32*>(a.x + q * PTILE + ptile(wb + (int64_t)g * 64 * V) + (int64_t)lane * V));
       __builtin_memcpy(&dst[q][g * V], &t, 16);
     }
 }
@@ -375,10 +376,10 @@ __device__ __forceinline__ void push_chunk_bc(const PushF32& a, int64_t wb, int6
   // chunk's loads included -- before registers are reused in the deposit.
   // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15)
   __builtin_amdgcn_s_waitcnt(0x0F70);
-  st_run<float, PPT>(a.vx, wb, lane, n, r.vx);
-  st_run<float, PPT>(a.vy, wb, lane, n, r.vy);
-  st_run<float, PPT>(a.x, wb, lane, n, r.x);
-  st_run<float, PPT>(a.y, wb, lane, n, r.y);
+  st_run<float, PPT>(a.xo + 2 * PTILE, wb, lane, n, r.vx);
+  st_run<float, PPT>(a.xo + 3 * PTILE, wb, lane, n, r.vy);
+  st_run<float, PPT>(a.xo, wb, lane, n, r.x);
+  st_run<float, PPT>(a.xo + PTILE, wb, lane, n, r.y);
   // one lane adds the wavefront's counts
   if ((n_oob | n_nan) && lane == __builtin_amdgcn_readfirstlane(lane)) {
     if (n_oob) atomicAdd(&a.errc[EPIC_ERRC_GATHER_OOB], (unsigned long long)n_oob);
