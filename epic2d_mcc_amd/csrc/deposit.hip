@@ -12,51 +12,17 @@
 
 namespace epic {
 
-constexpr int MAXSP = 16;
-
-struct FinArgs {
-  unsigned long long* Q[MAXSP];
-  double* den[MAXSP];
-  double scale[MAXSP];   // 2^-B * spwt
-  double factor[MAXSP];  // charge * (1/QE)
-  int nsp;
-  unsigned mask;         // species whose den is (re)computed from Q
-  int do_rho;            // 1: rho from den of all species
-  double norm;           // dx*dy*L^2*density
-  int nx, ny, bc;
-  double* rho;
-  unsigned long long* errc;
-  int clear_q;           // clear each read accumulator (walls without images)
-};
-
 __global__ void rho_finalize_kernel(FinArgs a) {
   const int64_t nn = (int64_t)a.nx * a.ny;
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nn; k += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(k / a.ny), j = (int)(k % a.ny);
-    double rho = 0;
-    for (int s = 0; s < a.nsp; s++) {
-      double den;
-      if (a.mask & (1u << s)) {
-        const long long qs = (long long)fold_q(a.Q[s], i, j, a.nx, a.ny, a.bc);
-        // without periodic images a node reads only its own accumulator
-        if (a.clear_q) a.Q[s][k] = 0ull;
-        if (qs < 0) atomicAdd(&a.errc[EPIC_ERRC_FIXED_OVF], 1ull);
-        den = den_from_q(qs, a.scale[s], a.norm, i, j, a.nx, a.ny, a.bc);
-        a.den[s][k] = den;
-      } else {
-        den = a.den[s][k];
-      }
-      if (a.do_rho) rho += den * a.factor[s];
-    }
-    if (a.do_rho) a.rho[k] = rho;
-  }
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nn; k += (int64_t)gridDim.x * blockDim.x)
+    fin_node(a, k);
 }
 
 // Clears the accumulators of every species in mask in one launch (under
 // periodic walls a node's fold reads its images, which other threads read
 // too, so rho_finalize_kernel cannot clear them as it reads).
 struct ZeroQArgs {
-  unsigned long long* Q[MAXSP];
+  unsigned long long* Q[FIN_MAXSP];
   int nsp;
   unsigned mask;
   int64_t nn;
@@ -83,7 +49,7 @@ static hipError_t zero_q(Ctx* c, unsigned mask) {
   return hipGetLastError();
 }
 
-static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
+FinArgs fin_args(Ctx* c, unsigned mask, int do_rho) {
   const epic_domain_desc& d = c->dom;
   FinArgs a{};
   a.nsp = c->nsp;
@@ -102,6 +68,14 @@ static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
   a.rho = c->rho;
   a.errc = c->errc;
   a.clear_q = d.bc != EPIC_BC_PBC;
+  return a;
+}
+
+hipError_t zero_q_after_fin(Ctx* c, const FinArgs& a) { return a.clear_q ? hipSuccess : zero_q(c, a.mask); }
+
+static hipError_t finalize(Ctx* c, unsigned mask, int do_rho) {
+  const epic_domain_desc& d = c->dom;
+  const FinArgs a = fin_args(c, mask, do_rho);
   const int64_t nn = (int64_t)d.nx * d.ny;
   int grid = (int)std::min<int64_t>((nn + 255) / 256, 4096);
   hipLaunchKernelGGL(rho_finalize_kernel, dim3(grid), dim3(256), 0, c->stream, a);

@@ -174,6 +174,9 @@ struct Ctx {
   // efield_done: the launched E-field is that of the final phi
   bool efield_in_solve = false;
   bool efield_done = false;
+  // fused step: the charge density is finalised by the sine-transform
+  // solve's first kernel (rho_finalize's work in the residual's launch)
+  bool rho_in_solve = false;
   int* ws_iter = nullptr;
   int max_solver_blocks = 0;
   // co-resident workgroups per CU of each particle kernel (queried once)
@@ -269,6 +272,15 @@ hipError_t launch_solve(Ctx* c, int* iters, double* err);
 hipError_t launch_spectral_solve_noalloc(Ctx* c, int* solves, double* relres, int* status, const char** why);
 hipError_t launch_conductor_rho(Ctx* c);
 hipError_t launch_efield(Ctx* c);
+struct NormsArgs;
+struct FinArgs;
+// The finalisation's arguments (all species: mask of every bit, do_rho 1) and
+// the accumulator clear that follows it under periodic walls.
+FinArgs fin_args(Ctx* c, unsigned mask, int do_rho);
+hipError_t zero_q_after_fin(Ctx* c, const FinArgs& a);
+// E-field (and the FP32 gather table) with a mixed solve's convergence norms
+// in the same launch where the precision allows (FP32), else after them.
+hipError_t launch_efield_norms(Ctx* c, const NormsArgs& na);
 hipError_t launch_pack_E2(Ctx* c);
 // diag.hip
 hipError_t launch_particle_moments(Ctx* c, int s, double out7[7]);

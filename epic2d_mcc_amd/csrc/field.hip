@@ -426,10 +426,56 @@ static hipError_t launch_cell_table(Ctx* c) {
   return hipGetLastError();
 }
 
-hipError_t launch_efield(Ctx* c) {
+static __global__ void norms_kernel(NormsArgs a) { norms_wave(a); }
+
+// FP32 contexts: the E-field of efield_kernel and the gather table of
+// cell_table_kernel in one launch.  The thread of node (i, j) writes the
+// node's field and, for i < nx - 1 and j < ny - 1, the table of cell (i, j),
+// evaluating the field at the cell's three other corners itself (the same
+// arithmetic, so the same values as through E2).  Optionally one more
+// workgroup forms the convergence norms of a mixed solve (NormsArgs.n > 0),
+// which do not depend on the field.
+__global__ void efield_cells_kernel(const double* phi, double* efx, double* efy, float2* E2, float4* Ec, int nx,
+                                    int ny, double dx, double dy, int bc, NormsArgs na) {
+  if (na.n > 0 && blockIdx.x == gridDim.x - 1) {
+    if (threadIdx.x < 64) norms_wave(na);
+    return;
+  }
+  const int n = nx * ny;
+  const int nthr = (gridDim.x - (na.n > 0 ? 1 : 0)) * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += nthr) {
+    const int i = k / ny, j = k % ny;
+    double ex, ey;
+    efield_at(phi, i, j, nx, ny, dx, dy, bc, &ex, &ey);
+    efx[k] = ex;
+    efy[k] = ey;
+    const float2 e00 = make_float2((float)ex, (float)ey);
+    E2[k] = e00;
+    if (i < nx - 1 && j < ny - 1) {
+      double ax, ay, bx, by, cx, cy;
+      efield_at(phi, i + 1, j, nx, ny, dx, dy, bc, &ax, &ay);
+      efield_at(phi, i, j + 1, nx, ny, dx, dy, bc, &bx, &by);
+      efield_at(phi, i + 1, j + 1, nx, ny, dx, dy, bc, &cx, &cy);
+      float4* out = Ec + 2 * (size_t)(i * (ny - 1) + j);
+      out[0] = bilinear_coeffs(e00.x, (float)ax, (float)bx, (float)cx);
+      out[1] = bilinear_coeffs(e00.y, (float)ay, (float)by, (float)cy);
+    }
+  }
+}
+
+hipError_t launch_efield_norms(Ctx* c, const NormsArgs& na) {
   const epic_domain_desc& d = c->dom;
   const int n = d.nx * d.ny;
   const int grid = std::min((n + 255) / 256, 4096);
+  if (d.precision == EPIC_FP32 && c->Ec) {
+    hipLaunchKernelGGL(efield_cells_kernel, dim3(grid + (na.n > 0 ? 1 : 0)), dim3(256), 0, c->stream, c->phi,
+                       c->efx, c->efy, (float2*)c->E2, (float4*)c->Ec, d.nx, d.ny, d.dx, d.dy, d.bc, na);
+    return hipGetLastError();
+  }
+  if (na.n > 0) {
+    hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, c->stream, na);
+    EPIC_HIP_TRY(hipGetLastError());
+  }
   if (d.precision == EPIC_FP64)
     hipLaunchKernelGGL(efield_kernel<double>, dim3(grid), dim3(256), 0, c->stream, c->phi, c->efx, c->efy,
                        (double*)c->E2, d.nx, d.ny, d.dx, d.dy, d.bc);
@@ -439,6 +485,8 @@ hipError_t launch_efield(Ctx* c) {
   EPIC_HIP_TRY(hipGetLastError());
   return launch_cell_table(c);
 }
+
+hipError_t launch_efield(Ctx* c) { return launch_efield_norms(c, NormsArgs{}); }
 
 template <typename P>
 __global__ void pack_E2_kernel(const double* efx, const double* efy, P* E2, int n) {

@@ -7,8 +7,45 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/epic2d.h"
+#include "device_common.hpp"
+#include "epic_internal.hpp"
 
 namespace epic {
+
+// Convergence norms of a mixed solve from the residual kernels' per-block
+// partial sums (pr: the latest residual's, pb: the first residual's, whose
+// r is b): ||b||^2 and ||r||^2 to out[0], out[1]; hs (when given): both, then
+// seq with a system-scope release, to pinned host memory for the host's
+// polling wait.  One wavefront; n = 0: nothing to form.
+struct NormsArgs {
+  const double* pr;
+  const double* pb;
+  int n;
+  double* out;
+  HostScal* hs;
+  unsigned long long seq;
+};
+
+__device__ __forceinline__ void norms_wave(const NormsArgs& a) {
+  double sr = 0, sb = 0;
+  for (int b = threadIdx.x & 63; b < a.n; b += 64) {
+    sr += a.pr[b];
+    sb += a.pb[b];
+  }
+  sr = wave_sum(sr);
+  sb = wave_sum(sb);
+  if ((threadIdx.x & 63) == 0) {
+    a.out[0] = sb;
+    a.out[1] = sr;
+    if (a.hs) {
+      __hip_atomic_store(&a.hs->bb, (unsigned long long)__double_as_longlong(sb), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&a.hs->rr, (unsigned long long)__double_as_longlong(sr), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&a.hs->seq, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
 
 // Fixed-point density of node (i, j) from one species' accumulator Q
 // (species.cpp:188-225): pbc takes the sum over the node's periodic images on
@@ -39,6 +76,46 @@ __device__ __forceinline__ double den_from_q(long long q, double scale, double n
     if (j == 0 || j == ny - 1) den *= 0.5;
   }
   return den;
+}
+
+// Q -> den and rho of one node: the arguments of the finalisation.
+constexpr int FIN_MAXSP = 16;
+struct FinArgs {
+  unsigned long long* Q[FIN_MAXSP];
+  double* den[FIN_MAXSP];
+  double scale[FIN_MAXSP];   // 2^-B * spwt
+  double factor[FIN_MAXSP];  // charge * (1/QE)
+  int nsp;
+  unsigned mask;         // species whose den is (re)computed from Q
+  int do_rho;            // 1: rho from den of all species
+  double norm;           // dx*dy*L^2*density
+  int nx, ny, bc;
+  double* rho;
+  unsigned long long* errc;
+  int clear_q;           // clear each read accumulator (walls without images)
+};
+
+// den_s of node k from Q_s (species in mask) and rho = sum den_s q_s/e
+// (domain.cpp:194-202); returns rho (0 when do_rho is 0).
+__device__ __forceinline__ double fin_node(const FinArgs& a, int64_t k) {
+  const int i = (int)(k / a.ny), j = (int)(k % a.ny);
+  double rho = 0;
+  for (int s = 0; s < a.nsp; s++) {
+    double den;
+    if (a.mask & (1u << s)) {
+      const long long qs = (long long)fold_q(a.Q[s], i, j, a.nx, a.ny, a.bc);
+      // without periodic images a node reads only its own accumulator
+      if (a.clear_q) a.Q[s][k] = 0ull;
+      if (qs < 0) atomicAdd(&a.errc[EPIC_ERRC_FIXED_OVF], 1ull);
+      den = den_from_q(qs, a.scale[s], a.norm, i, j, a.nx, a.ny, a.bc);
+      a.den[s][k] = den;
+    } else {
+      den = a.den[s][k];
+    }
+    if (a.do_rho) rho += den * a.factor[s];
+  }
+  if (a.do_rho) a.rho[k] = rho;
+  return rho;
 }
 
 // Right-hand side of pcgsolver (field.cpp:234-251): -rho dx^2 (L/LDe)^2,

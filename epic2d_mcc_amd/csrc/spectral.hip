@@ -61,9 +61,11 @@ struct RhsSrc {
   double dx2, s;
 };
 
-template <int OP>
+// FIN: rho is first finalised here from the species' accumulators (the work
+// of rho_finalize_kernel, node by node, in the same launch).
+template <int OP, bool FIN>
 __global__ void spec_residual_kernel(const double* b, const double* phi, double* r, int nx, int ny, double* part,
-                                     float* X, SpecLayout L, double* zero_phi, RhsSrc src) {
+                                     float* X, SpecLayout L, double* zero_phi, RhsSrc src, FinArgs fin) {
   __shared__ double sh[64];
   const int n = nx * ny;
   double acc = 0;
@@ -71,7 +73,10 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
     // zero_phi: the solve's start, phi = 0 (written here, not read)
     double rk;
     if (zero_phi) {
-      if (src.rho) src.b[k] = rhs_at(src.rho[k], k / ny, k % ny, nx, ny, src.dx2, src.s);
+      if (src.rho) {
+        const double rho = FIN ? fin_node(fin, k) : src.rho[k];
+        src.b[k] = rhs_at(rho, k / ny, k % ny, nx, ny, src.dx2, src.s);
+      }
       rk = residual_at_zero(OP, b, k, nx, ny);
       zero_phi[k] = 0.0;
     } else {
@@ -89,30 +94,7 @@ __global__ void spec_residual_kernel(const double* b, const double* phi, double*
   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
-// ||r||^2 and ||b||^2 from the residual kernels' partials (pr: the latest
-// residual's, pb: the first residual's, r = b) into out[1], out[0]; hs (when
-// given): both, then seq, to pinned host memory for the host's polling wait.
-__global__ void norms_kernel(const double* pr, const double* pb, int n, double* out, HostScal* hs,
-                             unsigned long long seq) {
-  double sr = 0, sb = 0;
-  for (int b = threadIdx.x; b < n; b += 64) {
-    sr += pr[b];
-    sb += pb[b];
-  }
-  sr = wave_sum(sr);
-  sb = wave_sum(sb);
-  if (threadIdx.x == 0) {
-    out[0] = sb;
-    out[1] = sr;
-    if (hs) {
-      __hip_atomic_store(&hs->bb, (unsigned long long)__double_as_longlong(sb), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&hs->rr, (unsigned long long)__double_as_longlong(sr), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&hs->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-  }
-}
+static __global__ void norms_kernel(NormsArgs a) { norms_wave(a); }
 
 struct SpecState {
   SpecLayout L{};
@@ -210,25 +192,34 @@ static int residual_blocks(const Ctx* c) { return std::min((c->dom.nx * c->dom.n
 // partials to the second region; b is built from rho there when src.rho is
 // given.
 static hipError_t residual(Ctx* c, const SpecLayout& L, const double* b, const double* phi, double* r, float* X,
-                           double* zero_phi = nullptr, RhsSrc src = RhsSrc{nullptr, nullptr, 0, 0}) {
+                           double* zero_phi = nullptr, RhsSrc src = RhsSrc{nullptr, nullptr, 0, 0},
+                           const FinArgs* fin = nullptr) {
   const epic_domain_desc& d = c->dom;
   const int g = residual_blocks(c);
   double* part = c->ws_part + (zero_phi ? SPEC_PART_B : 0);
-  if (d.poisson_op == EPIC_OP_REF_GHOST)
-    hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_REF_GHOST>, dim3(g), dim3(256), 0, c->stream, b, phi, r, d.nx,
-                       d.ny, part, X, L, zero_phi, src);
-  else
-    hipLaunchKernelGGL(spec_residual_kernel<EPIC_OP_DIRICHLET_WALL>, dim3(g), dim3(256), 0, c->stream, b, phi, r,
-                       d.nx, d.ny, part, X, L, zero_phi, src);
+  const FinArgs fa = fin ? *fin : FinArgs{};
+#define EPIC_R(OP, FIN)                                                                                      \
+  hipLaunchKernelGGL((spec_residual_kernel<OP, FIN>), dim3(g), dim3(256), 0, c->stream, b, phi, r, d.nx, d.ny, \
+                     part, X, L, zero_phi, src, fa)
+  if (d.poisson_op == EPIC_OP_REF_GHOST) {
+    if (fin) EPIC_R(EPIC_OP_REF_GHOST, true);
+    else EPIC_R(EPIC_OP_REF_GHOST, false);
+  } else {
+    if (fin) EPIC_R(EPIC_OP_DIRICHLET_WALL, true);
+    else EPIC_R(EPIC_OP_DIRICHLET_WALL, false);
+  }
+#undef EPIC_R
   return hipGetLastError();
 }
 
 // ||b||^2 and the latest residual's ||r||^2 into out (first_only: no
 // correction ran, r = b); seq > 0: also to the pinned host words.
-static hipError_t norms(Ctx* c, double* out, bool first_only, unsigned long long seq) {
+static NormsArgs norms_args(Ctx* c, double* out, bool first_only, unsigned long long seq) {
   const double* pb = c->ws_part + SPEC_PART_B;
-  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, c->stream, first_only ? pb : c->ws_part, pb,
-                     residual_blocks(c), out, seq ? c->hscal : nullptr, seq);
+  return NormsArgs{first_only ? pb : c->ws_part, pb, residual_blocks(c), out, seq ? c->hscal : nullptr, seq};
+}
+static hipError_t norms(Ctx* c, double* out, bool first_only, unsigned long long seq) {
+  hipLaunchKernelGGL(norms_kernel, dim3(1), dim3(64), 0, c->stream, norms_args(c, out, first_only, seq));
   return hipGetLastError();
 }
 
@@ -269,7 +260,15 @@ static hipError_t spectral_solve_on(Ctx* c, SpecState* S, double* b, double* phi
   const SpecLayout& L = S->L;
   // b from rho, phi := 0 and r = b in one kernel (the corrections write
   // phi's interior)
-  EPIC_HIP_TRY(residual(c, L, b, nullptr, c->r64, Xpack, phi, RhsSrc{rho, b, dx2, s}));
+  // (fused step: rho finalised from the deposit in the same launch)
+  if (c->rho_in_solve && rho) {
+    const FinArgs fa = fin_args(c, (1u << c->nsp) - 1u, 1);
+    EPIC_HIP_TRY(residual(c, L, b, nullptr, c->r64, Xpack, phi, RhsSrc{rho, b, dx2, s}, &fa));
+    EPIC_HIP_TRY(zero_q_after_fin(c, fa));
+  } else {
+    EPIC_HIP_TRY(residual(c, L, b, nullptr, c->r64, Xpack, phi, RhsSrc{rho, b, dx2, s}));
+  }
+  c->rho_in_solve = false;
   int k = 0;
   bool checked = false;
   double host[2] = {0, 0};
@@ -286,10 +285,10 @@ static hipError_t spectral_solve_on(Ctx* c, SpecState* S, double* b, double* phi
     if (k + 1 >= (fp32_inner ? 2 : 1)) {
       if (poll) {
         const unsigned long long seq = ++c->hscal_seq;
-        EPIC_HIP_TRY(norms(c, dscal, false, seq));
-        // E of this phi, in case it is the final one (a further correction
-        // launches it again)
-        if (c->efield_in_solve) EPIC_HIP_TRY(launch_efield(c));
+        // with E of this phi, in case it is the final one (a further
+        // correction launches it again)
+        if (c->efield_in_solve) EPIC_HIP_TRY(launch_efield_norms(c, norms_args(c, dscal, false, seq)));
+        else EPIC_HIP_TRY(norms(c, dscal, false, seq));
         c->efield_done = c->efield_in_solve;
         EPIC_HIP_TRY(wait_host_scalars(c, seq, &host[0], &host[1]));
       } else {
