@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # workload (profiles/push_f32_multi_kernel_bounded512_pmc.md): FETCH_SIZE x 2
 # + WRITE_SIZE x 1 per dispatch (the factors a plain 1 GiB device copy reads
 # back as), divided by the 104,857,600 particles of a dispatch.
-PUSH_TRAFFIC_BYTES_PER_PARTICLE = (856875.02 * 2 + 1718744.54) * 1024 / 104857600
+PUSH_TRAFFIC_BYTES_PER_PARTICLE = (881959.4 * 2 + 1747068.1) * 1024 / 104857600
 
 
 def gpu_run(st, steps, warmup, solver_name, mcc=None):
@@ -686,7 +686,7 @@ def main():
         main_single(args, st, "periodic 256x256 cells, electron + Ar+ at 1e15 m^-3, 200 ppc each (26,214,400), "
                               "FP32, Ar neutrals 1e21 m^-3 at 300 K (null-collision MCC, e-Ar elastic/excitation/"
                               "ionisation, Ar+-Ar elastic/CX), mixed Poisson 1e-10, sort every 20 steps", "fp32", 32,
-                    "push_f32_multi_kernel<4,false,8> (gather + push + wrap + deposit, all species in one launch)", mcc=mcc, gas=(1, 1, 1, 1e21),
+                    "push_f32_multi_kernel<4,false,8,PBC> (gather + push + wrap + deposit, all species in one launch, tiled storage, out of place)", mcc=mcc, gas=(1, 1, 1, 1e21),
                     cpu_note=", electron collisions by the reference's direct method")
         return
     solvers = {"pcg_mixed": _lib.SOLVER_PCG_MIXED, "spectral": _lib.SOLVER_SPECTRAL, "pcg": _lib.SOLVER_PCG,
@@ -744,7 +744,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": PUSH_TRAFFIC_BYTES_PER_PARTICLE * parts_per_launch,
-                     "kernel": "push_f32_multi_kernel<4,false,8> (CIC gather + leapfrog push + walls + deposit, all species in one launch)",
+                     "kernel": "push_f32_multi_kernel<4,false,8,OPEN> (CIC gather + leapfrog push + walls + deposit, all species in one launch, tiled storage, out of place)",
                      "bytes_per_particle_step": bytes_per_ps, "avg_launch_ms": avg_launch_s * 1e3,
                      "particles_per_launch": parts_per_launch},
         "device_errors": r["errors"],
