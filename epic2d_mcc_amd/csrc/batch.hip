@@ -226,7 +226,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_batch_kernel(const Pus
       full = wbn + 64 * PPT <= nn;
       if (full) load_chunk_full<PPT>(an, wbn, lane, r);
     }
-    if (active) deposit_chunk<PPT>(a, lane, wl, dp);
+    if (active) deposit_chunk<PPT, BC == EPIC_BC_PBC>(a, lane, wl, dp);
     if (cn < nchunks && !full && wbn < nn) load_chunk<PPT>(an, wbn, nn, lane, r);
     a = an;
     n = nn;

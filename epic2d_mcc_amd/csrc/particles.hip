@@ -655,7 +655,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_kernel(PushF32 a, 
     const bool more = chunk + cs < ce_;
     const bool full = more && nb + 64 * PPT <= n;
     if (full) load_chunk_full<PPT>(a, nb, lane, r);
-    if (DEP) deposit_chunk<PPT>(a, lane, wl, dp);
+    if (DEP) deposit_chunk<PPT, BC == EPIC_BC_PBC>(a, lane, wl, dp);
     if (more && !full) load_chunk<PPT>(a, nb, n, lane, r);
   }
 }
@@ -736,7 +736,7 @@ __global__ __launch_bounds__(PART_BLOCK, WAVES) void push_f32_multi_kernel(PushF
     const PushF32 an = S.sp[sn];
     const bool full = more && wbn + CHW <= nn;
     if (full) load_chunk_full<PPT>(an, wbn, lane, r);
-    if (active) deposit_chunk<PPT>(a, lane, wl, dp);
+    if (active) deposit_chunk<PPT, BC == EPIC_BC_PBC>(a, lane, wl, dp);
     if (more && !full && wbn < nn) load_chunk<PPT>(an, wbn, nn, lane, r);
     a = an;
     s = sn;

@@ -412,16 +412,75 @@ __device__ __forceinline__ void push_chunk_regs(const PushF32& a, int64_t wb, in
 // Wall kind of a kernel instantiation: -1 reads it per chunk.
 constexpr int BC_ANY = -1;
 
+// Periodic walls: offset d of a cell index from the anchor's, taken modulo
+// the period C (cells) into [-C/2, C/2], and a node index brought back into
+// [0, C) (node C is node 0's periodic image; the finalisation folds the two).
+__device__ __forceinline__ int per_off(int d, int C) { return d > C / 2 ? d - C : (d < -(C / 2) ? d + C : d); }
+__device__ __forceinline__ int per_wrap(int g, int C) { return g < 0 ? g + C : (g >= C ? g - C : g); }
+
+// Node-window accumulator flush of a run under periodic walls: (ri, rj) is
+// the run's cell as offsets from the anchor (ai, aj); the global fallback
+// adds to the wrapped nodes.
+__device__ __forceinline__ void flush_block_per(int ri, int rj, int ny, const unsigned long long (&acc)[4],
+                                                const Window& w, unsigned long long* lds, unsigned long long* Q,
+                                                int ai, int aj, int Ci, int Cj) {
+  const int li = ri - w.i0, lj = rj - w.j0;
+  if (li >= 0 && lj >= 0 && li + 1 < w.wi && lj + 1 < w.wj) {
+    unsigned long long* b = lds + li * w.wj + lj;
+    if (acc[0]) lds_add(b, acc[0]);
+    if (acc[1]) lds_add(b + w.wj, acc[1]);
+    if (acc[2]) lds_add(b + 1, acc[2]);
+    if (acc[3]) lds_add(b + w.wj + 1, acc[3]);
+  } else {
+    const int gi = per_wrap(ai + ri, Ci), gj = per_wrap(aj + rj, Cj);
+    const int gi1 = per_wrap(gi + 1, Ci), gj1 = per_wrap(gj + 1, Cj);
+    if (acc[0]) global_add(Q + (size_t)gi * ny + gj, acc[0]);
+    if (acc[1]) global_add(Q + (size_t)gi1 * ny + gj, acc[1]);
+    if (acc[2]) global_add(Q + (size_t)gi * ny + gj1, acc[2]);
+    if (acc[3]) global_add(Q + (size_t)gi1 * ny + gj1, acc[3]);
+  }
+}
+
 // Deposit of a pushed chunk through the wavefront's LDS window wl: each
 // lane sums runs of equal cells in registers, runs go to the window (or
-// straight to Q outside it), the window goes to Q.
-template <int PPT>
+// straight to Q outside it), the window goes to Q.  PER (periodic walls):
+// the window is placed in cell offsets from an anchor cell (the first
+// deposit of the first lane that has one), each offset taken modulo the
+// period, so that a particle that has just wrapped across the boundary stays
+// next to its chunk's other particles instead of stretching the window
+// across the grid; nodes are mapped back modulo the period when they reach Q
+// (node C then lands on its image, node 0, which changes no folded density).
+template <int PPT, bool PER = false>
 __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsigned long long* wl,
                                               const ChunkDep<PPT>& dp) {
   const int ny = a.ny;
-  const int gi0 = wave_min_dpp(dp.imin), gi1 = wave_max_dpp(dp.imax);
-  if (gi1 < 0) return;
-  const int gj0 = wave_min_dpp(dp.jmin), gj1 = wave_max_dpp(dp.jmax);
+  const int Ci = a.nx - 1, Cj = a.ny - 1;
+  int ai = 0, aj = 0;
+  int gi0, gi1, gj0, gj1;
+  if (PER) {
+    unsigned first = NO_DEPOSIT;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) first = first == NO_DEPOSIT ? dp.cp[k] : first;
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(first != NO_DEPOSIT);
+    if (!m) return;
+    const unsigned anc = (unsigned)__builtin_amdgcn_readlane((int)first, (int)__builtin_ctzll(m));
+    ai = (int)(anc >> 16);
+    aj = (int)(anc & 0xffffu);
+    int imin = 1 << 30, imax = -(1 << 30), jmin = 1 << 30, jmax = -(1 << 30);
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      if (dp.cp[k] == NO_DEPOSIT) continue;
+      const int ri = per_off((int)(dp.cp[k] >> 16) - ai, Ci), rj = per_off((int)(dp.cp[k] & 0xffffu) - aj, Cj);
+      imin = min(imin, ri); imax = max(imax, ri);
+      jmin = min(jmin, rj); jmax = max(jmax, rj);
+    }
+    gi0 = wave_min_dpp(imin); gi1 = wave_max_dpp(imax);
+    gj0 = wave_min_dpp(jmin); gj1 = wave_max_dpp(jmax);
+  } else {
+    gi0 = wave_min_dpp(dp.imin); gi1 = wave_max_dpp(dp.imax);
+    if (gi1 < 0) return;
+    gj0 = wave_min_dpp(dp.jmin); gj1 = wave_max_dpp(dp.jmax);
+  }
   const int wi = gi1 - gi0 + 2, wj = gj1 - gj0 + 2;
   Window win;
   if (wi * wj <= WAVE_WIN) {
@@ -442,14 +501,19 @@ __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsign
   static_assert((unsigned long long)PPT * (1ull << 28) <= (unsigned long long)(acc_t)~(acc_t)0,
                 "run accumulator too narrow for PPT particles");
   acc_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+  auto flush = [&]() {
+    const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
+    if (PER)
+      flush_block_per(per_off((int)(rk >> 16) - ai, Ci), per_off((int)(rk & 0xffffu) - aj, Cj), ny, acc, win, wl,
+                      a.Q, ai, aj, Ci, Cj);
+    else
+      flush_block((int)(rk >> 16), (int)(rk & 0xffffu), ny, acc, 4, win, wl, a.Q);
+  };
 #pragma unroll
   for (int k = 0; k < PPT; k++) {
     if (dp.cp[k] == NO_DEPOSIT) continue;
     if (dp.cp[k] != rk) {
-      if (rk != NO_DEPOSIT) {
-        const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
-        flush_block((int)(rk >> 16), (int)(rk & 0xffffu), ny, acc, 4, win, wl, a.Q);
-      }
+      if (rk != NO_DEPOSIT) flush();
       rk = dp.cp[k];
       acc0 = acc1 = acc2 = acc3 = 0;
     }
@@ -460,10 +524,7 @@ __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsign
     acc2 += (acc_t)(oI * dJ);
     acc3 += (acc_t)(dI * dJ);
   }
-  if (rk != NO_DEPOSIT) {
-    const unsigned long long acc[4] = {acc0, acc1, acc2, acc3};
-    flush_block((int)(rk >> 16), (int)(rk & 0xffffu), ny, acc, 4, win, wl, a.Q);
-  }
+  if (rk != NO_DEPOSIT) flush();
   wave_fence();
   const int W = win.wi * win.wj;
   // row of window slot k as floor((k + 1/2) / wj) in FP32: exact, since
@@ -473,8 +534,12 @@ __device__ __forceinline__ void deposit_chunk(const PushF32& a, int lane, unsign
     const unsigned long long v = wl[k];
     if (v) {
       const int ri = (int)(((float)k + 0.5f) * rwj);
-      const int gi = win.i0 + ri, gj = win.j0 + (k - ri * win.wj);
-      if (gi >= 0 && gj >= 0 && gi < a.nx && gj < a.ny) atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
+      int gi = win.i0 + ri, gj = win.j0 + (k - ri * win.wj);
+      if (PER) {
+        atomicAdd(&a.Q[(size_t)per_wrap(ai + gi, Ci) * ny + per_wrap(aj + gj, Cj)], v);
+      } else if (gi >= 0 && gj >= 0 && gi < a.nx && gj < a.ny) {
+        atomicAdd(&a.Q[(size_t)gi * ny + gj], v);
+      }
     }
   }
   wave_fence();
