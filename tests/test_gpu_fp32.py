@@ -222,3 +222,52 @@ def test_fp32_positions_stay_below_upper_wall(bc):
     ctx.step(0)
     assert not ctx.errors().any(), ctx.errors()
     ctx.close()
+
+
+def test_periodic_deposit_across_the_boundary():
+    """Periodic walls, every chunk straddling the boundary: the slots
+    alternate between particles just inside the low edges and just inside
+    the high edges, moving outward, so that a quarter of them wrap in the
+    first push.  The fused push's deposit (its window placed in cell offsets
+    modulo the period) gives the rho that a separate scatter of the pushed
+    positions gives, bit for bit."""
+    print("[fp32] periodic deposit across the boundary", flush=True)
+    n = 64 * 4 * 256
+    st = small_setup(N=n, Nx=128, Ny=128, precision=_lib.FP32, bc="pbc", tol=1e-8)
+    d = st.domain
+    Lx, Ly = (d.nx - 1) * d.dx, (d.ny - 1) * d.dy
+    rng = np.random.default_rng(7)
+    k = np.arange(n)
+    hi = (k % 2) == 1
+    # within 0.3 cells of the low or the high edge in x; y spread, with every
+    # fourth slot also at an edge in y
+    x = np.where(hi, Lx - 0.3 * d.dx * rng.random(n), 0.3 * d.dx * rng.random(n))
+    y = rng.random(n) * Ly
+    yedge = (k % 4) >= 2
+    y = np.where(yedge & hi, Ly - 0.3 * d.dy * rng.random(n), np.where(yedge, 0.3 * d.dy * rng.random(n), y))
+    # outward at up to ~0.6 cells per step (in units of the push's kx)
+    a = _lib.Context(st.domain, st.species)
+    step_len = 0.6 * d.dx / (d.vel_norm / (d.L * d.W) * d.DT)
+    vx = np.where(hi, 1.0, -1.0) * step_len * rng.random(n)
+    vy = np.where(yedge, np.where(hi, 1.0, -1.0) * step_len * rng.random(n), 0.0)
+    z = np.zeros(n)
+    parts = [(x, y, vx, vy, z), (x.copy(), y.copy(), z, z, z)]
+    for s, p in enumerate(parts):
+        a.upload(s, *p)
+    a.step(0)          # push of the uploaded state with the fused deposit
+    pushed = [a.download(s) for s in range(2)]
+    wrapped = np.sum((~hi) & (pushed[0][0] > Lx / 2)) + np.sum(hi & (pushed[0][0] < Lx / 2))
+    print(f"  electrons wrapped in x: {wrapped} of {n}", flush=True)
+    assert wrapped > n // 8
+    a.step(1)          # its rho, from the fused deposit
+    rho_fused = a.get_field(0)
+    assert not a.errors().any()
+    b = _lib.Context(st.domain, st.species)
+    for s in range(2):
+        g = pushed[s]
+        b.upload(s, g[0], g[1], g[2], g[3], g[4])
+        b.scatter(s)
+    b.compute_rho()
+    assert np.array_equal(rho_fused, b.get_field(0))
+    a.close()
+    b.close()
