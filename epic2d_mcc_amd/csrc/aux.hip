@@ -246,22 +246,21 @@ int64_t sort_key_space(int nx, int ny) { return key_map(nx - 1, ny - 1).K; }
 
 template <typename P>
 __global__ __launch_bounds__(256) void sort_hist_kernel(const P* x, const P* y, const int64_t* d_n, int* count, P x0,
-                                                        P y0, P dx, P dy, KeyMap m) {
+                                                        P y0, P dx, P dy, KeyMap m, uint32_t* pks) {
   __shared__ int win[HWIN];
   __shared__ int wbase_s;
   const int64_t n = *d_n;
   const int64_t stride = (int64_t)gridDim.x * SORT_CHUNK;
   for (int64_t base = blockIdx.x * (int64_t)SORT_CHUNK; base < n; base += stride)
-    hist_chunk(x, y, n, base, count, x0, y0, dx, dy, m, win, &wbase_s);
+    hist_chunk(x, y, n, base, count, x0, y0, dx, dy, m, win, &wbase_s, pks);
 }
 
-template <typename P>
-__global__ __launch_bounds__(256) void sort_fill_kernel(const P* x, const P* y, const int64_t* d_n, const int* start,
-                                                        int* fill, uint32_t* perm, P x0, P y0, P dx, P dy, KeyMap m) {
+__global__ __launch_bounds__(256) void sort_fill_kernel(const uint32_t* pks, const int64_t* d_n, const int* start,
+                                                        int* fill, uint32_t* perm, KeyMap m) {
   const int64_t n = *d_n;
   const int64_t stride = (int64_t)gridDim.x * SORT_CHUNK;
   for (int64_t base = blockIdx.x * (int64_t)SORT_CHUNK; base < n; base += stride)
-    fill_chunk(x, y, n, base, start, fill, perm, x0, y0, dx, dy, m);
+    fill_chunk(pks, n, base, start, fill, perm, m);
 }
 
 // One wavefront per FIX_GROUP cells.
@@ -304,13 +303,16 @@ static hipError_t sort_t(Ctx* c, int s) {
   const int gc = std::max(1, std::min((ncell + 2 + 255) / 256, 4096));
   hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_count, (int64_t)ncell + 2);
   hipLaunchKernelGGL(zero_int_kernel, dim3(gc), dim3(256), 0, c->stream, c->sort_fill, (int64_t)ncell + 2);
+  // the histogram's sorted (key, lane) words go to perm's second half (the
+  // per-cell fix's scratch, which is not used before the fix)
+  uint32_t* pks = c->sort_perm + c->sort_cap;
   hipLaunchKernelGGL(sort_hist_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
-                     c->sort_count, x0, y0, dx, dy, km);
+                     c->sort_count, x0, y0, dx, dy, km, pks);
   EPIC_HIP_TRY(hipGetLastError());
   // start[k] = exclusive scan; start[ncell] = live total; start[ncell+1] = all
   EPIC_HIP_TRY(scan_ints(c, c->sort_count, (int64_t)ncell + 1));
-  hipLaunchKernelGGL(sort_fill_kernel<P>, dim3(g), dim3(256), 0, c->stream, (const P*)S.x, (const P*)S.y, S.d_n,
-                     c->sort_count, c->sort_fill, c->sort_perm, x0, y0, dx, dy, km);
+  hipLaunchKernelGGL(sort_fill_kernel, dim3(g), dim3(256), 0, c->stream, (const uint32_t*)pks, S.d_n,
+                     c->sort_count, c->sort_fill, c->sort_perm, km);
   EPIC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(sort_fix_kernel, dim3(std::min((ncell + 4 * FIX_GROUP - 1) / (4 * FIX_GROUP), 65536)), dim3(256),
                      0, c->stream,

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void bsort_hist_kernel(const SortSeg* segs, in
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
     const int64_t n = *S.d_n, base = (c - S.chunk0) * SORT_CHUNK;
-    if (base < n) hist_chunk(S.x, S.y, n, base, S.count, S.x0, S.y0, S.dx, S.dy, S.km, win, &wbase_s);
+    if (base < n) hist_chunk(S.x, S.y, n, base, S.count, S.x0, S.y0, S.dx, S.dy, S.km, win, &wbase_s, S.scratch);
   }
 }
 
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void bsort_fill_kernel(const SortSeg* segs, in
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
     const int64_t n = *S.d_n, base = (c - S.chunk0) * SORT_CHUNK;
-    if (base < n) fill_chunk(S.x, S.y, n, base, S.count, S.fill, S.perm, S.x0, S.y0, S.dx, S.dy, S.km);
+    if (base < n) fill_chunk(S.scratch, n, base, S.count, S.fill, S.perm, S.km);
   }
 }
 

@@ -12,7 +12,9 @@
 // Stages: histogram of keys -> exclusive scan (start) -> slot reservation
 // (perm: destination slot -> source index; order inside a wavefront kept,
 // order between wavefronts arbitrary) -> per-cell fix (each cell's sources
-// ascending) -> gather into the second buffer.
+// ascending) -> gather into the second buffer.  The histogram keeps its
+// wavefront-sorted (key, lane) words, from which the reservation works
+// without computing the keys or sorting again.
 #pragma once
 
 #include "device_common.hpp"
@@ -110,11 +112,17 @@ __device__ __forceinline__ int wave_sort(int v) {
 // then adds the window's non-zero counts to the global histogram: particles
 // that moved less than a tile since the previous sort cost no global atomic.
 // Keys outside the window go straight to the global histogram.
+//
+// The wavefront sorts its (key, lane) words (key * 64 + lane; key spaces
+// below 2^25, multiples of 256, so no word reaches PK_NONE; slots past the
+// end as PK_NONE) and stores them at the chunk's own slots of pks (those
+// below n), for the slot reservation.
 constexpr int HWIN = 4096;
+constexpr int PK_NONE = 0x7fffffff;
 
 template <typename P>
 __device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, int64_t base, int* count, P x0, P y0,
-                                           P dx, P dy, const KeyMap& m, int* win, int* wbase_s) {
+                                           P dx, P dy, const KeyMap& m, int* win, int* wbase_s, uint32_t* pks) {
   const int lane = threadIdx.x & 63;
   int key[SORT_U];
 #pragma unroll
@@ -128,7 +136,11 @@ __device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, in
   const int wbase = *wbase_s;
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
-    const int ks = wave_sort(key[u]);
+    const int pk = wave_sort(key[u] >= 0 ? key[u] * 64 + lane : PK_NONE);
+    const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+    // a group's slots past the end hold its PK_NONE words (they sort last)
+    if (p < n) pks[p] = (uint32_t)pk;
+    const int ks = pk == PK_NONE ? -1 : pk >> 6;
     const LaneRun r = lane_run(ks);
     if (r.head == lane && ks >= 0) {
       const int loc = ks - wbase;
@@ -144,32 +156,30 @@ __device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, in
   __syncthreads();
 }
 
-// Slots of one chunk: the wavefront sorts its (key, lane) pairs (packed as
-// key * 64 + lane; key spaces below 2^25), so equal keys form one run in lane
-// (= index) order; the run's first lane reserves len consecutive slots with
-// one atomic.
-template <typename P>
-__device__ __forceinline__ void fill_chunk(const P* x, const P* y, int64_t n, int64_t base, const int* start,
-                                           int* fill, uint32_t* perm, P x0, P y0, P dx, P dy, const KeyMap& m) {
+// Slots of one chunk from the histogram's sorted (key, lane) words: equal
+// keys form one run in lane (= source) order; the run's first lane reserves
+// len consecutive slots with one atomic.  Absorbed particles (key K) are not
+// placed.
+__device__ __forceinline__ void fill_chunk(const uint32_t* pks, int64_t n, int64_t base, const int* start, int* fill,
+                                           uint32_t* perm, const KeyMap& m) {
   const int dead = m.K;
   const int lane = threadIdx.x & 63;
-  int key[SORT_U];
+  int pk[SORT_U];
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-    key[u] = p < n ? cell_key(x[pslot<P>(p)], y[pslot<P>(p)], x0, y0, dx, dy, m) : -1;
+    pk[u] = p < n ? (int)pks[p] : PK_NONE;
   }
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
-    const bool place = key[u] >= 0 && key[u] != dead;
-    const int pk = wave_sort(place ? key[u] * 64 + lane : 0x7fffffff);
-    const int ks = pk == 0x7fffffff ? -1 : pk >> 6;
+    const int key = pk[u] == PK_NONE ? -1 : pk[u] >> 6;
+    const int ks = key != dead ? key : -1;
     const LaneRun r = lane_run(ks);
     int b = 0;
     if (r.head == lane && ks >= 0) b = atomicAdd(&fill[ks], r.len);
     b = __shfl(b, r.head, 64);
     if (ks >= 0) {
-      const int64_t p = base + (int64_t)u * blockDim.x + (threadIdx.x & ~63) + (pk & 63);
+      const int64_t p = base + (int64_t)u * blockDim.x + (threadIdx.x & ~63) + (pk[u] & 63);
       perm[start[ks] + b + (lane - r.head)] = (uint32_t)p;
     }
   }
@@ -243,10 +253,70 @@ __device__ __forceinline__ void fix_unordered(uint32_t* perm, uint32_t* scratch,
   }
 }
 
-// FIX_GROUP consecutive cells per wavefront: their FIX_GROUP + 1 starts
-// are read at once and all their slots checked in one strided pass (a
-// descent between two slots of one cell marks the cell), so the dependent
-// loads of the check are shared by the group; marked cells are then sorted.
+// Ranks of one cell of m <= 64 * CNT entries from its runs.  An entry's
+// source lies in the 64-slot group (source / 64) of the wavefront that
+// reserved its slot; a group's entries of one cell are one run of
+// consecutive slots in ascending order (one reservation), and groups are
+// disjoint ranges of sources.  So an entry's rank in the sorted cell is the
+// length of every run of a smaller group plus its index in its own run: the
+// cell is ordered by walking its runs (a handful) instead of comparing every
+// pair of entries.  Entry t of the cell is lane t % 64, register t / 64.
+template <int CNT>
+__device__ __forceinline__ void fix_by_runs(uint32_t* perm, int b, int m) {
+  const int lane = threadIdx.x & 63;
+  uint32_t v[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; q++) {
+    const int t = lane + 64 * q;
+    v[q] = t < m ? perm[b + t] : 0xFFFFFFFFu;
+  }
+  // run heads: the first entry and every entry whose group differs from the
+  // previous entry's
+  unsigned long long H[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; q++) {
+    const int t = lane + 64 * q;
+    uint32_t prev = (uint32_t)__shfl_up((int)v[q], 1, 64);
+    if (lane == 0) prev = q > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)v[q > 0 ? q - 1 : 0], 63) : 0xFFFFFFFFu;
+    H[q] = __builtin_amdgcn_ballot_w64(t < m && (t == 0 || (v[q] >> 6) != (prev >> 6)));
+  }
+  int rank[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; q++) rank[q] = 0;
+  auto close_run = [&](int th, uint32_t g, int end) {
+#pragma unroll
+    for (int q = 0; q < CNT; q++) {
+      const int t = lane + 64 * q;
+      rank[q] += (v[q] >> 6) > g ? end - th : 0;
+      rank[q] += (t >= th && t < end) ? t - th : 0;
+    }
+  };
+  int prev_th = -1;
+  uint32_t prev_g = 0;
+#pragma unroll
+  for (int qh = 0; qh < CNT; qh++) {
+    for (unsigned long long h = H[qh]; h; h &= h - 1) {
+      const int lh = (int)__builtin_ctzll(h);
+      const int th = lh + 64 * qh;
+      const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)v[qh], lh) >> 6;
+      if (prev_th >= 0) close_run(prev_th, prev_g, th);
+      prev_th = th;
+      prev_g = g;
+    }
+  }
+  if (prev_th >= 0) close_run(prev_th, prev_g, m);
+  bool moved = false;
+#pragma unroll
+  for (int q = 0; q < CNT; q++) moved = moved || (lane + 64 * q < m && rank[q] != lane + 64 * q);
+  if (!__builtin_amdgcn_ballot_w64(moved)) return;
+#pragma unroll
+  for (int q = 0; q < CNT; q++)
+    if (lane + 64 * q < m) perm[b + rank[q]] = v[q];
+}
+
+// FIX_GROUP consecutive cells per wavefront, each read once: cells of up to
+// 256 entries are ordered by their runs (fix_by_runs), larger ones are
+// checked for a descent and sorted by rank counting when they have one.
 constexpr int FIX_GROUP = 8;
 
 __device__ __forceinline__ void fix_cells(const int* start, uint32_t* perm, uint32_t* scratch, int c0, int ncell,
@@ -254,22 +324,21 @@ __device__ __forceinline__ void fix_cells(const int* start, uint32_t* perm, uint
   const int lane = threadIdx.x & 63;
   const int nc = min(FIX_GROUP, ncell - c0);
   const int sv = lane <= nc ? start[c0 + lane] : 0;
-  int st[FIX_GROUP + 1];
-#pragma unroll
-  for (int q = 0; q <= FIX_GROUP; q++) st[q] = __builtin_amdgcn_readlane(sv, q);
-  const int b0 = st[0], e0 = st[nc];
-  unsigned mark = 0;
-  for (int t = b0 + lane; t + 1 < e0; t += 64) {
-    if (perm[t] > perm[t + 1]) {
-      // the cell of slot t, when slot t + 1 belongs to it too
-#pragma unroll
-      for (int q = 0; q < FIX_GROUP; q++)
-        if (q < nc && t >= st[q] && t + 1 < st[q + 1]) mark |= 1u << q;
+  for (int q = 0; q < nc; q++) {
+    const int b = __builtin_amdgcn_readlane(sv, q);
+    const int m = __builtin_amdgcn_readlane(sv, q + 1) - b;
+    if (m <= 1) continue;
+    if (m <= 64) {
+      fix_by_runs<1>(perm, b, m);
+    } else if (m <= 128) {
+      fix_by_runs<2>(perm, b, m);
+    } else if (m <= 256) {
+      fix_by_runs<4>(perm, b, m);
+    } else {
+      bool desc = false;
+      for (int t = b + lane; t + 1 < b + m; t += 64) desc = desc || perm[t] > perm[t + 1];
+      if (__builtin_amdgcn_ballot_w64(desc)) fix_unordered(perm, scratch, b, m, buf);
     }
-  }
-#pragma unroll
-  for (int q = 0; q < FIX_GROUP; q++) {
-    if (q < nc && __ballot((mark >> q) & 1u)) fix_unordered(perm, scratch, st[q], st[q + 1] - st[q], buf);
   }
 }
 
