@@ -106,7 +106,7 @@ static void free_all(Ctx* c) {
   void* gs[] = {c->rho, c->phi, c->efx, c->efy, c->E2, c->Ec, c->b64, c->r64, c->ws_v, c->ws_w, c->ws_x,
                 c->ws_part, c->ws_scalar, c->ws_iter, c->sort_count, c->sort_fill, c->sort_perm, c->blk_count,
                 c->red_part, c->red_out, c->errc, c->scan_tiles, c->mcc.sigma, c->mcc.counts, c->mcc.events,
-                c->mcc.event_count, c->mcc.cand, c->mcc.cand_n, c->sp_S, c->sp_T, c->sp_lam,
+                c->mcc.event_count, c->mcc.list, c->mcc.list_n, c->mcc.chunk_ev, c->sp_S, c->sp_T, c->sp_lam,
                 c->cond.table, c->cond.cmask, c->cond.node, c->cond.value, c->nr_ws};
   for (void* p : gs)
     if (p) hipFree(p);
@@ -331,20 +331,22 @@ int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp
   return EPIC_OK;
 }
 
-// What epic_mcc_setup allocates: table, counters, event list, candidate
-// lists (16-bit offsets, capacity rounded up to MCC_CHUNK) and their
-// per-chunk counts.
+// What epic_mcc_setup allocates: table, counters, event list, the
+// real-collision lists (8 bytes per slot of each colliding species, capacity
+// rounded up to MCC_CHUNK) with their two counts, and the electrons'
+// per-chunk event counts.
 // Default ionisation event list: 16384 events, or one per 64 electron slots
 // for large species (a hot 1e8-electron plasma ionises ~1e5 per step).
 static int mcc_event_cap(int event_cap, int64_t len_e) {
-  return event_cap > 0 ? event_cap : (int)std::max<int64_t>(16384, (len_e + 63) / 64);
+  return event_cap > 0 ? event_cap
+                       : (int)std::min<int64_t>(MCC_EVENT_MAX, std::max<int64_t>(16384, (len_e + 63) / 64));
 }
 
 static size_t mcc_bytes_of(int ntab, int event_cap, int64_t len, int64_t len_e) {
   const int cap = mcc_event_cap(event_cap, len_e);
   return (size_t)EPIC_MCC_NPROC * std::max(ntab, 1) * sizeof(double) + (EPIC_MCC_NPROC + 2) * sizeof(uint64_t) +
-         (size_t)cap * 8 * sizeof(double) + sizeof(int) + (size_t)std::max<int64_t>(len, 1) * sizeof(uint16_t) +
-         (size_t)std::max<int64_t>(len / MCC_CHUNK, 1) * sizeof(int);
+         (size_t)cap * 8 * sizeof(double) + sizeof(int) + (size_t)std::max<int64_t>(len, 1) * sizeof(uint2) +
+         2 * sizeof(int) + (size_t)std::max<int64_t>(len_e / MCC_CHUNK, 1) * sizeof(int);
 }
 
 static int64_t mcc_list_len(const epic_species_desc* sp, int s) {
@@ -738,6 +740,7 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   if (!h || !d) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
   if (d->ntab < 2 || !d->sigma || !(d->e_min > 0) || !(d->e_max > d->e_min)) return fail(c, EPIC_ERR_ARG, "bad table");
+  if (d->event_cap > MCC_EVENT_MAX) return fail(c, EPIC_ERR_ARG, "event_cap above 2^28");
   if (d->electron_sp >= c->nsp || d->ion_sp >= c->nsp) return fail(c, EPIC_ERR_ARG, "bad species");
   MccState& M = c->mcc;
   // a repeated setup replaces the table and the event list
@@ -758,27 +761,30 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   M.events = nullptr;
   CK(dalloc_c(c, &M.events, (size_t)M.event_cap * 8), "mcc events");
   if (!M.event_count) CK(dalloc_c(c, &M.event_count, 1), "mcc events");
-  // candidate lists (16-bit offsets) and per-chunk counts of the colliding
-  // species, sized by their capacities
-  auto cand_bytes = [](int64_t len) -> size_t {
-    return (size_t)std::max<int64_t>(len, 1) * sizeof(uint16_t) +
-           (size_t)std::max<int64_t>(len / MCC_CHUNK, 1) * sizeof(int);
+  // real-collision lists of the colliding species, sized by their
+  // capacities, their counts and the electrons' per-chunk event counts
+  auto list_bytes = [](int64_t len, int64_t len_e) -> size_t {
+    return (size_t)std::max<int64_t>(len, 1) * sizeof(uint2) + 2 * sizeof(int) +
+           (size_t)std::max<int64_t>(len_e / MCC_CHUNK, 1) * sizeof(int);
   };
-  if (M.cand) {
-    c->bytes -= cand_bytes(M.cand_len_e + M.cand_len_i);
-    hipFree(M.cand);
-    hipFree(M.cand_n);
-    M.cand = nullptr;
-    M.cand_n = nullptr;
+  if (M.list) {
+    c->bytes -= list_bytes(M.list_len_e + M.list_len_i, M.list_len_e);
+    hipFree(M.list);
+    hipFree(M.list_n);
+    hipFree(M.chunk_ev);
+    M.list = nullptr;
+    M.list_n = nullptr;
+    M.chunk_ev = nullptr;
   }
   auto list_len = [&](int s) -> int64_t {
     return s >= 0 ? (c->sp[s].cap + MCC_CHUNK - 1) / MCC_CHUNK * MCC_CHUNK : 0;
   };
-  M.cand_len_e = list_len(d->electron_sp);
-  M.cand_len_i = list_len(d->ion_sp);
-  const int64_t len = M.cand_len_e + M.cand_len_i;
-  CK(dalloc_c(c, &M.cand, (size_t)len), "mcc candidates");
-  CK(dalloc_c(c, &M.cand_n, (size_t)(len / MCC_CHUNK)), "mcc candidates");
+  M.list_len_e = list_len(d->electron_sp);
+  M.list_len_i = list_len(d->ion_sp);
+  const int64_t len = M.list_len_e + M.list_len_i;
+  CK(dalloc_c(c, &M.list, (size_t)len), "mcc collision lists");
+  CK(dalloc_c(c, &M.list_n, 2), "mcc collision lists");
+  CK(dalloc_c(c, &M.chunk_ev, (size_t)(M.list_len_e / MCC_CHUNK)), "mcc collision lists");
   M.nu_max_e = d->electron_sp >= 0 ? mcc_nu_max(M.d, sig, ME, true) : 0.0;
   M.nu_max_i = d->ion_sp >= 0 ? mcc_nu_max(M.d, sig, c->sp[d->ion_sp].desc.mass, false) : 0.0;
   M.on = true;

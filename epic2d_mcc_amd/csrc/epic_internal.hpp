@@ -31,6 +31,7 @@ constexpr int FIX_BITS_FP32 = 28;
 // LDS deposit window: 8192 int64 node accumulators (64 KiB) per workgroup.
 constexpr int DEP_WINDOW = 8192;
 constexpr int MCC_CHUNK = 2048;  // particle slots per MCC selection chunk
+constexpr int MCC_EVENT_MAX = 1 << 28;  // ionisation event list entries at most
 constexpr int PART_BLOCK = 256;  // threads per workgroup in particle kernels
 
 // FP32 particle storage is tiled: the x, y, vx, vy of 256 consecutive slots
@@ -84,11 +85,14 @@ struct MccState {
   double* events = nullptr;              // device event_cap * 8
   int* event_count = nullptr;            // device
   int event_cap = 0;
-  // candidate lists of the two-pass null-collision step: electron list
-  // (cand_len_e offsets) then ion list, and a count per MCC_CHUNK slots
-  uint16_t* cand = nullptr;
-  int* cand_n = nullptr;
-  int64_t cand_len_e = 0, cand_len_i = 0;
+  // real-collision lists of the two-pass null-collision step (pass 1 appends
+  // (slot, process | (event slot + 1) << 3), pass 2 reads): electron list
+  // (list_len_e entries) then ion list; their two counts; the electrons'
+  // stored ionisation events per MCC_CHUNK slots
+  uint2* list = nullptr;
+  int* list_n = nullptr;
+  int* chunk_ev = nullptr;
+  int64_t list_len_e = 0, list_len_i = 0;
 };
 
 // One absorbing shape on the device: a disk (one electrode, a, b centre and

@@ -69,8 +69,9 @@ struct MccArgs {
   int* event_count;
   int event_cap;
   int slot;  // batched runs: index of the simulation's step counter
-  uint16_t* cand;  // candidate offsets, MCC_CHUNK per selection chunk
-  int* cand_n;     // candidates per selection chunk
+  uint2* list;     // real collisions: (slot, process | (event slot + 1) << 3)
+  int* list_n;     // entries in list (cleared by the finish kernel)
+  int* chunk_ev;   // electrons: stored ionisation events per MCC_CHUNK slots
   unsigned long long* errc;  // EPIC_ERRC_MCC_BOUND: collision probability above p_null
 };
 
@@ -89,152 +90,192 @@ __device__ __forceinline__ double gauss(uint32_t a, uint32_t b, uint32_t c, uint
   return m * cos(2.0 * M_PI * u2);
 }
 
-// One candidate of the null-collision method (R0 < P_null already drawn):
-// the real/null decision, the process and the kinematics.  Returns the
-// index of the ionisation event it stored, or -1.
+// The neutral partner of an ion collision: a Maxwellian at gas_temp_K from
+// the particle's (id, step) streams 2 and 3.
 template <typename P>
-__device__ __forceinline__ int mcc_candidate(const MccArgs<P>& a, uint32_t step, int64_t p,
-                                             unsigned long long (&cnt)[EPIC_MCC_NPROC]) {
-  int stored = -1;
-  do {
-    const uint32_t pid = a.id[p];
-    const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
-    // x, y, vx, vy at slot q (tiled FP32 storage), vz and id at p
-    const int64_t q = pslot<P>(p);
-    const P xp = a.x[q];
-    if (isnan(xp)) continue;
-    double vxe = a.vx[q], vye = a.vy[q], vze = a.vz[p];
-    const double R1 = u01_53(r.c, r.d);
+__device__ __forceinline__ void neutral_velocity(const MccArgs<P>& a, uint32_t pid, uint32_t step, double& vnx,
+                                                 double& vny, double& vnz) {
+  const u32x4 r2 = philox4x32_10({pid, step, RNG_MCC2, a.sp}, a.k0, a.k1);
+  const u32x4 r3 = philox4x32_10({pid, step, RNG_MCC3, a.sp}, a.k0, a.k1);
+  double g2nd, g3nd;
+  vnx = a.gas_vth * gauss(r2.a, r2.b, r2.c, r2.d, &g2nd);
+  vny = a.gas_vth * g2nd;
+  vnz = a.gas_vth * gauss(r3.a, r3.b, r3.c, r3.d, &g3nd);
+}
+
+// One candidate of the null-collision method (R0 < P_null already drawn):
+// the real/null decision and, for a real collision, its process (0-2
+// electron-neutral elastic / excitation / ionisation, 3-4 ion-neutral
+// elastic / charge exchange).  Returns -1 for a null collision or an
+// absorbed slot.
+template <typename P>
+__device__ __forceinline__ int mcc_null_test(const MccArgs<P>& a, uint32_t step, int64_t p) {
+  // x, y, vx, vy at slot q (tiled FP32 storage), vz and id at p; every load
+  // is issued before the first test (p is a live slot, so all are in bounds)
+  const int64_t q = pslot<P>(p);
+  const uint32_t pid = a.id[p];
+  const P xp = a.x[q];
+  const double vxe = a.vx[q], vye = a.vy[q], vze = a.vz[p];
+  const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
+  if (isnan(xp)) return -1;
+  const double R1 = u01_53(r.c, r.d);
+  if (a.kind == 0) {
+    // electron - neutral (neutral at rest)
+    const double v_sqr = (vxe * vxe + vye * vye + vze * vze) * a.vel_norm * a.vel_norm;
+    const double velocity = sqrt(v_sqr);
+    const double energy = (0.5 * ME * v_sqr) / eV;
+    double s_el = (a.enable & 1) ? a.tab.at(0, energy) : 0.0;
+    double s_ex = (a.enable & 2) && energy > a.e_exc ? a.tab.at(1, energy) : 0.0;
+    double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, energy) : 0.0;
+    const double t0 = s_el, t1 = t0 + s_ex, t2 = t1 + s_io;
+    const double nu = a.gas_density * t2 * velocity;
+    const double pc = -expm1(-nu * a.dt);
+    // a speed beyond the one nu_max covers: the particle collides with
+    // p_null instead of pc (counted, as an undercount)
+    if (pc > a.p_null) atomicAdd(&a.errc[EPIC_ERRC_MCC_BOUND], 1ull);
+    if (R1 * a.p_null >= pc) return -1;  // null collision
     const u32x4 r2 = philox4x32_10({pid, step, RNG_MCC2, a.sp}, a.k0, a.k1);
+    if (a.ref_selection) {
+      // two independent draws (collision.cpp:117,125)
+      if (u01_53(r2.a, r2.b) < (t0 / t2)) return 0;
+      if (u01_53(r2.c, r2.d) < (t1 / t2)) return 1;
+      return 2;
+    }
+    const double R = u01_53(r2.a, r2.b) * t2;
+    return R < t0 ? 0 : (R < t1 ? 1 : 2);
+  }
+  // ion - neutral against a Maxwellian neutral
+  double vnx, vny, vnz;
+  neutral_velocity(a, pid, step, vnx, vny, vnz);
+  const double gx = vxe - vnx, gy = vye - vny, gz = vze - vnz;
+  const double g2 = gx * gx + gy * gy + gz * gz;
+  const double g = sqrt(g2);
+  const double energy = 0.5 * a.mass * g2 * a.vel_norm * a.vel_norm / eV;
+  const double s_el = (a.enable & 8) ? a.tab.at(3, energy) : 0.0;
+  const double s_cx = (a.enable & 16) ? a.tab.at(4, energy) : 0.0;
+  const double nu_el = a.gas_density * s_el * g * a.vel_norm;
+  const double nu_cx = a.gas_density * s_cx * g * a.vel_norm;
+  const double nu = nu_el + nu_cx;
+  const double pc = -expm1(-nu * a.dt);
+  if (pc > a.p_null) atomicAdd(&a.errc[EPIC_ERRC_MCC_BOUND], 1ull);
+  if (R1 * a.p_null >= pc) return -1;  // null collision
+  // real collision: process by frequency share
+  const double R = (R1 * a.p_null / pc) * nu;
+  return R < nu_el ? 3 : 4;
+}
+
+// The kinematics of one real collision (slot p, process from pass 1).  An
+// ionisation with ev >= 0 fills event ev (its rank in the chunk, ev[7], is
+// pass 1's).
+template <typename P>
+__device__ __forceinline__ void mcc_collide(const MccArgs<P>& a, uint32_t step, int64_t p, int type, int evi,
+                                            unsigned long long (&cnt)[EPIC_MCC_NPROC]) {
+  const int64_t q = pslot<P>(p);
+  const uint32_t pid = a.id[p];
+  const double vxe = a.vx[q], vye = a.vy[q], vze = a.vz[p];
+  if (a.kind == 0) {
     const u32x4 r3 = philox4x32_10({pid, step, RNG_MCC3, a.sp}, a.k0, a.k1);
-    if (a.kind == 0) {
-      // electron - neutral (neutral at rest)
-      const double v_sqr = (vxe * vxe + vye * vye + vze * vze) * a.vel_norm * a.vel_norm;
-      const double velocity = sqrt(v_sqr);
-      const double energy = (0.5 * ME * v_sqr) / eV;
-      double s_el = (a.enable & 1) ? a.tab.at(0, energy) : 0.0;
-      double s_ex = (a.enable & 2) && energy > a.e_exc ? a.tab.at(1, energy) : 0.0;
-      double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, energy) : 0.0;
-      const double t0 = s_el, t1 = t0 + s_ex, t2 = t1 + s_io;
-      const double nu = a.gas_density * t2 * velocity;
-      const double pc = -expm1(-nu * a.dt);
-      // a speed beyond the one nu_max covers: the particle collides with
-      // p_null instead of pc (counted, as an undercount)
-      if (pc > a.p_null) atomicAdd(&a.errc[EPIC_ERRC_MCC_BOUND], 1ull);
-      if (R1 * a.p_null >= pc) continue;  // null collision
-      int type;
-      if (a.ref_selection) {
-        // two independent draws (collision.cpp:117,125)
-        if (u01_53(r2.a, r2.b) < (t0 / t2)) type = 0;
-        else if (u01_53(r2.c, r2.d) < (t1 / t2)) type = 1;
-        else type = 2;
-      } else {
-        const double R = u01_53(r2.a, r2.b) * t2;
-        type = R < t0 ? 0 : (R < t1 ? 1 : 2);
-      }
-      const double F1 = ME / (ME + a.gas_mass), F2 = a.gas_mass / (ME + a.gas_mass);
-      double gx = vxe, gy = vye, gz = vze;
-      double g = sqrt(gx * gx + gy * gy + gz * gz);
-      const double wx = F1 * vxe, wy = F1 * vye, wz = F1 * vze;
-      const double theta = (gx == 0) ? 0.5 * PI : atan2(sqrt(gy * gy + gz * gz), gx);
-      const double phi = (gy == 0) ? (gz > 0 ? 0.5 * PI : -0.5 * PI) : atan2(gz, gy);
-      const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
-      double chi, eta;
-      const double Ra = u01_53(r3.a, r3.b), Rb = u01_53(r3.c, r3.d);
-      if (type == 0) {
-        chi = acos(1.0 - 2.0 * Ra);
-        eta = 2 * PI * Rb;
-      } else if (type == 1) {
-        double en = 0.5 * ME * g * g * a.vel_norm * a.vel_norm;
-        en = fabs(en - a.e_exc * eV);
-        g = sqrt(2.0 * en / ME) / a.vel_norm;
-        chi = acos(1.0 - 2.0 * Ra);
-        eta = 2 * PI * Rb;
-      } else {
-        double en = 0.5 * ME * g * g * a.vel_norm * a.vel_norm;
-        en = fabs(en - a.e_ion * eV);
-        const double e_ej = 10.0 * tan(Ra * atan(en / eV / 20.0)) * eV;
-        const double e_sc = fabs(en - e_ej);
-        g = sqrt(2.0 * e_sc / ME) / a.vel_norm;
-        const double g2 = sqrt(2.0 * e_ej / ME) / a.vel_norm;
-        chi = acos(sqrt(e_sc / en));
-        const double chi2 = acos(sqrt(e_ej / en));
-        eta = 2 * PI * Rb;
-        const double eta2 = eta + PI;
-        double hx, hy, hz;
-        rotate(g2, chi2, eta2, st, ct, sp, cp, hx, hy, hz);
-        const int e = atomicAdd(a.event_count, 1);
-        if (e < a.event_cap) {
-          double* ev = a.events + (size_t)e * EVW;
-          ev[0] = (double)p;
-          ev[1] = (double)xp;
-          ev[2] = (double)a.y[q];
-          ev[3] = wx + F2 * hx;
-          ev[4] = wy + F2 * hy;
-          ev[5] = wz + F2 * hz;
-          ev[6] = (double)pid;
-          stored = e;  // ev[7]: the event's rank within its chunk, set by the caller
-        }
-      }
-      double ngx, ngy, ngz;
-      rotate(g, chi, eta, st, ct, sp, cp, ngx, ngy, ngz);
-      a.vx[q] = (P)(wx + F2 * ngx);
-      a.vy[q] = (P)(wy + F2 * ngy);
-      a.vz[p] = (P)(wz + F2 * ngz);
-      cnt[type]++;
+    const double F1 = ME / (ME + a.gas_mass), F2 = a.gas_mass / (ME + a.gas_mass);
+    double gx = vxe, gy = vye, gz = vze;
+    double g = sqrt(gx * gx + gy * gy + gz * gz);
+    const double wx = F1 * vxe, wy = F1 * vye, wz = F1 * vze;
+    const double theta = (gx == 0) ? 0.5 * PI : atan2(sqrt(gy * gy + gz * gz), gx);
+    const double phi = (gy == 0) ? (gz > 0 ? 0.5 * PI : -0.5 * PI) : atan2(gz, gy);
+    const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+    double chi, eta;
+    const double Ra = u01_53(r3.a, r3.b), Rb = u01_53(r3.c, r3.d);
+    if (type == 0) {
+      chi = acos(1.0 - 2.0 * Ra);
+      eta = 2 * PI * Rb;
+    } else if (type == 1) {
+      double en = 0.5 * ME * g * g * a.vel_norm * a.vel_norm;
+      en = fabs(en - a.e_exc * eV);
+      g = sqrt(2.0 * en / ME) / a.vel_norm;
+      chi = acos(1.0 - 2.0 * Ra);
+      eta = 2 * PI * Rb;
     } else {
-      // ion - neutral against a Maxwellian neutral
-      double g2nd, g3nd;
-      const double vnx = a.gas_vth * gauss(r2.a, r2.b, r2.c, r2.d, &g2nd);
-      const double vny = a.gas_vth * g2nd;
-      const double vnz = a.gas_vth * gauss(r3.a, r3.b, r3.c, r3.d, &g3nd);
-      const double gx = vxe - vnx, gy = vye - vny, gz = vze - vnz;
-      const double g2 = gx * gx + gy * gy + gz * gz;
-      const double g = sqrt(g2);
-      const double energy = 0.5 * a.mass * g2 * a.vel_norm * a.vel_norm / eV;
-      const double s_el = (a.enable & 8) ? a.tab.at(3, energy) : 0.0;
-      const double s_cx = (a.enable & 16) ? a.tab.at(4, energy) : 0.0;
-      const double nu_el = a.gas_density * s_el * g * a.vel_norm;
-      const double nu_cx = a.gas_density * s_cx * g * a.vel_norm;
-      const double nu = nu_el + nu_cx;
-      const double pc = -expm1(-nu * a.dt);
-      if (pc > a.p_null) atomicAdd(&a.errc[EPIC_ERRC_MCC_BOUND], 1ull);
-      if (R1 * a.p_null >= pc) continue;  // null collision
-      // real collision: process by frequency share
-      const double R = (R1 * a.p_null / pc) * nu;
-      if (R < nu_el) {
-        const u32x4 r4 = philox4x32_10({pid, step, RNG_MCC_ION_SCATTER, a.sp}, a.k0, a.k1);
-        const double M = a.gas_mass, m = a.mass;
-        const double cx = (m * vxe + M * vnx) / (m + M), cy = (m * vye + M * vny) / (m + M),
-                     cz = (m * vze + M * vnz) / (m + M);
-        const double ct = 1.0 - 2.0 * u01_53(r4.a, r4.b), stt = sqrt(fmax(0.0, 1.0 - ct * ct));
-        const double ph = 2.0 * M_PI * u01_53(r4.c, r4.d);
-        const double f = M / (m + M);
-        a.vx[q] = (P)(cx + f * g * stt * cos(ph));
-        a.vy[q] = (P)(cy + f * g * stt * sin(ph));
-        a.vz[p] = (P)(cz + f * g * ct);
-        cnt[3]++;
-      } else {
-        a.vx[q] = (P)vnx;
-        a.vy[q] = (P)vny;
-        a.vz[p] = (P)vnz;
-        cnt[4]++;
+      double en = 0.5 * ME * g * g * a.vel_norm * a.vel_norm;
+      en = fabs(en - a.e_ion * eV);
+      const double e_ej = 10.0 * tan(Ra * atan(en / eV / 20.0)) * eV;
+      const double e_sc = fabs(en - e_ej);
+      g = sqrt(2.0 * e_sc / ME) / a.vel_norm;
+      const double g2 = sqrt(2.0 * e_ej / ME) / a.vel_norm;
+      chi = acos(sqrt(e_sc / en));
+      const double chi2 = acos(sqrt(e_ej / en));
+      eta = 2 * PI * Rb;
+      const double eta2 = eta + PI;
+      double hx, hy, hz;
+      rotate(g2, chi2, eta2, st, ct, sp, cp, hx, hy, hz);
+      if (evi >= 0) {
+        double* ev = a.events + (size_t)evi * EVW;
+        ev[0] = (double)p;
+        ev[1] = (double)a.x[q];
+        ev[2] = (double)a.y[q];
+        ev[3] = wx + F2 * hx;
+        ev[4] = wy + F2 * hy;
+        ev[5] = wz + F2 * hz;
+        ev[6] = (double)pid;
       }
     }
-  } while (false);
-  return stored;
+    double ngx, ngy, ngz;
+    rotate(g, chi, eta, st, ct, sp, cp, ngx, ngy, ngz);
+    a.vx[q] = (P)(wx + F2 * ngx);
+    a.vy[q] = (P)(wy + F2 * ngy);
+    a.vz[p] = (P)(wz + F2 * ngz);
+  } else {
+    double vnx, vny, vnz;
+    neutral_velocity(a, pid, step, vnx, vny, vnz);
+    if (type == 3) {
+      const double gx = vxe - vnx, gy = vye - vny, gz = vze - vnz;
+      const double g = sqrt(gx * gx + gy * gy + gz * gz);
+      const u32x4 r4 = philox4x32_10({pid, step, RNG_MCC_ION_SCATTER, a.sp}, a.k0, a.k1);
+      const double M = a.gas_mass, m = a.mass;
+      const double cx = (m * vxe + M * vnx) / (m + M), cy = (m * vye + M * vny) / (m + M),
+                   cz = (m * vze + M * vnz) / (m + M);
+      const double ct = 1.0 - 2.0 * u01_53(r4.a, r4.b), stt = sqrt(fmax(0.0, 1.0 - ct * ct));
+      const double ph = 2.0 * M_PI * u01_53(r4.c, r4.d);
+      const double f = M / (m + M);
+      a.vx[q] = (P)(cx + f * g * stt * cos(ph));
+      a.vy[q] = (P)(cy + f * g * stt * sin(ph));
+      a.vz[p] = (P)(cz + f * g * ct);
+    } else {
+      a.vx[q] = (P)vnx;
+      a.vy[q] = (P)vny;
+      a.vz[p] = (P)vnz;
+    }
+  }
+  cnt[type]++;
 }
 
 // A selection chunk (MCC_CHUNK consecutive slots) is one wavefront's work
-// in both passes.
+// in pass 1; a real collision's list entry packs its process (3 bits) and
+// its event slot + 1 (29 bits).
 static_assert(MCC_CHUNK <= 65536, "candidate offsets are 16-bit");
+static_assert(MCC_EVENT_MAX < (1 << 29) - 1, "event slot + 1 in 29 bits");
+
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Pass 1 of a colliding species: every particle draws R0 from its (id, step)
-// stream; those with R0 < P_null (a fraction ~nu_max dt) are candidates.
-// Chunk c's candidate offsets go to cand[c*MCC_CHUNK + 0 .. cand_n[c]), in
-// slot order.  Wavefront w of the grid takes chunks w, w + W, ...
+// stream; those with R0 < P_null (a fraction ~nu_max dt) are candidates,
+// listed in LDS (cand: MCC_CHUNK offsets of this wavefront) in slot order.
+// The wavefront then runs the null test on its chunk's candidates, 64 at a
+// time, and appends the real collisions to the species' list (one atomic per
+// 64 candidates).  Electron ionisations reserve their event slots here, in
+// slot order within the chunk, so each stored event gets its rank among the
+// chunk's stored events (ev[7]) and chunk_ev[c] their count: the birth
+// append orders births by (chunk, rank), i.e. by parent slot, whatever order
+// the list has.  Wavefront w of the grid takes chunks w, w + W, ...
 template <typename P>
-__device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
+__device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk,
+                                                uint16_t* cand) {
   const int64_t n = a.d_n[0];
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
@@ -252,53 +293,61 @@ __device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t st
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int off = (k0 + j) * 64 + lane;
-        bool cand = false;
+        bool is_cand = false;
         if (base + off < n) {
           const u32x4 r = philox4x32_10({pid[j], step, RNG_MCC, a.sp}, a.k0, a.k1);
-          cand = u01_53(r.a, r.b) < a.p_null;
+          is_cand = u01_53(r.a, r.b) < a.p_null;
         }
-        const unsigned long long m = __ballot(cand);
-        const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-        if (cand) a.cand[base + cnt + rank] = (uint16_t)off;
+        const unsigned long long m = __ballot(is_cand);
+        if (is_cand) cand[cnt + lane_rank(m)] = (uint16_t)off;
         cnt += (int)__popcll(m);
       }
     }
-    if (lane == 0) a.cand_n[c] = cnt;
+    wave_lds_sync();
+    int nstored = 0;
+    for (int t0 = 0; t0 < cnt; t0 += 64) {
+      const int t = t0 + lane;
+      const int64_t p = base + (t < cnt ? (int)cand[t] : 0);
+      const int type = t < cnt ? mcc_null_test(a, step, p) : -1;
+      const unsigned long long real = __ballot(type >= 0);
+      if (!real) continue;
+      int evi = -1;
+      if (a.kind == 0) {
+        const unsigned long long ion = __ballot(type == 2);
+        if (ion) {
+          const int first = __builtin_ctzll(ion), k = (int)__popcll(ion);
+          int e0 = 0;
+          if (lane == first) e0 = atomicAdd(a.event_count, k);
+          e0 = __shfl(e0, first, 64);
+          const int i = lane_rank(ion);
+          if (type == 2 && e0 + i < a.event_cap) {
+            evi = e0 + i;
+            a.events[(size_t)evi * EVW + 7] = (double)(nstored + i);
+          }
+          nstored += max(0, min(k, a.event_cap - e0));
+        }
+      }
+      const int first = __builtin_ctzll(real);
+      int b0 = 0;
+      if (lane == first) b0 = atomicAdd(a.list_n, (int)__popcll(real));
+      b0 = __shfl(b0, first, 64);
+      if (type >= 0) a.list[b0 + lane_rank(real)] = make_uint2((uint32_t)p, (uint32_t)type | ((uint32_t)(evi + 1) << 3));
+    }
+    if (a.kind == 0 && lane == 0) a.chunk_ev[c] = nstored;
+    wave_lds_sync();  // the next chunk overwrites the candidate list
   }
 }
 
-// Pass 2: the candidates, one per lane, so the collision arithmetic runs on
-// full wavefronts instead of on the few lanes of each wavefront that drew a
-// candidate.  Each particle's draws are keyed by (id, step): the order of
-// processing changes nothing.  Electrons: a chunk's candidates are taken in
-// slot order, so each stored ionisation event gets its rank among the
-// chunk's events (ev[7]), and the chunk's event count replaces its
-// candidate count in cand_n[c] (read at the start, not needed after); the
-// birth append orders the births by (chunk, rank), i.e. by parent slot.
+// Pass 2: the kinematics of the species' real collisions, one per lane over
+// the list pass 1 built.  Each particle's draws are keyed by (id, step): the
+// list's order changes nothing.
 template <typename P>
 __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk) {
-  const int64_t n = a.d_n[0];
-  const int lane = threadIdx.x & 63;
-  const int wpb = blockDim.x >> 6;
-  const int64_t nch = (n + MCC_CHUNK - 1) / MCC_CHUNK;
+  const int m = *a.list_n;
   unsigned long long cnt[EPIC_MCC_NPROC] = {0, 0, 0, 0, 0};
-  for (int64_t c = (int64_t)bid * wpb + (threadIdx.x >> 6); c < nch; c += (int64_t)nblk * wpb) {
-    const int64_t base = c * MCC_CHUNK;
-    const int nc = a.cand_n[c];
-    int nev = 0;
-    for (int t0 = 0; t0 < nc; t0 += 64) {
-      const int t = t0 + lane;
-      const int e = t < nc ? mcc_candidate(a, step, base + a.cand[base + t], cnt) : -1;
-      if (a.kind == 0) {
-        const unsigned long long m = __ballot(e >= 0);
-        if (e >= 0) {
-          const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-          a.events[(size_t)e * EVW + 7] = (double)(nev + r);
-        }
-        nev += (int)__popcll(m);
-      }
-    }
-    if (a.kind == 0 && lane == 0) a.cand_n[c] = nev;
+  for (int k = bid * (int)blockDim.x + (int)threadIdx.x; k < m; k += nblk * (int)blockDim.x) {
+    const uint2 e = a.list[k];
+    mcc_collide(a, step, (int64_t)e.x, (int)(e.y & 7u), (int)(e.y >> 3) - 1, cnt);
   }
 #pragma unroll
   for (int q = 0; q < EPIC_MCC_NPROC; q++) {
@@ -309,14 +358,24 @@ __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int
   }
 }
 
+// A context's colliding species (electrons, ions) in one launch per pass:
+// blockIdx.y selects the species.
 template <typename P>
-__global__ __launch_bounds__(MCC_BLOCK) void mcc_select_kernel(MccArgs<P> a) {
-  mcc_select_body(a, a.step, blockIdx.x, gridDim.x);
+struct MccPair {
+  MccArgs<P> a[2];
+};
+
+template <typename P>
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_select_kernel(MccPair<P> s) {
+  __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_CHUNK];
+  const MccArgs<P>& a = s.a[blockIdx.y];
+  mcc_select_body(a, a.step, blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
 }
 
 // OCC: workgroups per CU the register allocation is held to (1: none)
 template <typename P, int OCC>
-__global__ __launch_bounds__(MCC_BLOCK, OCC) void mcc_kernel(MccArgs<P> a) {
+__global__ __launch_bounds__(MCC_BLOCK, OCC) void mcc_kernel(MccPair<P> s) {
+  const MccArgs<P>& a = s.a[blockIdx.y];
   mcc_body(a, a.step, blockIdx.x, gridDim.x);
 }
 
@@ -324,8 +383,9 @@ __global__ __launch_bounds__(MCC_BLOCK, OCC) void mcc_kernel(MccArgs<P> a) {
 // from the simulation's device step counter.
 template <typename P>
 __global__ __launch_bounds__(MCC_BLOCK) void mcc_select_batch_kernel(const MccArgs<P>* args, const int* steps) {
+  __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_CHUNK];
   const MccArgs<P>& a = args[blockIdx.y];
-  mcc_select_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x);
+  mcc_select_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
 }
 
 template <typename P>
@@ -366,10 +426,12 @@ struct AppendArgs {
   unsigned long long* errc;
   int slot;  // batched runs: index of the simulation's step counter
   int* chunk_n;  // per selection chunk: its ionisation events (scanned in place)
+  int births;    // the electrons ionise: append their events
+  int* list_n;   // the context's real-collision list counts (electrons, ions)
 };
 
 template <typename P>
-__device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t step, int* scan) {
+__device__ __forceinline__ void mcc_births(const AppendArgs<P>& A, uint32_t step, int* scan) {
   const double* events = A.events;
   int* chunk_n = A.chunk_n;
   int* event_count = A.event_count;
@@ -478,10 +540,22 @@ __device__ __forceinline__ void mcc_append_body(const AppendArgs<P>& A, uint32_t
   if (dropped) atomicAdd(&errc[EPIC_ERRC_CAPACITY], (unsigned long long)dropped);
 }
 
+// The call's last MCC kernel, one workgroup per context: the births when the
+// electrons ionise, then the real-collision list counts are cleared for the
+// next call (the collision pass before it has read them).
+template <typename P>
+__device__ __forceinline__ void mcc_finish_body(const AppendArgs<P>& A, uint32_t step, int* scan) {
+  if (A.births) mcc_births(A, step, scan);
+  if (threadIdx.x == 0) {
+    A.list_n[0] = 0;
+    A.list_n[1] = 0;
+  }
+}
+
 template <typename P>
 __global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_kernel(AppendArgs<P> A) {
   __shared__ int scan[APPEND_BLOCK];
-  mcc_append_body(A, A.step, scan);
+  mcc_finish_body(A, A.step, scan);
 }
 
 // Batched: one workgroup per simulation with ionisation.
@@ -489,7 +563,7 @@ template <typename P>
 __global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_batch_kernel(const AppendArgs<P>* args, const int* steps) {
   __shared__ int scan[APPEND_BLOCK];
   const AppendArgs<P>& A = args[blockIdx.x];
-  mcc_append_body(A, (uint32_t)steps[A.slot], scan);
+  mcc_finish_body(A, (uint32_t)steps[A.slot], scan);
 }
 
 // Arguments of one species' collision pass (kind 0 electron, 1 ion).
@@ -529,30 +603,34 @@ static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   a.events = M.events;
   a.event_count = M.event_count;
   a.event_cap = M.event_cap;
-  a.cand = kind == 0 ? M.cand : M.cand + M.cand_len_e;
-  a.cand_n = kind == 0 ? M.cand_n : M.cand_n + M.cand_len_e / MCC_CHUNK;
+  a.list = kind == 0 ? M.list : M.list + M.list_len_e;
+  a.list_n = M.list_n + kind;
+  a.chunk_ev = kind == 0 ? M.chunk_ev : nullptr;
   a.errc = c->errc;
   return a;
 }
 
 // Arguments of the ionisation-birth append (electron species -> ion species).
 template <typename P>
-static AppendArgs<P> append_args(Ctx* c, int step) {
+static AppendArgs<P> append_args(Ctx* c, int step, bool births) {
   MccState& M = c->mcc;
   const epic_domain_desc& d = c->dom;
+  AppendArgs<P> A{};
+  A.step = (uint32_t)step;
+  A.list_n = M.list_n;
+  A.births = births ? 1 : 0;
+  if (!births) return A;
   SpeciesDev& S = c->sp[M.d.electron_sp];
   SpeciesDev& I = c->sp[M.d.ion_sp];
-  AppendArgs<P> A{};
   A.events = M.events;
   A.event_count = M.event_count;
   A.cap_events = M.event_cap;
-  A.chunk_n = M.cand_n;  // the electrons' per-chunk event counts (see mcc_body)
+  A.chunk_n = M.chunk_ev;  // the electrons' per-chunk event counts (pass 1)
   A.ex = (P*)S.x; A.ey = (P*)S.y; A.evx = (P*)S.vx; A.evy = (P*)S.vy; A.evz = (P*)S.vz;
   A.eid = S.id; A.e_n = S.d_n; A.e_cap = S.cap;
   A.ix = (P*)I.x; A.iy = (P*)I.y; A.ivx = (P*)I.vx; A.ivy = (P*)I.vy; A.ivz = (P*)I.vz;
   A.iid = I.id; A.i_n = I.d_n; A.i_cap = I.cap;
   A.ion_vth = sqrt(I.desc.temp * QE / I.desc.mass) / d.vel_norm;
-  A.step = (uint32_t)step;
   A.k0 = (uint32_t)d.seed;
   A.k1 = (uint32_t)(d.seed >> 32);
   A.Qe = S.Q;
@@ -565,27 +643,46 @@ static AppendArgs<P> append_args(Ctx* c, int step) {
   return A;
 }
 
+// One context's collision step: pass 1 of both colliding species in one
+// launch, pass 2 of both in one launch, then the finish kernel (births,
+// list counts cleared).
 template <typename P>
-static hipError_t mcc_species(Ctx* c, int s, int kind, int step) {
-  const MccArgs<P> a = mcc_args<P>(c, s, kind, step);
-  if (a.nu_max <= 0) return hipSuccess;
-  // one wavefront per selection chunk in pass 1; pass 2 runs at the
-  // occupancy of the collision arithmetic
-  const int64_t nch = (c->sp[s].cap + MCC_CHUNK - 1) / MCC_CHUNK;
+static hipError_t mcc_launch(Ctx* c, int step) {
+  const epic_mcc_desc& d = c->mcc.d;
+  MccPair<P> s{};
+  int np = 0;
+  int64_t nch = 0;
+  bool births = false;
+  if (d.electron_sp >= 0) {
+    const MccArgs<P> a = mcc_args<P>(c, d.electron_sp, 0, step);
+    if (a.nu_max > 0) {
+      s.a[np++] = a;
+      nch = std::max<int64_t>(nch, (c->sp[d.electron_sp].cap + MCC_CHUNK - 1) / MCC_CHUNK);
+      births = (a.enable & 4) != 0;
+    }
+  }
+  if (d.ion_sp >= 0 && (d.enable[EPIC_MCC_I_ELASTIC] || d.enable[EPIC_MCC_I_CX])) {
+    const MccArgs<P> a = mcc_args<P>(c, d.ion_sp, 1, step);
+    if (a.nu_max > 0) {
+      s.a[np++] = a;
+      nch = std::max<int64_t>(nch, (c->sp[d.ion_sp].cap + MCC_CHUNK - 1) / MCC_CHUNK);
+    }
+  }
+  if (np == 0) return hipSuccess;
+  // one wavefront per selection chunk in pass 1; pass 2 at the occupancy of
+  // the collision arithmetic, every workgroup resident
   const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(c->num_cu * 16, (nch + 3) / 4));
-  hipLaunchKernelGGL(mcc_select_kernel<P>, dim3(g1), dim3(MCC_BLOCK), 0, c->stream, a);
+  hipLaunchKernelGGL(mcc_select_kernel<P>, dim3(g1, np), dim3(MCC_BLOCK), 0, c->stream, s);
   EPIC_HIP_TRY(hipGetLastError());
   if (c->mcc_occ == 1)
-    hipLaunchKernelGGL((mcc_kernel<P, 1>), dim3(c->num_cu * 4), dim3(MCC_BLOCK), 0, c->stream, a);
+    hipLaunchKernelGGL((mcc_kernel<P, 1>), dim3(c->num_cu, np), dim3(MCC_BLOCK), 0, c->stream, s);
   else
-    hipLaunchKernelGGL((mcc_kernel<P, 2>), dim3(c->num_cu * 8), dim3(MCC_BLOCK), 0, c->stream, a);
+    hipLaunchKernelGGL((mcc_kernel<P, 2>), dim3(c->num_cu * 2, np), dim3(MCC_BLOCK), 0, c->stream, s);
   EPIC_HIP_TRY(hipGetLastError());
-  if (kind == 0 && (a.enable & 4)) {
-    AppendArgs<P> A = append_args<P>(c, step);
-    if (!c->deposit_valid) A.Qe = A.Qi = nullptr;
-    hipLaunchKernelGGL(mcc_append_kernel<P>, dim3(1), dim3(APPEND_BLOCK), 0, c->stream, A);
-    EPIC_HIP_TRY(hipGetLastError());
-  }
+  AppendArgs<P> A = append_args<P>(c, step, births);
+  if (!c->deposit_valid) A.Qe = A.Qi = nullptr;
+  hipLaunchKernelGGL(mcc_append_kernel<P>, dim3(1), dim3(APPEND_BLOCK), 0, c->stream, A);
+  EPIC_HIP_TRY(hipGetLastError());
   return hipSuccess;
 }
 
@@ -599,27 +696,35 @@ void mcc_batch_collect(Ctx* c, int slot, std::vector<unsigned char>& args, std::
     const unsigned char* b = (const unsigned char*)p;
     v.insert(v.end(), b, b + n);
   };
+  bool any = false, births = false;
   if (d.electron_sp >= 0) {
     MccArgs<float> a = mcc_args<float>(c, d.electron_sp, 0, 0);
     a.slot = slot;
     if (a.nu_max > 0) {
       put(args, &a, sizeof(a));
-      if (a.enable & 4) {
-        AppendArgs<float> A = append_args<float>(c, 0);
-        A.slot = slot;
-        put(app, &A, sizeof(A));
-      }
+      any = true;
+      births = (a.enable & 4) != 0;
     }
   }
   if (d.ion_sp >= 0 && (d.enable[EPIC_MCC_I_ELASTIC] || d.enable[EPIC_MCC_I_CX])) {
     MccArgs<float> a = mcc_args<float>(c, d.ion_sp, 1, 0);
     a.slot = slot;
-    if (a.nu_max > 0) put(args, &a, sizeof(a));
+    if (a.nu_max > 0) {
+      put(args, &a, sizeof(a));
+      any = true;
+    }
+  }
+  if (any) {
+    // every simulation with a collision pass gets a finish workgroup (its
+    // list counts are cleared there)
+    AppendArgs<float> A = append_args<float>(c, 0, births);
+    A.slot = slot;
+    put(app, &A, sizeof(A));
   }
 }
 
-// All collision passes of a batch in one launch (blockIdx.y = pass), then
-// every simulation's birth append in one launch (one workgroup each).
+// All collision passes of a batch in one launch per pass (blockIdx.y =
+// pass), then every simulation's finish workgroup in one launch.
 hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int nargs, const void* d_app, int napp,
                             const int* d_steps) {
   if (nargs > 0) {
@@ -627,7 +732,7 @@ hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int 
     hipLaunchKernelGGL(mcc_select_batch_kernel<float>, dim3(gx, nargs), dim3(MCC_BLOCK), 0, st,
                        (const MccArgs<float>*)d_args, d_steps);
     EPIC_HIP_TRY(hipGetLastError());
-    const int g2 = std::max(1, std::min(num_cu * 8, (num_cu * 16 + nargs - 1) / nargs));
+    const int g2 = std::max(1, std::min(num_cu * 2, (num_cu * 4 + nargs - 1) / nargs));
     hipLaunchKernelGGL(mcc_batch_kernel<float>, dim3(g2, nargs), dim3(MCC_BLOCK), 0, st,
                        (const MccArgs<float>*)d_args, d_steps);
     EPIC_HIP_TRY(hipGetLastError());
@@ -642,15 +747,7 @@ hipError_t launch_mcc_batch(hipStream_t st, int num_cu, const void* d_args, int 
 
 hipError_t launch_mcc(Ctx* c, int step) {
   if (!c->mcc.on) return hipSuccess;
-  const epic_mcc_desc& d = c->mcc.d;
-  const bool fp64 = c->dom.precision == EPIC_FP64;
-  if (d.electron_sp >= 0) {
-    EPIC_HIP_TRY(fp64 ? mcc_species<double>(c, d.electron_sp, 0, step) : mcc_species<float>(c, d.electron_sp, 0, step));
-  }
-  if (d.ion_sp >= 0 && (d.enable[EPIC_MCC_I_ELASTIC] || d.enable[EPIC_MCC_I_CX])) {
-    EPIC_HIP_TRY(fp64 ? mcc_species<double>(c, d.ion_sp, 1, step) : mcc_species<float>(c, d.ion_sp, 1, step));
-  }
-  return hipSuccess;
+  return c->dom.precision == EPIC_FP64 ? mcc_launch<double>(c, step) : mcc_launch<float>(c, step);
 }
 
 // nu_max over the table for a particle of mass m (s^-1), 5% margin for the

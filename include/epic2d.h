@@ -176,7 +176,8 @@ typedef struct epic_mcc_desc {
                                     (collision.cpp:117,125); 0: one draw against
                                     cumulative frequencies */
   int32_t event_cap;             /* ionisation births per step (0: 16384, or one per
-                                    64 electron slots when that is more) */
+                                    64 electron slots when that is more; at most
+                                    2^28, larger values are rejected) */
 } epic_mcc_desc;
 
 typedef struct epic_mcc_stats {
