@@ -209,6 +209,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PART_PPT")) c->part_ppt = atoi(v);
   if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
   if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
+  if (const char* v = getenv("EPIC_MCC_SEL_GRID")) c->mcc_sel_grid = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
   if (const char* v = getenv("EPIC_COND_CELL_MASK")) c->cond_cell_mask = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI_WAVES")) c->push_multi_waves = atoi(v);
@@ -333,8 +334,8 @@ int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp
 
 // What epic_mcc_setup allocates: table, counters, event list, the
 // real-collision lists (8 bytes per slot of each colliding species, capacity
-// rounded up to MCC_CHUNK) with their two counts, and the electrons'
-// per-chunk event counts.
+// rounded up to MCC_SUB) with their two counts, and the electrons' event
+// counts per selection range (ranges of 64 slots at least).
 // Default ionisation event list: 16384 events, or one per 64 electron slots
 // for large species (a hot 1e8-electron plasma ionises ~1e5 per step).
 static int mcc_event_cap(int event_cap, int64_t len_e) {
@@ -346,13 +347,13 @@ static size_t mcc_bytes_of(int ntab, int event_cap, int64_t len, int64_t len_e) 
   const int cap = mcc_event_cap(event_cap, len_e);
   return (size_t)EPIC_MCC_NPROC * std::max(ntab, 1) * sizeof(double) + (EPIC_MCC_NPROC + 2) * sizeof(uint64_t) +
          (size_t)cap * 8 * sizeof(double) + sizeof(int) + (size_t)std::max<int64_t>(len, 1) * sizeof(uint2) +
-         2 * sizeof(int) + (size_t)std::max<int64_t>(len_e / MCC_CHUNK, 1) * sizeof(int);
+         2 * sizeof(int) + (size_t)std::max<int64_t>(len_e / 64, 1) * sizeof(int);
 }
 
 static int64_t mcc_list_len(const epic_species_desc* sp, int s) {
   if (s < 0) return 0;
   const int64_t cap = sp[s].capacity > 0 ? sp[s].capacity : std::max<int64_t>(sp[s].numparticle, 1);
-  return (cap + MCC_CHUNK - 1) / MCC_CHUNK * MCC_CHUNK;
+  return (cap + MCC_SUB - 1) / MCC_SUB * MCC_SUB;
 }
 
 int epic_mcc_estimate_bytes(const epic_mcc_desc* d, const epic_species_desc* sp, int nsp, int64_t* bytes) {
@@ -753,8 +754,8 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   CK(dalloc_c(c, &M.sigma, sig.size()), "mcc table");
   CK(hipMemcpy(M.sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice), "mcc table");
   M.d.sigma = nullptr;
-  M.event_cap = mcc_event_cap(d->event_cap, d->electron_sp >= 0 ? (c->sp[d->electron_sp].cap + MCC_CHUNK - 1) /
-                                                                      MCC_CHUNK * MCC_CHUNK
+  M.event_cap = mcc_event_cap(d->event_cap, d->electron_sp >= 0 ? (c->sp[d->electron_sp].cap + MCC_SUB - 1) /
+                                                                      MCC_SUB * MCC_SUB
                                                                 : 0);
   if (!M.counts) CK(dalloc_c(c, &M.counts, EPIC_MCC_NPROC + 2), "mcc counts");
   if (M.events) hipFree(M.events);
@@ -765,7 +766,7 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
   // capacities, their counts and the electrons' per-chunk event counts
   auto list_bytes = [](int64_t len, int64_t len_e) -> size_t {
     return (size_t)std::max<int64_t>(len, 1) * sizeof(uint2) + 2 * sizeof(int) +
-           (size_t)std::max<int64_t>(len_e / MCC_CHUNK, 1) * sizeof(int);
+           (size_t)std::max<int64_t>(len_e / 64, 1) * sizeof(int);
   };
   if (M.list) {
     c->bytes -= list_bytes(M.list_len_e + M.list_len_i, M.list_len_e);
@@ -777,14 +778,14 @@ int epic_mcc_setup(epic_ctx* h, const epic_mcc_desc* d) {
     M.chunk_ev = nullptr;
   }
   auto list_len = [&](int s) -> int64_t {
-    return s >= 0 ? (c->sp[s].cap + MCC_CHUNK - 1) / MCC_CHUNK * MCC_CHUNK : 0;
+    return s >= 0 ? (c->sp[s].cap + MCC_SUB - 1) / MCC_SUB * MCC_SUB : 0;
   };
   M.list_len_e = list_len(d->electron_sp);
   M.list_len_i = list_len(d->ion_sp);
   const int64_t len = M.list_len_e + M.list_len_i;
   CK(dalloc_c(c, &M.list, (size_t)len), "mcc collision lists");
   CK(dalloc_c(c, &M.list_n, 2), "mcc collision lists");
-  CK(dalloc_c(c, &M.chunk_ev, (size_t)(M.list_len_e / MCC_CHUNK)), "mcc collision lists");
+  CK(dalloc_c(c, &M.chunk_ev, (size_t)(M.list_len_e / 64)), "mcc collision lists");
   M.nu_max_e = d->electron_sp >= 0 ? mcc_nu_max(M.d, sig, ME, true) : 0.0;
   M.nu_max_i = d->ion_sp >= 0 ? mcc_nu_max(M.d, sig, c->sp[d->ion_sp].desc.mass, false) : 0.0;
   M.on = true;

@@ -18,23 +18,38 @@ struct u32x4 {
   uint32_t a, b, c, d;
 };
 
+// The 32 x 32 -> 64-bit products are written as one 64-bit multiply, which
+// gfx950 does in one v_mad_u64_u32 (instead of a multiply-hi and a
+// multiply-lo).
 __device__ __forceinline__ u32x4 philox4x32_10(u32x4 ctr, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; r++) {
-    const uint32_t hi0 = __umulhi(0xD2511F53u, ctr.a);
-    const uint32_t lo0 = 0xD2511F53u * ctr.a;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, ctr.c);
-    const uint32_t lo1 = 0xCD9E8D57u * ctr.c;
+    const uint64_t p0 = (uint64_t)0xD2511F53u * ctr.a;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * ctr.c;
     u32x4 n;
-    n.a = hi1 ^ ctr.b ^ k0;
-    n.b = lo1;
-    n.c = hi0 ^ ctr.d ^ k1;
-    n.d = lo0;
+    n.a = (uint32_t)(p1 >> 32) ^ ctr.b ^ k0;
+    n.b = (uint32_t)p1;
+    n.c = (uint32_t)(p0 >> 32) ^ ctr.d ^ k1;
+    n.d = (uint32_t)p0;
     ctr = n;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
   return ctr;
+}
+
+// Philox2x32-10 of the same family (one multiply-hi/lo per round, 32-bit key
+// with a Weyl increment): 64 random bits for half the multiplies of
+// Philox4x32-10, for draws that need no more.
+__device__ __forceinline__ uint2 philox2x32_10(uint32_t c0, uint32_t c1, uint32_t k) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p = (uint64_t)0xD256D193u * c0;
+    c0 = (uint32_t)(p >> 32) ^ k ^ c1;
+    c1 = (uint32_t)p;
+    k += 0x9E3779B9u;
+  }
+  return make_uint2(c0, c1);
 }
 
 // 53 random bits -> [0, 1)
@@ -54,6 +69,7 @@ enum : uint32_t {
   RNG_MCC2 = 5,
   RNG_MCC3 = 6,
   RNG_INJECT = 7,
+  RNG_MCC_KEY = 8,  // host-side derivation of the candidate draws' Philox2x32 key
   RNG_MCC_ION_SCATTER = 12,
   RNG_MCC_NEWION = 13,
   RNG_MCC_NEWION2 = 14,

@@ -30,7 +30,11 @@ constexpr int FIX_BITS_FP32 = 28;
 
 // LDS deposit window: 8192 int64 node accumulators (64 KiB) per workgroup.
 constexpr int DEP_WINDOW = 8192;
-constexpr int MCC_CHUNK = 2048;  // particle slots per MCC selection chunk
+// MCC selection pass: a wavefront selects MCC_SUB slots at a time within its
+// range of slots (a multiple of 64: one range per resident wavefront for a
+// single simulation, MCC_RANGE_BATCH slots in batched runs)
+constexpr int MCC_SUB = 512;
+constexpr int MCC_RANGE_BATCH = 2048;
 constexpr int MCC_EVENT_MAX = 1 << 28;  // ionisation event list entries at most
 constexpr int PART_BLOCK = 256;  // threads per workgroup in particle kernels
 
@@ -88,7 +92,7 @@ struct MccState {
   // real-collision lists of the two-pass null-collision step (pass 1 appends
   // (slot, process | (event slot + 1) << 3), pass 2 reads): electron list
   // (list_len_e entries) then ion list; their two counts; the electrons'
-  // stored ionisation events per MCC_CHUNK slots
+  // stored ionisation events per selection range (ranges of 64 slots at least)
   uint2* list = nullptr;
   int* list_n = nullptr;
   int* chunk_ev = nullptr;
@@ -225,6 +229,7 @@ struct Ctx {
   int part_ppt = 2;            // FP64 particle kernels: particles per thread (2 or 4; generic also 8)
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
+  int mcc_sel_grid = 1;        // MCC selection pass: 1 resident workgroups, 0 one chunk per wavefront
   int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
   int push_multi_waves = 8;    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
   int cond_cell_mask = 1;      // 0: every cell flagged (the exact test everywhere)

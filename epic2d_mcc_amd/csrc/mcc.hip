@@ -2,10 +2,11 @@
 //
 // Replaces CollisionHandler::handle_collisions (collision.cpp:190-207), which
 // draws P = 1 - exp(-nu(E) dt) per electron.  Here every particle of the
-// colliding species draws from its own Philox stream keyed by
-// (particle id, step): it is a candidate with probability
-// P_null = 1 - exp(-nu_max dt), and a candidate collides for real with
-// probability (1 - exp(-nu(E) dt)) / P_null (else a null collision), so each
+// colliding species draws from its own counter-based streams keyed by
+// (particle id, step): R0 from Philox2x32-10 under a per-(seed, species) key
+// makes it a candidate with probability P_null = 1 - exp(-nu_max dt), and a
+// candidate collides for real with probability (1 - exp(-nu(E) dt)) / P_null
+// (R1 and the later draws from Philox4x32-10; else a null collision), so each
 // particle's collision probability is exactly the reference's; only the
 // candidates (a fraction ~nu_max dt) evaluate cross sections.
 //
@@ -35,14 +36,24 @@ struct MccTab {
   const double* sigma;  // NPROC * n
   int n;
   double loge0, inv_dloge;
-  __device__ __forceinline__ double at(int p, double e) const {
-    if (!(e > 0)) return sigma[p * n];
-    double t = (log(e) - loge0) * inv_dloge;
-    if (t <= 0) return sigma[p * n];
-    if (t >= n - 1) return sigma[p * n + n - 1];
+  // where energy e falls on the log-spaced grid, found once for all the
+  // processes read at e: entry k and the fraction f towards k + 1 (f < 0:
+  // outside the grid, entry k as it is)
+  struct Pos {
+    int k;
+    double f;
+  };
+  __device__ __forceinline__ Pos pos(double e) const {
+    if (!(e > 0)) return {0, -1.0};
+    const double t = (log(e) - loge0) * inv_dloge;
+    if (t <= 0) return {0, -1.0};
+    if (t >= n - 1) return {n - 1, -1.0};
     const int k = (int)t;
-    const double f = t - k;
-    return sigma[p * n + k] * (1 - f) + sigma[p * n + k + 1] * f;
+    return {k, t - k};
+  }
+  __device__ __forceinline__ double at(int p, Pos q) const {
+    if (q.f < 0) return sigma[p * n + q.k];
+    return sigma[p * n + q.k] * (1 - q.f) + sigma[p * n + q.k + 1] * q.f;
   }
 };
 
@@ -64,6 +75,8 @@ struct MccArgs {
   double e_exc, e_ion;  // eV
   double ion_vth;       // normalised thermal spread of newborn ions
   uint32_t step, sp, k0, k1;
+  uint32_t key_r0;  // Philox2x32 key of the candidate draw R0 (per seed and species)
+  uint64_t r0_below;  // R0 < p_null exactly when its 53-bit integer is below this (ceil(p_null 2^53))
   unsigned long long* counts;
   double* events;
   int* event_count;
@@ -71,7 +84,8 @@ struct MccArgs {
   int slot;  // batched runs: index of the simulation's step counter
   uint2* list;     // real collisions: (slot, process | (event slot + 1) << 3)
   int* list_n;     // entries in list (cleared by the finish kernel)
-  int* chunk_ev;   // electrons: stored ionisation events per MCC_CHUNK slots
+  int* chunk_ev;   // electrons: stored ionisation events per range
+  int waves;       // pass 1: wavefronts of the grid, one range each (0: MCC_RANGE_BATCH-slot ranges)
   unsigned long long* errc;  // EPIC_ERRC_MCC_BOUND: collision probability above p_null
 };
 
@@ -83,11 +97,15 @@ __device__ __forceinline__ void rotate(double g, double chi, double eta, double 
   gz = g * (st * sp * cc + ct * sp * sc * ce + cp * sc * se);
 }
 
+// Box-Muller pair; the angle 2 pi u2 through sincospi(2 u2) (2 u2 in [0, 2):
+// no large-argument reduction).
 __device__ __forceinline__ double gauss(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double* second) {
   const double u1 = 1.0 - u01_53(a, b), u2 = u01_53(c, d);
   const double m = sqrt(-2.0 * log(u1));
-  *second = m * sin(2.0 * M_PI * u2);
-  return m * cos(2.0 * M_PI * u2);
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
+  *second = m * sn;
+  return m * cs;
 }
 
 // The neutral partner of an ion collision: a Maxwellian at gas_temp_K from
@@ -108,7 +126,9 @@ __device__ __forceinline__ void neutral_velocity(const MccArgs<P>& a, uint32_t p
 // electron-neutral elastic / excitation / ionisation, 3-4 ion-neutral
 // elastic / charge exchange).  Returns -1 for a null collision or an
 // absorbed slot.
-template <typename P>
+// KIND: the species kind when fixed at compile time (0 electron, 1 ion), -1
+// to read it from the arguments.
+template <typename P, int KIND>
 __device__ __forceinline__ int mcc_null_test(const MccArgs<P>& a, uint32_t step, int64_t p) {
   // x, y, vx, vy at slot q (tiled FP32 storage), vz and id at p; every load
   // is issued before the first test (p is a live slot, so all are in bounds)
@@ -119,14 +139,15 @@ __device__ __forceinline__ int mcc_null_test(const MccArgs<P>& a, uint32_t step,
   const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
   if (isnan(xp)) return -1;
   const double R1 = u01_53(r.c, r.d);
-  if (a.kind == 0) {
+  if ((KIND >= 0 ? KIND : a.kind) == 0) {
     // electron - neutral (neutral at rest)
     const double v_sqr = (vxe * vxe + vye * vye + vze * vze) * a.vel_norm * a.vel_norm;
     const double velocity = sqrt(v_sqr);
     const double energy = (0.5 * ME * v_sqr) / eV;
-    double s_el = (a.enable & 1) ? a.tab.at(0, energy) : 0.0;
-    double s_ex = (a.enable & 2) && energy > a.e_exc ? a.tab.at(1, energy) : 0.0;
-    double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, energy) : 0.0;
+    const MccTab::Pos at_e = a.tab.pos(energy);
+    double s_el = (a.enable & 1) ? a.tab.at(0, at_e) : 0.0;
+    double s_ex = (a.enable & 2) && energy > a.e_exc ? a.tab.at(1, at_e) : 0.0;
+    double s_io = (a.enable & 4) && energy > a.e_ion ? a.tab.at(2, at_e) : 0.0;
     const double t0 = s_el, t1 = t0 + s_ex, t2 = t1 + s_io;
     const double nu = a.gas_density * t2 * velocity;
     const double pc = -expm1(-nu * a.dt);
@@ -151,8 +172,9 @@ __device__ __forceinline__ int mcc_null_test(const MccArgs<P>& a, uint32_t step,
   const double g2 = gx * gx + gy * gy + gz * gz;
   const double g = sqrt(g2);
   const double energy = 0.5 * a.mass * g2 * a.vel_norm * a.vel_norm / eV;
-  const double s_el = (a.enable & 8) ? a.tab.at(3, energy) : 0.0;
-  const double s_cx = (a.enable & 16) ? a.tab.at(4, energy) : 0.0;
+  const MccTab::Pos at_e = a.tab.pos(energy);
+  const double s_el = (a.enable & 8) ? a.tab.at(3, at_e) : 0.0;
+  const double s_cx = (a.enable & 16) ? a.tab.at(4, at_e) : 0.0;
   const double nu_el = a.gas_density * s_el * g * a.vel_norm;
   const double nu_cx = a.gas_density * s_cx * g * a.vel_norm;
   const double nu = nu_el + nu_cx;
@@ -247,11 +269,13 @@ __device__ __forceinline__ void mcc_collide(const MccArgs<P>& a, uint32_t step, 
   cnt[type]++;
 }
 
-// A selection chunk (MCC_CHUNK consecutive slots) is one wavefront's work
-// in pass 1; a real collision's list entry packs its process (3 bits) and
-// its event slot + 1 (29 bits).
-static_assert(MCC_CHUNK <= 65536, "candidate offsets are 16-bit");
+// Pass 1 walks a species in ranges of slots (a multiple of 64, at most
+// MCC_RMAX), one wavefront per range at a time; a real collision's list
+// entry packs its process (3 bits) and its event slot + 1 (29 bits).
+static_assert(MCC_SUB == 512, "the selection loads 8 groups of 64 slots at a time");
 static_assert(MCC_EVENT_MAX < (1 << 29) - 1, "event slot + 1 in 29 bits");
+constexpr int MCC_RMAX = 3072;  // LDS candidate list of a wavefront (16-bit offsets)
+static_assert(MCC_RMAX % 64 == 0 && MCC_RMAX <= 65536 && MCC_RMAX >= MCC_RANGE_BATCH, "range bounds");
 
 __device__ __forceinline__ int lane_rank(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
@@ -263,79 +287,114 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pass 1 of a colliding species: every particle draws R0 from its (id, step)
-// stream; those with R0 < P_null (a fraction ~nu_max dt) are candidates,
-// listed in LDS (cand: MCC_CHUNK offsets of this wavefront) in slot order.
-// The wavefront then runs the null test on its chunk's candidates, 64 at a
-// time, and appends the real collisions to the species' list (one atomic per
-// 64 candidates).  Electron ionisations reserve their event slots here, in
-// slot order within the chunk, so each stored event gets its rank among the
-// chunk's stored events (ev[7]) and chunk_ev[c] their count: the birth
-// append orders births by (chunk, rank), i.e. by parent slot, whatever order
-// the list has.  Wavefront w of the grid takes chunks w, w + W, ...
-template <typename P>
+// One null-test round: up to 64 candidates (cand[t0 + lane], t < todo; slot
+// offsets from base), real collisions appended to the species' list with
+// one atomic.  Electron ionisations reserve their event slots here, in slot
+// order within the range, so each stored event gets its rank among the
+// range's stored events (ev[7]; nstored counts them).
+template <typename P, int KIND>
+__device__ __forceinline__ void mcc_null_round(const MccArgs<P>& a, uint32_t step, int64_t base, const uint16_t* cand,
+                                               int t0, int todo, int& nstored) {
+  const int lane = threadIdx.x & 63;
+  const int kind = KIND >= 0 ? KIND : a.kind;
+  const int t = t0 + lane;
+  const int64_t p = base + (t < todo ? (int64_t)cand[t] : 0);
+  const int type = t < todo ? mcc_null_test<P, KIND>(a, step, p) : -1;
+  const unsigned long long real = __ballot(type >= 0);
+  if (!real) return;
+  int evi = -1;
+  if (kind == 0) {
+    const unsigned long long ion = __ballot(type == 2);
+    if (ion) {
+      const int first = __builtin_ctzll(ion), k = (int)__popcll(ion);
+      int e0 = 0;
+      if (lane == first) e0 = atomicAdd(a.event_count, k);
+      e0 = __shfl(e0, first, 64);
+      const int i = lane_rank(ion);
+      if (type == 2 && e0 + i < a.event_cap) {
+        evi = e0 + i;
+        a.events[(size_t)evi * EVW + 7] = (double)(nstored + i);
+      }
+      nstored += max(0, min(k, a.event_cap - e0));
+    }
+  }
+  const int first = __builtin_ctzll(real);
+  int b0 = 0;
+  if (lane == first) b0 = atomicAdd(a.list_n, (int)__popcll(real));
+  b0 = __shfl(b0, first, 64);
+  if (type >= 0) a.list[b0 + lane_rank(real)] = make_uint2((uint32_t)p, (uint32_t)type | ((uint32_t)(evi + 1) << 3));
+}
+
+// Slots per selection range: n spread over `waves` ranges (rounded up to 64
+// slots, at most MCC_RMAX), or MCC_RANGE_BATCH when waves is 0.  The append
+// computes the same.
+__device__ __forceinline__ int64_t mcc_range(int64_t n, int waves) {
+  if (waves <= 0) return MCC_RANGE_BATCH;
+  const int64_t per = (n + waves - 1) / waves;
+  return min((int64_t)MCC_RMAX, max((int64_t)64, (per + 63) / 64 * 64));
+}
+
+// Pass 1 of a colliding species over range rg, in two phases.  Selection:
+// every particle draws R0 from its (id, step) Philox2x32 stream, MCC_SUB
+// slots at a time (the next MCC_SUB slots' ids load while these are drawn);
+// those with R0 < P_null (a fraction ~nu_max dt) are candidates, listed in
+// LDS (cand) in slot order.  Null test: the candidates in rounds of 64.
+// chunk_ev[rg] gets the range's stored electron ionisation events: the birth
+// append orders births by (range, rank), i.e. by parent slot, whatever order
+// the list has.
+template <typename P, int KIND>
+__device__ __forceinline__ void mcc_select_range(const MccArgs<P>& a, uint32_t step, int64_t rg, int64_t range,
+                                                 int64_t n, uint16_t* cand) {
+  const int lane = threadIdx.x & 63;
+  const int kind = KIND >= 0 ? KIND : a.kind;
+  const int64_t base = rg * range;
+  const int64_t end = min(base + range, n);
+  int cnt = 0;
+  // loads and draws run on every lane (slots past the end read the range's
+  // last slot and are not candidates): no branches, so the next slots' loads
+  // stay in flight through the draws
+  uint32_t nxt[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) nxt[j] = a.id[min(base + j * 64 + lane, end - 1)];
+  for (int64_t sb = base; sb < end; sb += MCC_SUB) {
+    uint32_t pid[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) pid[j] = nxt[j];
+#pragma unroll
+    for (int j = 0; j < 8; j++) nxt[j] = a.id[min(sb + MCC_SUB + j * 64 + lane, end - 1)];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int64_t p = sb + j * 64 + lane;
+      // the key goes through an empty asm so that its ten round values are
+      // formed by scalar adds here rather than kept live (and spilled to
+      // VGPR lanes) across the loop
+      uint32_t key = __builtin_amdgcn_readfirstlane(a.key_r0);
+      asm volatile("" : "+s"(key));
+      const uint2 r0 = philox2x32_10(pid[j], step, key);
+      // u01_53(r0) < p_null, as an exact integer comparison
+      const uint64_t m53 = ((uint64_t)r0.x << 21) ^ (uint64_t)(r0.y >> 11);
+      const bool is_cand = p < end && m53 < a.r0_below;
+      const unsigned long long m = __ballot(is_cand);
+      if (is_cand) cand[cnt + lane_rank(m)] = (uint16_t)(p - base);
+      cnt += (int)__popcll(m);
+    }
+  }
+  wave_lds_sync();
+  int nstored = 0;
+  for (int t0 = 0; t0 < cnt; t0 += 64) mcc_null_round<P, KIND>(a, step, base, cand, t0, cnt, nstored);
+  if (kind == 0 && lane == 0) a.chunk_ev[rg] = nstored;
+  wave_lds_sync();  // the next range overwrites the candidate list
+}
+
+template <typename P, int KIND>
 __device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk,
                                                 uint16_t* cand) {
   const int64_t n = a.d_n[0];
-  const int lane = threadIdx.x & 63;
+  const int64_t range = mcc_range(n, a.waves);
   const int wpb = blockDim.x >> 6;
-  const int64_t nch = (n + MCC_CHUNK - 1) / MCC_CHUNK;
-  for (int64_t c = (int64_t)bid * wpb + (threadIdx.x >> 6); c < nch; c += (int64_t)nblk * wpb) {
-    const int64_t base = c * MCC_CHUNK;
-    int cnt = 0;
-    for (int k0 = 0; k0 < MCC_CHUNK / 64; k0 += 8) {
-      uint32_t pid[8];
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int64_t p = base + (k0 + j) * 64 + lane;
-        pid[j] = p < n ? a.id[p] : 0u;
-      }
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int off = (k0 + j) * 64 + lane;
-        bool is_cand = false;
-        if (base + off < n) {
-          const u32x4 r = philox4x32_10({pid[j], step, RNG_MCC, a.sp}, a.k0, a.k1);
-          is_cand = u01_53(r.a, r.b) < a.p_null;
-        }
-        const unsigned long long m = __ballot(is_cand);
-        if (is_cand) cand[cnt + lane_rank(m)] = (uint16_t)off;
-        cnt += (int)__popcll(m);
-      }
-    }
-    wave_lds_sync();
-    int nstored = 0;
-    for (int t0 = 0; t0 < cnt; t0 += 64) {
-      const int t = t0 + lane;
-      const int64_t p = base + (t < cnt ? (int)cand[t] : 0);
-      const int type = t < cnt ? mcc_null_test(a, step, p) : -1;
-      const unsigned long long real = __ballot(type >= 0);
-      if (!real) continue;
-      int evi = -1;
-      if (a.kind == 0) {
-        const unsigned long long ion = __ballot(type == 2);
-        if (ion) {
-          const int first = __builtin_ctzll(ion), k = (int)__popcll(ion);
-          int e0 = 0;
-          if (lane == first) e0 = atomicAdd(a.event_count, k);
-          e0 = __shfl(e0, first, 64);
-          const int i = lane_rank(ion);
-          if (type == 2 && e0 + i < a.event_cap) {
-            evi = e0 + i;
-            a.events[(size_t)evi * EVW + 7] = (double)(nstored + i);
-          }
-          nstored += max(0, min(k, a.event_cap - e0));
-        }
-      }
-      const int first = __builtin_ctzll(real);
-      int b0 = 0;
-      if (lane == first) b0 = atomicAdd(a.list_n, (int)__popcll(real));
-      b0 = __shfl(b0, first, 64);
-      if (type >= 0) a.list[b0 + lane_rank(real)] = make_uint2((uint32_t)p, (uint32_t)type | ((uint32_t)(evi + 1) << 3));
-    }
-    if (a.kind == 0 && lane == 0) a.chunk_ev[c] = nstored;
-    wave_lds_sync();  // the next chunk overwrites the candidate list
-  }
+  const int64_t nr = (n + range - 1) / range;
+  for (int64_t rg = (int64_t)bid * wpb + (threadIdx.x >> 6); rg < nr; rg += (int64_t)nblk * wpb)
+    mcc_select_range<P, KIND>(a, step, rg, range, n, cand);
 }
 
 // Pass 2: the kinematics of the species' real collisions, one per lane over
@@ -363,13 +422,16 @@ __device__ __forceinline__ void mcc_body(const MccArgs<P>& a, uint32_t step, int
 template <typename P>
 struct MccPair {
   MccArgs<P> a[2];
+  int np;  // species in a (1 or 2)
 };
 
-template <typename P>
-__global__ __launch_bounds__(MCC_BLOCK) void mcc_select_kernel(MccPair<P> s) {
-  __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_CHUNK];
-  const MccArgs<P>& a = s.a[blockIdx.y];
-  mcc_select_body(a, a.step, blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
+// Pass 1 of one species, its kind fixed at compile time (the electron and the
+// ion null tests need different registers: each launch runs at its own
+// occupancy).  Every workgroup resident; wavefront w takes chunks w, w + W, ...
+template <typename P, int KIND>
+__global__ __launch_bounds__(MCC_BLOCK) void mcc_select_kernel(MccArgs<P> a) {
+  __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_RMAX];
+  mcc_select_body<P, KIND>(a, a.step, blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
 }
 
 // OCC: workgroups per CU the register allocation is held to (1: none)
@@ -383,9 +445,9 @@ __global__ __launch_bounds__(MCC_BLOCK, OCC) void mcc_kernel(MccPair<P> s) {
 // from the simulation's device step counter.
 template <typename P>
 __global__ __launch_bounds__(MCC_BLOCK) void mcc_select_batch_kernel(const MccArgs<P>* args, const int* steps) {
-  __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_CHUNK];
+  __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_RMAX];
   const MccArgs<P>& a = args[blockIdx.y];
-  mcc_select_body(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
+  mcc_select_body<P, -1>(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
 }
 
 template <typename P>
@@ -425,7 +487,8 @@ struct AppendArgs {
   unsigned long long* counts;
   unsigned long long* errc;
   int slot;  // batched runs: index of the simulation's step counter
-  int* chunk_n;  // per selection chunk: its ionisation events (scanned in place)
+  int* chunk_n;  // per selection range: its ionisation events (scanned in place)
+  int waves;     // the selection grid's wavefronts (mcc_range)
   int births;    // the electrons ionise: append their events
   int* list_n;   // the context's real-collision list counts (electrons, ions)
 };
@@ -455,10 +518,11 @@ __device__ __forceinline__ void mcc_births(const AppendArgs<P>& A, uint32_t step
   const int E = min(*event_count, cap_events);
   const int64_t ne = e_n[0], ni = i_n[0];
   const uint32_t ide = (uint32_t)e_n[1], idi = (uint32_t)i_n[1];
-  // exclusive scan of the per-chunk event counts the collision pass left in
-  // chunk_n (chunk order = slot order): each thread a contiguous run of
-  // chunks, then a scan of the run totals
-  const int nch = (int)((ne + MCC_CHUNK - 1) / MCC_CHUNK);
+  // exclusive scan of the per-range event counts the selection pass left in
+  // chunk_n (range order = slot order): each thread a contiguous run of
+  // ranges, then a scan of the run totals
+  const int64_t range = mcc_range(ne, A.waves);
+  const int nch = (int)((ne + range - 1) / range);
   const int per = (nch + (int)blockDim.x - 1) / (int)blockDim.x;
   const int c0 = min((int)threadIdx.x * per, nch), c1 = min(c0 + per, nch);
   int run = 0;
@@ -481,7 +545,7 @@ __device__ __forceinline__ void mcc_births(const AppendArgs<P>& A, uint32_t step
   int dropped = 0;
   for (int k = threadIdx.x; k < E; k += blockDim.x) {
     const double* ev = events + (size_t)k * EVW;
-    const int rank = chunk_n[(int64_t)ev[0] / MCC_CHUNK] + (int)ev[7];
+    const int rank = chunk_n[(int64_t)ev[0] / range] + (int)ev[7];
     const int64_t se = ne + rank, si = ni + rank;
     if (se >= e_cap || si >= i_cap) {
       dropped++;
@@ -566,6 +630,22 @@ __global__ __launch_bounds__(APPEND_BLOCK) void mcc_append_batch_kernel(const Ap
   mcc_finish_body(A, (uint32_t)steps[A.slot], scan);
 }
 
+// The candidate draws' Philox2x32 key of a (seed, species): the first word of
+// Philox4x32-10 at counter (species, 0, RNG_MCC_KEY, 0) under the seed, on the
+// host (a counter no device draw uses).
+static uint32_t mcc_r0_key(uint32_t k0, uint32_t k1, uint32_t sp) {
+  uint32_t c[4] = {sp, 0u, RNG_MCC_KEY, 0u};
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c[0];
+}
+
 // Arguments of one species' collision pass (kind 0 electron, 1 ion).
 template <typename P>
 static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
@@ -587,6 +667,7 @@ static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   const double dt = d.DT / d.W;  // seconds (collision.cpp:201)
   a.nu_max = kind == 0 ? M.nu_max_e : M.nu_max_i;
   a.p_null = -expm1(-a.nu_max * dt);
+  a.r0_below = (uint64_t)std::ceil(std::ldexp(a.p_null, 53));
   a.dt = dt;
   a.vel_norm = d.vel_norm;
   a.mass = S.desc.mass;
@@ -599,6 +680,7 @@ static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   a.sp = (uint32_t)s;
   a.k0 = (uint32_t)d.seed;
   a.k1 = (uint32_t)(d.seed >> 32);
+  a.key_r0 = mcc_r0_key(a.k0, a.k1, a.sp);
   a.counts = M.counts;
   a.events = M.events;
   a.event_count = M.event_count;
@@ -625,7 +707,7 @@ static AppendArgs<P> append_args(Ctx* c, int step, bool births) {
   A.events = M.events;
   A.event_count = M.event_count;
   A.cap_events = M.event_cap;
-  A.chunk_n = M.chunk_ev;  // the electrons' per-chunk event counts (pass 1)
+  A.chunk_n = M.chunk_ev;  // the electrons' per-range event counts (pass 1)
   A.ex = (P*)S.x; A.ey = (P*)S.y; A.evx = (P*)S.vx; A.evy = (P*)S.vy; A.evz = (P*)S.vz;
   A.eid = S.id; A.e_n = S.d_n; A.e_cap = S.cap;
   A.ix = (P*)I.x; A.iy = (P*)I.y; A.ivx = (P*)I.vx; A.ivy = (P*)I.vy; A.ivz = (P*)I.vz;
@@ -643,21 +725,18 @@ static AppendArgs<P> append_args(Ctx* c, int step, bool births) {
   return A;
 }
 
-// One context's collision step: pass 1 of both colliding species in one
-// launch, pass 2 of both in one launch, then the finish kernel (births,
-// list counts cleared).
+// One context's collision step: pass 1 of each colliding species, pass 2 of
+// both in one launch, then the finish kernel (births, list counts cleared).
 template <typename P>
 static hipError_t mcc_launch(Ctx* c, int step) {
   const epic_mcc_desc& d = c->mcc.d;
   MccPair<P> s{};
   int np = 0;
-  int64_t nch = 0;
   bool births = false;
   if (d.electron_sp >= 0) {
     const MccArgs<P> a = mcc_args<P>(c, d.electron_sp, 0, step);
     if (a.nu_max > 0) {
       s.a[np++] = a;
-      nch = std::max<int64_t>(nch, (c->sp[d.electron_sp].cap + MCC_CHUNK - 1) / MCC_CHUNK);
       births = (a.enable & 4) != 0;
     }
   }
@@ -665,21 +744,47 @@ static hipError_t mcc_launch(Ctx* c, int step) {
     const MccArgs<P> a = mcc_args<P>(c, d.ion_sp, 1, step);
     if (a.nu_max > 0) {
       s.a[np++] = a;
-      nch = std::max<int64_t>(nch, (c->sp[d.ion_sp].cap + MCC_CHUNK - 1) / MCC_CHUNK);
     }
   }
   if (np == 0) return hipSuccess;
-  // one wavefront per selection chunk in pass 1; pass 2 at the occupancy of
-  // the collision arithmetic, every workgroup resident
-  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(c->num_cu * 16, (nch + 3) / 4));
-  hipLaunchKernelGGL(mcc_select_kernel<P>, dim3(g1, np), dim3(MCC_BLOCK), 0, c->stream, s);
-  EPIC_HIP_TRY(hipGetLastError());
+  s.np = np;
+  // pass 1 per species: every workgroup resident, each wavefront one range
+  // of the species (EPIC_MCC_SEL_GRID=0: ranges of MCC_RANGE_BATCH slots, one
+  // workgroup per four), then pass 2 of both species
+  static int sel_per_cu[2][2] = {{0, 0}, {0, 0}};
+  int waves_e = 0;
+  for (int k = 0; k < np; k++) {
+    MccArgs<P>& a = s.a[k];
+    const void* kern = a.kind == 0 ? (const void*)mcc_select_kernel<P, 0> : (const void*)mcc_select_kernel<P, 1>;
+    int& per_cu = sel_per_cu[sizeof(P) == 8][a.kind];
+    if (per_cu == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, MCC_BLOCK, 0) != hipSuccess ||
+                        per_cu < 1))
+      per_cu = 1;
+    const int64_t cap = std::max<int64_t>(c->sp[a.sp].cap, 1);
+    int g1;
+    if (c->mcc_sel_grid == 0) {
+      a.waves = 0;
+      g1 = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * 16,
+                                                       ((cap + MCC_RANGE_BATCH - 1) / MCC_RANGE_BATCH + 3) / 4));
+    } else {
+      g1 = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * per_cu, (cap + 255) / 256));
+      a.waves = g1 * (MCC_BLOCK / 64);
+    }
+    if (a.kind == 0) {
+      waves_e = a.waves;
+      hipLaunchKernelGGL((mcc_select_kernel<P, 0>), dim3(g1), dim3(MCC_BLOCK), 0, c->stream, a);
+    } else {
+      hipLaunchKernelGGL((mcc_select_kernel<P, 1>), dim3(g1), dim3(MCC_BLOCK), 0, c->stream, a);
+    }
+    EPIC_HIP_TRY(hipGetLastError());
+  }
   if (c->mcc_occ == 1)
     hipLaunchKernelGGL((mcc_kernel<P, 1>), dim3(c->num_cu, np), dim3(MCC_BLOCK), 0, c->stream, s);
   else
     hipLaunchKernelGGL((mcc_kernel<P, 2>), dim3(c->num_cu * 2, np), dim3(MCC_BLOCK), 0, c->stream, s);
   EPIC_HIP_TRY(hipGetLastError());
   AppendArgs<P> A = append_args<P>(c, step, births);
+  A.waves = waves_e;
   if (!c->deposit_valid) A.Qe = A.Qi = nullptr;
   hipLaunchKernelGGL(mcc_append_kernel<P>, dim3(1), dim3(APPEND_BLOCK), 0, c->stream, A);
   EPIC_HIP_TRY(hipGetLastError());
@@ -700,6 +805,7 @@ void mcc_batch_collect(Ctx* c, int slot, std::vector<unsigned char>& args, std::
   if (d.electron_sp >= 0) {
     MccArgs<float> a = mcc_args<float>(c, d.electron_sp, 0, 0);
     a.slot = slot;
+    a.waves = 0;
     if (a.nu_max > 0) {
       put(args, &a, sizeof(a));
       any = true;
@@ -709,6 +815,7 @@ void mcc_batch_collect(Ctx* c, int slot, std::vector<unsigned char>& args, std::
   if (d.ion_sp >= 0 && (d.enable[EPIC_MCC_I_ELASTIC] || d.enable[EPIC_MCC_I_CX])) {
     MccArgs<float> a = mcc_args<float>(c, d.ion_sp, 1, 0);
     a.slot = slot;
+    a.waves = 0;
     if (a.nu_max > 0) {
       put(args, &a, sizeof(a));
       any = true;
@@ -719,6 +826,7 @@ void mcc_batch_collect(Ctx* c, int slot, std::vector<unsigned char>& args, std::
     // list counts are cleared there)
     AppendArgs<float> A = append_args<float>(c, 0, births);
     A.slot = slot;
+    A.waves = 0;
     put(app, &A, sizeof(A));
   }
 }

@@ -231,6 +231,22 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   for (int i = 0; i < 4; i++) out[i] = c[i];
 }
 
+// Philox2x32-10 of the same paper: one multiply-hi/lo by 0xD256D193 per round,
+// a 32-bit key advanced by 0x9E3779B9 after each round (the HIP path's
+// candidate draw of the collision step).
+void orc_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1];
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p = (uint64_t)0xD256D193u * c0;
+    const uint32_t n0 = (uint32_t)(p >> 32) ^ key ^ c1;
+    c1 = (uint32_t)p;
+    c0 = n0;
+    key += 0x9E3779B9u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+}
+
 }  // extern "C"
 
 namespace {

@@ -82,6 +82,8 @@ def oracle():
         o.orc_mt19937_nth.argtypes = [ctypes.c_uint, ctypes.c_int]
         o.orc_mt19937_nth.restype = ctypes.c_uint
         o.orc_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+        o.orc_philox2x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_uint32)]
         o.orc_compute_setup.argtypes = [ctypes.POINTER(OrcInputs), ctypes.POINTER(OrcSetup)]
         for n in ("orc_sigma_elastic", "orc_sigma_excitation", "orc_sigma_ionization"):
             getattr(o, n).argtypes = [ctypes.c_double]

@@ -6,6 +6,9 @@ null-collision scheme with Philox streams on log-spaced tables of the same
 analytic cross sections.  Per-process counts must agree within 3 sigma
 (sigma^2 = sum of the two Poisson counts); energy removed by thresholds is
 checked against the electrons' kinetic-energy drop; runs are repeatable."""
+import ctypes
+import math
+
 import numpy as np
 import pytest
 
@@ -72,6 +75,100 @@ def test_counts_match_reference_direct_method():
     # births: one electron and one ion per ionisation
     assert ctx.count(0) == N + g[2]
     assert ctx.count(1) == 1 + g[2]
+
+
+def _u53(a, b):
+    return float(((int(a) << 21) ^ (int(b) >> 11)) * 2.0 ** -53)
+
+
+def _philox4(ctr, key):
+    C = (ctypes.c_uint32 * 4)(*[int(v) & 0xffffffff for v in ctr])
+    K = (ctypes.c_uint32 * 2)(*[int(v) & 0xffffffff for v in key])
+    O = (ctypes.c_uint32 * 4)()
+    oracle().orc_philox4x32_10(C, K, O)
+    return list(O)
+
+
+def _philox2(ctr, key):
+    C = (ctypes.c_uint32 * 2)(*[int(v) & 0xffffffff for v in ctr])
+    O = (ctypes.c_uint32 * 2)()
+    oracle().orc_philox2x32_10(C, int(key) & 0xffffffff, O)
+    return list(O)
+
+
+def test_collision_selection_is_the_keyed_draws():
+    """One collision step, elastic only: particle id collides exactly when
+    R0 < P_null and R1 P_null < 1 - exp(-nu(E) dt), with R0 from Philox2x32-10
+    at counter (id, step) under the key Philox4x32-10 gives at counter
+    (species, 0, 8, 0) under the seed, R1 from Philox4x32-10 at counter (id,
+    step, 4, species), nu(E) = n_g sigma(E) v from the log-spaced table and
+    P_null = 1 - exp(-nu_max dt).  The draws come from the oracle's Philox
+    restatements (pinned to the Random123 known answers); the set of particles
+    whose velocity changed must equal the predicted set.  Particles within
+    1e-12 (relative) of either threshold are left out of the comparison."""
+    print("[mcc] exact selection from the keyed draws", flush=True)
+    N, step, gas = 30000, 7, 2e22
+    st = _setup(N, 8.0)
+    d = st.domain
+    e = _electrons(st, N, 8.0, 11)
+    table = xsec.synthetic_ar()
+    e_min, e_max, tab = table
+    m = xsec.mcc_desc(table, gas_density=gas, gas_temp_K=0.0, enable=(1, 0, 0, 0, 0))
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, *e)
+    ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    ctx.mcc_setup(m)
+    x0, y0, vx0, vy0, vz0, ids = ctx.download(0)
+    ctx.collide(step)
+    _, _, vx1, vy1, vz1, ids1 = ctx.download(0)
+    ctx.close()
+    assert np.array_equal(ids, ids1)
+    hit = (vx1 != vx0) | (vy1 != vy0) | (vz1 != vz0)
+    # P_null from nu_max over the table (and 4x the top speed), 5% margin
+    n = tab.shape[1]
+    l0, l1 = math.log(e_min), math.log(e_max)
+    best = 0.0
+    for k in range(n):
+        en = math.exp(l0 + (l1 - l0) * k / (n - 1))
+        best = max(best, gas * tab[0][k] * math.sqrt(2.0 * en * config.eV / ME))
+    best = max(best, gas * tab[0][n - 1] * math.sqrt(2.0 * e_max * config.eV / ME) * 4.0)
+    dt = d.DT / d.W
+    p_null = -math.expm1(-best * 1.05 * dt)
+    assert 0.1 < p_null < 0.6
+    k0, k1 = d.seed & 0xffffffff, d.seed >> 32
+    key = _philox4([0, 0, 8, 0], [k0, k1])[0]
+    loge0, inv = math.log(e_min), (n - 1) / (math.log(e_max) - math.log(e_min))
+    want = np.zeros(len(ids), dtype=bool)
+    keep = np.ones(len(ids), dtype=bool)
+    for i, pid in enumerate(ids):
+        r0 = _philox2([pid, step], key)
+        R0 = _u53(r0[0], r0[1])
+        if abs(R0 - p_null) < 1e-12 * p_null:
+            keep[i] = False
+        if R0 >= p_null:
+            continue
+        r = _philox4([pid, step, 4, 0], [k0, k1])
+        R1 = _u53(r[2], r[3])
+        v_sqr = (vx0[i] ** 2 + vy0[i] ** 2 + vz0[i] ** 2) * d.vel_norm ** 2
+        en = 0.5 * ME * v_sqr / config.eV
+        t = (math.log(en) - loge0) * inv
+        if t <= 0:
+            sig = tab[0][0]
+        elif t >= n - 1:
+            sig = tab[0][n - 1]
+        else:
+            kk = int(t)
+            f = t - kk
+            sig = tab[0][kk] * (1 - f) + tab[0][kk + 1] * f
+        pc = -math.expm1(-gas * sig * math.sqrt(v_sqr) * dt)
+        if abs(R1 * p_null - pc) < 1e-12 * pc:
+            keep[i] = False
+        want[i] = R1 * p_null < pc
+    print(f"  p_null {p_null:.3f}: {int(want.sum())} predicted, {int(hit.sum())} collided, "
+          f"{int((~keep).sum())} at a threshold")
+    assert want.sum() > 1000
+    assert (~keep).sum() < 5
+    assert np.array_equal(hit[keep], want[keep])
 
 
 def test_energy_removed_equals_thresholds():
@@ -176,10 +273,10 @@ def test_mcc_device_bytes_match_estimate():
 
 
 def test_candidate_lists_independent_of_layout():
-    """The two-pass collision step (candidate selection per 2048-slot chunk,
-    then one candidate per lane) gives each particle the outcome of its own
-    (id, step) draws: the same particles, reordered in memory by the cell
-    sort, end with the same velocities."""
+    """The two-pass collision step (candidate selection and null test per
+    512-slot chunk, then one real collision per lane) gives each particle
+    the outcome of its own (id, step) draws: the same particles, reordered in
+    memory by the cell sort, end with the same velocities."""
     print("[mcc] layout independence", flush=True)
     N = 70001  # a partial last chunk
     st = _setup(N, 8.0)

@@ -44,6 +44,19 @@ def test_philox_known_answers(ctr, key, out):
     assert list(O) == out
 
 
+@pytest.mark.parametrize("ctr,key,out", [
+    ([0, 0], 0, [0xff1dae59, 0x6cd10df2]),
+    ([0xffffffff] * 2, 0xffffffff, [0x2c3f628b, 0xab4fd7ad]),
+    ([0x243f6a88, 0x85a308d3], 0x13198a2e, [0xdd7ce038, 0xf62a4c12]),
+])
+def test_philox2x32_known_answers(ctr, key, out):
+    """Random123 kat_vectors for philox2x32_10."""
+    C = (ctypes.c_uint32 * 2)(*ctr)
+    O = (ctypes.c_uint32 * 2)()
+    oracle().orc_philox2x32_10(C, key, O)
+    assert list(O) == out
+
+
 # ---- set-up / normalisation -------------------------------------------------------
 def _orc_setup(specs, Nx, Ny, density, DT_coeff, norm_scheme=1):
     n = len(specs)
