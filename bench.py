@@ -44,10 +44,14 @@ def gpu_run(st, steps, warmup, solver_name, mcc=None):
     if mcc is not None:
         ctx.mcc_setup(mcc)
     ctx.initial_field()
+    ctx.sync()
+    print("[bench] set up: particles loaded and sorted, initial field", file=sys.stderr, flush=True)
     ts = 0
     for _ in range(warmup):
         ctx.step(ts)
         ts += 1
+    ctx.sync()
+    print(f"[bench] {warmup} warm-up steps done", file=sys.stderr, flush=True)
     n0 = sum(ctx.count(s) for s in range(nsp))
     per_sp0 = [ctx.count(s) for s in range(nsp)]
     # the timed steps record only the push's event pair (the roofline's
@@ -61,6 +65,7 @@ def gpu_run(st, steps, warmup, solver_name, mcc=None):
         ts += 1
     ctx.sync()
     t1 = time.perf_counter()
+    print(f"[bench] {steps} timed steps done", file=sys.stderr, flush=True)
     tm = ctx.timing_read()
     n1 = sum(ctx.count(s) for s in range(nsp))
     per_sp1 = [ctx.count(s) for s in range(nsp)]
@@ -72,6 +77,7 @@ def gpu_run(st, steps, warmup, solver_name, mcc=None):
         ctx.step(ts)
         ts += 1
     ctx.sync()
+    print(f"[bench] {steps} stage-timed steps done", file=sys.stderr, flush=True)
     stages = ctx.timing_read()
     ctx.timing(False)
     errs = ctx.errors()
