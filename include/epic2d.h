@@ -291,7 +291,7 @@ int epic_timing_read(epic_ctx* ctx, double ms[EPIC_NSTAGE], int64_t calls[EPIC_N
 int epic_estimate_bytes(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp, int64_t* bytes);
 int epic_device_bytes(epic_ctx* ctx, int64_t* bytes);
 /* Device bytes epic_mcc_setup adds to a context of these species (tables,
- * ionisation event list, candidate lists of the colliding species). */
+ * ionisation event list, real-collision lists of the colliding species). */
 int epic_mcc_estimate_bytes(const epic_mcc_desc* d, const epic_species_desc* sp, int nsp, int64_t* bytes);
 
 /* ---- batched ensemble --------------------------------------------------- */
