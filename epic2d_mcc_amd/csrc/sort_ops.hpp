@@ -117,7 +117,10 @@ __device__ __forceinline__ int wave_sort(int v) {
 // below 2^25, multiples of 256, so no word reaches PK_NONE; slots past the
 // end as PK_NONE) and stores them at the chunk's own slots of pks (those
 // below n), for the slot reservation.
-constexpr int HWIN = 4096;
+// HWIN covers a chunk's keys (2048 slots span a few dozen cells, and a
+// particle that changed cell within its tile moved its key by at most a
+// tile's 256) with a window that is cheap to clear and flush per chunk.
+constexpr int HWIN = 1024;
 constexpr int PK_NONE = 0x7fffffff;
 
 template <typename P>
@@ -253,23 +256,18 @@ __device__ __forceinline__ void fix_unordered(uint32_t* perm, uint32_t* scratch,
   }
 }
 
-// Ranks of one cell of m <= 64 * CNT entries from its runs.  An entry's
-// source lies in the 64-slot group (source / 64) of the wavefront that
-// reserved its slot; a group's entries of one cell are one run of
-// consecutive slots in ascending order (one reservation), and groups are
-// disjoint ranges of sources.  So an entry's rank in the sorted cell is the
-// length of every run of a smaller group plus its index in its own run: the
-// cell is ordered by walking its runs (a handful) instead of comparing every
-// pair of entries.  Entry t of the cell is lane t % 64, register t / 64.
+// Ranks of one cell of m <= 64 * CNT entries (v: entry t in lane t % 64,
+// register t / 64; entries past m all-ones) from its runs.  An entry's source
+// lies in the 64-slot group (source / 64) of the wavefront that reserved its
+// slot; a group's entries of one cell are one run of consecutive slots in
+// ascending order (one reservation), and groups are disjoint ranges of
+// sources.  So an entry's rank in the sorted cell is the length of every run
+// of a smaller group plus its index in its own run: the cell is ordered by
+// walking its runs (a handful) instead of comparing every pair of entries.
+// Writes only when some entry moves.
 template <int CNT>
-__device__ __forceinline__ void fix_by_runs(uint32_t* perm, int b, int m) {
+__device__ __forceinline__ void fix_runs_regs(uint32_t* perm, int b, int m, const uint32_t (&v)[CNT]) {
   const int lane = threadIdx.x & 63;
-  uint32_t v[CNT];
-#pragma unroll
-  for (int q = 0; q < CNT; q++) {
-    const int t = lane + 64 * q;
-    v[q] = t < m ? perm[b + t] : 0xFFFFFFFFu;
-  }
   // run heads: the first entry and every entry whose group differs from the
   // previous entry's
   unsigned long long H[CNT];
@@ -314,9 +312,24 @@ __device__ __forceinline__ void fix_by_runs(uint32_t* perm, int b, int m) {
     if (lane + 64 * q < m) perm[b + rank[q]] = v[q];
 }
 
-// FIX_GROUP consecutive cells per wavefront, each read once: cells of up to
-// 256 entries are ordered by their runs (fix_by_runs), larger ones are
-// checked for a descent and sorted by rank counting when they have one.
+template <int CNT>
+__device__ __forceinline__ void fix_by_runs(uint32_t* perm, int b, int m) {
+  const int lane = threadIdx.x & 63;
+  uint32_t v[CNT];
+#pragma unroll
+  for (int q = 0; q < CNT; q++) {
+    const int t = lane + 64 * q;
+    v[q] = t < m ? perm[b + t] : 0xFFFFFFFFu;
+  }
+  fix_runs_regs<CNT>(perm, b, m, v);
+}
+
+// FIX_GROUP consecutive cells per wavefront, each read once.  When every cell
+// of the group has at most 128 entries (the common case), all the group's
+// entries are loaded at once, so the wavefront waits for memory once per
+// group rather than once per cell.  Otherwise cell by cell: cells of up to
+// 256 entries are ordered by their runs, larger ones checked for a descent
+// and sorted by rank counting when they have one.
 constexpr int FIX_GROUP = 8;
 
 __device__ __forceinline__ void fix_cells(const int* start, uint32_t* perm, uint32_t* scratch, int c0, int ncell,
@@ -324,6 +337,27 @@ __device__ __forceinline__ void fix_cells(const int* start, uint32_t* perm, uint
   const int lane = threadIdx.x & 63;
   const int nc = min(FIX_GROUP, ncell - c0);
   const int sv = lane <= nc ? start[c0 + lane] : 0;
+  // the group's largest cell: lanes q < nc hold cell q's size (the shuffle
+  // runs on every lane: lane nc - 1 reads lane nc's start)
+  const int sv_next = __shfl_down(sv, 1, 64);
+  const int msz = lane < nc ? sv_next - sv : 0;
+  if (wave_max(msz) <= 128) {
+    uint32_t v[FIX_GROUP][2];
+#pragma unroll
+    for (int q = 0; q < FIX_GROUP; q++) {
+      const int b = __builtin_amdgcn_readlane(sv, q);
+      const int m = q < nc ? __builtin_amdgcn_readlane(sv, q + 1) - b : 0;
+      v[q][0] = lane < m ? perm[b + lane] : 0xFFFFFFFFu;
+      v[q][1] = lane + 64 < m ? perm[b + 64 + lane] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int q = 0; q < FIX_GROUP; q++) {
+      const int b = __builtin_amdgcn_readlane(sv, q);
+      const int m = q < nc ? __builtin_amdgcn_readlane(sv, q + 1) - b : 0;
+      if (m > 1) fix_runs_regs<2>(perm, b, m, v[q]);
+    }
+    return;
+  }
   for (int q = 0; q < nc; q++) {
     const int b = __builtin_amdgcn_readlane(sv, q);
     const int m = __builtin_amdgcn_readlane(sv, q + 1) - b;
