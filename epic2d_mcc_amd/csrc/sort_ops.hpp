@@ -256,53 +256,151 @@ __device__ __forceinline__ void fix_unordered(uint32_t* perm, uint32_t* scratch,
   }
 }
 
+// Ascending bitonic sort of one unsigned per lane across the wavefront.
+__device__ __forceinline__ uint32_t wave_sort_u(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#define EPIC_BSTEP_U(K, J)                                     \
+  {                                                            \
+    const uint32_t o = (uint32_t)xor_lane<J>((int)v);          \
+    const bool asc = (lane & K) == 0, lower = (lane & J) == 0; \
+    v = (lower == asc) ? min(v, o) : max(v, o);                \
+  }
+  EPIC_BSTEP_U(2, 1)
+  EPIC_BSTEP_U(4, 2) EPIC_BSTEP_U(4, 1)
+  EPIC_BSTEP_U(8, 4) EPIC_BSTEP_U(8, 2) EPIC_BSTEP_U(8, 1)
+  EPIC_BSTEP_U(16, 8) EPIC_BSTEP_U(16, 4) EPIC_BSTEP_U(16, 2) EPIC_BSTEP_U(16, 1)
+  EPIC_BSTEP_U(32, 16) EPIC_BSTEP_U(32, 8) EPIC_BSTEP_U(32, 4) EPIC_BSTEP_U(32, 2) EPIC_BSTEP_U(32, 1)
+  EPIC_BSTEP_U(64, 32) EPIC_BSTEP_U(64, 16) EPIC_BSTEP_U(64, 8) EPIC_BSTEP_U(64, 4) EPIC_BSTEP_U(64, 2)
+  EPIC_BSTEP_U(64, 1)
+#undef EPIC_BSTEP_U
+  return v;
+}
+
+// Set bits of m strictly below position p (0 <= p <= 64).
+__device__ __forceinline__ int popc_below(unsigned long long m, int p) {
+  return p >= 64 ? (int)__popcll(m) : (int)__popcll(m & ((1ull << p) - 1));
+}
+
 // Ranks of one cell of m <= 64 * CNT entries (v: entry t in lane t % 64,
 // register t / 64; entries past m all-ones) from its runs.  An entry's source
 // lies in the 64-slot group (source / 64) of the wavefront that reserved its
 // slot; a group's entries of one cell are one run of consecutive slots in
 // ascending order (one reservation), and groups are disjoint ranges of
 // sources.  So an entry's rank in the sorted cell is the length of every run
-// of a smaller group plus its index in its own run: the cell is ordered by
-// walking its runs (a handful) instead of comparing every pair of entries.
-// Writes only when some entry moves.
+// of a smaller group plus its index in its own run.  A few runs are walked
+// one by one; more (up to 64) are gathered one per lane, sorted by group
+// with a bitonic sort across the wavefront, and their lengths scanned in
+// that order, which gives each run's first rank.  Writes only when some
+// entry moves.
 template <int CNT>
 __device__ __forceinline__ void fix_runs_regs(uint32_t* perm, int b, int m, const uint32_t (&v)[CNT]) {
   const int lane = threadIdx.x & 63;
   // run heads: the first entry and every entry whose group differs from the
   // previous entry's
   unsigned long long H[CNT];
+  int cb[CNT + 1];  // heads in the registers before q
+  cb[0] = 0;
 #pragma unroll
   for (int q = 0; q < CNT; q++) {
     const int t = lane + 64 * q;
     uint32_t prev = (uint32_t)__shfl_up((int)v[q], 1, 64);
     if (lane == 0) prev = q > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)v[q > 0 ? q - 1 : 0], 63) : 0xFFFFFFFFu;
     H[q] = __builtin_amdgcn_ballot_w64(t < m && (t == 0 || (v[q] >> 6) != (prev >> 6)));
+    cb[q + 1] = cb[q] + (int)__popcll(H[q]);
   }
-  int rank[CNT];
+  const int R = cb[CNT];
+  // the groups' spread, for the packed sort keys (group offset << 8 | start)
+  uint32_t gmin = 0xFFFFFFFFu, gmax = 0;
 #pragma unroll
-  for (int q = 0; q < CNT; q++) rank[q] = 0;
-  auto close_run = [&](int th, uint32_t g, int end) {
+  for (int q = 0; q < CNT; q++) {
+    const bool in = lane + 64 * q < m;
+    gmin = min(gmin, in ? v[q] >> 6 : 0xFFFFFFFFu);
+    gmax = max(gmax, in ? v[q] >> 6 : 0u);
+  }
+  gmin = (uint32_t)wave_min((int)min(gmin, 0x7FFFFFFFu));
+  gmax = (uint32_t)wave_max((int)min(gmax, 0x7FFFFFFFu));
+  int rank[CNT];
+  if (R > 8 && R <= 64 && gmax - gmin < (1u << 24) && 64 * CNT <= 256) {
+    // lane r < R: run r (in slot order) -- its head's register and lane by a
+    // binary search for the (r - cb[q])-th set bit of H[q]
+    int qr = 0;
+#pragma unroll
+    for (int q = 1; q < CNT; q++) qr += lane >= cb[q] ? 1 : 0;
+    unsigned long long hm = H[0];
+    int kb = 0;
+#pragma unroll
+    for (int q = 1; q < CNT; q++)
+      if (qr == q) {
+        hm = H[q];
+        kb = cb[q];
+      }
+    const int k = lane - kb;
+    int lo = 0;
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1)
+      if (popc_below(hm, lo + st) <= k) lo += st;
+    // the head's group (read from its lane of every register, the right one kept)
+    uint32_t gh = 0;
 #pragma unroll
     for (int q = 0; q < CNT; q++) {
-      const int t = lane + 64 * q;
-      rank[q] += (v[q] >> 6) > g ? end - th : 0;
-      rank[q] += (t >= th && t < end) ? t - th : 0;
+      const uint32_t x = (uint32_t)__shfl((int)v[q], lo, 64);
+      gh = qr == q ? x : gh;
     }
-  };
-  int prev_th = -1;
-  uint32_t prev_g = 0;
+    const int t_r = 64 * qr + lo;  // run r's first entry (lanes r < R)
+    const int t_next = __shfl_down(t_r, 1, 64);
+    const int len_r = lane + 1 < R ? t_next - t_r : m - t_r;
+    const uint32_t key = lane < R ? (((gh >> 6) - gmin) << 8) | (uint32_t)t_r : 0xFFFFFFFFu;
+    const uint32_t sk = wave_sort_u(key);
+    // sorted position k: its run's index in slot order (from its start) and length
+    const int ts = (int)(sk & 255u);
+    int rs = 0;
 #pragma unroll
-  for (int qh = 0; qh < CNT; qh++) {
-    for (unsigned long long h = H[qh]; h; h &= h - 1) {
-      const int lh = (int)__builtin_ctzll(h);
-      const int th = lh + 64 * qh;
-      const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)v[qh], lh) >> 6;
-      if (prev_th >= 0) close_run(prev_th, prev_g, th);
-      prev_th = th;
-      prev_g = g;
+    for (int q = 0; q < CNT; q++) rs += (ts >> 6) > q ? cb[q + 1] - cb[q] : ((ts >> 6) == q ? popc_below(H[q], ts & 63) : 0);
+    const int lk = __shfl(len_r, lane < R ? rs : 0, 64);
+    // exclusive scan of the lengths in group order
+    int inc = lane < R ? lk : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(inc, off, 64);
+      inc += lane >= off ? o : 0;
     }
+    const int base_k = inc - (lane < R ? lk : 0);
+    // each run's first rank back to lane r (a permutation: sorted lanes k < R
+    // send to their run's lane, the rest to themselves)
+    const int base_r = __builtin_amdgcn_ds_permute((lane < R ? rs : lane) * 4, base_k);
+#pragma unroll
+    for (int q = 0; q < CNT; q++) {
+      // entry t's run: the heads at or before it, minus one
+      const int r = cb[q] + popc_below(H[q], lane + 1) - 1;
+      const int rr = max(r, 0);
+      rank[q] = __shfl(base_r, rr, 64) + (lane + 64 * q) - __shfl(t_r, rr, 64);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < CNT; q++) rank[q] = 0;
+    auto close_run = [&](int th, uint32_t g, int end) {
+#pragma unroll
+      for (int q = 0; q < CNT; q++) {
+        const int t = lane + 64 * q;
+        rank[q] += (v[q] >> 6) > g ? end - th : 0;
+        rank[q] += (t >= th && t < end) ? t - th : 0;
+      }
+    };
+    int prev_th = -1;
+    uint32_t prev_g = 0;
+#pragma unroll
+    for (int qh = 0; qh < CNT; qh++) {
+      for (unsigned long long h = H[qh]; h; h &= h - 1) {
+        const int lh = (int)__builtin_ctzll(h);
+        const int th = lh + 64 * qh;
+        const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)v[qh], lh) >> 6;
+        if (prev_th >= 0) close_run(prev_th, prev_g, th);
+        prev_th = th;
+        prev_g = g;
+      }
+    }
+    if (prev_th >= 0) close_run(prev_th, prev_g, m);
   }
-  if (prev_th >= 0) close_run(prev_th, prev_g, m);
   bool moved = false;
 #pragma unroll
   for (int q = 0; q < CNT; q++) moved = moved || (lane + 64 * q < m && rank[q] != lane + 64 * q);
