@@ -44,13 +44,13 @@ struct TileGemmT {
 };
 using TileGemm = TileGemmT<float>;
 
-// Epilogue: the four wavefronts' partial tiles summed in wavefront order.
-template <typename T>
+// Epilogue: the NW wavefronts' partial tiles summed in wavefront order.
+template <typename T, int NW = 4>
 __device__ __forceinline__ void tile_store(const TileGemmT<T>& t, T (*part)[32 * 32], int tm, int tn, T* C) {
-  for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+  for (int q = threadIdx.x; q < 32 * 32; q += blockDim.x) {
     T v = part[0][q] + part[1][q];
-    v += part[2][q];
-    v += part[3][q];
+#pragma unroll
+    for (int w = 2; w < NW; w++) v += part[w][q];
     const int i = tm + (q & 31), j = tn + (q >> 5);
     const size_t o = (size_t)i + (size_t)j * t.ldc;
     if (t.G) v *= t.G[o];
@@ -179,6 +179,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   tile_store<double>(t, part, tm, tn, C);
 }
 
+// FP64, K = 128 NB: eight wavefronts, each a contiguous eighth of K (NB
+// 16-deep blocks), every operand load of that slice issued before the first
+// product (160 registers at NB = 4: two wavefronts per SIMD), so a
+// wavefront waits for memory once; the eight partial tiles summed in LDS in
+// wavefront order.
+template <int NB>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) void dgemm_tile32_w8_kernel(
+    TileGemmT<double> t) {
+  __shared__ double part[8][32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tm = blockIdx.x * 32, tn = blockIdx.y * 32;
+  const double* A = t.A + blockIdx.z * t.sA;
+  const double* B = t.B + blockIdx.z * t.sB;
+  double* C = t.C + blockIdx.z * t.sC;
+  const int r16 = lane & 15, g = lane >> 4;
+  const int k0 = w * NB * 16;
+  v4d b[NB][2];
+  double a[NB][2][4];
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int kb = k0 + 16 * u;
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = tn + cb * 16 + r16;
+      b[u][cb] = *reinterpret_cast<const v4d*>(B + (size_t)col * t.ldb + kb + 4 * g);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) a[u][rb][e] = A[(size_t)(tm + rb * 16 + r16) + (size_t)(kb + 4 * g + e) * t.lda];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  v4d acc[2][2];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) acc[rb][cb] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < NB; u++)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][rb][e], b[u][cb][e], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) part[w][(rb * 16 + 4 * g + r) + (cb * 16 + r16) * 32] = acc[rb][cb][r];
+  __syncthreads();
+  tile_store<double, 8>(t, part, tm, tn, C);
+}
+
 // FP32 for K that is not a multiple of 64 or beyond the register-resident
 // slice (K > TILE32_MAX_K): the FP64 kernel's round structure, U 16-deep
 // blocks per round.
@@ -265,7 +321,15 @@ inline hipError_t gemm_tile32(hipStream_t st, int m, int n, int k, const TileGem
 inline hipError_t gemm_tile32(hipStream_t st, int m, int n, int k, const TileGemmT<double>& t, int nbatch = 1) {
   if (!tile32_ok(m, n, k) || nbatch < 1) return hipErrorInvalidValue;
   const dim3 grid(m / 32, n / 32, nbatch);
-  hipLaunchKernelGGL(dgemm_tile32_kernel<2>, grid, dim3(256), 0, st, t, k);
+  // K = 128 NB up to 512: the eight-wavefront kernel with the whole slice in
+  // registers; other K the round-by-round kernel
+  switch (k % 128 == 0 ? k / 128 : 0) {
+    case 1: hipLaunchKernelGGL(dgemm_tile32_w8_kernel<1>, grid, dim3(512), 0, st, t); break;
+    case 2: hipLaunchKernelGGL(dgemm_tile32_w8_kernel<2>, grid, dim3(512), 0, st, t); break;
+    case 3: hipLaunchKernelGGL(dgemm_tile32_w8_kernel<3>, grid, dim3(512), 0, st, t); break;
+    case 4: hipLaunchKernelGGL(dgemm_tile32_w8_kernel<4>, grid, dim3(512), 0, st, t); break;
+    default: hipLaunchKernelGGL(dgemm_tile32_kernel<2>, grid, dim3(256), 0, st, t, k); break;
+  }
   return hipGetLastError();
 }
 
