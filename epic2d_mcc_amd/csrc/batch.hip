@@ -350,8 +350,12 @@ __global__ __launch_bounds__(256) void bsort_fix_kernel(const SortSeg* segs, int
   }
 }
 
+// Each XCD walks a contiguous range of chunks with one workgroup per CU, as
+// the single-simulation gather does (sort_gather_kernel in aux.hip).
 __global__ __launch_bounds__(256) void bsort_gather_kernel(const SortSeg* segs, int nseg, int64_t nchunks) {
-  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+  int64_t cb, ce, cs;
+  xcd_chunk_range(nchunks, &cb, &ce, &cs);
+  for (int64_t c = cb; c < ce; c += cs) {
     const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
     const int64_t n = S.count[S.km.K], base = (c - S.chunk0) * SORT_CHUNK;
     if (base < n)
@@ -895,7 +899,8 @@ static hipError_t batch_sort_round(Batch* B, const std::vector<Ctx*>& due, int r
   hipLaunchKernelGGL(bsort_fix_kernel, dim3((int)std::min<int64_t>(items, 65536)), dim3(256), 0, B->stream, S, nseg,
                      items);
   EPIC_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bsort_gather_kernel, dim3(gpart), dim3(256), 0, B->stream, S, nseg, chunks);
+  hipLaunchKernelGGL(bsort_gather_kernel, dim3((int)std::min<int64_t>(chunks, (int64_t)B->num_cu)), dim3(256), 0,
+                     B->stream, S, nseg, chunks);
   EPIC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bsort_setn_kernel, dim3(1), dim3(256), 0, B->stream, S, nseg);
   EPIC_HIP_TRY(hipGetLastError());

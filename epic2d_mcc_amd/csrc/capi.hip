@@ -210,6 +210,7 @@ int ctx_create(const epic_domain_desc* dom, const epic_species_desc* sp, int nsp
   if (const char* v = getenv("EPIC_PUSH64")) c->push64 = atoi(v);
   if (const char* v = getenv("EPIC_MCC_OCC")) c->mcc_occ = atoi(v);
   if (const char* v = getenv("EPIC_MCC_SEL_GRID")) c->mcc_sel_grid = atoi(v);
+  if (const char* v = getenv("EPIC_SORT_GATHER_WG")) c->sort_gather_wg = std::max(1, atoi(v));
   if (const char* v = getenv("EPIC_PUSH_MULTI")) c->push_multi = atoi(v);
   if (const char* v = getenv("EPIC_COND_CELL_MASK")) c->cond_cell_mask = atoi(v);
   if (const char* v = getenv("EPIC_PUSH_MULTI_WAVES")) c->push_multi_waves = atoi(v);

@@ -230,6 +230,7 @@ struct Ctx {
   int push64 = 1;              // FP64 CIC windowed push: 1 dedicated kernel, 0 generic
   int mcc_occ = 2;             // MCC collision pass: register budget for 1 or 2 workgroups per CU
   int mcc_sel_grid = 1;        // MCC selection pass: 1 resident workgroups, 0 one chunk per wavefront
+  int sort_gather_wg = 1;      // sort gather: workgroups per CU
   int push_multi = 1;          // one launch for all pushed FP32 species (EPIC_PUSH_MULTI=0: one per species)
   int push_multi_waves = 8;    // register budget of that launch: 8 waves per SIMD (64 VGPRs), 1: the compiler's choice
   int cond_cell_mask = 1;      // 0: every cell flagged (the exact test everywhere)
