@@ -79,10 +79,22 @@ __device__ __forceinline__ LaneRun lane_run(int key) {
   return r;
 }
 
-// Lane-xor exchange: ds_swizzle's xor mode inside 32-lane halves, a
-// permute across them.
+// Lane-xor exchange.  Inside a 16-lane row the exchange is a DPP operand
+// (no LDS round trip): xor 1 and xor 2 are quad permutations, xor 8 a
+// rotation of the row by 8, xor 4 the row shifted down 4 for lanes with bit
+// 2 clear and up 4 for the others; xor 16 is ds_swizzle's xor mode inside
+// 32-lane halves, xor 32 a permute across them.
 template <int J>
 __device__ __forceinline__ int xor_lane(int v) {
+  if (J == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, true);   // quad_perm [1,0,3,2]
+  if (J == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, true);   // quad_perm [2,3,0,1]
+  if (J == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, true);  // row_ror:8
+  if (J == 4) {
+    // row_shl:4 reads lane + 4 (0 past the row's end: those lanes take the other)
+    const int from_above = __builtin_amdgcn_mov_dpp(v, 0x104, 0xf, 0xf, true);
+    const int from_below = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, true);  // row_shr:4, lane - 4
+    return (threadIdx.x & 4) ? from_below : from_above;
+  }
   if (J < 32) return __builtin_amdgcn_ds_swizzle(v, (J << 10) | 0x1F);
   return __shfl_xor(v, J, 64);
 }
