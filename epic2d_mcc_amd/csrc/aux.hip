@@ -257,10 +257,12 @@ __global__ __launch_bounds__(256) void sort_hist_kernel(const P* x, const P* y, 
 
 __global__ __launch_bounds__(256) void sort_fill_kernel(const uint32_t* pks, const int64_t* d_n, const int* start,
                                                         int* fill, uint32_t* perm, KeyMap m) {
+  __shared__ int win[HWIN];
+  __shared__ int wbase_s;
   const int64_t n = *d_n;
   const int64_t stride = (int64_t)gridDim.x * SORT_CHUNK;
   for (int64_t base = blockIdx.x * (int64_t)SORT_CHUNK; base < n; base += stride)
-    fill_chunk(pks, n, base, start, fill, perm, m);
+    fill_chunk(pks, n, base, start, fill, perm, m, win, &wbase_s);
 }
 
 // One wavefront per FIX_GROUP cells.
@@ -288,8 +290,14 @@ __global__ __launch_bounds__(256) void sort_gather_kernel(const uint32_t* perm, 
   const int64_t n = *total;
   int64_t cb, ce, cs;
   xcd_chunk_range((n + SORT_CHUNK - 1) / SORT_CHUNK, &cb, &ce, &cs);
-  for (int64_t chunk = cb; chunk < ce; chunk += cs)
-    gather_chunk(perm, n, chunk * SORT_CHUNK, x, y, vx, vy, vz, id, x2, y2, vx2, vy2, vz2, id2);
+  uint32_t p[SORT_U], pn[SORT_U];
+  if (cb < ce) gather_sources(perm, n, cb * SORT_CHUNK, pn);
+  for (int64_t chunk = cb; chunk < ce; chunk += cs) {
+#pragma unroll
+    for (int u = 0; u < SORT_U; u++) p[u] = pn[u];
+    const int64_t next = chunk + cs < ce ? (chunk + cs) * SORT_CHUNK : -1;
+    gather_move(p, n, chunk * SORT_CHUNK, x, y, vx, vy, vz, id, x2, y2, vx2, vy2, vz2, id2, perm, next, pn);
+  }
 }
 
 __global__ void zero_int_kernel(int* p, int64_t n) {

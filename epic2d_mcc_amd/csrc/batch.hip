@@ -333,10 +333,12 @@ __global__ __launch_bounds__(1024) void bsort_scan_kernel(const SortSeg* segs) {
 }
 
 __global__ __launch_bounds__(256) void bsort_fill_kernel(const SortSeg* segs, int nseg, int64_t nchunks) {
+  __shared__ int win[HWIN];
+  __shared__ int wbase_s;
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const SortSeg& S = segs[find_seg(segs, nseg, c, false)];
     const int64_t n = *S.d_n, base = (c - S.chunk0) * SORT_CHUNK;
-    if (base < n) fill_chunk(S.scratch, n, base, S.count, S.fill, S.perm, S.km);
+    if (base < n) fill_chunk(S.scratch, n, base, S.count, S.fill, S.perm, S.km, win, &wbase_s);
   }
 }
 

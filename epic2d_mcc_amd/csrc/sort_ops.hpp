@@ -139,11 +139,21 @@ template <typename P>
 __device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, int64_t base, int* count, P x0, P y0,
                                            P dx, P dy, const KeyMap& m, int* win, int* wbase_s, uint32_t* pks) {
   const int lane = threadIdx.x & 63;
+  // every load first, from in-range slots (past the end: the last live one,
+  // not used), so that all 2 SORT_U are in flight together
+  P xv[SORT_U], yv[SORT_U];
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
+    const int64_t q = pslot<P>(p < n ? p : n - 1);
+    xv[u] = x[q];
+    yv[u] = y[q];
+  }
   int key[SORT_U];
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-    key[u] = p < n ? cell_key(x[pslot<P>(p)], y[pslot<P>(p)], x0, y0, dx, dy, m) : -1;
+    key[u] = p < n ? cell_key(xv[u], yv[u], x0, y0, dx, dy, m) : -1;
   }
   if (threadIdx.x == 0) *wbase_s = key[0] >= 0 && key[0] < m.K ? max(0, key[0] - HWIN / 2) : 0;
   for (int t = threadIdx.x; t < HWIN; t += blockDim.x) win[t] = 0;
@@ -172,32 +182,64 @@ __device__ __forceinline__ void hist_chunk(const P* x, const P* y, int64_t n, in
 }
 
 // Slots of one chunk from the histogram's sorted (key, lane) words: equal
-// keys form one run in lane (= source) order; the run's first lane reserves
-// len consecutive slots with one atomic.  Absorbed particles (key K) are not
-// placed.
+// keys form one run in lane (= source) order, which takes len consecutive
+// slots.  The runs of keys inside an LDS window of HWIN keys (centred on the
+// chunk's first key, as the histogram's) take their offsets inside the
+// chunk's block of that key with an LDS atomic; one global atomic per key
+// then reserves the chunk's block.  Runs of keys outside the window reserve
+// their slots with a global atomic each.  Absorbed particles (key K) are
+// not placed.  (Block-uniform call; win and wbase_s in LDS.)
 __device__ __forceinline__ void fill_chunk(const uint32_t* pks, int64_t n, int64_t base, const int* start, int* fill,
-                                           uint32_t* perm, const KeyMap& m) {
+                                           uint32_t* perm, const KeyMap& m, int* win, int* wbase_s) {
   const int dead = m.K;
   const int lane = threadIdx.x & 63;
   int pk[SORT_U];
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t p = base + (int64_t)u * blockDim.x + threadIdx.x;
-    pk[u] = p < n ? (int)pks[p] : PK_NONE;
+    const int v = (int)pks[p < n ? p : n - 1];
+    pk[u] = p < n ? v : PK_NONE;
   }
+  if (threadIdx.x == 0) {
+    const int k0 = pk[0] == PK_NONE ? -1 : pk[0] >> 6;
+    *wbase_s = k0 >= 0 && k0 < m.K ? max(0, k0 - HWIN / 2) : 0;
+  }
+  for (int t = threadIdx.x; t < HWIN; t += blockDim.x) win[t] = 0;
+  __syncthreads();
+  const int wbase = *wbase_s;
+  int off[SORT_U];
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int key = pk[u] == PK_NONE ? -1 : pk[u] >> 6;
     const int ks = key != dead ? key : -1;
     const LaneRun r = lane_run(ks);
-    int b = 0;
-    if (r.head == lane && ks >= 0) b = atomicAdd(&fill[ks], r.len);
-    b = __shfl(b, r.head, 64);
+    int o = 0;
+    if (r.head == lane && ks >= 0) {
+      const int loc = ks - wbase;
+      o = loc >= 0 && loc < HWIN ? atomicAdd(&win[loc], r.len) : atomicAdd(&fill[ks], r.len);
+    }
+    off[u] = o;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < HWIN; t += blockDim.x) {
+    const int c = win[t];
+    if (c) win[t] = atomicAdd(&fill[wbase + t], c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < SORT_U; u++) {
+    const int key = pk[u] == PK_NONE ? -1 : pk[u] >> 6;
+    const int ks = key != dead ? key : -1;
+    const LaneRun r = lane_run(ks);
+    const int o = __shfl(off[u], r.head, 64);
     if (ks >= 0) {
+      const int loc = ks - wbase;
+      const int blk = loc >= 0 && loc < HWIN ? win[loc] : 0;
       const int64_t p = base + (int64_t)u * blockDim.x + (threadIdx.x & ~63) + (pk[u] & 63);
-      perm[start[ks] + b + (lane - r.head)] = (uint32_t)p;
+      perm[start[ks] + blk + o + (lane - r.head)] = (uint32_t)p;
     }
   }
+  __syncthreads();
 }
 
 // One cell's sources ascending (one wavefront): rank counting (rank = number
@@ -491,26 +533,34 @@ __device__ __forceinline__ void fix_cells(const int* start, uint32_t* perm, uint
 // stores, so that the L2 keeps the source lines, which neighbouring chunks
 // read again (a particle that changed cell is read from its old cell's run):
 // 5 % faster than plain stores on a 26.2M-particle species.
-template <typename P>
-__device__ __forceinline__ void gather_chunk(const uint32_t* perm, int64_t n, int64_t base, const P* x, const P* y,
-                                             const P* vx, const P* vy, const P* vz, const uint32_t* id, P* x2,
-                                             P* y2, P* vx2, P* vy2, P* vz2, uint32_t* id2) {
-  uint32_t p[SORT_U];
+// Source indices of one chunk.
+// Source indices of one chunk (base < n); a slot past the end takes the
+// last live slot's source, so that every load of the move is in range and
+// none needs a branch (a branch per particle makes each wait for the last).
+__device__ __forceinline__ void gather_sources(const uint32_t* perm, int64_t n, int64_t base, uint32_t (&p)[SORT_U]) {
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-    p[u] = i < n ? __builtin_nontemporal_load(&perm[i]) : 0u;
+    p[u] = __builtin_nontemporal_load(&perm[i < n ? i : n - 1]);
   }
+}
+
+// Moves one chunk's particles from their sources p.  When nbase >= 0 the
+// next chunk's source indices are loaded into pn after this chunk's reads
+// are issued, so that their latency overlaps this chunk's.
+template <typename P>
+__device__ __forceinline__ void gather_move(const uint32_t (&p)[SORT_U], int64_t n, int64_t base, const P* x,
+                                            const P* y, const P* vx, const P* vy, const P* vz, const uint32_t* id,
+                                            P* x2, P* y2, P* vx2, P* vy2, P* vz2, uint32_t* id2,
+                                            const uint32_t* perm, int64_t nbase, uint32_t (&pn)[SORT_U]) {
   P a[SORT_U], b[SORT_U], c[SORT_U], d[SORT_U], e[SORT_U];
   uint32_t f[SORT_U];
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
-    const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
-    if (i < n) {
-      const int64_t q = pslot<P>(p[u]);
-      a[u] = x[q]; b[u] = y[q]; c[u] = vx[q]; d[u] = vy[q]; e[u] = vz[p[u]]; f[u] = id[p[u]];
-    }
+    const int64_t q = pslot<P>(p[u]);
+    a[u] = x[q]; b[u] = y[q]; c[u] = vx[q]; d[u] = vy[q]; e[u] = vz[p[u]]; f[u] = id[p[u]];
   }
+  if (nbase >= 0) gather_sources(perm, n, nbase, pn);
 #pragma unroll
   for (int u = 0; u < SORT_U; u++) {
     const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
@@ -524,6 +574,15 @@ __device__ __forceinline__ void gather_chunk(const uint32_t* perm, int64_t n, in
       __builtin_nontemporal_store(f[u], &id2[i]);
     }
   }
+}
+
+template <typename P>
+__device__ __forceinline__ void gather_chunk(const uint32_t* perm, int64_t n, int64_t base, const P* x, const P* y,
+                                             const P* vx, const P* vy, const P* vz, const uint32_t* id, P* x2,
+                                             P* y2, P* vx2, P* vy2, P* vz2, uint32_t* id2) {
+  uint32_t p[SORT_U];
+  gather_sources(perm, n, base, p);
+  gather_move(p, n, base, x, y, vx, vy, vz, id, x2, y2, vx2, vy2, vz2, id2, perm, -1, p);
 }
 
 }  // namespace epic
