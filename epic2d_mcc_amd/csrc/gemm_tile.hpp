@@ -121,6 +121,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   tile_store<float>(t, part, tm, tn, C);
 }
 
+// FP32 for K = 32 (2j + 1) (K / 16 blocks not a multiple of 4): wavefront w
+// takes the contiguous blocks [w nb / 4, (w + 1) nb / 4) of the nb = K / 16,
+// at most NSTEP of them, every operand load issued before the first product
+// as in sgemm_tile32_kernel.  A wavefront's unused steps load an in-range
+// block and multiply zeros (selected, not branched around, so that no load
+// waits for another).
+template <int NSTEP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void sgemm_tile32_part_kernel(
+    TileGemm t, int K) {
+  __shared__ float part[4][32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tm = blockIdx.x * 32, tn = blockIdx.y * 32;
+  const float* A = t.A + blockIdx.z * t.sA;
+  const float* B = t.B + blockIdx.z * t.sB;
+  float* C = t.C + blockIdx.z * t.sC;
+  const int r16 = lane & 15, g = lane >> 4;
+  const int nb = K / 16, b0 = (w * nb) / 4, cw = ((w + 1) * nb) / 4 - b0;
+  v4f b[NSTEP][2];
+  float a[NSTEP][2][4];
+#pragma unroll
+  for (int st = 0; st < NSTEP; st++) {
+    const int kb = 16 * min(b0 + st, nb - 1);
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = tn + cb * 16 + r16;
+      b[st][cb] = *reinterpret_cast<const v4f*>(B + (size_t)col * t.ldb + kb + 4 * g);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        a[st][rb][e] = A[(size_t)(tm + rb * 16 + r16) + (size_t)(kb + 4 * g + e) * t.lda];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  v4f acc[2][2];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = 0; st < NSTEP; st++) {
+    const bool used = st < cw;
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(used ? a[st][rb][e] : 0.f, b[st][cb][e], acc[rb][cb],
+                                                              0, 0, 0);
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) part[w][(rb * 16 + 4 * g + r) + (cb * 16 + r16) * 32] = acc[rb][cb][r];
+  __syncthreads();
+  tile_store<float>(t, part, tm, tn, C);
+}
+
 // FP64: wavefront w takes the 16-deep k blocks w, w + 4, w + 8, ... (K/16
 // blocks in all), U of them per round (all loads of a round issued before
 // its products).
@@ -235,9 +296,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   tile_store<double, 8>(t, part, tm, tn, C);
 }
 
-// FP32 for K that is not a multiple of 64 or beyond the register-resident
-// slice (K > TILE32_MAX_K): the FP64 kernel's round structure, U 16-deep
-// blocks per round.
+// FP32 for K beyond the register-resident slice (K > TILE32_MAX_K): the
+// FP64 kernel's round structure, U 16-deep blocks per round.
 template <int U>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void sgemm_tile32_loop_kernel(
     TileGemm t, int K) {
@@ -293,8 +353,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   tile_store<float>(t, part, tm, tn, C);
 }
 
-// Largest K of the register-resident FP32 kernel (K = 64 NSTEP, NSTEP <= 9);
-// other K take the round-by-round kernel.
+// Largest K of the register-resident FP32 kernels (K = 64 NSTEP, NSTEP <= 9,
+// or K = 32 (2j + 1) up to it); larger K take the round-by-round kernel.
 constexpr int TILE32_MAX_K = 576;
 
 inline bool tile32_ok(int m, int n, int k) { return m % 32 == 0 && n % 32 == 0 && k % 32 == 0 && k >= 32; }
@@ -304,6 +364,16 @@ inline bool tile32_ok(int m, int n, int k) { return m % 32 == 0 && n % 32 == 0 &
 inline hipError_t gemm_tile32(hipStream_t st, int m, int n, int k, const TileGemmT<float>& t, int nbatch = 1) {
   if (!tile32_ok(m, n, k) || nbatch < 1) return hipErrorInvalidValue;
   const dim3 grid(m / 32, n / 32, nbatch);
+  if (k % 64 != 0 && k <= TILE32_MAX_K) {
+    switch ((k / 16 + 3) / 4) {
+#define EPIC_P(N) \
+  case N: hipLaunchKernelGGL(sgemm_tile32_part_kernel<N>, grid, dim3(256), 0, st, t, k); break;
+      EPIC_P(1) EPIC_P(2) EPIC_P(3) EPIC_P(4) EPIC_P(5) EPIC_P(6) EPIC_P(7) EPIC_P(8) EPIC_P(9)
+#undef EPIC_P
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   if (k > TILE32_MAX_K || k % 64 != 0) {
     hipLaunchKernelGGL(sgemm_tile32_loop_kernel<4>, grid, dim3(256), 0, st, t, k);
     return hipGetLastError();
