@@ -233,6 +233,7 @@ static hipError_t compact_t(Ctx* c, int s) {
   EPIC_HIP_TRY(hipMemcpyAsync(&nn, S.d_n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
   S.n_host = nn;
+  S.maybe_dead = false;  // compaction removed every NaN slot
   return hipSuccess;
 }
 
@@ -345,6 +346,7 @@ static hipError_t sort_t(Ctx* c, int s) {
   EPIC_HIP_TRY(hipMemcpyAsync(&nn, S.d_n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   EPIC_HIP_TRY(hipStreamSynchronize(c->stream));
   S.n_host = nn;
+  S.maybe_dead = false;  // the sort dropped every NaN slot
   return hipSuccess;
 }
 

@@ -434,6 +434,8 @@ int epic_upload_particles(epic_ctx* h, int s, const double* x, const double* y, 
   int64_t cnt[2] = {n, n};
   CK(hipMemcpy(S.d_n, cnt, sizeof(cnt), hipMemcpyHostToDevice), "upload count");
   S.n_host = n;
+  S.maybe_dead = false;
+  for (int64_t i = 0; i < n && !S.maybe_dead; i++) S.maybe_dead = std::isnan(x[i]);
   c->deposit_valid = false;
   return EPIC_OK;
 }
@@ -1022,6 +1024,14 @@ int epic_error_counters(epic_ctx* h, int64_t out[EPIC_NERR]) {
 int epic_clear_error_counters(epic_ctx* h) {
   if (!h) return EPIC_ERR_ARG;
   Ctx* c = &h->c;
+  // particles dropped as non-finite stay as NaN slots until their species'
+  // next sort or compaction; the collision pass learns of them from this count
+  unsigned long long nan_drops = 0;
+  CK(hipMemcpyAsync(&nan_drops, c->errc + EPIC_ERRC_NAN, sizeof(nan_drops), hipMemcpyDeviceToHost, c->stream),
+     "errc");
+  CK(hipStreamSynchronize(c->stream), "errc");
+  if (nan_drops || c->errc_host[EPIC_ERRC_NAN])
+    for (int s = 0; s < c->nsp; s++) c->sp[s].maybe_dead = true;
   CK(hipMemsetAsync(c->errc, 0, EPIC_NERR * sizeof(unsigned long long), c->stream), "errc");
   CK(hipStreamSynchronize(c->stream), "errc");
   for (int k = 0; k < EPIC_NERR; k++) c->errc_host[k] = 0;

@@ -58,6 +58,10 @@ struct SpeciesDev {
   int64_t cap = 0;
   int sort_mult = 1;        // sorted every dom.sort_interval * sort_mult steps
   int64_t n_host = 0;       // host copy of the live+dead slot count (authoritative unless births)
+  // absorbed (NaN x) slots may exist that neither the walls, the conductors
+  // nor the device's non-finite count account for: an uploaded NaN position,
+  // or a non-finite count cleared before this species' next sort/compaction
+  bool maybe_dead = false;
   void* x = nullptr;        // P*  (P = double | float)
   void* y = nullptr;
   void* vx = nullptr;

@@ -87,6 +87,10 @@ struct MccArgs {
   int* chunk_ev;   // electrons: stored ionisation events per range
   int waves;       // pass 1: wavefronts of the grid, one range each (0: MCC_RANGE_BATCH-slot ranges)
   unsigned long long* errc;  // EPIC_ERRC_MCC_BOUND: collision probability above p_null
+  // 1: the species holds no absorbed (NaN x) slot unless the push dropped a
+  // non-finite particle (errc[EPIC_ERRC_NAN] > 0): walls that do not absorb,
+  // no conductors, nothing uploaded or cleared since (SpeciesDev::maybe_dead)
+  int no_dead;
 };
 
 __device__ __forceinline__ void rotate(double g, double chi, double eta, double st, double ct, double sp, double cp,
@@ -128,16 +132,18 @@ __device__ __forceinline__ void neutral_velocity(const MccArgs<P>& a, uint32_t p
 // absorbed slot.
 // KIND: the species kind when fixed at compile time (0 electron, 1 ion), -1
 // to read it from the arguments.
-template <typename P, int KIND>
+// DEAD: absorbed slots (NaN x) may exist and are skipped; without, x is not
+// read (a quarter of the candidates' lines).
+template <typename P, int KIND, bool DEAD>
 __device__ __forceinline__ int mcc_null_test(const MccArgs<P>& a, uint32_t step, int64_t p) {
   // x, y, vx, vy at slot q (tiled FP32 storage), vz and id at p; every load
   // is issued before the first test (p is a live slot, so all are in bounds)
   const int64_t q = pslot<P>(p);
   const uint32_t pid = a.id[p];
-  const P xp = a.x[q];
+  const P xp = DEAD ? a.x[q] : (P)0;
   const double vxe = a.vx[q], vye = a.vy[q], vze = a.vz[p];
   const u32x4 r = philox4x32_10({pid, step, RNG_MCC, a.sp}, a.k0, a.k1);
-  if (isnan(xp)) return -1;
+  if (DEAD && isnan(xp)) return -1;
   const double R1 = u01_53(r.c, r.d);
   if ((KIND >= 0 ? KIND : a.kind) == 0) {
     // electron - neutral (neutral at rest)
@@ -292,14 +298,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 // one atomic.  Electron ionisations reserve their event slots here, in slot
 // order within the range, so each stored event gets its rank among the
 // range's stored events (ev[7]; nstored counts them).
-template <typename P, int KIND>
+template <typename P, int KIND, bool DEAD>
 __device__ __forceinline__ void mcc_null_round(const MccArgs<P>& a, uint32_t step, int64_t base, const uint16_t* cand,
                                                int t0, int todo, int& nstored) {
   const int lane = threadIdx.x & 63;
   const int kind = KIND >= 0 ? KIND : a.kind;
   const int t = t0 + lane;
   const int64_t p = base + (t < todo ? (int64_t)cand[t] : 0);
-  const int type = t < todo ? mcc_null_test<P, KIND>(a, step, p) : -1;
+  const int type = t < todo ? mcc_null_test<P, KIND, DEAD>(a, step, p) : -1;
   const unsigned long long real = __ballot(type >= 0);
   if (!real) return;
   int evi = -1;
@@ -342,7 +348,7 @@ __device__ __forceinline__ int64_t mcc_range(int64_t n, int waves) {
 // chunk_ev[rg] gets the range's stored electron ionisation events: the birth
 // append orders births by (range, rank), i.e. by parent slot, whatever order
 // the list has.
-template <typename P, int KIND>
+template <typename P, int KIND, bool DEAD>
 __device__ __forceinline__ void mcc_select_range(const MccArgs<P>& a, uint32_t step, int64_t rg, int64_t range,
                                                  int64_t n, uint16_t* cand) {
   const int lane = threadIdx.x & 63;
@@ -381,12 +387,12 @@ __device__ __forceinline__ void mcc_select_range(const MccArgs<P>& a, uint32_t s
   }
   wave_lds_sync();
   int nstored = 0;
-  for (int t0 = 0; t0 < cnt; t0 += 64) mcc_null_round<P, KIND>(a, step, base, cand, t0, cnt, nstored);
+  for (int t0 = 0; t0 < cnt; t0 += 64) mcc_null_round<P, KIND, DEAD>(a, step, base, cand, t0, cnt, nstored);
   if (kind == 0 && lane == 0) a.chunk_ev[rg] = nstored;
   wave_lds_sync();  // the next range overwrites the candidate list
 }
 
-template <typename P, int KIND>
+template <typename P, int KIND, bool DEAD>
 __device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t step, int bid, int nblk,
                                                 uint16_t* cand) {
   const int64_t n = a.d_n[0];
@@ -394,7 +400,7 @@ __device__ __forceinline__ void mcc_select_body(const MccArgs<P>& a, uint32_t st
   const int wpb = blockDim.x >> 6;
   const int64_t nr = (n + range - 1) / range;
   for (int64_t rg = (int64_t)bid * wpb + (threadIdx.x >> 6); rg < nr; rg += (int64_t)nblk * wpb)
-    mcc_select_range<P, KIND>(a, step, rg, range, n, cand);
+    mcc_select_range<P, KIND, DEAD>(a, step, rg, range, n, cand);
 }
 
 // Pass 2: the kinematics of the species' real collisions, one per lane over
@@ -431,7 +437,12 @@ struct MccPair {
 template <typename P, int KIND>
 __global__ __launch_bounds__(MCC_BLOCK) void mcc_select_kernel(MccArgs<P> a) {
   __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_RMAX];
-  mcc_select_body<P, KIND>(a, a.step, blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
+  // the push counts every particle it drops as non-finite (a NaN slot); the
+  // count is read after that push's launch, so it is complete here
+  if (a.no_dead && __builtin_amdgcn_readfirstlane((int)(a.errc[EPIC_ERRC_NAN] != 0)) == 0)
+    mcc_select_body<P, KIND, false>(a, a.step, blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
+  else
+    mcc_select_body<P, KIND, true>(a, a.step, blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
 }
 
 // OCC: workgroups per CU the register allocation is held to (1: none)
@@ -447,7 +458,7 @@ template <typename P>
 __global__ __launch_bounds__(MCC_BLOCK) void mcc_select_batch_kernel(const MccArgs<P>* args, const int* steps) {
   __shared__ uint16_t cand[MCC_BLOCK / 64][MCC_RMAX];
   const MccArgs<P>& a = args[blockIdx.y];
-  mcc_select_body<P, -1>(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
+  mcc_select_body<P, -1, true>(a, (uint32_t)steps[a.slot], blockIdx.x, gridDim.x, cand[threadIdx.x >> 6]);
 }
 
 template <typename P>
@@ -689,6 +700,7 @@ static MccArgs<P> mcc_args(Ctx* c, int s, int kind, int step) {
   a.list_n = M.list_n + kind;
   a.chunk_ev = kind == 0 ? M.chunk_ev : nullptr;
   a.errc = c->errc;
+  a.no_dead = d.bc != EPIC_BC_OPEN && c->cond.desc.empty() && !S.maybe_dead;
   return a;
 }
 
