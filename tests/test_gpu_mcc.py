@@ -470,3 +470,48 @@ def test_many_ionisations_per_step_appended_in_parent_order():
     assert len(xi) == 1 + nion
     assert np.array_equal(xi[1:], x[N:]) and np.array_equal(yi[1:], y[N:])
     ctx.close()
+
+
+@pytest.mark.parametrize("how", ["upload", "push", "push+clear"])
+def test_absorbed_slots_never_collide(how):
+    """A slot holding an absorbed particle (NaN x) takes no collision, on
+    periodic walls where the selection pass skips reading the candidates' x
+    when the species provably holds no such slot.  The NaN comes with an
+    upload, or from a push that drops particles with a non-finite velocity
+    (counted on the device as EPIC_ERRC_NAN), or from such a push whose count
+    was cleared before the collision step.  Elastic only: a selected absorbed
+    slot would add to the collision count, which must equal the number of
+    live particles whose velocity changed."""
+    print(f"[mcc] absorbed slots never collide ({how})", flush=True)
+    N, gas = 30000, 2e22
+    st = _setup(N, 8.0)
+    d = st.domain
+    m = xsec.mcc_desc(xsec.synthetic_ar(), gas_density=gas, gas_temp_K=0.0, enable=(1, 0, 0, 0, 0))
+    x, y, vx, vy, vz = [np.array(a) for a in _electrons(st, N, 8.0, 5)]
+    dead = np.zeros(N, dtype=bool)
+    dead[::3] = True
+    if how == "upload":
+        x[dead] = np.nan
+    else:
+        vx[dead] = np.nan
+    ctx = _lib.Context(d, st.species)
+    ctx.upload(0, x, y, vx, vy, vz)
+    ctx.upload(1, [1.0], [1.0], [0.0], [0.0], [0.0])
+    ctx.mcc_setup(m)
+    if how != "upload":
+        ctx.push(0)
+        assert int(ctx.errors()[5]) == int(dead.sum())  # EPIC_ERRC_NAN
+        if how == "push+clear":
+            ctx.clear_errors()
+            assert int(ctx.errors()[5]) == 0
+    # the download returns the live particles only
+    _, _, vx0, vy0, vz0, ids0 = ctx.download(0)
+    assert len(ids0) == N - int(dead.sum())
+    ctx.collide(7)
+    _, _, vx1, vy1, vz1, ids1 = ctx.download(0)
+    count = ctx.mcc_stats()["count"]
+    ctx.close()
+    assert np.array_equal(ids0, ids1)
+    hit = (vx1 != vx0) | (vy1 != vy0) | (vz1 != vz0)
+    assert hit.sum() > 1000
+    assert int(count[0]) == int(hit.sum())
